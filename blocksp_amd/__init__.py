@@ -1,0 +1,364 @@
+"""blocksp_amd -- MI355X-native block-sparse solve path (BlockSp drop-in).
+
+Python bindings over the C ABI of libblocksp_amd.so (include/blocksp_c.h).
+The names follow the reference's surface (blocksp_solvers.hpp,
+blocksp_preconditioners.hpp): ``bsmv``, ``pGMRES``, ``pCG``, and the
+preconditioners ``NoPre``, ``Diag``, ``SOR``, ``ILU0``, ``IChol0`` whose
+``__call__`` overwrites a device vector r with M^{-1} r.
+
+All compute runs in the HIP library; loading fails loudly if it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "BspError", "lib", "Context", "DeviceArray", "BSR", "NoPre", "Diag", "SOR", "ILU0", "IChol0",
+    "bsmv", "bsmv_dim", "bsmm_rhs", "pGMRES", "pCG", "Stats", "TOL_REFERENCE", "TOL_TRUE_RESIDUAL",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libblocksp_amd.so")
+
+TOL_REFERENCE = 0
+TOL_TRUE_RESIDUAL = 1
+
+_ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
+           -5: "numerical failure", -6: "sync-free wait timed out"}
+
+
+class BspError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"blocksp_amd error {code} ({_ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
+                ("precond_applies", C.c_int), ("res0", C.c_double), ("res_est", C.c_double),
+                ("true_relres", C.c_double), ("setup_ms", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_VP = C.c_void_p
+_SIGS = {
+    "bsp_ctx_create": [C.c_int, _VP, C.POINTER(_VP)],
+    "bsp_ctx_destroy": [_VP],
+    "bsp_ctx_sync": [_VP],
+    "bsp_ctx_set_option": [_VP, C.c_char_p, C.c_int64],
+    "bsp_malloc": [_VP, C.c_size_t, C.POINTER(_VP)],
+    "bsp_free": [_VP, _VP],
+    "bsp_memcpy_h2d": [_VP, _VP, _VP, C.c_size_t],
+    "bsp_memcpy_d2h": [_VP, _VP, _VP, C.c_size_t],
+    "bsp_memcpy_d2d": [_VP, _VP, _VP, C.c_size_t],
+    "bsp_memset_zero": [_VP, _VP, C.c_size_t],
+    "bsp_event_record": [_VP, C.c_int],
+    "bsp_event_elapsed_ms": [_VP, C.c_int, C.c_int, C.POINTER(C.c_float)],
+    "bsp_bsr_create": [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_bsr_destroy": [_VP],
+    "bsp_bsr_info": [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)],
+    "bsp_bsr_download": [_VP, _VP, _VP, _VP, _VP],
+    "bsp_bsr_set_values_dev": [_VP, _VP, _VP],
+    "bsp_bsmv": [_VP, _VP, _VP, _VP, C.c_int],
+    "bsp_bsmv_dim": [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int],
+    "bsp_bsmv_transpose": [_VP, _VP, _VP, _VP, C.c_int],
+    "bsp_bsmm_rhs": [_VP, _VP, C.c_int, _VP, _VP],
+    "bsp_bsmm": [_VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_dot": [_VP, C.c_int64, _VP, _VP, C.POINTER(C.c_double)],
+    "bsp_nrm2": [_VP, C.c_int64, _VP, C.POINTER(C.c_double)],
+    "bsp_infnorm": [_VP, C.c_int64, _VP, C.POINTER(C.c_double)],
+    "bsp_axpy": [_VP, C.c_int64, C.c_double, _VP, _VP],
+    "bsp_scal": [_VP, C.c_int64, C.c_double, _VP],
+    "bsp_precond_create_none": [_VP, _VP, C.POINTER(_VP)],
+    "bsp_precond_create_diag": [_VP, _VP, C.c_int, C.POINTER(_VP)],
+    "bsp_precond_create_sor": [_VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)],
+    "bsp_precond_create_ilu0": [_VP, _VP, C.POINTER(_VP)],
+    "bsp_precond_create_ichol0": [_VP, _VP, C.POINTER(_VP)],
+    "bsp_precond_destroy": [_VP],
+    "bsp_precond_apply": [_VP, _VP, _VP],
+    "bsp_precond_levels": [_VP, C.c_int, _VP, C.POINTER(C.c_int)],
+    "bsp_precond_factor_info": [_VP, C.POINTER(C.c_int64)],
+    "bsp_precond_download_factor": [_VP, _VP, _VP, _VP, _VP],
+    "bsp_pgmres": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Stats)],
+    "bsp_pcg": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.POINTER(Stats)],
+}
+
+
+def lib():
+    """Load libblocksp_amd.so (build it with `python -m blocksp_amd.build`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m blocksp_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.bsp_last_error.argtypes = [_VP]
+        L.bsp_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS) + ["bsp_last_error"]
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """A device + stream (bsp_ctx)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.bsp_ctx_create(device, stream, C.byref(h))
+        if rc != 0:
+            raise BspError(rc, f"bsp_ctx_create(device={device}) failed")
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc < 0:
+            raise BspError(rc, f"{what}: {self.L.bsp_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bsp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.bsp_ctx_sync(self.h), "sync")
+
+    def set_option(self, name: str, value: int):
+        self.check(self.L.bsp_ctx_set_option(self.h, name.encode(), int(value)), "set_option")
+
+    def empty(self, shape, dtype=np.float64):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def zeros(self, shape, dtype=np.float64):
+        d = DeviceArray(self, shape, dtype)
+        d.zero()
+        return d
+
+    def record(self, slot):
+        self.check(self.L.bsp_event_record(self.h, slot), "event_record")
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        self.check(self.L.bsp_event_elapsed_ms(self.h, a, b, C.byref(ms)), "event_elapsed")
+        return ms.value
+
+
+class DeviceArray:
+    """Device memory owned by Python (bsp_malloc / bsp_free)."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.check(ctx.L.bsp_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)), "malloc")
+        self.ptr = p
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        self.ctx.check(self.ctx.L.bsp_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a), self.nbytes), "h2d")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(self.ctx.L.bsp_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, self.nbytes), "d2h")
+        return out
+
+    def zero(self):
+        self.ctx.check(self.ctx.L.bsp_memset_zero(self.ctx.h, self.ptr, self.nbytes), "memset")
+
+    def copy_from(self, other: "DeviceArray"):
+        assert other.nbytes == self.nbytes
+        self.ctx.check(self.ctx.L.bsp_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes), "d2d")
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.ctx.L.bsp_free(self.ctx.h, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class BSR:
+    """BlockSp<T,B*D> on the device (bsp_bsr)."""
+
+    def __init__(self, ctx: Context, n_rows, n_cols, bs, rowptr, colidx, vals):
+        self.ctx = ctx
+        self.n_rows, self.n_cols, self.bs = int(n_rows), int(n_cols), int(bs)
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        ci = np.ascontiguousarray(colidx, np.int32)
+        v = np.ascontiguousarray(vals, np.float64)
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_bsr_create(ctx.h, self.n_rows, self.n_cols, self.bs, _ptr(rp), _ptr(ci), _ptr(v),
+                                       C.byref(h)), "bsr_create")
+        self.h = h
+        self.nnzb = int(ci.shape[0])
+
+    @classmethod
+    def from_problem(cls, ctx, p):
+        return cls(ctx, p.n, p.n, p.bs, p.rowptr, p.colidx, p.vals)
+
+    def download(self):
+        rp = np.empty(self.n_rows + 1, np.int32)
+        ci = np.empty(self.nnzb, np.int32)
+        v = np.empty((self.nnzb, self.bs, self.bs))
+        self.ctx.check(self.ctx.L.bsp_bsr_download(self.ctx.h, self.h, _ptr(rp), _ptr(ci), _ptr(v)), "bsr_download")
+        return rp, ci, v
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.L.bsp_bsr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def bsmv(ctx, A: BSR, x: DeviceArray, y: DeviceArray, accumulate=False):
+    """y (+)= A x  (bsmv, blocksp_multiply.hpp:365-390)."""
+    ctx.check(ctx.L.bsp_bsmv(ctx.h, A.h, x.ptr, y.ptr, int(accumulate)), "bsmv")
+
+
+def bsmv_dim(ctx, A: BSR, D, dim, x: DeviceArray, y: DeviceArray, accumulate=False):
+    ctx.check(ctx.L.bsp_bsmv_dim(ctx.h, A.h, int(D), int(dim), x.ptr, y.ptr, int(accumulate)), "bsmv_dim")
+
+
+def bsmm_rhs(ctx, A: BSR, nrhs, X: DeviceArray, Y: DeviceArray):
+    ctx.check(ctx.L.bsp_bsmm_rhs(ctx.h, A.h, int(nrhs), X.ptr, Y.ptr), "bsmm_rhs")
+
+
+class _Precond:
+    kind = None
+
+    def __call__(self, r: DeviceArray):
+        self.ctx.check(self.ctx.L.bsp_precond_apply(self.ctx.h, self.h, r.ptr), "precond_apply")
+
+    def levels(self, lower=True):
+        n = self.A.n_rows
+        lev = np.empty(n, np.int32)
+        nl = C.c_int()
+        self.ctx.check(self.ctx.L.bsp_precond_levels(self.h, int(lower), _ptr(lev), C.byref(nl)), "levels")
+        return nl.value, lev
+
+    def factor(self):
+        nz = C.c_int64()
+        self.ctx.check(self.ctx.L.bsp_precond_factor_info(self.h, C.byref(nz)), "factor_info")
+        rp = np.empty(self.A.n_rows + 1, np.int32)
+        ci = np.empty(nz.value, np.int32)
+        v = np.empty((nz.value, self.A.bs, self.A.bs))
+        self.ctx.check(self.ctx.L.bsp_precond_download_factor(self.ctx.h, self.h, _ptr(rp), _ptr(ci), _ptr(v)),
+                       "download_factor")
+        return rp, ci, v
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.L.bsp_precond_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class NoPre(_Precond):
+    """preConditioners::NoPre (blocksp_preconditioners.hpp:66-71)."""
+
+    def __init__(self, ctx, A):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_precond_create_none(ctx.h, A.h, C.byref(h)), "NoPre")
+        self.h = h
+
+
+class Diag(_Precond):
+    """preConditioners::Diag<Inv> (blocksp_preconditioners.hpp:75-111); cholesky=True for CholeskyFactor."""
+
+    def __init__(self, ctx, A, cholesky=False):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_precond_create_diag(ctx.h, A.h, int(cholesky), C.byref(h)), "Diag")
+        self.h = h
+
+
+class SOR(_Precond):
+    """preConditioners::SOR<Inv>(A, w, nu, type) (blocksp_preconditioners.hpp:116-240)."""
+
+    def __init__(self, ctx, A, w, nu, type=0, cholesky=False):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_precond_create_sor(ctx.h, A.h, float(w), int(nu), int(type), int(cholesky), C.byref(h)),
+                  "SOR")
+        self.h = h
+
+
+class ILU0(_Precond):
+    """preConditioners::ILU0(A) (blocksp_preconditioners.hpp:246-357)."""
+
+    def __init__(self, ctx, A):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_precond_create_ilu0(ctx.h, A.h, C.byref(h)), "ILU0")
+        self.h = h
+
+
+class IChol0(_Precond):
+    """preConditioners::IChol0(A) (blocksp_preconditioners.hpp:364-538)."""
+
+    def __init__(self, ctx, A):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_precond_create_ichol0(ctx.h, A.h, C.byref(h)), "IChol0")
+        self.h = h
+
+
+def pGMRES(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=500, restart=-1, k=-1, tol_mode=TOL_REFERENCE, x=None):
+    """pGMRES (blocksp_solvers.hpp:156-295). Returns (x DeviceArray, Stats, return code)."""
+    x = x if x is not None else ctx.empty(b.shape)
+    st = Stats()
+    rc = ctx.L.bsp_pgmres(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(restart), int(k),
+                          int(tol_mode), C.byref(st))
+    ctx.check(rc, "pgmres")
+    return x, st, rc
+
+
+def pCG(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=1000, tol_mode=TOL_REFERENCE, x=None):
+    """pCG (blocksp_solvers.hpp:51-105). Returns (x DeviceArray, Stats, return code)."""
+    x = x if x is not None else ctx.empty(b.shape)
+    st = Stats()
+    rc = ctx.L.bsp_pcg(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(tol_mode), C.byref(st))
+    ctx.check(rc, "pcg")
+    return x, st, rc

@@ -1,0 +1,188 @@
+// bsp_internal.hpp -- internal types of libblocksp_amd (MI355X / gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/blocksp_c.h"
+
+struct bsp_ctx;
+
+namespace bsp {
+
+// Records `msg` as the context's last error and returns `code`.
+int fail(const bsp_ctx* ctx, int code, const std::string& msg);
+
+#define BSP_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::bsp::fail(ctx, BSP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define BSP_CHECK(cond, code, msg)                       \
+    do {                                                 \
+        if (!(cond)) return ::bsp::fail(ctx, code, msg); \
+    } while (0)
+
+// Device allocation owned by a context (freed with hipFree).
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t b) {
+        release();
+        bytes = b;
+        if (b == 0) return hipSuccess;
+        return hipMalloc(&p, b);
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Strictly-triangular (or full) block structure on the device plus the level
+// schedule used by the sweep kernels.
+struct Sweep {
+    int n = 0, bs = 0;
+    int64_t nnzb = 0;
+    DevMem rp, ci, val;          // owned block rows (original per-row order), if any
+    const int* drp = nullptr;    // the structure the kernels read: the owned arrays or
+    const int* dci = nullptr;    // (SOR) the matrix's own arrays
+    const double* dval = nullptr;
+    // schedule: rows sorted by level, each level padded with -1 to a multiple of
+    // the rows handled by one wavefront, so a wavefront never holds two levels.
+    DevMem order;                // int32 [nslots]
+    std::vector<int> level_ptr;  // host: slot offsets of each level (padded)
+    std::vector<int> level_of;   // host: level of each row (unpadded)
+    int nslots = 0;
+    int nlevels = 0;
+    DevMem flags;                // uint32 [n], epoch-stamped completion flags
+    DevMem ticket;               // uint64 workgroup ticket counter
+};
+
+}  // namespace bsp
+
+struct bsp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    mutable std::string last_error;
+    int num_cus = 256;
+    // options
+    int trsv_mode = 1;               // 0 level launches, 1 sync-free single launch
+    int64_t spin_limit = int64_t(1) << 22;
+    int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
+    // reduction scratch
+    bsp::DevMem red_partials;        // double [kMaxRedBlocks]
+    bsp::DevMem red_counter;         // uint32 [kRedSlots]
+    bsp::DevMem scalars;             // double [kScalarSlots]
+    bsp::DevMem err_flag;            // uint32 [1], sync-free wait timeouts
+    double* pinned = nullptr;        // host pinned [kScalarSlots]
+    hipEvent_t events[16] = {};
+    static constexpr int kMaxRedBlocks = 1024;
+    static constexpr int kScalarSlots = 4096;
+};
+
+struct bsp_bsr {
+    bsp_ctx* ctx = nullptr;
+    int n_rows = 0, n_cols = 0, bs = 0;
+    int64_t nnzb = 0;
+    bsp::DevMem rp, ci, val;
+    std::vector<int32_t> h_rp, h_ci;   // host copy of the structure
+    std::vector<double> h_val;         // host copy of the values (factorisations start here)
+    // lazily built transpose (for bsmv_transpose)
+    std::unique_ptr<bsp_bsr> transposed;
+};
+
+struct bsp_precond {
+    bsp_ctx* ctx = nullptr;
+    int kind = BSP_PRE_NONE;
+    int n = 0, bs = 0;
+    const bsp_bsr* A = nullptr;       // SOR / Diag: non-owning (reference keeps a pointer too)
+    double w = 1.0;
+    int nu = 0, type = 0, inv_kind = 0;
+    bsp::Sweep lower, upper;          // ILU0/IChol0 factors (strict parts) or SOR schedules
+    bsp::DevMem dinv_lower;           // [n][bs][bs] applied after the forward sweep (IChol0) or none
+    bsp::DevMem dinv_upper;           // [n][bs][bs] applied after the backward sweep / SOR / Diag
+    bsp::DevMem work0, work1;         // vectors for SOR ping-pong
+    uint32_t epoch = 0;
+    // host copy of the factor in the reference layout (for parity download)
+    std::vector<int32_t> f_rp, f_ci;
+    std::vector<double> f_val;
+};
+
+namespace bsp {
+
+// ---- kernel launchers (bsp_kernels.hip) ----
+int launch_bsmv(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* val,
+                const double* x, double* y, int accumulate);
+int launch_bsmv_dim(const bsp_ctx* ctx, int n, int B, int D, int dim, const int* rp, const int* ci,
+                    const double* val, const double* x, double* y, int accumulate);
+int launch_bsmm_rhs(const bsp_ctx* ctx, int n, int bs, int nrhs, const int* rp, const int* ci,
+                    const double* val, const double* X, double* Y);
+int launch_block_diag(const bsp_ctx* ctx, int n, int bs, const double* dinv, double* x);
+
+// Deterministic reductions; the result lands in device scalar slot `slot`
+// (ctx->scalars) and, if host_out != nullptr, is copied back synchronously.
+// kind: 0 dot(x,y), 1 sqrt(|dot(x,x)|), 2 max|x|.
+int reduce(const bsp_ctx* ctx, int kind, int64_t len, const double* x, const double* y, int slot,
+           double* host_out);
+int launch_axpy(const bsp_ctx* ctx, int64_t len, double a, const double* x, double* y);
+int launch_scal(const bsp_ctx* ctx, int64_t len, double a, double* x);
+
+// Sweeps: trsv computes x_i <- Dinv_i (x_i - sum_j T_ij x_j) over the schedule.
+int launch_trsv(bsp_ctx* ctx, Sweep& S, const double* dinv, double* x, uint32_t epoch);
+// SOR sweep: xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j!=i} A_ij x_j)), x_j from xn on the
+// dependency side (forward: j < i) and from xo otherwise.
+int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const double* dinv,
+                     const double* b, const double* xo, double* xn, uint32_t epoch);
+
+// Krylov helpers (all operands device-resident; h values read from device memory).
+// mgs step: if qprev: v -= h[hprev_slot] * qprev;  then h[hout_slot] = dot(qnext, v) or
+// (qnext == nullptr) sqrt(dot(v, v)).
+int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qprev, const double* h_prev,
+                    const double* qnext, double* h_out);
+// q = v / h[0]
+int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
+// x += sum_i c[i] * Q[i]  (c on device, m vectors)
+int launch_combine_add(const bsp_ctx* ctx, int64_t len, int m, const double* const* Q, const double* c,
+                       double* x);
+// Qout[j] = sum_i Q[i] * P[i*kp + j]  (P on device, m x kp)
+int launch_basis_rotate(const bsp_ctx* ctx, int64_t len, int m, int kp, const double* const* Q,
+                        const double* P, double* const* Qout);
+// r = b - A x  (overwrite)
+int launch_residual(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* val,
+                    const double* b, const double* x, double* r);
+int launch_copy(const bsp_ctx* ctx, int64_t len, const double* x, double* y);
+int launch_fill_u32(const bsp_ctx* ctx, int64_t len, uint32_t v, uint32_t* p);
+// pCG updates: x += alpha p; r -= alpha Ap with alpha = num/den from device scalars
+int launch_cg_update_xr(const bsp_ctx* ctx, int64_t len, const double* num, const double* den,
+                        const double* p, const double* Ap, double* x, double* r);
+// p = z + (num/den) p
+int launch_cg_update_p(const bsp_ctx* ctx, int64_t len, const double* num, const double* den,
+                       const double* z, double* p);
+
+// Scalar slots used by the solvers.
+enum : int {
+    SLOT_TMP0 = 0,
+    SLOT_TMP1 = 1,
+    SLOT_TMP2 = 2,
+    SLOT_TMP3 = 3,
+    SLOT_H = 64,      // Hessenberg column scratch (<= 1024 entries)
+    SLOT_Y = 1200,    // combination coefficients
+    SLOT_P = 1400,    // restart rotation matrix (<= 2048 entries)
+};
+
+}  // namespace bsp

@@ -1,0 +1,881 @@
+// bsp_kernels.hip -- CDNA4 (gfx950) kernels of the block-sparse solve path.
+//
+// Mapping used throughout: a block row of a BSR matrix with block size BS is
+// handled by BS consecutive lanes of a 64-wide wavefront (lane r owns row r of
+// every block of that block row), so one wavefront covers 64/BS block rows.
+// Lane r reads the BS contiguous doubles of its block row with 16-byte loads;
+// the BS lanes of a group therefore stream the whole BS*BS block contiguously.
+// Within a block row, lane r accumulates sum_blocks (sum_c a_rc x_c) in the
+// reference's order: blocks in stored order, columns c = 0..BS-1
+// (blocksp_multiply.hpp:289-325, blocksp_dense_multiply.hpp:29-48), with FMA.
+#include "bsp_internal.hpp"
+
+#include <algorithm>
+
+namespace bsp {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+template <int BS>
+__device__ __forceinline__ void load_vec(const double* __restrict__ src, double (&a)[BS]) {
+    if constexpr (BS % 2 == 0) {
+        const double2* s2 = reinterpret_cast<const double2*>(src);
+#pragma unroll
+        for (int c = 0; c < BS / 2; ++c) {
+            double2 t = s2[c];
+            a[2 * c] = t.x;
+            a[2 * c + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) a[c] = src[c];
+    }
+}
+
+template <int BS>
+__device__ __forceinline__ double row_dot(const double (&a)[BS], const double (&x)[BS]) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) s = fma(a[c], x[c], s);
+    return s;
+}
+
+// Apply a BS x BS block (row r of it held by lane r of the group) to the
+// group's vector s (element c held by lane base+c): returns sum_c D[r][c] s_c.
+template <int BS>
+__device__ __forceinline__ double group_matvec(const double* __restrict__ D, int r, int base, double s) {
+    double d[BS];
+    load_vec<BS>(D + r * BS, d);
+    double out = 0.0;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) out = fma(d[c], __shfl(s, base + c, kWave), out);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// bsmv
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bsmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const double* __restrict__ val, const double* __restrict__ x,
+                                                 double* __restrict__ y, int accumulate) {
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    double acc = accumulate ? y[(size_t)i * BS + r] : 0.0;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    int p = p0;
+    for (; p + 1 < p1; p += 2) {
+        const int c0 = ci[p], c1 = ci[p + 1];
+        double a0[BS], a1[BS], x0[BS], x1[BS];
+        load_vec<BS>(val + ((size_t)p * BS + r) * BS, a0);
+        load_vec<BS>(val + ((size_t)(p + 1) * BS + r) * BS, a1);
+        load_vec<BS>(x + (size_t)c0 * BS, x0);
+        load_vec<BS>(x + (size_t)c1 * BS, x1);
+        acc += row_dot<BS>(a0, x0);
+        acc += row_dot<BS>(a1, x1);
+    }
+    if (p < p1) {
+        const int c0 = ci[p];
+        double a0[BS], x0[BS];
+        load_vec<BS>(val + ((size_t)p * BS + r) * BS, a0);
+        load_vec<BS>(x + (size_t)c0 * BS, x0);
+        acc += row_dot<BS>(a0, x0);
+    }
+    y[(size_t)i * BS + r] = acc;
+}
+
+// Runtime block size (bs <= 64): same mapping, scalar loads.
+__global__ __launch_bounds__(kBlock) void k_bsmv_generic(int n, int bs, const int* __restrict__ rp,
+                                                         const int* __restrict__ ci,
+                                                         const double* __restrict__ val,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         int accumulate) {
+    const int rpw = kWave / bs;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int g = lane / bs;
+    const int r = lane - g * bs;
+    if (g >= rpw) return;
+    const int i = wave * rpw + g;
+    if (i >= n) return;
+    double acc = accumulate ? y[(size_t)i * bs + r] : 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        const double* a = val + ((size_t)p * bs + r) * bs;
+        const double* xc = x + (size_t)ci[p] * bs;
+        double s = 0.0;
+        for (int c = 0; c < bs; ++c) s = fma(a[c], xc[c], s);
+        acc += s;
+    }
+    y[(size_t)i * bs + r] = acc;
+}
+
+// Per-dimension product: A has blocks B x B; x, y have n*B*D entries; each of the
+// D dimension slices of a block row is a separate lane group.
+__global__ __launch_bounds__(kBlock) void k_bsmv_dim(int n, int B, int D, int dim, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const double* __restrict__ val,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     int accumulate) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t total = (int64_t)n * D * B;
+    if (t >= total) return;
+    const int r = (int)(t % B);
+    const int d = (int)((t / B) % D);
+    const int i = (int)(t / ((int64_t)B * D));
+    if (dim >= 0 && d != dim) return;
+    const int stride = B * D;
+    double acc = accumulate ? y[(size_t)i * stride + d * B + r] : 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        const double* a = val + ((size_t)p * B + r) * B;
+        const double* xc = x + (size_t)ci[p] * stride + d * B;
+        double s = 0.0;
+        for (int c = 0; c < B; ++c) s = fma(a[c], xc[c], s);
+        acc += s;
+    }
+    y[(size_t)i * stride + d * B + r] = acc;
+}
+
+// Multi-RHS: Y[(i*bs + r)*nrhs + k] = sum_blocks sum_c a_rc X[(col*bs + c)*nrhs + k].
+// One thread per (row entry, rhs); RHS-minor layout keeps the X reads coalesced.
+__global__ __launch_bounds__(kBlock) void k_bsmm_rhs(int n, int bs, int nrhs, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const double* __restrict__ val,
+                                                     const double* __restrict__ X, double* __restrict__ Y) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t total = (int64_t)n * bs * nrhs;
+    if (t >= total) return;
+    const int k = (int)(t % nrhs);
+    const int64_t er = t / nrhs;
+    const int r = (int)(er % bs);
+    const int i = (int)(er / bs);
+    double acc = 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        const double* a = val + ((size_t)p * bs + r) * bs;
+        const double* xc = X + (size_t)ci[p] * bs * nrhs + k;
+        double s = 0.0;
+        for (int c = 0; c < bs; ++c) s = fma(a[c], xc[(size_t)c * nrhs], s);
+        acc += s;
+    }
+    Y[t] = acc;
+}
+
+// x_i <- D_i x_i for every block row (Diag preconditioner).
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_block_diag(int n, const double* __restrict__ dinv, double* __restrict__ x) {
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    const int i = wave * RPW + g;
+    const bool active = g < RPW && i < n;
+    double s = active ? x[(size_t)i * BS + r] : 0.0;
+    double out = 0.0;
+    if (active) out = 0.0;
+    // all lanes take part in the shuffles
+    const int base = g * BS;
+    {
+        double d[BS];
+        if (active) load_vec<BS>(dinv + ((size_t)i * BS + r) * BS, d);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            double sc = __shfl(s, (base + c) & (kWave - 1), kWave);
+            if (active) out = fma(d[c], sc, out);
+        }
+    }
+    if (active) x[(size_t)i * BS + r] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void k_block_diag_generic(int n, int bs, const double* __restrict__ dinv,
+                                                               const double* __restrict__ xin,
+                                                               double* __restrict__ xout) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)n * bs) return;
+    const int r = (int)(t % bs);
+    const int64_t i = t / bs;
+    const double* d = dinv + ((size_t)i * bs + r) * bs;
+    const double* s = xin + (size_t)i * bs;
+    double out = 0.0;
+    for (int c = 0; c < bs; ++c) out = fma(d[c], s[c], out);
+    xout[t] = out;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (deterministic: fixed grid, fixed partial order, last block sums)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+template <int KIND>
+__device__ __forceinline__ double block_reduce(double v, double* sh) {
+    v = KIND == 2 ? wave_max(v) : wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = sh[0];
+        for (int i = 1; i < kBlock / kWave; ++i) r = KIND == 2 ? fmax(r, sh[i]) : r + sh[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+// Finish a grid reduction: every block stores its partial; the last block to
+// arrive (counted with a device-scope atomic) combines the partials in block
+// order and writes *out.
+template <int KIND>
+__device__ __forceinline__ void grid_finish(double part, double* partials, unsigned* counter, double* out,
+                                            double* sh) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = part;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double v = KIND == 2 ? 0.0 : 0.0;
+    // fixed order: thread t sums partials t, t+256, ...; then block tree
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+        double pv = __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = KIND == 2 ? fmax(v, pv) : v + pv;
+    }
+    v = block_reduce<KIND>(v, sh);
+    if (threadIdx.x == 0) {
+        if (KIND == 1) v = sqrt(fabs(v));
+        *out = v;
+        *counter = 0u;
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_reduce(int64_t len, const double* __restrict__ x,
+                                                   const double* __restrict__ y, double* partials,
+                                                   unsigned* counter, double* out) {
+    __shared__ double sh[kBlock / kWave];
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        if (KIND == 0) s = fma(x[i], y[i], s);
+        else if (KIND == 1) s = fma(x[i], x[i], s);
+        else s = fmax(s, fabs(x[i]));
+    }
+    s = block_reduce<KIND>(s, sh);
+    grid_finish<KIND>(s, partials, counter, out, sh);
+}
+
+// v -= h_prev * qprev (if qprev), then out = dot(qnext, v) or ||v||.
+__global__ __launch_bounds__(kBlock) void k_mgs_step(int64_t len, double* __restrict__ v,
+                                                     const double* __restrict__ qprev,
+                                                     const double* __restrict__ h_prev,
+                                                     const double* __restrict__ qnext, double* partials,
+                                                     unsigned* counter, double* out) {
+    __shared__ double sh[kBlock / kWave];
+    const double h = qprev ? *h_prev : 0.0;
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        double vi = v[i];
+        if (qprev) {
+            vi = vi + (-h) * qprev[i];
+            v[i] = vi;
+        }
+        s = qnext ? fma(qnext[i], vi, s) : fma(vi, vi, s);
+    }
+    s = block_reduce<0>(s, sh);
+    if (qnext) grid_finish<0>(s, partials, counter, out, sh);
+    else grid_finish<1>(s, partials, counter, out, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy(int64_t len, double a, const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scal(int64_t len, double a, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) x[i] *= a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_div(int64_t len, const double* __restrict__ v,
+                                                      const double* __restrict__ h, double* __restrict__ q) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) q[i] = v[i] / *h;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t len, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) y[i] = x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(int64_t len, uint32_t v, uint32_t* __restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) p[i] = v;
+}
+
+constexpr int kMaxBasis = 64;
+
+struct PtrTable {
+    const double* q[kMaxBasis];
+};
+struct OutTable {
+    double* q[kMaxBasis];
+};
+
+// x += sum_i c_i Q_i, summed in index order (solvers.hpp:290-294).
+__global__ __launch_bounds__(kBlock) void k_combine_add(int64_t len, int m, PtrTable Q, const double* __restrict__ c,
+                                                        double* __restrict__ x) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= len) return;
+    double acc = x[e];
+    for (int i = 0; i < m; ++i) acc += c[i] * Q.q[i][e];
+    x[e] = acc;
+}
+
+// Qout_j = sum_i Q_i P_ij (deflated_restart.hpp:250-258), m <= 64, kp <= 64.
+__global__ __launch_bounds__(kBlock) void k_basis_rotate(int64_t len, int m, int kp, PtrTable Q,
+                                                         const double* __restrict__ P, OutTable Out) {
+    __shared__ double sP[kMaxBasis * 16];
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int j0 = 0; j0 < kp; j0 += 16) {
+        const int jn = min(16, kp - j0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < m * jn; t += kBlock) sP[t] = P[(t / jn) * kp + j0 + (t % jn)];
+        __syncthreads();
+        if (e < len) {
+            double acc[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double qv = Q.q[i][e];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j < jn) acc[j] = fma(qv, sP[i * jn + j], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < jn) Out.q[j0 + j][e] = acc[j];
+        }
+    }
+}
+
+// r = b - A x
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                     const double* __restrict__ val, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r_out) {
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        double a[BS], xv[BS];
+        load_vec<BS>(val + ((size_t)p * BS + r) * BS, a);
+        load_vec<BS>(x + (size_t)ci[p] * BS, xv);
+        acc += row_dot<BS>(a, xv);
+    }
+    r_out[(size_t)i * BS + r] = b[(size_t)i * BS + r] - acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(int64_t len, const double* __restrict__ num,
+                                                         const double* __restrict__ den, const double* __restrict__ p,
+                                                         const double* __restrict__ Ap, double* __restrict__ x,
+                                                         double* __restrict__ r) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= len) return;
+    const double alpha = *num / *den;
+    x[i] += alpha * p[i];
+    r[i] -= alpha * Ap[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(int64_t len, const double* __restrict__ num,
+                                                        const double* __restrict__ den, const double* __restrict__ z,
+                                                        double* __restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= len) return;
+    const double beta = *num / *den;
+    p[i] = z[i] + beta * p[i];
+}
+
+// ---------------------------------------------------------------------------
+// Triangular and SOR sweeps over a level schedule
+// ---------------------------------------------------------------------------
+// Slot s of the (padded) schedule holds row order[s] (or -1).  Each lane group
+// handles one slot.  In the sync-free variant every workgroup first takes a
+// ticket (a device-scope counter) and processes the slots of that ticket, so
+// any row it waits on belongs to a workgroup that is already running; a level
+// never shares a wavefront with another level (the schedule is padded), so no
+// wavefront waits on itself.  Completion of row i is published by a release
+// store flags[i] = epoch after x_i is written; readers acquire the flag before
+// reading x_j.  Waits are bounded by spin_limit polls; on timeout the row sets
+// *err and proceeds, so every wavefront always exits.
+
+__device__ __forceinline__ bool wait_flag(const uint32_t* flags, int j, uint32_t epoch, int64_t spin_limit,
+                                          uint32_t* err) {
+    int64_t it = 0;
+    while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        ++it;
+        // give up after spin_limit polls, or as soon as any other wait has given up
+        if (it > spin_limit ||
+            ((it & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
+            __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+template <int BS, bool SYNCFREE>
+__global__ __launch_bounds__(kBlock) void k_trsv(int slot0, int nslots, const int* __restrict__ order,
+                                                 const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const double* __restrict__ val, const double* __restrict__ dinv,
+                                                 double* x, uint32_t* flags, uint32_t epoch,
+                                                 unsigned long long* ticket, int64_t spin_limit, uint32_t* err) {
+    constexpr int RPW = kWave / BS;
+    constexpr int SPB = (kBlock / kWave) * RPW;   // slots per workgroup
+    __shared__ unsigned long long s_tk;
+    int wg = blockIdx.x;
+    if constexpr (SYNCFREE) {
+        if (threadIdx.x == 0)
+            s_tk = __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        wg = (int)(s_tk % gridDim.x);
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    const int slot = slot0 + wg * SPB + wv * RPW + g;
+    const bool have = g < RPW && slot < slot0 + nslots;
+    const int i = have ? order[slot] : -1;
+    const bool active = i >= 0;
+    double s = 0.0;
+    if (active) {
+        s = x[(size_t)i * BS + r];
+        const int p0 = rp[i], p1 = rp[i + 1];
+        for (int p = p0; p < p1; ++p) {
+            const int c = ci[p];
+            double a[BS];
+            load_vec<BS>(val + ((size_t)p * BS + r) * BS, a);
+            if constexpr (SYNCFREE) wait_flag(flags, c, epoch, spin_limit, err);
+            double xv[BS];
+            load_vec<BS>(x + (size_t)c * BS, xv);
+            s -= row_dot<BS>(a, xv);
+        }
+    }
+    double out = s;
+    if (dinv) {
+        const int base = (g < RPW ? g : 0) * BS;
+        double d[BS];
+        if (active) load_vec<BS>(dinv + ((size_t)i * BS + r) * BS, d);
+        out = 0.0;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const double sc = __shfl(s, base + c, kWave);
+            if (active) out = fma(d[c], sc, out);
+        }
+    }
+    if (active) {
+        x[(size_t)i * BS + r] = out;
+        if constexpr (SYNCFREE) {
+            if (r == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <int BS, bool SYNCFREE>
+__global__ __launch_bounds__(kBlock) void k_sor(int slot0, int nslots, const int* __restrict__ order,
+                                                const int* __restrict__ rp, const int* __restrict__ ci,
+                                                const double* __restrict__ val, const double* __restrict__ dinv,
+                                                int forward, double w, const double* __restrict__ b,
+                                                const double* __restrict__ xo, double* xn, uint32_t* flags,
+                                                uint32_t epoch, unsigned long long* ticket, int64_t spin_limit,
+                                                uint32_t* err) {
+    constexpr int RPW = kWave / BS;
+    constexpr int SPB = (kBlock / kWave) * RPW;
+    __shared__ unsigned long long s_tk;
+    int wg = blockIdx.x;
+    if constexpr (SYNCFREE) {
+        if (threadIdx.x == 0)
+            s_tk = __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        wg = (int)(s_tk % gridDim.x);
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    const int slot = slot0 + wg * SPB + wv * RPW + g;
+    const bool have = g < RPW && slot < slot0 + nslots;
+    const int i = have ? order[slot] : -1;
+    const bool active = i >= 0;
+    double ph = 0.0, xold = 0.0;
+    if (active) {
+        xold = xo ? xo[(size_t)i * BS + r] : 0.0;
+        const int p0 = rp[i], p1 = rp[i + 1];
+        for (int p = p0; p < p1; ++p) {
+            const int c = ci[p];
+            if (c == i) continue;
+            const bool dep = forward ? (c < i) : (c > i);
+            double a[BS];
+            load_vec<BS>(val + ((size_t)p * BS + r) * BS, a);
+            double xv[BS];
+            if (dep) {
+                if constexpr (SYNCFREE) wait_flag(flags, c, epoch, spin_limit, err);
+                load_vec<BS>(xn + (size_t)c * BS, xv);
+            } else if (xo) {
+                load_vec<BS>(xo + (size_t)c * BS, xv);
+            } else {
+#pragma unroll
+                for (int k = 0; k < BS; ++k) xv[k] = 0.0;
+            }
+            ph += row_dot<BS>(a, xv);
+        }
+        ph = b[(size_t)i * BS + r] - ph;
+    }
+    const int base = (g < RPW ? g : 0) * BS;
+    double d[BS];
+    if (active) load_vec<BS>(dinv + ((size_t)i * BS + r) * BS, d);
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+        const double sc = __shfl(ph, base + c, kWave);
+        if (active) t = fma(d[c], sc, t);
+    }
+    if (active) {
+        xn[(size_t)i * BS + r] = xold + (-w * xold + w * t);
+        if constexpr (SYNCFREE) {
+            if (r == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
+
+int reduce_blocks(int64_t len) {
+    int64_t b = (len + kBlock * 8 - 1) / (kBlock * 8);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, bsp_ctx::kMaxRedBlocks));
+}
+
+#define BSP_LAUNCH_CHECK()                                                                               \
+    do {                                                                                                 \
+        hipError_t e_ = hipGetLastError();                                                               \
+        if (e_ != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <template <int> class F, class... Args>
+bool dispatch_bs(int bs, Args&&... args) {
+    switch (bs) {
+        case 1: F<1>::run(args...); return true;
+        case 2: F<2>::run(args...); return true;
+        case 3: F<3>::run(args...); return true;
+        case 4: F<4>::run(args...); return true;
+        case 5: F<5>::run(args...); return true;
+        case 6: F<6>::run(args...); return true;
+        case 7: F<7>::run(args...); return true;
+        case 8: F<8>::run(args...); return true;
+        case 16: F<16>::run(args...); return true;
+        case 32: F<32>::run(args...); return true;
+        default: return false;
+    }
+}
+
+template <int BS>
+struct BsmvL {
+    static void run(hipStream_t st, int n, const int* rp, const int* ci, const double* val, const double* x,
+                    double* y, int acc) {
+        constexpr int RPW = kWave / BS;
+        const int64_t waves = (n + RPW - 1) / RPW;
+        hipLaunchKernelGGL(k_bsmv<BS>, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, st, n, rp, ci, val, x, y, acc);
+    }
+};
+
+template <int BS>
+struct ResidL {
+    static void run(hipStream_t st, int n, const int* rp, const int* ci, const double* val, const double* b,
+                    const double* x, double* r) {
+        constexpr int RPW = kWave / BS;
+        const int64_t waves = (n + RPW - 1) / RPW;
+        hipLaunchKernelGGL(k_residual<BS>, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, st, n, rp, ci, val, b, x, r);
+    }
+};
+
+template <int BS>
+struct DiagL {
+    static void run(hipStream_t st, int n, const double* dinv, double* x) {
+        constexpr int RPW = kWave / BS;
+        const int64_t waves = (n + RPW - 1) / RPW;
+        hipLaunchKernelGGL(k_block_diag<BS>, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, st, n, dinv, x);
+    }
+};
+
+template <int BS>
+struct TrsvL {
+    static void run(hipStream_t st, bool syncfree, int slot0, int nslots, const int* order, const int* rp,
+                    const int* ci, const double* val, const double* dinv, double* x, uint32_t* flags,
+                    uint32_t epoch, unsigned long long* ticket, int64_t spin, uint32_t* err) {
+        constexpr int RPW = kWave / BS;
+        constexpr int SPB = (kBlock / kWave) * RPW;
+        const unsigned grid = (unsigned)((nslots + SPB - 1) / SPB);
+        if (grid == 0) return;
+        if (syncfree)
+            hipLaunchKernelGGL((k_trsv<BS, true>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci, val,
+                               dinv, x, flags, epoch, ticket, spin, err);
+        else
+            hipLaunchKernelGGL((k_trsv<BS, false>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci,
+                               val, dinv, x, flags, epoch, ticket, spin, err);
+    }
+};
+
+template <int BS>
+struct SorL {
+    static void run(hipStream_t st, bool syncfree, int slot0, int nslots, const int* order, const int* rp,
+                    const int* ci, const double* val, const double* dinv, int fwd, double w, const double* b,
+                    const double* xo, double* xn, uint32_t* flags, uint32_t epoch, unsigned long long* ticket,
+                    int64_t spin, uint32_t* err) {
+        constexpr int RPW = kWave / BS;
+        constexpr int SPB = (kBlock / kWave) * RPW;
+        const unsigned grid = (unsigned)((nslots + SPB - 1) / SPB);
+        if (grid == 0) return;
+        if (syncfree)
+            hipLaunchKernelGGL((k_sor<BS, true>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci, val,
+                               dinv, fwd, w, b, xo, xn, flags, epoch, ticket, spin, err);
+        else
+            hipLaunchKernelGGL((k_sor<BS, false>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci, val,
+                               dinv, fwd, w, b, xo, xn, flags, epoch, ticket, spin, err);
+    }
+};
+
+}  // namespace
+
+int sweep_rows_per_wave(int bs) { return kWave / bs; }
+
+int launch_bsmv(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* val,
+                const double* x, double* y, int accumulate) {
+    if (n <= 0) return BSP_OK;
+    if (!dispatch_bs<BsmvL>(bs, ctx->stream, n, rp, ci, val, x, y, accumulate)) {
+        BSP_CHECK(bs <= kWave, BSP_E_UNSUPPORTED, "block size > 64 is not supported");
+        const int rpw = kWave / bs;
+        const int64_t waves = (n + rpw - 1) / rpw;
+        hipLaunchKernelGGL(k_bsmv_generic, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, ctx->stream, n, bs, rp, ci,
+                           val, x, y, accumulate);
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_bsmv_dim(const bsp_ctx* ctx, int n, int B, int D, int dim, const int* rp, const int* ci,
+                    const double* val, const double* x, double* y, int accumulate) {
+    if (n <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_bsmv_dim, dim3(grid_for((int64_t)n * B * D)), dim3(kBlock), 0, ctx->stream, n, B, D, dim, rp,
+                       ci, val, x, y, accumulate);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_bsmm_rhs(const bsp_ctx* ctx, int n, int bs, int nrhs, const int* rp, const int* ci, const double* val,
+                    const double* X, double* Y) {
+    if (n <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_bsmm_rhs, dim3(grid_for((int64_t)n * bs * nrhs)), dim3(kBlock), 0, ctx->stream, n, bs, nrhs,
+                       rp, ci, val, X, Y);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_block_diag(const bsp_ctx* ctx, int n, int bs, const double* dinv, double* x) {
+    if (n <= 0) return BSP_OK;
+    if (!dispatch_bs<DiagL>(bs, ctx->stream, n, dinv, x)) {
+        return fail(ctx, BSP_E_UNSUPPORTED, "block diagonal apply: unsupported block size");
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int reduce(const bsp_ctx* ctx, int kind, int64_t len, const double* x, const double* y, int slot, double* host_out) {
+    double* out = ctx->scalars.as<double>() + slot;
+    const int nb = reduce_blocks(len);
+    double* part = ctx->red_partials.as<double>();
+    unsigned* cnt = ctx->red_counter.as<unsigned>();
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_reduce<0>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out); break;
+        case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out); break;
+        default: hipLaunchKernelGGL(k_reduce<2>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out); break;
+    }
+    BSP_LAUNCH_CHECK();
+    if (host_out) {
+        BSP_HIP(hipMemcpyAsync(ctx->pinned, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BSP_HIP(hipStreamSynchronize(ctx->stream));
+        *host_out = ctx->pinned[0];
+    }
+    return BSP_OK;
+}
+
+int launch_axpy(const bsp_ctx* ctx, int64_t len, double a, const double* x, double* y) {
+    if (len <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, a, x, y);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_scal(const bsp_ctx* ctx, int64_t len, double a, double* x) {
+    if (len <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_scal, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, a, x);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_trsv(bsp_ctx* ctx, Sweep& S, const double* dinv, double* x, uint32_t epoch) {
+    if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
+    const bool sf = ctx->trsv_mode == 1;
+    uint32_t* err = ctx->err_flag.as<uint32_t>();
+    auto* tk = S.ticket.as<unsigned long long>();
+    if (sf) {
+        if (!dispatch_bs<TrsvL>(S.bs, ctx->stream, true, 0, S.nslots, S.order.as<int>(), S.drp,
+                                S.dci, S.dval, dinv, x, S.flags.as<uint32_t>(), epoch, tk,
+                                ctx->spin_limit, err))
+            return fail(ctx, BSP_E_UNSUPPORTED, "triangular sweep: unsupported block size");
+    } else {
+        for (int l = 0; l < S.nlevels; ++l) {
+            const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
+            if (!dispatch_bs<TrsvL>(S.bs, ctx->stream, false, s0, cnt, S.order.as<int>(), S.drp,
+                                    S.dci, S.dval, dinv, x, S.flags.as<uint32_t>(), epoch, tk,
+                                    ctx->spin_limit, err))
+                return fail(ctx, BSP_E_UNSUPPORTED, "triangular sweep: unsupported block size");
+        }
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const double* dinv, const double* b,
+                     const double* xo, double* xn, uint32_t epoch) {
+    if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
+    const bool sf = ctx->trsv_mode == 1;
+    uint32_t* err = ctx->err_flag.as<uint32_t>();
+    auto* tk = S.ticket.as<unsigned long long>();
+    if (sf) {
+        if (!dispatch_bs<SorL>(S.bs, ctx->stream, true, 0, S.nslots, S.order.as<int>(), S.drp,
+                               S.dci, S.dval, dinv, forward, w, b, xo, xn,
+                               S.flags.as<uint32_t>(), epoch, tk, ctx->spin_limit, err))
+            return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweep: unsupported block size");
+    } else {
+        for (int l = 0; l < S.nlevels; ++l) {
+            const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
+            if (!dispatch_bs<SorL>(S.bs, ctx->stream, false, s0, cnt, S.order.as<int>(), S.drp,
+                                   S.dci, S.dval, dinv, forward, w, b, xo, xn,
+                                   S.flags.as<uint32_t>(), epoch, tk, ctx->spin_limit, err))
+                return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweep: unsupported block size");
+        }
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qprev, const double* h_prev,
+                    const double* qnext, double* h_out) {
+    const int nb = reduce_blocks(len);
+    hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kBlock), 0, ctx->stream, len, v, qprev, h_prev, qnext,
+                       ctx->red_partials.as<double>(), ctx->red_counter.as<unsigned>(), h_out);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q) {
+    hipLaunchKernelGGL(k_scale_div, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, v, h, q);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_combine_add(const bsp_ctx* ctx, int64_t len, int m, const double* const* Q, const double* c, double* x) {
+    // chunks of 64 vectors, in index order
+    for (int i0 = 0; i0 < m; i0 += kMaxBasis) {
+        const int mm = std::min(kMaxBasis, m - i0);
+        PtrTable t{};
+        for (int i = 0; i < mm; ++i) t.q[i] = Q[i0 + i];
+        hipLaunchKernelGGL(k_combine_add, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, mm, t, c + i0, x);
+        BSP_LAUNCH_CHECK();
+    }
+    return BSP_OK;
+}
+
+int launch_basis_rotate(const bsp_ctx* ctx, int64_t len, int m, int kp, const double* const* Q, const double* P,
+                        double* const* Qout) {
+    BSP_CHECK(m <= kMaxBasis && kp <= kMaxBasis, BSP_E_UNSUPPORTED, "Krylov basis larger than 64 vectors");
+    PtrTable t{};
+    OutTable o{};
+    for (int i = 0; i < m; ++i) t.q[i] = Q[i];
+    for (int j = 0; j < kp; ++j) o.q[j] = Qout[j];
+    hipLaunchKernelGGL(k_basis_rotate, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, m, kp, t, P, o);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_residual(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* val,
+                    const double* b, const double* x, double* r) {
+    if (n <= 0) return BSP_OK;
+    if (!dispatch_bs<ResidL>(bs, ctx->stream, n, rp, ci, val, b, x, r)) {
+        // generic: r = A x then r = b - r
+        int rc = launch_bsmv(ctx, n, bs, rp, ci, val, x, r, 0);
+        if (rc) return rc;
+        rc = launch_scal(ctx, (int64_t)n * bs, -1.0, r);
+        if (rc) return rc;
+        return launch_axpy(ctx, (int64_t)n * bs, 1.0, b, r);
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_copy(const bsp_ctx* ctx, int64_t len, const double* x, double* y) {
+    if (len <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, x, y);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_fill_u32(const bsp_ctx* ctx, int64_t len, uint32_t v, uint32_t* p) {
+    if (len <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, v, p);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_cg_update_xr(const bsp_ctx* ctx, int64_t len, const double* num, const double* den, const double* p,
+                        const double* Ap, double* x, double* r) {
+    hipLaunchKernelGGL(k_cg_update_xr, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, num, den, p, Ap, x, r);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_cg_update_p(const bsp_ctx* ctx, int64_t len, const double* num, const double* den, const double* z,
+                       double* p) {
+    hipLaunchKernelGGL(k_cg_update_p, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, num, den, z, p);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+}  // namespace bsp

@@ -1,0 +1,452 @@
+// bsp_krylov.cpp -- pGMRES with deflated restart and pCG on the device.
+//
+// Vectors of length n*bs live on the device for the whole solve; every
+// vector operation (bsmv, preconditioner, modified Gram-Schmidt, updates,
+// norms) is a kernel on the context stream.  Per GMRES iteration the host
+// receives only the new Hessenberg column (iter+2 doubles) to run the
+// least-squares update and the stopping test; per pCG iteration one scalar.
+//
+// pGMRES follows blocksp_solvers.hpp:156-295: left preconditioning, Arnoldi
+// with modified Gram-Schmidt (one_arnoldi, 109-136), least squares on the full
+// Hessenberg each iteration (the reference calls gels, 192-232), restart at
+// iter == restart through deflated_restart (deflated_restart.hpp:38-269),
+// final x = x0 + Q y.  pCG follows blocksp_solvers.hpp:51-105.
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <limits>
+
+#include "bsp_dense.hpp"
+#include "bsp_internal.hpp"
+
+namespace bsp {
+
+namespace {
+
+struct VecPool {
+    std::deque<DevMem> bufs;
+    size_t len = 0;
+    hipError_t grow(size_t count) {
+        while (bufs.size() < count) {
+            bufs.emplace_back();
+            hipError_t e = bufs.back().alloc(sizeof(double) * std::max<size_t>(len, 1));
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+};
+
+struct Timer {
+    bsp_ctx* ctx;
+    explicit Timer(bsp_ctx* c) : ctx(c) { (void)hipEventRecord(ctx->events[14], ctx->stream); }
+    double stop() {
+        float ms = 0.f;
+        (void)hipEventRecord(ctx->events[15], ctx->stream);
+        (void)hipEventSynchronize(ctx->events[15]);
+        (void)hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]);
+        return ms;
+    }
+};
+
+int check_sweep_errors(bsp_ctx* ctx) {
+    uint32_t err = 0;
+    BSP_HIP(hipMemcpyAsync(&err, ctx->err_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) {
+        BSP_HIP(hipMemsetAsync(ctx->err_flag.p, 0, sizeof(uint32_t), ctx->stream));
+        return fail(ctx, BSP_E_TIMEOUT, "a sync-free sweep wait exceeded the spin limit");
+    }
+    return BSP_OK;
+}
+
+// One pGMRES run in the reference's semantics, starting from x (x0 = x on entry).
+// tolerance is absolute on the preconditioned least-squares residual estimate.
+int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
+               bool tol_is_relative, int maxIters, int restart, int k, int& iters_out, int& restarts_out,
+               double& res0_out, double& res_out, bool& converged, int& applies) {
+    const int n = A->n_rows, bs = A->bs;
+    const int64_t len = (int64_t)n * bs;
+    const double eps = std::numeric_limits<double>::epsilon();
+    double* sc = ctx->scalars.as<double>();
+    const int maxm = restart > 0 ? restart : maxIters + 1;
+    // Hessenberg column (device) and its pinned host mirror, sized for the largest basis
+    DevMem hcol_mem;
+    BSP_HIP(hcol_mem.alloc(sizeof(double) * (maxm + 2)));
+    struct Pinned {
+        double* p = nullptr;
+        ~Pinned() { if (p) (void)hipHostFree(p); }
+    } hpin, ypin;
+    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * (maxm + 2), hipHostMallocDefault));
+    BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (maxm + 2), hipHostMallocDefault));
+    DevMem ydev;
+    BSP_HIP(ydev.alloc(sizeof(double) * (maxm + 2)));
+    VecPool pool;
+    pool.len = (size_t)len;
+    // basis slots: Q[0..m] plus k+2 spare vectors for the restart rotation, plus v
+    BSP_HIP(pool.grow(2));
+    std::vector<double*> Q;             // current basis (device pointers into pool)
+    std::vector<double*> spare;
+    auto take = [&](double*& p) -> int {
+        if (!spare.empty()) {
+            p = spare.back();
+            spare.pop_back();
+            return BSP_OK;
+        }
+        BSP_HIP(pool.grow(pool.bufs.size() + 1));
+        p = pool.bufs.back().as<double>();
+        return BSP_OK;
+    };
+    double* v = pool.bufs[0].as<double>();
+    double* q0 = pool.bufs[1].as<double>();
+    // r0 = M^{-1}(b - A x0)
+    int rc = launch_residual(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x, q0);
+    if (rc) return rc;
+    rc = bsp_precond_apply(ctx, M, q0);
+    ++applies;
+    if (rc) return rc;
+    double beta = 0.0;
+    rc = reduce(ctx, 1, len, q0, q0, SLOT_TMP0, &beta);
+    if (rc) return rc;
+    res0_out = beta;
+    iters_out = 0;
+    restarts_out = 0;
+    converged = false;
+    if (!(beta > 0.0) || !std::isfinite(beta)) {
+        res_out = beta;
+        converged = beta == 0.0;
+        return beta == 0.0 ? BSP_OK : fail(ctx, BSP_E_NUMERIC, "initial residual is not finite");
+    }
+    rc = launch_scale_div(ctx, len, q0, sc + SLOT_TMP0, q0);
+    if (rc) return rc;
+    Q.push_back(q0);
+    const double zero_thresh = std::min(beta * eps, eps);
+    const double tolerance = tol_is_relative ? beta * tol : tol;
+    // host Hessenberg (row-major, capacity (maxm+1) x maxm) and LS right-hand side
+    std::vector<double> H, c_r, y, res_vec;
+    int Hm = 0, Hn = 0;
+    auto H_resize = [&](int m, int nn) {
+        std::vector<double> Hnew((size_t)m * nn, 0.0);
+        for (int i = 0; i < std::min(m, Hm); ++i)
+            for (int j = 0; j < std::min(nn, Hn); ++j) Hnew[(size_t)i * nn + j] = H[(size_t)i * Hn + j];
+        H.swap(Hnew);
+        Hm = m;
+        Hn = nn;
+    };
+    bool restarted = false;
+    double res = beta;
+    int iter = 0, tot = 0;
+    double* hcol_dev = hcol_mem.as<double>();
+    while (res > tolerance) {
+        // ---- one_arnoldi (solvers.hpp:109-136) ----
+        H_resize(iter + 2, iter + 1);
+        rc = launch_bsmv(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), Q[iter], v, 0);
+        if (rc) return rc;
+        rc = bsp_precond_apply(ctx, M, v);
+        ++applies;
+        if (rc) return rc;
+        for (int j = 0; j <= iter; ++j) {
+            rc = launch_mgs_step(ctx, len, v, j ? Q[j - 1] : nullptr, j ? hcol_dev + (j - 1) : nullptr, Q[j],
+                                 hcol_dev + j);
+            if (rc) return rc;
+        }
+        rc = launch_mgs_step(ctx, len, v, Q[iter], hcol_dev + iter, nullptr, hcol_dev + iter + 1);
+        if (rc) return rc;
+        double* qn = nullptr;
+        rc = take(qn);
+        if (rc) return rc;
+        rc = launch_scale_div(ctx, len, v, hcol_dev + iter + 1, qn);
+        if (rc) return rc;
+        BSP_HIP(hipMemcpyAsync(hpin.p, hcol_dev, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
+        BSP_HIP(hipStreamSynchronize(ctx->stream));
+        for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hpin.p[j];
+        const double hlast = hpin.p[iter + 1];
+        const bool breakdown = std::fabs(hlast) < zero_thresh || !std::isfinite(hlast);
+        if (!breakdown) Q.push_back(qn);
+        else spare.push_back(qn);
+        // ---- least squares (solvers.hpp:192-232) ----
+        const int m = iter + 2, nn = iter + 1;
+        std::vector<double> c(m, 0.0);
+        if (!restarted) c[0] = beta;
+        else
+            for (int i = 0; i < (int)c_r.size() && i < m; ++i) c[i] = c_r[i];
+        y.assign(nn, 0.0);
+        dense::lstsq(m, nn, H.data(), c.data(), y.data());
+        res_vec.assign(m, 0.0);
+        double r2 = 0.0;
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < nn; ++j) s += -1.0 * H[(size_t)i * nn + j] * y[j];
+            res_vec[i] = s + c[i];
+            r2 += res_vec[i] * res_vec[i];
+        }
+        res = std::sqrt(r2);
+        ++iter;
+        ++tot;
+        if (ctx->verbose > 0 && tot % ctx->verbose == 0) {
+            std::fprintf(stderr, "[pgmres] iter %d res %.3e (target %.3e) restarts %d\n", tot, res, tolerance,
+                         restarts_out);
+            std::fflush(stderr);
+        }
+        if (tot > maxIters || breakdown || !std::isfinite(res)) break;
+        if (iter == restart && res > tolerance) {
+            // x0 += Q y (solvers.hpp:242-245); the reference's r0 recomputation (248) is only
+            // consumed by a pure restart and is skipped for k > 0.
+            for (int i = 0; i < nn; ++i) ypin.p[i] = y[i];
+            BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+            rc = launch_combine_add(ctx, len, nn, Q.data(), ydev.as<double>(), x);
+            if (rc) return rc;
+            ++restarts_out;
+            if (k == 0) {
+                // pure restart: r0 = M^{-1}(b - A x0), Q = [r0/||r0||]
+                for (double* p : Q) spare.push_back(p);
+                Q.clear();
+                double* r0 = nullptr;
+                rc = take(r0);
+                if (rc) return rc;
+                rc = launch_residual(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x, r0);
+                if (rc) return rc;
+                rc = bsp_precond_apply(ctx, M, r0);
+                ++applies;
+                if (rc) return rc;
+                rc = reduce(ctx, 1, len, r0, r0, SLOT_TMP0, &beta);
+                if (rc) return rc;
+                rc = launch_scale_div(ctx, len, r0, sc + SLOT_TMP0, r0);
+                if (rc) return rc;
+                Q.push_back(r0);
+                H.clear();
+                Hm = Hn = 0;
+                iter = 0;
+                restarted = false;
+                res = beta;
+                y.clear();
+                continue;
+            }
+            std::vector<double> P, Hnew;
+            const int kout = dense::deflate(nn, H.data(), res_vec.data(), k, P, Hnew, c_r);
+            BSP_CHECK(kout >= 0, BSP_E_NUMERIC, "deflated restart: small eigenproblem failed");
+            const int kp = kout + 1;
+            // Q_new = Q P  (Q has m = nn+1 vectors)
+            BSP_CHECK((size_t)m * kp <= (size_t)(bsp_ctx::kScalarSlots - SLOT_P), BSP_E_UNSUPPORTED,
+                      "restart rotation too large");
+            for (size_t t = 0; t < P.size(); ++t) ctx->pinned[SLOT_P + t] = P[t];
+            BSP_HIP(hipMemcpyAsync(sc + SLOT_P, ctx->pinned + SLOT_P, sizeof(double) * P.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            std::vector<double*> Qnew(kp);
+            for (int j = 0; j < kp; ++j) {
+                rc = take(Qnew[j]);
+                if (rc) return rc;
+            }
+            BSP_CHECK((int)Q.size() == m, BSP_E_NUMERIC, "basis size mismatch at restart");
+            rc = launch_basis_rotate(ctx, len, m, kp, Q.data(), sc + SLOT_P, Qnew.data());
+            if (rc) return rc;
+            BSP_HIP(hipStreamSynchronize(ctx->stream));   // pinned buffer reuse
+            for (double* p : Q) spare.push_back(p);
+            Q = Qnew;
+            H = Hnew;
+            Hm = kp;
+            Hn = kout;
+            iter = kout;
+            restarted = true;
+            y.clear();
+        }
+    }
+    // x = x0 + Q y (solvers.hpp:290-294)
+    if (!y.empty()) {
+        for (size_t i = 0; i < y.size(); ++i) ypin.p[i] = y[i];
+        BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
+        rc = launch_combine_add(ctx, len, (int)y.size(), Q.data(), ydev.as<double>(), x);
+        if (rc) return rc;
+    }
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    iters_out = tot;
+    res_out = res;
+    converged = res <= tolerance;
+    return check_sweep_errors(ctx);
+}
+
+int true_relres(bsp_ctx* ctx, const bsp_bsr* A, const double* b, const double* x, double* work, double bnorm,
+                double& out) {
+    const int64_t len = (int64_t)A->n_rows * A->bs;
+    int rc = launch_residual(ctx, A->n_rows, A->bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x, work);
+    if (rc) return rc;
+    double rn = 0.0;
+    rc = reduce(ctx, 1, len, work, work, SLOT_TMP1, &rn);
+    if (rc) return rc;
+    out = bnorm > 0.0 ? rn / bnorm : rn;
+    return BSP_OK;
+}
+
+}  // namespace
+
+}  // namespace bsp
+
+using namespace bsp;
+
+extern "C" {
+
+int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
+               int maxIters, int restart, int k, int tol_mode, bsp_stats* stats) {
+    if (!ctx || !A || !M || !b || !x) return BSP_E_INVALID;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "pGMRES needs a square matrix");
+    BSP_CHECK(M->n == A->n_rows && M->bs == A->bs, BSP_E_INVALID, "preconditioner does not match the matrix");
+    BSP_CHECK(b != x, BSP_E_INVALID, "b and x must not alias");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int64_t len = (int64_t)A->n_rows * A->bs;
+    Timer timer(ctx);
+    BSP_HIP(hipMemsetAsync(x, 0, sizeof(double) * len, ctx->stream));   // x0 = 0
+    int iters = 0, restarts = 0, applies = 0;
+    double res0 = 0.0, res = 0.0;
+    bool conv = false;
+    int rc = gmres_core(ctx, A, M, b, x, tol, true, maxIters, restart, k, iters, restarts, res0, res, conv, applies);
+    if (rc < 0) return rc;
+    double bnorm = 0.0, rr = 0.0;
+    DevMem work;
+    BSP_HIP(work.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    rc = reduce(ctx, 1, len, b, b, SLOT_TMP2, &bnorm);
+    if (rc) return rc;
+    rc = true_relres(ctx, A, b, x, work.as<double>(), bnorm, rr);
+    if (rc) return rc;
+    int total_iters = iters, total_restarts = restarts;
+    if (tol_mode == BSP_TOL_TRUE_RESIDUAL) {
+        // iterative refinement in FP64 until the true relative residual meets tol
+        for (int outer = 0; outer < 20 && rr > tol && total_iters < maxIters && std::isfinite(rr); ++outer) {
+            // solve A e = r with a target that brings ||b - A x|| below tol ||b||
+            DevMem e;
+            BSP_HIP(e.alloc(sizeof(double) * len));
+            rc = launch_residual(ctx, A->n_rows, A->bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x,
+                                 work.as<double>());
+            if (rc) return rc;
+            BSP_HIP(hipMemsetAsync(e.p, 0, sizeof(double) * len, ctx->stream));
+            const double inner_tol = std::max(0.25 * tol / rr, 1e-15);
+            int it2 = 0, rs2 = 0;
+            double r02 = 0.0, res2 = 0.0;
+            bool c2 = false;
+            rc = gmres_core(ctx, A, M, work.as<double>(), e.as<double>(), inner_tol, true, maxIters - total_iters,
+                            restart, k, it2, rs2, r02, res2, c2, applies);
+            if (rc < 0) return rc;
+            rc = launch_axpy(ctx, len, 1.0, e.as<double>(), x);
+            if (rc) return rc;
+            total_iters += it2;
+            total_restarts += rs2;
+            rc = true_relres(ctx, A, b, x, work.as<double>(), bnorm, rr);
+            if (rc) return rc;
+            res = res2;
+        }
+        conv = rr <= tol;
+    }
+    const double ms = timer.stop();
+    if (stats) {
+        stats->iters = total_iters;
+        stats->restarts = total_restarts;
+        stats->converged = conv ? 1 : 0;
+        stats->precond_applies = applies;
+        stats->res0 = res0;
+        stats->res_est = res;
+        stats->true_relres = rr;
+        stats->solve_ms = ms;
+    }
+    return conv ? BSP_OK : BSP_NOT_CONVERGED;
+}
+
+int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
+            int maxIters, int tol_mode, bsp_stats* stats) {
+    if (!ctx || !A || !M || !b || !x) return BSP_E_INVALID;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "pCG needs a square matrix");
+    BSP_CHECK(M->n == A->n_rows && M->bs == A->bs, BSP_E_INVALID, "preconditioner does not match the matrix");
+    BSP_CHECK(b != x, BSP_E_INVALID, "b and x must not alias");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int n = A->n_rows, bs = A->bs;
+    const int64_t len = (int64_t)n * bs;
+    Timer timer(ctx);
+    DevMem mr, mp, mz, mAp;
+    BSP_HIP(mr.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    BSP_HIP(mp.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    BSP_HIP(mz.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    BSP_HIP(mAp.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    double *r = mr.as<double>(), *p = mp.as<double>(), *z = mz.as<double>(), *Ap = mAp.as<double>();
+    double* sc = ctx->scalars.as<double>();
+    const int* rp = A->rp.as<int>();
+    const int* ci = A->ci.as<int>();
+    const double* av = A->val.as<double>();
+    BSP_HIP(hipMemsetAsync(x, 0, sizeof(double) * len, ctx->stream));
+    double bnorm = 0.0;
+    int rc = reduce(ctx, 1, len, b, b, SLOT_TMP2, &bnorm);
+    if (rc) return rc;
+    int total = 0, applies = 0;
+    double res0 = 0.0, res = 0.0, rr = 0.0;
+    bool conv = false;
+    const double target = tol;
+    for (int outer = 0; outer < (tol_mode == BSP_TOL_TRUE_RESIDUAL ? 20 : 1); ++outer) {
+        // r = b - A x (x = 0 on the first pass: r = b, solvers.hpp:60)
+        rc = launch_residual(ctx, n, bs, rp, ci, av, b, x, r);
+        if (rc) return rc;
+        rc = launch_copy(ctx, len, r, p);
+        if (rc) return rc;
+        rc = bsp_precond_apply(ctx, M, p);
+        ++applies;
+        if (rc) return rc;
+        rc = launch_copy(ctx, len, p, z);
+        if (rc) return rc;
+        double pn = 0.0;
+        rc = reduce(ctx, 1, len, p, p, SLOT_TMP0, &pn);
+        if (rc) return rc;
+        if (outer == 0) res0 = pn;
+        double inner_tol = tol;
+        if (outer > 0) inner_tol = std::max(0.25 * target / rr, 1e-15);
+        const double tolerance = pn * inner_tol;
+        res = pn;
+        int iter = 0;
+        // rz in SLOT 0.. : s0 = rz, s1 = pAp, s3 = rz_new
+        while (res > tolerance) {
+            rc = launch_bsmv(ctx, n, bs, rp, ci, av, p, Ap, 0);
+            if (rc) return rc;
+            rc = reduce(ctx, 0, len, r, z, SLOT_TMP0, nullptr);
+            if (rc) return rc;
+            rc = reduce(ctx, 0, len, p, Ap, SLOT_TMP1, nullptr);
+            if (rc) return rc;
+            rc = launch_cg_update_xr(ctx, len, sc + SLOT_TMP0, sc + SLOT_TMP1, p, Ap, x, r);
+            if (rc) return rc;
+            rc = reduce(ctx, 1, len, r, r, SLOT_TMP2 + 1, &res);
+            if (rc) return rc;
+            if (!std::isfinite(res)) break;
+            if (res > tolerance) {
+                rc = launch_copy(ctx, len, r, z);
+                if (rc) return rc;
+                rc = bsp_precond_apply(ctx, M, z);
+                ++applies;
+                if (rc) return rc;
+                rc = reduce(ctx, 0, len, r, z, SLOT_TMP3, nullptr);
+                if (rc) return rc;
+                rc = launch_cg_update_p(ctx, len, sc + SLOT_TMP3, sc + SLOT_TMP0, z, p);
+                if (rc) return rc;
+                ++iter;
+                if (ctx->verbose > 0 && iter % ctx->verbose == 0) {
+                    std::fprintf(stderr, "[pcg] iter %d res %.3e (target %.3e)\n", total + iter, res, tolerance);
+                    std::fflush(stderr);
+                }
+                if (total + iter > maxIters) break;
+            }
+        }
+        total += iter;
+        rc = true_relres(ctx, A, b, x, Ap, bnorm, rr);
+        if (rc) return rc;
+        conv = tol_mode == BSP_TOL_TRUE_RESIDUAL ? rr <= target : res <= tolerance;
+        if (tol_mode != BSP_TOL_TRUE_RESIDUAL || conv || total > maxIters || !std::isfinite(rr)) break;
+    }
+    rc = check_sweep_errors(ctx);
+    if (rc) return rc;
+    const double ms = timer.stop();
+    if (stats) {
+        stats->iters = total;
+        stats->restarts = 0;
+        stats->converged = conv ? 1 : 0;
+        stats->precond_applies = applies;
+        stats->res0 = res0;
+        stats->res_est = res;
+        stats->true_relres = rr;
+        stats->solve_ms = ms;
+    }
+    return conv ? BSP_OK : BSP_NOT_CONVERGED;
+}
+
+}  // extern "C"

@@ -1,0 +1,489 @@
+// bsp_precond.cpp -- preconditioner setup (host) and application (device sweeps).
+//
+// Setup follows the reference algorithms (blocksp_preconditioners.hpp):
+//  * ILU0 (255-291): right-looking block elimination restricted to A's pattern,
+//    L_ki = A_ki U_ii^{-1}, A_kj -= L_ki A_ij when (i,j) is in the pattern.
+//  * IChol0 (374-429): on lowTri(A), L_ii = chol(A_ii), L_ki = A_ki L_ii^{-T},
+//    L_kj -= L_ki L_ji^T for i < j <= k when A(i,j) is in the pattern.
+//  * SOR (116-240) / Diag (75-111): inverses of the diagonal blocks.
+// The diagonal-block inverses are formed explicitly so that the device applies
+// them as one small matrix-vector product per block row.
+// Application runs entirely on the device: forward / backward block sweeps
+// over level schedules (bsp_kernels.hip).
+#include <algorithm>
+#include <cstring>
+
+#include "bsp_dense.hpp"
+#include "bsp_internal.hpp"
+
+namespace bsp {
+
+namespace {
+
+constexpr int kWaveRows = 64;
+
+int find_col(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, int i, int j) {
+    for (int p = rp[i]; p < rp[i + 1]; ++p)
+        if (ci[p] == j) return p;
+    return -1;
+}
+
+// Level schedule of a sweep.  lower: row i waits on columns < i (processed in
+// increasing level); otherwise on columns > i.  Each level is padded to a
+// multiple of the rows one wavefront handles.
+void build_schedule(Sweep& S, int n, int bs, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                    bool lower) {
+    S.n = n;
+    S.bs = bs;
+    S.level_of.assign(n, 0);
+    int nlev = 0;
+    if (lower) {
+        for (int i = 0; i < n; ++i) {
+            int l = 0;
+            for (int p = rp[i]; p < rp[i + 1]; ++p)
+                if (ci[p] < i) l = std::max(l, S.level_of[ci[p]] + 1);
+            S.level_of[i] = l;
+            nlev = std::max(nlev, l + 1);
+        }
+    } else {
+        for (int i = n - 1; i >= 0; --i) {
+            int l = 0;
+            for (int p = rp[i]; p < rp[i + 1]; ++p)
+                if (ci[p] > i) l = std::max(l, S.level_of[ci[p]] + 1);
+            S.level_of[i] = l;
+            nlev = std::max(nlev, l + 1);
+        }
+    }
+    S.nlevels = nlev;
+    const int rpw = std::max(1, kWaveRows / bs);
+    std::vector<int> cnt(nlev, 0);
+    for (int i = 0; i < n; ++i) cnt[S.level_of[i]]++;
+    S.level_ptr.assign(nlev + 1, 0);
+    for (int l = 0; l < nlev; ++l) S.level_ptr[l + 1] = S.level_ptr[l] + (cnt[l] + rpw - 1) / rpw * rpw;
+    S.nslots = S.level_ptr[nlev];
+    std::vector<int> order(S.nslots, -1), fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
+    for (int i = 0; i < n; ++i) order[fill[S.level_of[i]]++] = i;
+    (void)S.order.alloc(sizeof(int) * std::max(S.nslots, 1));
+    (void)hipMemcpy(S.order.p, order.data(), sizeof(int) * S.nslots, hipMemcpyHostToDevice);
+    (void)S.flags.alloc(sizeof(uint32_t) * std::max(n, 1));
+    (void)hipMemset(S.flags.p, 0, sizeof(uint32_t) * std::max(n, 1));
+    (void)S.ticket.alloc(sizeof(unsigned long long));
+    (void)hipMemset(S.ticket.p, 0, sizeof(unsigned long long));
+}
+
+hipError_t upload_structure(Sweep& S, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                            const std::vector<double>& val, int bs) {
+    const size_t nnz = ci.size();
+    S.nnzb = (int64_t)nnz;
+    hipError_t e;
+    if ((e = S.rp.alloc(sizeof(int) * rp.size())) != hipSuccess) return e;
+    if ((e = S.ci.alloc(sizeof(int) * std::max<size_t>(nnz, 1))) != hipSuccess) return e;
+    if ((e = S.val.alloc(sizeof(double) * std::max<size_t>(nnz * bs * bs, 1))) != hipSuccess) return e;
+    if ((e = hipMemcpy(S.rp.p, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if (nnz) {
+        if ((e = hipMemcpy(S.ci.p, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return e;
+        if ((e = hipMemcpy(S.val.p, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return e;
+    }
+    S.drp = S.rp.as<int>();
+    S.dci = S.ci.as<int>();
+    S.dval = S.val.as<double>();
+    return hipSuccess;
+}
+
+// Split a square block matrix (host) into its strictly lower / strictly upper parts,
+// preserving each row's stored order.
+void split_strict(int n, int bs, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                  const std::vector<double>& v, bool lower, std::vector<int32_t>& orp, std::vector<int32_t>& oci,
+                  std::vector<double>& ov) {
+    const size_t b2 = (size_t)bs * bs;
+    orp.assign(n + 1, 0);
+    oci.clear();
+    ov.clear();
+    for (int i = 0; i < n; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            const int c = ci[p];
+            if (lower ? (c < i) : (c > i)) {
+                oci.push_back(c);
+                ov.insert(ov.end(), v.begin() + p * b2, v.begin() + (p + 1) * b2);
+            }
+        }
+        orp[i + 1] = (int32_t)oci.size();
+    }
+}
+
+int diag_inverses(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, std::vector<double>& dinv) {
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    dinv.assign(n * b2, 0.0);
+    std::vector<double> blk(b2), linv(b2);
+    for (int i = 0; i < n; ++i) {
+        const int p = find_col(A->h_rp, A->h_ci, i, i);
+        BSP_CHECK(p >= 0, BSP_E_NUMERIC, "missing diagonal block in row " + std::to_string(i));
+        const double* a = A->h_val.data() + p * b2;
+        if (inv_kind == 1) {
+            std::memcpy(blk.data(), a, sizeof(double) * b2);
+            BSP_CHECK(dense::cholesky_lower(bs, blk.data()), BSP_E_NUMERIC,
+                      "diagonal block " + std::to_string(i) + " is not positive definite");
+            BSP_CHECK(dense::invert(bs, blk.data(), linv.data()), BSP_E_NUMERIC, "singular Cholesky factor");
+            // A^{-1} = L^{-T} L^{-1}
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) {
+                    double s = 0.0;
+                    for (int k = 0; k < bs; ++k) s += linv[k * bs + r] * linv[k * bs + c];
+                    dinv[i * b2 + r * bs + c] = s;
+                }
+        } else {
+            BSP_CHECK(dense::invert(bs, a, dinv.data() + i * b2), BSP_E_NUMERIC,
+                      "singular diagonal block in row " + std::to_string(i));
+        }
+    }
+    return BSP_OK;
+}
+
+hipError_t upload(DevMem& m, const std::vector<double>& v) {
+    hipError_t e = m.alloc(sizeof(double) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(m.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+}  // namespace bsp
+
+using namespace bsp;
+
+extern "C" {
+
+int bsp_precond_create_none(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_NONE;
+    M->n = A->n_rows;
+    M->bs = A->bs;
+    M->A = A;
+    *out = M.release();
+    return BSP_OK;
+}
+
+int bsp_precond_create_diag(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "Diag needs a square matrix");
+    BSP_HIP(hipSetDevice(ctx->device));
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_DIAG;
+    M->n = A->n_rows;
+    M->bs = A->bs;
+    M->A = A;
+    M->inv_kind = inv_kind;
+    std::vector<double> dinv;
+    int rc = diag_inverses(ctx, A, inv_kind, dinv);
+    if (rc) return rc;
+    BSP_HIP(upload(M->dinv_upper, dinv));
+    *out = M.release();
+    return BSP_OK;
+}
+
+int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int type, int inv_kind,
+                           bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "SOR needs a square matrix");
+    BSP_CHECK(nu >= 0, BSP_E_INVALID, "nu must be >= 0");
+    BSP_HIP(hipSetDevice(ctx->device));
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_SOR;
+    M->n = A->n_rows;
+    M->bs = A->bs;
+    M->A = A;
+    M->w = w;
+    M->nu = nu;
+    M->type = type;
+    M->inv_kind = inv_kind;
+    std::vector<double> dinv;
+    int rc = diag_inverses(ctx, A, inv_kind, dinv);
+    if (rc) return rc;
+    BSP_HIP(upload(M->dinv_upper, dinv));
+    build_schedule(M->lower, M->n, M->bs, A->h_rp, A->h_ci, true);
+    build_schedule(M->upper, M->n, M->bs, A->h_rp, A->h_ci, false);
+    for (Sweep* S : {&M->lower, &M->upper}) {
+        S->drp = A->rp.as<int>();
+        S->dci = A->ci.as<int>();
+        S->dval = A->val.as<double>();
+        S->nnzb = A->nnzb;
+    }
+    const size_t len = (size_t)M->n * M->bs;
+    BSP_HIP(M->work0.alloc(sizeof(double) * std::max<size_t>(len, 1)));
+    BSP_HIP(M->work1.alloc(sizeof(double) * std::max<size_t>(len, 1)));
+    *out = M.release();
+    return BSP_OK;
+}
+
+int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    const auto& rp = A->h_rp;
+    const auto& ci = A->h_ci;
+    std::vector<double> LU(A->h_val);
+    // rows holding a block in column c (ascending), from A's pattern
+    std::vector<int32_t> cptr(n + 1, 0), crow, cpos;
+    for (int p = 0; p < (int)ci.size(); ++p) cptr[ci[p] + 1]++;
+    for (int c = 0; c < n; ++c) cptr[c + 1] += cptr[c];
+    crow.resize(ci.size());
+    cpos.resize(ci.size());
+    {
+        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                crow[fill[ci[p]]] = i;
+                cpos[fill[ci[p]]++] = p;
+            }
+    }
+    std::vector<double> inv(b2), tmp(b2), prod(b2);
+    for (int i = 0; i < n; ++i) {
+        const int pd = find_col(rp, ci, i, i);
+        BSP_CHECK(pd >= 0, BSP_E_NUMERIC, "ILU0: missing diagonal block in row " + std::to_string(i));
+        BSP_CHECK(dense::invert(bs, LU.data() + pd * b2, inv.data()), BSP_E_NUMERIC,
+                  "ILU0: singular pivot block in row " + std::to_string(i));
+        for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
+            if (crow[q] <= i) continue;
+            double* lki = LU.data() + (size_t)cpos[q] * b2;
+            std::memcpy(tmp.data(), lki, sizeof(double) * b2);
+            dense::matmul(bs, tmp.data(), inv.data(), lki);
+        }
+        for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
+            const int k = crow[q];
+            if (k <= i) continue;
+            const double* lki = LU.data() + (size_t)cpos[q] * b2;
+            for (int p = rp[k]; p < rp[k + 1]; ++p) {
+                const int j = ci[p];
+                if (j <= i) continue;
+                const int pij = find_col(rp, ci, i, j);
+                if (pij < 0) continue;
+                dense::matmul(bs, lki, LU.data() + (size_t)pij * b2, prod.data());
+                double* akj = LU.data() + (size_t)p * b2;
+                for (size_t e = 0; e < b2; ++e) akj[e] -= prod[e];
+            }
+        }
+    }
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_ILU0;
+    M->n = n;
+    M->bs = bs;
+    std::vector<double> dinv(n * b2);
+    for (int i = 0; i < n; ++i) {
+        const int pd = find_col(rp, ci, i, i);
+        BSP_CHECK(dense::invert(bs, LU.data() + pd * b2, dinv.data() + i * b2), BSP_E_NUMERIC,
+                  "ILU0: singular U block in row " + std::to_string(i));
+    }
+    std::vector<int32_t> lrp, lci, urp, uci;
+    std::vector<double> lv, uv;
+    split_strict(n, bs, rp, ci, LU, true, lrp, lci, lv);
+    split_strict(n, bs, rp, ci, LU, false, urp, uci, uv);
+    BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs));
+    BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs));
+    build_schedule(M->lower, n, bs, lrp, lci, true);
+    build_schedule(M->upper, n, bs, urp, uci, false);
+    BSP_HIP(upload(M->dinv_upper, dinv));
+    M->f_rp = rp;
+    M->f_ci = ci;
+    M->f_val.swap(LU);
+    *out = M.release();
+    return BSP_OK;
+}
+
+int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "IChol0 needs a square matrix");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    // lowTri (blocksp_containers.hpp:277-285): blocks with col <= row, in row order
+    std::vector<int32_t> Lrp(n + 1, 0), Lci;
+    std::vector<double> Lv;
+    for (int i = 0; i < n; ++i) {
+        for (int p = A->h_rp[i]; p < A->h_rp[i + 1]; ++p)
+            if (A->h_ci[p] <= i) {
+                Lci.push_back(A->h_ci[p]);
+                Lv.insert(Lv.end(), A->h_val.begin() + p * b2, A->h_val.begin() + (p + 1) * b2);
+            }
+        Lrp[i + 1] = (int32_t)Lci.size();
+    }
+    // rows k > c holding L(k, c)
+    std::vector<int32_t> cptr(n + 1, 0), crow, cpos;
+    for (int i = 0; i < n; ++i)
+        for (int p = Lrp[i]; p < Lrp[i + 1]; ++p)
+            if (Lci[p] < i) cptr[Lci[p] + 1]++;
+    for (int c = 0; c < n; ++c) cptr[c + 1] += cptr[c];
+    crow.resize(cptr[n]);
+    cpos.resize(cptr[n]);
+    {
+        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int p = Lrp[i]; p < Lrp[i + 1]; ++p)
+                if (Lci[p] < i) {
+                    crow[fill[Lci[p]]] = i;
+                    cpos[fill[Lci[p]]++] = p;
+                }
+    }
+    std::vector<double> linv(b2), tmp(b2), prod(b2), ljt(b2);
+    for (int i = 0; i < n; ++i) {
+        const int pd = find_col(Lrp, Lci, i, i);
+        BSP_CHECK(pd >= 0, BSP_E_NUMERIC, "IChol0: missing diagonal block in row " + std::to_string(i));
+        double* lii = Lv.data() + (size_t)pd * b2;
+        BSP_CHECK(dense::cholesky_lower(bs, lii), BSP_E_NUMERIC,
+                  "IChol0: pivot block " + std::to_string(i) + " is not positive definite");
+        BSP_CHECK(dense::invert(bs, lii, linv.data()), BSP_E_NUMERIC, "IChol0: singular pivot");
+        // L_ki <- L_ki L_ii^{-T}
+        for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
+            double* lki = Lv.data() + (size_t)cpos[q] * b2;
+            std::memcpy(tmp.data(), lki, sizeof(double) * b2);
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) {
+                    double s = 0.0;
+                    for (int t = 0; t < bs; ++t) s += tmp[r * bs + t] * linv[c * bs + t];
+                    lki[r * bs + c] = s;
+                }
+        }
+        for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
+            const int k = crow[q];
+            const double* lki = Lv.data() + (size_t)cpos[q] * b2;
+            for (int p = Lrp[k]; p < Lrp[k + 1]; ++p) {
+                const int j = Lci[p];
+                if (j <= i || j > k) continue;
+                if (find_col(A->h_rp, A->h_ci, i, j) < 0) continue;
+                const int pji = find_col(Lrp, Lci, j, i);
+                if (pji < 0) continue;
+                dense::transpose_to(bs, Lv.data() + (size_t)pji * b2, ljt.data());
+                dense::matmul(bs, lki, ljt.data(), prod.data());
+                double* lkj = Lv.data() + (size_t)p * b2;
+                for (size_t e = 0; e < b2; ++e) lkj[e] -= prod[e];
+            }
+        }
+    }
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_ICHOL0;
+    M->n = n;
+    M->bs = bs;
+    std::vector<double> dlo(n * b2), dup(n * b2);
+    for (int i = 0; i < n; ++i) {
+        const int pd = find_col(Lrp, Lci, i, i);
+        BSP_CHECK(dense::invert(bs, Lv.data() + (size_t)pd * b2, dlo.data() + i * b2), BSP_E_NUMERIC,
+                  "IChol0: singular factor block");
+        dense::transpose_to(bs, dlo.data() + i * b2, dup.data() + i * b2);
+    }
+    std::vector<int32_t> lrp, lci;
+    std::vector<double> lv;
+    split_strict(n, bs, Lrp, Lci, Lv, true, lrp, lci, lv);
+    // L^T strictly upper: row i holds (L_ki)^T for k > i, ascending k
+    std::vector<int32_t> urp(n + 1, 0), uci(cptr[n]);
+    std::vector<double> uv((size_t)cptr[n] * b2);
+    for (int i = 0; i < n; ++i) {
+        urp[i + 1] = cptr[i + 1];
+        for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
+            uci[q] = crow[q];
+            dense::transpose_to(bs, Lv.data() + (size_t)cpos[q] * b2, uv.data() + (size_t)q * b2);
+        }
+    }
+    BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs));
+    BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs));
+    build_schedule(M->lower, n, bs, lrp, lci, true);
+    build_schedule(M->upper, n, bs, urp, uci, false);
+    BSP_HIP(upload(M->dinv_lower, dlo));
+    BSP_HIP(upload(M->dinv_upper, dup));
+    M->f_rp.swap(Lrp);
+    M->f_ci.swap(Lci);
+    M->f_val.swap(Lv);
+    *out = M.release();
+    return BSP_OK;
+}
+
+int bsp_precond_destroy(bsp_precond* M) {
+    delete M;
+    return BSP_OK;
+}
+
+int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* Mc, double* r) {
+    if (!ctx || !Mc) return BSP_E_INVALID;
+    auto* M = const_cast<bsp_precond*>(Mc);   // epochs/tickets advance; the operator stays const
+    const int64_t len = (int64_t)M->n * M->bs;
+    switch (M->kind) {
+        case BSP_PRE_NONE: return BSP_OK;
+        case BSP_PRE_DIAG: return launch_block_diag(ctx, M->n, M->bs, M->dinv_upper.as<double>(), r);
+        case BSP_PRE_ILU0: {
+            int rc = launch_trsv(ctx, M->lower, nullptr, r, ++M->epoch);
+            if (rc) return rc;
+            return launch_trsv(ctx, M->upper, M->dinv_upper.as<double>(), r, ++M->epoch);
+        }
+        case BSP_PRE_ICHOL0: {
+            int rc = launch_trsv(ctx, M->lower, M->dinv_lower.as<double>(), r, ++M->epoch);
+            if (rc) return rc;
+            return launch_trsv(ctx, M->upper, M->dinv_upper.as<double>(), r, ++M->epoch);
+        }
+        case BSP_PRE_SOR: {
+            double* buf[2] = {M->work0.as<double>(), M->work1.as<double>()};
+            const double* xo = nullptr;   // x0 = 0 (preconditioners.hpp:227-234)
+            int cur = 0;
+            auto sweep = [&](int fwd) -> int {
+                double* xn = buf[cur];
+                int rc = launch_sor_sweep(ctx, fwd ? M->lower : M->upper, fwd, M->w, M->dinv_upper.as<double>(),
+                                          r, xo, xn, ++M->epoch);
+                xo = xn;
+                cur ^= 1;
+                return rc;
+            };
+            int rc = BSP_OK;
+            if (M->type < 0) {
+                for (int s = 0; s < M->nu && !rc; ++s) rc = sweep(0);
+            } else if (M->type == 0) {
+                for (int s = 0; s < M->nu && !rc; ++s) rc = sweep(1);
+                for (int s = 0; s < M->nu && !rc; ++s) rc = sweep(0);
+            } else {
+                for (int s = 0; s < M->nu && !rc; ++s) rc = sweep(1);
+            }
+            if (rc) return rc;
+            if (xo) return launch_copy(ctx, len, xo, r);
+            BSP_HIP(hipMemsetAsync(r, 0, sizeof(double) * len, ctx->stream));
+            return BSP_OK;
+        }
+    }
+    return fail(ctx, BSP_E_INVALID, "unknown preconditioner");
+}
+
+int bsp_precond_levels(const bsp_precond* M, int lower, int32_t* level, int* nlevels) {
+    if (!M) return BSP_E_INVALID;
+    const Sweep& S = lower ? M->lower : M->upper;
+    if (S.level_of.empty() && M->n > 0) return BSP_E_UNSUPPORTED;
+    if (level) std::copy(S.level_of.begin(), S.level_of.end(), level);
+    if (nlevels) *nlevels = S.nlevels;
+    return BSP_OK;
+}
+
+int bsp_precond_factor_info(const bsp_precond* M, int64_t* nnzb) {
+    if (!M) return BSP_E_INVALID;
+    if (nnzb) *nnzb = (int64_t)M->f_ci.size();
+    return BSP_OK;
+}
+
+int bsp_precond_download_factor(bsp_ctx* ctx, const bsp_precond* M, int32_t* rowptr, int32_t* colidx, double* vals) {
+    if (!ctx || !M) return BSP_E_INVALID;
+    BSP_CHECK(!M->f_rp.empty(), BSP_E_UNSUPPORTED, "this preconditioner has no factor");
+    if (rowptr) std::copy(M->f_rp.begin(), M->f_rp.end(), rowptr);
+    if (colidx) std::copy(M->f_ci.begin(), M->f_ci.end(), colidx);
+    if (vals) std::copy(M->f_val.begin(), M->f_val.end(), vals);
+    return BSP_OK;
+}
+
+}  // extern "C"

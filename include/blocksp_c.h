@@ -1,0 +1,174 @@
+/*
+ * blocksp_c.h -- C ABI of the MI355X block-sparse solve library (libblocksp_amd.so).
+ *
+ * This is the drop-in boundary for BlockSp's solve path.  The reference is a
+ * header-only C++20 template library with no FFI of its own; each entry point
+ * below names the reference function or type it replaces (file:line, relative
+ * to the BlockSp v1.0 checkout).  The C++20 headers in include/blocksp/ mirror
+ * the reference's template surface on top of these calls.
+ *
+ * Conventions
+ *  - Every function returns int: 0 (BSP_OK) on success, < 0 on error, and
+ *    BSP_NOT_CONVERGED (1) when a solver stops at maxIters.  The message of the
+ *    last error on a context is returned by bsp_last_error().
+ *  - Pointers named *_dev are device memory owned by the caller (allocate with
+ *    bsp_malloc or any HIP allocator on the context's device).  Host arrays are
+ *    named *_host.  Handles are owned by the library until destroyed.
+ *  - A context is bound to one device and one HIP stream and is not
+ *    thread-safe; use one context per host thread.  All work is ordered on the
+ *    context's stream; functions that return host results synchronise it.
+ *  - BSR layout: rowptr[n_rows+1], colidx[nnzb] (int32) in the caller's per-row
+ *    order, which is kept (the reference keeps insertion order,
+ *    blocksp_containers.hpp:147-164); values are nnzb blocks of bs*bs doubles,
+ *    row-major within a block.  Vectors use the solArray flat layout
+ *    (ie, dim, ib) -> ie*B*D + dim*B + ib (blocksp_containers.hpp:572-736).
+ *  - No exceptions cross this boundary.
+ */
+#ifndef BLOCKSP_C_H
+#define BLOCKSP_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSP_OK 0
+#define BSP_NOT_CONVERGED 1
+#define BSP_E_INVALID (-1)
+#define BSP_E_HIP (-2)
+#define BSP_E_NOMEM (-3)
+#define BSP_E_UNSUPPORTED (-4)
+#define BSP_E_NUMERIC (-5)
+#define BSP_E_TIMEOUT (-6)
+
+typedef struct bsp_ctx bsp_ctx;
+typedef struct bsp_bsr bsp_bsr;
+typedef struct bsp_precond bsp_precond;
+
+/* ---- context and memory (no reference counterpart: device plumbing) ---- */
+
+/* Create a context on `device`.  `stream` is a hipStream_t to borrow, or NULL to
+ * let the library create its own (default priority). */
+int bsp_ctx_create(int device, void* stream, bsp_ctx** out);
+int bsp_ctx_destroy(bsp_ctx* ctx);
+const char* bsp_last_error(const bsp_ctx* ctx);
+int bsp_ctx_sync(bsp_ctx* ctx);
+/* Set a tuning option by name ("trsv" = 0 level launches / 1 single-launch
+ * sync-free; "spin_limit" = polls before a sync-free wait gives up). */
+int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
+
+int bsp_malloc(bsp_ctx* ctx, size_t bytes, void** out_dev);
+int bsp_free(bsp_ctx* ctx, void* ptr_dev);
+int bsp_memcpy_h2d(bsp_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int bsp_memcpy_d2h(bsp_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+int bsp_memcpy_d2d(bsp_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+int bsp_memset_zero(bsp_ctx* ctx, void* dst_dev, size_t bytes);
+
+/* ---- timing on the context stream (HIP events) ---- */
+int bsp_event_record(bsp_ctx* ctx, int slot);                  /* slot in [0, 16) */
+int bsp_event_elapsed_ms(bsp_ctx* ctx, int slot_a, int slot_b, float* ms);
+
+/* ---- matrices: BlockSp<T,B*D> = Sp::CSR<TinyMatrix> (blocksp_containers.hpp:52-53, 92-496) ---- */
+
+int bsp_bsr_create(bsp_ctx* ctx, int n_rows, int n_cols, int bs, const int32_t* rowptr_host,
+                   const int32_t* colidx_host, const double* vals_host, bsp_bsr** out);
+int bsp_bsr_destroy(bsp_bsr* A);
+int bsp_bsr_info(const bsp_bsr* A, int* n_rows, int* n_cols, int* bs, int64_t* nnzb);
+/* Copy structure and values back to host buffers sized by bsp_bsr_info. */
+int bsp_bsr_download(bsp_ctx* ctx, const bsp_bsr* A, int32_t* rowptr_host, int32_t* colidx_host,
+                     double* vals_host);
+/* Replace the values (same structure) from a device buffer. */
+int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev);
+
+/* ---- products ---- */
+
+/* y (+)= A x.  Replaces bsmv(A,x) / bsmv(b,A,x) for BlockSp<T,B*D> (blocksp_multiply.hpp:365-390,
+ * implementation 170-325).  accumulate=0 overwrites y (returning form), 1 adds (void form). */
+int bsp_bsmv(bsp_ctx* ctx, const bsp_bsr* A, const double* x_dev, double* y_dev, int accumulate);
+/* Per-dimension product for BlockSp<T,B> times solArray<T,B,D> (blocksp_multiply.hpp:170-193,
+ * 252-285); A has bs = B, vectors have n*B*D entries; dim < 0 multiplies all dimensions. */
+int bsp_bsmv_dim(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x_dev, double* y_dev,
+                 int accumulate);
+/* y (+)= A^T x (transpose=true in blocksp_multiply.hpp:196-204, 236-245). */
+int bsp_bsmv_transpose(bsp_ctx* ctx, const bsp_bsr* A, const double* x_dev, double* y_dev,
+                       int accumulate);
+/* Multi-RHS product Y = A X with nrhs right-hand sides stored RHS-minor
+ * (X[(row*bs + r)*nrhs + k]); no single reference call, it is nrhs bsmv calls. */
+int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X_dev, double* Y_dev);
+/* C = A * B, structure in the reference's first-touch order (blocksp_multiply.hpp:51-97). */
+int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C);
+
+/* ---- BLAS-1 on solArrays (blocksp_containers.hpp:858-864, 917-1012) ---- */
+int bsp_dot(bsp_ctx* ctx, int64_t len, const double* x_dev, const double* y_dev, double* out_host);
+int bsp_nrm2(bsp_ctx* ctx, int64_t len, const double* x_dev, double* out_host);
+int bsp_infnorm(bsp_ctx* ctx, int64_t len, const double* x_dev, double* out_host);
+int bsp_axpy(bsp_ctx* ctx, int64_t len, double a, const double* x_dev, double* y_dev);
+int bsp_scal(bsp_ctx* ctx, int64_t len, double a, double* x_dev);
+
+/* ---- preconditioners (blocksp_preconditioners.hpp) ---- */
+/* The PreCond concept of the reference is `void operator()(solArray&) const`,
+ * overwriting r with M^{-1} r (preconditioners.hpp:70, 106, 227-234, 352-355, 492-495). */
+
+#define BSP_PRE_NONE 0
+#define BSP_PRE_DIAG 1
+#define BSP_PRE_SOR 2
+#define BSP_PRE_ILU0 3
+#define BSP_PRE_ICHOL0 4
+
+/* NoPre (preconditioners.hpp:66-71). */
+int bsp_precond_create_none(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
+/* Diag<Inv> (preconditioners.hpp:75-111); inv_kind 0 = LUFactor, 1 = CholeskyFactor. */
+int bsp_precond_create_diag(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, bsp_precond** out);
+/* SOR<Inv>(A, w, nu, type) (preconditioners.hpp:116-240).  A must outlive the preconditioner
+ * (the reference keeps a non-owning pointer too, preconditioners.hpp:120). */
+int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int type, int inv_kind,
+                           bsp_precond** out);
+/* ILU0(A) (preconditioners.hpp:246-357): factor + diagonal-block inverses. */
+int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
+/* IChol0(A) (preconditioners.hpp:364-538), A symmetric positive definite. */
+int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
+int bsp_precond_destroy(bsp_precond* M);
+/* r <- M^{-1} r in place (the operator() of every reference preconditioner). */
+int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* M, double* r_dev);
+/* Level sets of the forward (lower=1) or backward (lower=0) sweep: level_host[n]. */
+int bsp_precond_levels(const bsp_precond* M, int lower, int32_t* level_host, int* nlevels);
+/* Factor values in the structure of A (ILU0: LU in A's structure; IChol0: L in lowTri(A)). */
+int bsp_precond_factor_info(const bsp_precond* M, int64_t* nnzb);
+int bsp_precond_download_factor(bsp_ctx* ctx, const bsp_precond* M, int32_t* rowptr_host,
+                                 int32_t* colidx_host, double* vals_host);
+
+/* ---- Krylov solvers (blocksp_solvers.hpp) ---- */
+
+typedef struct {
+    int iters;          /* Krylov iterations (the reference's totIters / iter counter) */
+    int restarts;       /* deflated restarts */
+    int converged;      /* 1 when the stopping rule was met */
+    int precond_applies;
+    double res0;        /* ||M^{-1} b||_2 */
+    double res_est;     /* last residual estimate used by the stopping rule */
+    double true_relres; /* ||b - A x||_2 / ||b||_2 of the returned x */
+    double setup_ms;    /* not filled by the solvers (preconditioners are built separately) */
+    double solve_ms;    /* wall time of the solve on the context stream */
+} bsp_stats;
+
+/* Stopping rule: 0 = the reference's (pGMRES: least-squares residual estimate of the
+ * preconditioned system <= tol*||M^{-1}b||, solvers.hpp:177-183; pCG: ||r|| <= tol*||M^{-1}b||,
+ * solvers.hpp:64-65), 1 = true relative residual ||b-Ax||/||b|| <= tol (checked on the device,
+ * restarting from the current iterate when the estimate converged first). */
+#define BSP_TOL_REFERENCE 0
+#define BSP_TOL_TRUE_RESIDUAL 1
+
+/* pGMRES(A, b, tol, M, maxIters, restart, k) with deflated restart (solvers.hpp:156-295,
+ * blocksp_deflated_restart.hpp:38-269).  x_dev is output (initial guess 0, as in the reference). */
+int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
+               double tol, int maxIters, int restart, int k, int tol_mode, bsp_stats* stats);
+/* pCG(A, b, tol, M, maxIters) (solvers.hpp:51-105). */
+int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
+            double tol, int maxIters, int tol_mode, bsp_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLOCKSP_C_H */
