@@ -81,6 +81,7 @@ _SIGS = {
     "bsp_precond_create_sor": [_VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)],
     "bsp_precond_create_ilu0": [_VP, _VP, C.POINTER(_VP)],
     "bsp_precond_create_ichol0": [_VP, _VP, C.POINTER(_VP)],
+    "bsp_precond_create_host": [_VP, _VP, _VP, _VP, C.POINTER(_VP)],
     "bsp_precond_destroy": [_VP],
     "bsp_precond_apply": [_VP, _VP, _VP],
     "bsp_precond_levels": [_VP, C.c_int, _VP, C.POINTER(C.c_int)],

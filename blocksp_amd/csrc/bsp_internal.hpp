@@ -60,7 +60,9 @@ struct Sweep {
     DevMem rp, ci, val;          // owned block rows (original per-row order), if any
     const int* drp = nullptr;    // the structure the kernels read: the owned arrays or
     const int* dci = nullptr;    // (SOR) the matrix's own arrays
-    const double* dval = nullptr;
+    const void* dval = nullptr;
+    bool f32 = false;            // values / vectors in FP32 instead of FP64
+    unsigned long long tickets_used = 0;   // ticket counter value at the next sync-free launch
     // schedule: rows sorted by level, each level padded with -1 to a multiple of
     // the rows handled by one wavefront, so a wavefront never holds two levels.
     DevMem order;                // int32 [nslots]
@@ -84,6 +86,7 @@ struct bsp_ctx {
     int trsv_mode = 1;               // 0 level launches, 1 sync-free single launch
     int64_t spin_limit = int64_t(1) << 22;
     int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
+    int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     // reduction scratch
     bsp::DevMem red_partials;        // double [kMaxRedBlocks]
     bsp::DevMem red_counter;         // uint32 [kRedSlots]
@@ -118,6 +121,8 @@ struct bsp_precond {
     bsp::DevMem dinv_upper;           // [n][bs][bs] applied after the backward sweep / SOR / Diag
     bsp::DevMem work0, work1;         // vectors for SOR ping-pong
     uint32_t epoch = 0;
+    bsp_host_precond_fn host_fn = nullptr;   // BSP_PRE_HOST
+    void* host_user = nullptr;
     // host copy of the factor in the reference layout (for parity download)
     std::vector<int32_t> f_rp, f_ci;
     std::vector<double> f_val;
@@ -143,11 +148,12 @@ int launch_axpy(const bsp_ctx* ctx, int64_t len, double a, const double* x, doub
 int launch_scal(const bsp_ctx* ctx, int64_t len, double a, double* x);
 
 // Sweeps: trsv computes x_i <- Dinv_i (x_i - sum_j T_ij x_j) over the schedule.
-int launch_trsv(bsp_ctx* ctx, Sweep& S, const double* dinv, double* x, uint32_t epoch);
+// Vectors and values are double, or float when S.f32 (bsp_sweeps.hip).
+int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoch);
 // SOR sweep: xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j!=i} A_ij x_j)), x_j from xn on the
 // dependency side (forward: j < i) and from xo otherwise.
-int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const double* dinv,
-                     const double* b, const double* xo, double* xn, uint32_t epoch);
+int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b,
+                     const void* xo, void* xn, uint32_t epoch);
 
 // Krylov helpers (all operands device-resident; h values read from device memory).
 // mgs step: if qprev: v -= h[hprev_slot] * qprev;  then h[hout_slot] = dot(qnext, v) or

@@ -421,160 +421,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(int64_t len, const doubl
     p[i] = z[i] + beta * p[i];
 }
 
-// ---------------------------------------------------------------------------
-// Triangular and SOR sweeps over a level schedule
-// ---------------------------------------------------------------------------
-// Slot s of the (padded) schedule holds row order[s] (or -1).  Each lane group
-// handles one slot.  In the sync-free variant every workgroup first takes a
-// ticket (a device-scope counter) and processes the slots of that ticket, so
-// any row it waits on belongs to a workgroup that is already running; a level
-// never shares a wavefront with another level (the schedule is padded), so no
-// wavefront waits on itself.  Completion of row i is published by a release
-// store flags[i] = epoch after x_i is written; readers acquire the flag before
-// reading x_j.  Waits are bounded by spin_limit polls; on timeout the row sets
-// *err and proceeds, so every wavefront always exits.
-
-__device__ __forceinline__ bool wait_flag(const uint32_t* flags, int j, uint32_t epoch, int64_t spin_limit,
-                                          uint32_t* err) {
-    int64_t it = 0;
-    while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        ++it;
-        // give up after spin_limit polls, or as soon as any other wait has given up
-        if (it > spin_limit ||
-            ((it & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
-            __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return true;
-}
-
-template <int BS, bool SYNCFREE>
-__global__ __launch_bounds__(kBlock) void k_trsv(int slot0, int nslots, const int* __restrict__ order,
-                                                 const int* __restrict__ rp, const int* __restrict__ ci,
-                                                 const double* __restrict__ val, const double* __restrict__ dinv,
-                                                 double* x, uint32_t* flags, uint32_t epoch,
-                                                 unsigned long long* ticket, int64_t spin_limit, uint32_t* err) {
-    constexpr int RPW = kWave / BS;
-    constexpr int SPB = (kBlock / kWave) * RPW;   // slots per workgroup
-    __shared__ unsigned long long s_tk;
-    int wg = blockIdx.x;
-    if constexpr (SYNCFREE) {
-        if (threadIdx.x == 0)
-            s_tk = __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        wg = (int)(s_tk % gridDim.x);
-    }
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
-    const int g = lane / BS;
-    const int r = lane - g * BS;
-    const int slot = slot0 + wg * SPB + wv * RPW + g;
-    const bool have = g < RPW && slot < slot0 + nslots;
-    const int i = have ? order[slot] : -1;
-    const bool active = i >= 0;
-    double s = 0.0;
-    if (active) {
-        s = x[(size_t)i * BS + r];
-        const int p0 = rp[i], p1 = rp[i + 1];
-        for (int p = p0; p < p1; ++p) {
-            const int c = ci[p];
-            double a[BS];
-            load_vec<BS>(val + ((size_t)p * BS + r) * BS, a);
-            if constexpr (SYNCFREE) wait_flag(flags, c, epoch, spin_limit, err);
-            double xv[BS];
-            load_vec<BS>(x + (size_t)c * BS, xv);
-            s -= row_dot<BS>(a, xv);
-        }
-    }
-    double out = s;
-    if (dinv) {
-        const int base = (g < RPW ? g : 0) * BS;
-        double d[BS];
-        if (active) load_vec<BS>(dinv + ((size_t)i * BS + r) * BS, d);
-        out = 0.0;
-#pragma unroll
-        for (int c = 0; c < BS; ++c) {
-            const double sc = __shfl(s, base + c, kWave);
-            if (active) out = fma(d[c], sc, out);
-        }
-    }
-    if (active) {
-        x[(size_t)i * BS + r] = out;
-        if constexpr (SYNCFREE) {
-            if (r == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
-template <int BS, bool SYNCFREE>
-__global__ __launch_bounds__(kBlock) void k_sor(int slot0, int nslots, const int* __restrict__ order,
-                                                const int* __restrict__ rp, const int* __restrict__ ci,
-                                                const double* __restrict__ val, const double* __restrict__ dinv,
-                                                int forward, double w, const double* __restrict__ b,
-                                                const double* __restrict__ xo, double* xn, uint32_t* flags,
-                                                uint32_t epoch, unsigned long long* ticket, int64_t spin_limit,
-                                                uint32_t* err) {
-    constexpr int RPW = kWave / BS;
-    constexpr int SPB = (kBlock / kWave) * RPW;
-    __shared__ unsigned long long s_tk;
-    int wg = blockIdx.x;
-    if constexpr (SYNCFREE) {
-        if (threadIdx.x == 0)
-            s_tk = __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        wg = (int)(s_tk % gridDim.x);
-    }
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
-    const int g = lane / BS;
-    const int r = lane - g * BS;
-    const int slot = slot0 + wg * SPB + wv * RPW + g;
-    const bool have = g < RPW && slot < slot0 + nslots;
-    const int i = have ? order[slot] : -1;
-    const bool active = i >= 0;
-    double ph = 0.0, xold = 0.0;
-    if (active) {
-        xold = xo ? xo[(size_t)i * BS + r] : 0.0;
-        const int p0 = rp[i], p1 = rp[i + 1];
-        for (int p = p0; p < p1; ++p) {
-            const int c = ci[p];
-            if (c == i) continue;
-            const bool dep = forward ? (c < i) : (c > i);
-            double a[BS];
-            load_vec<BS>(val + ((size_t)p * BS + r) * BS, a);
-            double xv[BS];
-            if (dep) {
-                if constexpr (SYNCFREE) wait_flag(flags, c, epoch, spin_limit, err);
-                load_vec<BS>(xn + (size_t)c * BS, xv);
-            } else if (xo) {
-                load_vec<BS>(xo + (size_t)c * BS, xv);
-            } else {
-#pragma unroll
-                for (int k = 0; k < BS; ++k) xv[k] = 0.0;
-            }
-            ph += row_dot<BS>(a, xv);
-        }
-        ph = b[(size_t)i * BS + r] - ph;
-    }
-    const int base = (g < RPW ? g : 0) * BS;
-    double d[BS];
-    if (active) load_vec<BS>(dinv + ((size_t)i * BS + r) * BS, d);
-    double t = 0.0;
-#pragma unroll
-    for (int c = 0; c < BS; ++c) {
-        const double sc = __shfl(ph, base + c, kWave);
-        if (active) t = fma(d[c], sc, t);
-    }
-    if (active) {
-        xn[(size_t)i * BS + r] = xold + (-w * xold + w * t);
-        if constexpr (SYNCFREE) {
-            if (r == 0) __hip_atomic_store(&flags[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
 inline unsigned grid_for(int64_t threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
 
 int reduce_blocks(int64_t len) {
@@ -634,46 +480,7 @@ struct DiagL {
     }
 };
 
-template <int BS>
-struct TrsvL {
-    static void run(hipStream_t st, bool syncfree, int slot0, int nslots, const int* order, const int* rp,
-                    const int* ci, const double* val, const double* dinv, double* x, uint32_t* flags,
-                    uint32_t epoch, unsigned long long* ticket, int64_t spin, uint32_t* err) {
-        constexpr int RPW = kWave / BS;
-        constexpr int SPB = (kBlock / kWave) * RPW;
-        const unsigned grid = (unsigned)((nslots + SPB - 1) / SPB);
-        if (grid == 0) return;
-        if (syncfree)
-            hipLaunchKernelGGL((k_trsv<BS, true>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci, val,
-                               dinv, x, flags, epoch, ticket, spin, err);
-        else
-            hipLaunchKernelGGL((k_trsv<BS, false>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci,
-                               val, dinv, x, flags, epoch, ticket, spin, err);
-    }
-};
-
-template <int BS>
-struct SorL {
-    static void run(hipStream_t st, bool syncfree, int slot0, int nslots, const int* order, const int* rp,
-                    const int* ci, const double* val, const double* dinv, int fwd, double w, const double* b,
-                    const double* xo, double* xn, uint32_t* flags, uint32_t epoch, unsigned long long* ticket,
-                    int64_t spin, uint32_t* err) {
-        constexpr int RPW = kWave / BS;
-        constexpr int SPB = (kBlock / kWave) * RPW;
-        const unsigned grid = (unsigned)((nslots + SPB - 1) / SPB);
-        if (grid == 0) return;
-        if (syncfree)
-            hipLaunchKernelGGL((k_sor<BS, true>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci, val,
-                               dinv, fwd, w, b, xo, xn, flags, epoch, ticket, spin, err);
-        else
-            hipLaunchKernelGGL((k_sor<BS, false>), dim3(grid), dim3(kBlock), 0, st, slot0, nslots, order, rp, ci, val,
-                               dinv, fwd, w, b, xo, xn, flags, epoch, ticket, spin, err);
-    }
-};
-
 }  // namespace
-
-int sweep_rows_per_wave(int bs) { return kWave / bs; }
 
 int launch_bsmv(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* val,
                 const double* x, double* y, int accumulate) {
@@ -745,53 +552,6 @@ int launch_axpy(const bsp_ctx* ctx, int64_t len, double a, const double* x, doub
 int launch_scal(const bsp_ctx* ctx, int64_t len, double a, double* x) {
     if (len <= 0) return BSP_OK;
     hipLaunchKernelGGL(k_scal, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, a, x);
-    BSP_LAUNCH_CHECK();
-    return BSP_OK;
-}
-
-int launch_trsv(bsp_ctx* ctx, Sweep& S, const double* dinv, double* x, uint32_t epoch) {
-    if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
-    const bool sf = ctx->trsv_mode == 1;
-    uint32_t* err = ctx->err_flag.as<uint32_t>();
-    auto* tk = S.ticket.as<unsigned long long>();
-    if (sf) {
-        if (!dispatch_bs<TrsvL>(S.bs, ctx->stream, true, 0, S.nslots, S.order.as<int>(), S.drp,
-                                S.dci, S.dval, dinv, x, S.flags.as<uint32_t>(), epoch, tk,
-                                ctx->spin_limit, err))
-            return fail(ctx, BSP_E_UNSUPPORTED, "triangular sweep: unsupported block size");
-    } else {
-        for (int l = 0; l < S.nlevels; ++l) {
-            const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
-            if (!dispatch_bs<TrsvL>(S.bs, ctx->stream, false, s0, cnt, S.order.as<int>(), S.drp,
-                                    S.dci, S.dval, dinv, x, S.flags.as<uint32_t>(), epoch, tk,
-                                    ctx->spin_limit, err))
-                return fail(ctx, BSP_E_UNSUPPORTED, "triangular sweep: unsupported block size");
-        }
-    }
-    BSP_LAUNCH_CHECK();
-    return BSP_OK;
-}
-
-int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const double* dinv, const double* b,
-                     const double* xo, double* xn, uint32_t epoch) {
-    if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
-    const bool sf = ctx->trsv_mode == 1;
-    uint32_t* err = ctx->err_flag.as<uint32_t>();
-    auto* tk = S.ticket.as<unsigned long long>();
-    if (sf) {
-        if (!dispatch_bs<SorL>(S.bs, ctx->stream, true, 0, S.nslots, S.order.as<int>(), S.drp,
-                               S.dci, S.dval, dinv, forward, w, b, xo, xn,
-                               S.flags.as<uint32_t>(), epoch, tk, ctx->spin_limit, err))
-            return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweep: unsupported block size");
-    } else {
-        for (int l = 0; l < S.nlevels; ++l) {
-            const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
-            if (!dispatch_bs<SorL>(S.bs, ctx->stream, false, s0, cnt, S.order.as<int>(), S.drp,
-                                   S.dci, S.dval, dinv, forward, w, b, xo, xn,
-                                   S.flags.as<uint32_t>(), epoch, tk, ctx->spin_limit, err))
-                return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweep: unsupported block size");
-        }
-    }
     BSP_LAUNCH_CHECK();
     return BSP_OK;
 }

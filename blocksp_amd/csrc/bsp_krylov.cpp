@@ -63,7 +63,10 @@ int check_sweep_errors(bsp_ctx* ctx) {
 // tolerance is absolute on the preconditioned least-squares residual estimate.
 int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
                bool tol_is_relative, int maxIters, int restart, int k, int& iters_out, int& restarts_out,
-               double& res0_out, double& res_out, bool& converged, int& applies) {
+               double& res0_out, double& res_out, bool& converged, int& applies, int stag_window,
+               bool& stagnated) {
+    stagnated = false;
+    std::vector<double> res_hist;
     const int n = A->n_rows, bs = A->bs;
     const int64_t len = (int64_t)n * bs;
     const double eps = std::numeric_limits<double>::epsilon();
@@ -188,6 +191,16 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
             std::fflush(stderr);
         }
         if (tot > maxIters || breakdown || !std::isfinite(res)) break;
+        if (iter == restart && res > tolerance && stag_window > 0) {
+            // stagnation of the restarted recurrence (true-residual mode only): less than a
+            // factor 2 gained over the last stag_window restarts
+            res_hist.push_back(res);
+            const size_t h = res_hist.size();
+            if (h > (size_t)stag_window && res > 0.5 * res_hist[h - 1 - stag_window]) {
+                stagnated = true;
+                break;
+            }
+        }
         if (iter == restart && res > tolerance) {
             // x0 += Q y (solvers.hpp:242-245); the reference's r0 recomputation (248) is only
             // consumed by a pure restart and is skipped for k > 0.
@@ -296,8 +309,14 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     BSP_HIP(hipMemsetAsync(x, 0, sizeof(double) * len, ctx->stream));   // x0 = 0
     int iters = 0, restarts = 0, applies = 0;
     double res0 = 0.0, res = 0.0;
-    bool conv = false;
-    int rc = gmres_core(ctx, A, M, b, x, tol, true, maxIters, restart, k, iters, restarts, res0, res, conv, applies);
+    bool conv = false, stagnated = false;
+    // Reference semantics: one run with the reference's stopping rule.  True-residual mode
+    // additionally leaves a run whose restarted recurrence stagnates (8 restarts without a
+    // factor 2) and restarts from the current iterate with the exact residual.
+    const bool true_mode = tol_mode == BSP_TOL_TRUE_RESIDUAL;
+    const int stag_window = true_mode && restart > 0 ? 8 : 0;
+    int rc = gmres_core(ctx, A, M, b, x, tol, true, maxIters, restart, k, iters, restarts, res0, res, conv, applies,
+                        stag_window, stagnated);
     if (rc < 0) return rc;
     double bnorm = 0.0, rr = 0.0;
     DevMem work;
@@ -306,31 +325,27 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     if (rc) return rc;
     rc = true_relres(ctx, A, b, x, work.as<double>(), bnorm, rr);
     if (rc) return rc;
-    int total_iters = iters, total_restarts = restarts;
-    if (tol_mode == BSP_TOL_TRUE_RESIDUAL) {
-        // iterative refinement in FP64 until the true relative residual meets tol
-        for (int outer = 0; outer < 20 && rr > tol && total_iters < maxIters && std::isfinite(rr); ++outer) {
-            // solve A e = r with a target that brings ||b - A x|| below tol ||b||
-            DevMem e;
-            BSP_HIP(e.alloc(sizeof(double) * len));
-            rc = launch_residual(ctx, A->n_rows, A->bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x,
-                                 work.as<double>());
-            if (rc) return rc;
-            BSP_HIP(hipMemsetAsync(e.p, 0, sizeof(double) * len, ctx->stream));
-            const double inner_tol = std::max(0.25 * tol / rr, 1e-15);
+    int total_iters = iters, total_restarts = restarts, passes = 1;
+    if (true_mode) {
+        // FP64 refinement: restart from x with the exact residual until ||b - Ax|| <= tol ||b||
+        for (; passes < 50 && rr > tol && total_iters < maxIters && std::isfinite(rr); ++passes) {
+            const double pass_tol = std::min(0.5, std::max(0.25 * tol / rr, 1e-12));
             int it2 = 0, rs2 = 0;
             double r02 = 0.0, res2 = 0.0;
             bool c2 = false;
-            rc = gmres_core(ctx, A, M, work.as<double>(), e.as<double>(), inner_tol, true, maxIters - total_iters,
-                            restart, k, it2, rs2, r02, res2, c2, applies);
+            rc = gmres_core(ctx, A, M, b, x, pass_tol, true, maxIters - total_iters, restart, k, it2, rs2, r02, res2,
+                            c2, applies, stag_window, stagnated);
             if (rc < 0) return rc;
-            rc = launch_axpy(ctx, len, 1.0, e.as<double>(), x);
-            if (rc) return rc;
             total_iters += it2;
             total_restarts += rs2;
             rc = true_relres(ctx, A, b, x, work.as<double>(), bnorm, rr);
             if (rc) return rc;
             res = res2;
+            if (ctx->verbose > 0) {
+                std::fprintf(stderr, "[pgmres] refinement pass %d: +%d iterations, true relres %.3e\n", passes, it2,
+                             rr);
+                std::fflush(stderr);
+            }
         }
         conv = rr <= tol;
     }

@@ -410,6 +410,20 @@ int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out)
     return BSP_OK;
 }
 
+int bsp_precond_create_host(bsp_ctx* ctx, const bsp_bsr* A, bsp_host_precond_fn fn, void* user, bsp_precond** out) {
+    if (!ctx || !A || !out || !fn) return BSP_E_INVALID;
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_HOST;
+    M->n = A->n_rows;
+    M->bs = A->bs;
+    M->A = A;
+    M->host_fn = fn;
+    M->host_user = user;
+    *out = M.release();
+    return BSP_OK;
+}
+
 int bsp_precond_destroy(bsp_precond* M) {
     delete M;
     return BSP_OK;
@@ -422,6 +436,15 @@ int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* Mc, double* r) {
     switch (M->kind) {
         case BSP_PRE_NONE: return BSP_OK;
         case BSP_PRE_DIAG: return launch_block_diag(ctx, M->n, M->bs, M->dinv_upper.as<double>(), r);
+        case BSP_PRE_HOST: {
+            std::vector<double> h((size_t)len);
+            BSP_HIP(hipMemcpyAsync(h.data(), r, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream));
+            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            M->host_fn(h.data(), len, M->host_user);
+            BSP_HIP(hipMemcpyAsync(r, h.data(), sizeof(double) * len, hipMemcpyHostToDevice, ctx->stream));
+            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            return BSP_OK;
+        }
         case BSP_PRE_ILU0: {
             int rc = launch_trsv(ctx, M->lower, nullptr, r, ++M->epoch);
             if (rc) return rc;

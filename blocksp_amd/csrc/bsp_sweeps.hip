@@ -1,0 +1,286 @@
+// bsp_sweeps.hip -- level-scheduled block triangular solves and SOR sweeps (gfx950).
+//
+// A sweep visits the block rows of a schedule: rows sorted by level, every
+// level padded to whole wavefronts (one wavefront never holds two levels).  A
+// block row is owned by BS lanes (lane r owns row r of each BS x BS block), as
+// in the bsmv kernel.
+//
+//  * trsv: x_i <- Dinv_i (x_i - sum_j T_ij x_j) over the rows of T (the strict
+//    part of an ILU0 / IChol0 factor); Dinv may be absent (unit diagonal).
+//  * SOR:  xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j != i} A_ij x_j)), with x_j
+//    from xn on the dependency side (forward: j < i) and from xo elsewhere
+//    (blocksp_preconditioners.hpp:135-202).
+//
+// Two launch modes:
+//  * level launches: one kernel per level (kernel boundaries order the levels);
+//  * single launch ("sync-free"): a persistent grid takes chunks of the
+//    schedule in order from a device ticket counter, so every row a chunk waits
+//    on belongs to a chunk that is already held by a running workgroup.  A row
+//    polls its dependencies' completion flags with relaxed coherent loads, then
+//    issues one acquire fence; a wavefront publishes its rows with one release
+//    fence followed by relaxed flag stores (flag = epoch of this launch).  Every
+//    wait is bounded (spin_limit polls, or immediately once any wait has failed
+//    and set *err), so every wavefront always exits.
+#include "bsp_internal.hpp"
+
+#include <algorithm>
+
+namespace bsp {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+template <int BS, typename VT>
+__device__ __forceinline__ void load_row(const VT* __restrict__ src, VT (&a)[BS]) {
+    if constexpr (std::is_same_v<VT, double> && BS % 2 == 0) {
+        const double2* s2 = reinterpret_cast<const double2*>(src);
+#pragma unroll
+        for (int c = 0; c < BS / 2; ++c) {
+            const double2 t = s2[c];
+            a[2 * c] = t.x;
+            a[2 * c + 1] = t.y;
+        }
+    } else if constexpr (std::is_same_v<VT, float> && BS % 4 == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+        for (int c = 0; c < BS / 4; ++c) {
+            const float4 t = s4[c];
+            a[4 * c] = t.x;
+            a[4 * c + 1] = t.y;
+            a[4 * c + 2] = t.z;
+            a[4 * c + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) a[c] = src[c];
+    }
+}
+
+template <int BS, typename VT>
+__device__ __forceinline__ VT row_dot(const VT (&a)[BS], const VT (&x)[BS]) {
+    VT s = VT(0);
+#pragma unroll
+    for (int c = 0; c < BS; ++c) s = fma(a[c], x[c], s);
+    return s;
+}
+
+__device__ __forceinline__ void wait_relaxed(const uint32_t* flag, uint32_t epoch, int64_t spin_limit, uint32_t* err) {
+    int64_t it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        ++it;
+        if (it > spin_limit ||
+            ((it & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
+            __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+struct SweepArgs {
+    int nchunks;
+    const int* order;
+    const int* rp;
+    const int* ci;
+    const void* val;
+    const void* dinv;
+    void* x;            // trsv: in/out; SOR: xn (out)
+    const void* b;      // SOR right-hand side
+    const void* xo;     // SOR previous iterate (nullptr: zero)
+    double w;
+    int forward;
+    uint32_t* flags;
+    uint32_t epoch;
+    unsigned long long* ticket;
+    unsigned long long base;
+    int64_t spin_limit;
+    uint32_t* err;
+};
+
+// One chunk of slots: [slot_begin, slot_end) with kWavesPerBlock * RPW slots per block.
+template <int BS, typename VT, bool SF, bool SOR>
+__device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, int slot_end) {
+    constexpr int RPW = kWave / BS;
+    const VT* __restrict__ val = static_cast<const VT*>(a.val);
+    const VT* __restrict__ dinv = static_cast<const VT*>(a.dinv);
+    VT* x = static_cast<VT*>(a.x);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    const int slot = slot_begin + wv * RPW + g;
+    const bool have = g < RPW && slot < slot_end;
+    const int i = have ? a.order[slot] : -1;
+    const bool active = i >= 0;
+    const int p0 = active ? a.rp[i] : 0, p1 = active ? a.rp[i + 1] : 0;
+    if constexpr (SF) {
+        if (active) {
+            for (int p = p0; p < p1; ++p) {
+                const int c = a.ci[p];
+                const bool dep = SOR ? (a.forward ? c < i : c > i) : true;
+                if (dep) wait_relaxed(&a.flags[c], a.epoch, a.spin_limit, a.err);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    VT s = VT(0), xold = VT(0);
+    if (active) {
+        if constexpr (SOR) {
+            const VT* xo = static_cast<const VT*>(a.xo);
+            const VT* b = static_cast<const VT*>(a.b);
+            xold = xo ? xo[(size_t)i * BS + r] : VT(0);
+            VT ph = VT(0);
+            for (int p = p0; p < p1; ++p) {
+                const int c = a.ci[p];
+                if (c == i) continue;
+                const bool dep = a.forward ? c < i : c > i;
+                VT av[BS], xv[BS];
+                load_row<BS, VT>(val + ((size_t)p * BS + r) * BS, av);
+                if (dep) {
+                    load_row<BS, VT>(x + (size_t)c * BS, xv);
+                } else if (xo) {
+                    load_row<BS, VT>(xo + (size_t)c * BS, xv);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < BS; ++k) xv[k] = VT(0);
+                }
+                ph += row_dot<BS, VT>(av, xv);
+            }
+            s = b[(size_t)i * BS + r] - ph;
+        } else {
+            s = x[(size_t)i * BS + r];
+            for (int p = p0; p < p1; ++p) {
+                const int c = a.ci[p];
+                VT av[BS], xv[BS];
+                load_row<BS, VT>(val + ((size_t)p * BS + r) * BS, av);
+                load_row<BS, VT>(x + (size_t)c * BS, xv);
+                s -= row_dot<BS, VT>(av, xv);
+            }
+        }
+    }
+    VT out = s;
+    if (dinv) {
+        const int base = (g < RPW ? g : 0) * BS;
+        VT d[BS];
+        if (active) load_row<BS, VT>(dinv + ((size_t)i * BS + r) * BS, d);
+        out = VT(0);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const VT sc = __shfl(s, base + c, kWave);
+            if (active) out = fma(d[c], sc, out);
+        }
+    }
+    if constexpr (SOR) {
+        const VT w = (VT)a.w;
+        out = xold + (-w * xold + w * out);
+    }
+    if (active) x[(size_t)i * BS + r] = out;
+    if constexpr (SF) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (active && r == 0) __hip_atomic_store(&a.flags[i], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int BS, typename VT, bool SOR>
+__global__ __launch_bounds__(kBlock) void k_sweep_level(SweepArgs a, int slot0, int count) {
+    constexpr int SPB = kWavesPerBlock * (kWave / BS);
+    sweep_chunk<BS, VT, false, SOR>(a, slot0 + blockIdx.x * SPB, slot0 + count);
+}
+
+template <int BS, typename VT, bool SOR>
+__global__ __launch_bounds__(kBlock) void k_sweep_syncfree(SweepArgs a, int nslots) {
+    constexpr int SPB = kWavesPerBlock * (kWave / BS);
+    __shared__ long long s_chunk;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_chunk = (long long)(__hip_atomic_fetch_add(a.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                                  a.base);
+        __syncthreads();
+        const long long chunk = s_chunk;
+        __syncthreads();
+        if (chunk >= a.nchunks) break;
+        sweep_chunk<BS, VT, true, SOR>(a, (int)chunk * SPB, nslots);
+    }
+}
+
+template <int BS, typename VT, bool SOR>
+int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
+    constexpr int SPB = kWavesPerBlock * (kWave / BS);
+    const int nchunks = (S.nslots + SPB - 1) / SPB;
+    a.nchunks = nchunks;
+    if (ctx->trsv_mode == 1) {
+        const int grid = std::max(1, std::min(nchunks, ctx->num_cus * ctx->sweep_wg_per_cu));
+        a.base = S.tickets_used;
+        hipLaunchKernelGGL((k_sweep_syncfree<BS, VT, SOR>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, S.nslots);
+        S.tickets_used += (unsigned long long)(nchunks + grid);
+    } else {
+        for (int l = 0; l < S.nlevels; ++l) {
+            const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
+            if (cnt <= 0) continue;
+            hipLaunchKernelGGL((k_sweep_level<BS, VT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, ctx->stream,
+                               a, s0, cnt);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
+    return BSP_OK;
+}
+
+template <typename VT, bool SOR>
+int dispatch(bsp_ctx* ctx, Sweep& S, const SweepArgs& a) {
+    switch (S.bs) {
+        case 1: return run_sweep<1, VT, SOR>(ctx, S, a);
+        case 2: return run_sweep<2, VT, SOR>(ctx, S, a);
+        case 3: return run_sweep<3, VT, SOR>(ctx, S, a);
+        case 4: return run_sweep<4, VT, SOR>(ctx, S, a);
+        case 5: return run_sweep<5, VT, SOR>(ctx, S, a);
+        case 6: return run_sweep<6, VT, SOR>(ctx, S, a);
+        case 7: return run_sweep<7, VT, SOR>(ctx, S, a);
+        case 8: return run_sweep<8, VT, SOR>(ctx, S, a);
+        case 16: return run_sweep<16, VT, SOR>(ctx, S, a);
+        case 32: return run_sweep<32, VT, SOR>(ctx, S, a);
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "sweep: unsupported block size " + std::to_string(S.bs));
+    }
+}
+
+SweepArgs make_args(bsp_ctx* ctx, Sweep& S, uint32_t epoch) {
+    SweepArgs a{};
+    a.order = S.order.as<int>();
+    a.rp = S.drp;
+    a.ci = S.dci;
+    a.val = S.dval;
+    a.flags = S.flags.as<uint32_t>();
+    a.epoch = epoch;
+    a.ticket = S.ticket.as<unsigned long long>();
+    a.spin_limit = ctx->spin_limit;
+    a.err = ctx->err_flag.as<uint32_t>();
+    return a;
+}
+
+}  // namespace
+
+int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoch) {
+    if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
+    SweepArgs a = make_args(ctx, S, epoch);
+    a.dinv = dinv;
+    a.x = x;
+    return S.f32 ? dispatch<float, false>(ctx, S, a) : dispatch<double, false>(ctx, S, a);
+}
+
+int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b, const void* xo,
+                     void* xn, uint32_t epoch) {
+    if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
+    SweepArgs a = make_args(ctx, S, epoch);
+    a.dinv = dinv;
+    a.x = xn;
+    a.b = b;
+    a.xo = xo;
+    a.w = w;
+    a.forward = forward;
+    return S.f32 ? dispatch<float, true>(ctx, S, a) : dispatch<double, true>(ctx, S, a);
+}
+
+}  // namespace bsp

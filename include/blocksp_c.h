@@ -116,6 +116,7 @@ int bsp_scal(bsp_ctx* ctx, int64_t len, double a, double* x_dev);
 #define BSP_PRE_SOR 2
 #define BSP_PRE_ILU0 3
 #define BSP_PRE_ICHOL0 4
+#define BSP_PRE_HOST 5
 
 /* NoPre (preconditioners.hpp:66-71). */
 int bsp_precond_create_none(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
@@ -129,6 +130,14 @@ int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int
 int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
 /* IChol0(A) (preconditioners.hpp:364-538), A symmetric positive definite. */
 int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
+/* A user preconditioner (the reference's PreCond concept: any object with
+ * `void operator()(solArray&) const`, preconditioners.hpp:542-556, lambdas in
+ * blocksp_tests.hpp:113).  The library copies r to a host buffer, calls
+ * fn(r_host, len, user) which must overwrite it with M^{-1} r, and copies it back.
+ * This keeps arbitrary host preconditioners usable by the device solvers. */
+typedef void (*bsp_host_precond_fn)(double* r_host, int64_t len, void* user);
+int bsp_precond_create_host(bsp_ctx* ctx, const bsp_bsr* A, bsp_host_precond_fn fn, void* user,
+                            bsp_precond** out);
 int bsp_precond_destroy(bsp_precond* M);
 /* r <- M^{-1} r in place (the operator() of every reference preconditioner). */
 int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* M, double* r_dev);

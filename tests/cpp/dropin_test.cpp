@@ -1,0 +1,187 @@
+// Drop-in test: reference-style user code against include/blocksp_include.hpp.
+// Builds matrices through the inserting operator(), then calls bsmv, the solver
+// structs, ez:: one-call solvers and a user lambda preconditioner, checking the
+// true residual of every solve on the host.  Prints one line per check and
+// exits non-zero on the first failure.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "blocksp_include.hpp"
+
+// user code names the matrix type BlockSp::BlockSp<T, B*D> (README.md:48)
+template <typename T, int M>
+using BSM = BlockSp::BlockSp<T, M>;
+using BlockSp::solArray;
+using BlockSp::TinyMatrix;
+using BlockSp::TinyVector;
+using BlockSp::bsmv;
+using BlockSp::pCG;
+using BlockSp::pGMRES;
+using BlockSp::ILU0_pGMRES;
+using BlockSp::IChol0_pCG;
+namespace ez = BlockSp::ez;
+namespace preConditioners = BlockSp::preConditioners;
+namespace dense = BlockSp::dense;
+
+static unsigned long long g_state = 88172645463325252ull;
+static double urand() {   // xorshift64 in [-1, 1)
+    g_state ^= g_state << 13;
+    g_state ^= g_state >> 7;
+    g_state ^= g_state << 17;
+    return (double)(g_state >> 11) / (double)(1ull << 52) - 1.0;
+}
+
+static int failures = 0;
+static void report(const char* name, double value, double limit) {
+    const bool ok = value <= limit && value == value;
+    std::printf("%s %s %.3e (limit %.1e)\n", ok ? "OK" : "FAIL", name, value, limit);
+    if (!ok) ++failures;
+}
+
+// 2D 5-point block stencil, nx x ny, blocks BD x BD; spd selects a symmetric matrix.
+template <int BD>
+BSM<double, BD> stencil(int nx, int ny, bool spd) {
+    BSM<double, BD> A(nx * ny);
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            const int i = iy * nx + ix;
+            TinyMatrix<double, BD, BD> d(0.0);
+            for (int r = 0; r < BD; ++r) d(r, r) = 4.5;
+            A(i, i) = d;
+            const int nb[4] = {ix > 0 ? i - 1 : -1, ix < nx - 1 ? i + 1 : -1, iy > 0 ? i - nx : -1,
+                               iy < ny - 1 ? i + nx : -1};
+            for (int j : nb) {
+                if (j < 0) continue;
+                TinyMatrix<double, BD, BD> o(0.0);
+                for (int r = 0; r < BD; ++r) {
+                    o(r, r) = -1.0;
+                    if (!spd && r + 1 < BD) o(r, r + 1) = 0.1 * urand();
+                }
+                A(i, j) = o;
+            }
+        }
+    A.shrink_to_fit();
+    return A;
+}
+
+template <int B, int D>
+double relres(const BSM<double, B * D>& A, const solArray<double, B, D>& x, const solArray<double, B, D>& b) {
+    // host residual b - A x, independent of the device
+    std::vector<double> r(b.data(), b.data() + b.size_flat());
+    const int bs = B * D;
+    for (int i = 0; i < A.m(); ++i)
+        for (size_t k = 0; k < A.colSize(i); ++k) {
+            const int j = A.colInd(i, (int)k);
+            const auto& blk = A.get_data(i)[k];
+            for (int rr = 0; rr < bs; ++rr)
+                for (int c = 0; c < bs; ++c) r[i * bs + rr] -= blk(rr, c) * x.data()[j * bs + c];
+        }
+    double nr = 0, nb = 0;
+    for (size_t i = 0; i < r.size(); ++i) {
+        nr += r[i] * r[i];
+        nb += b.data()[i] * b.data()[i];
+    }
+    return std::sqrt(nr / nb);
+}
+
+int main() {
+    // ---- B=2, D=1 nonsymmetric
+    auto A = stencil<2>(20, 16, false);
+    solArray<double, 2> b(A.m());
+    for (auto& e : b) e(0) = TinyVector<double, 2>(urand(), urand());
+    {
+        auto y = bsmv(A, b);
+        double err = 0.0;
+        for (int i = 0; i < A.m(); ++i) {
+            TinyVector<double, 2> acc(0.0);
+            for (size_t k = 0; k < A.colSize(i); ++k) {
+                const auto& blk = A.get_data(i)[k];
+                const auto& xv = b(A.colInd(i, (int)k));
+                acc(0) += blk(0, 0) * xv(0) + blk(0, 1) * xv(1);
+                acc(1) += blk(1, 0) * xv(0) + blk(1, 1) * xv(1);
+            }
+            err = std::max(err, std::max(std::abs(acc(0) - y(i)(0)), std::abs(acc(1) - y(i)(1))));
+        }
+        report("bsmv_b2", err, 1e-13);
+        solArray<double, 2> acc = b;
+        bsmv(acc, A, b);   // accumulating form: acc = b + A b
+        double e2 = 0.0;
+        for (size_t i = 0; i < acc.size_flat(); ++i) e2 = std::max(e2, std::abs(acc.data()[i] - b.data()[i] - y.data()[i]));
+        report("bsmv_b2_accumulate", e2, 1e-12);
+    }
+    report("ilu0_pgmres_dr", relres<2, 1>(A, ILU0_pGMRES<double, 2>(A, 1e-10, 5000, 30, 10).solve(A, b), b), 1e-8);
+    report("ilu0_pgmres", relres<2, 1>(A, ez::ilu0_pGMRES<double, 2>(A, b), b), 1e-6);
+    report("sor_pgmres_dr", relres<2, 1>(A, ez::sor_pGMRES_dr<double, 2>(A, b), b), 1e-6);
+    report("sor_pgmres_robust", relres<2, 1>(A, ez::sor_pGMRES_robust<double, 2>(A, b), b), 1e-6);
+    {
+        preConditioners::SOR<dense::LUFactor<double, 2>, double, 2> sor(A, 1.2, 3, 0);
+        auto x = pGMRES<double, 2, 1>(A, b, 1e-10, sor, 5000, 30, 10);
+        report("pgmres_sor_struct", relres<2, 1>(A, x, b), 1e-8);
+    }
+    {
+        // user preconditioner: a lambda doing block-Jacobi on the host (the PreCond concept)
+        auto jac = [&](solArray<double, 2>& r) {
+            for (int i = 0; i < A.m(); ++i) {
+                const auto& d = A(i, i);
+                const double det = d(0, 0) * d(1, 1) - d(0, 1) * d(1, 0);
+                const double r0 = r(i)(0), r1 = r(i)(1);
+                r(i)(0) = (d(1, 1) * r0 - d(0, 1) * r1) / det;
+                r(i)(1) = (-d(1, 0) * r0 + d(0, 0) * r1) / det;
+            }
+        };
+        auto x = pGMRES<double, 2, 1>(A, b, 1e-10, jac, 5000, 30, 10);
+        report("pgmres_lambda_precond", relres<2, 1>(A, x, b), 1e-8);
+        auto x2 = pGMRES<double, 2, 1>(A, b, 1e-10, preConditioners::NoPre<double, 2>(), 5000, 30, 10);
+        report("pgmres_nopre", relres<2, 1>(A, x2, b), 1e-8);
+    }
+    // ---- SPD, B=4 D=1: pCG
+    auto S = stencil<4>(16, 12, true);
+    solArray<double, 4> bs4(S.m());
+    for (auto& e : bs4) e(0) = TinyVector<double, 4>(urand(), urand(), urand(), urand());
+    report("ichol0_pcg", relres<4, 1>(S, IChol0_pCG<double, 4>(S, 1e-10).solve(S, bs4), bs4), 1e-8);
+    report("ez_ichol0_pcg", relres<4, 1>(S, ez::ichol0_pCG<double, 4>(S, bs4), bs4), 1e-6);
+    report("ez_sor_pcg", relres<4, 1>(S, ez::sor_pCG<double, 4>(S, bs4), bs4), 1e-6);
+    {
+        preConditioners::Diag<dense::CholeskyFactor<double, 4>, double, 4> dg(S);
+        report("pcg_diag", relres<4, 1>(S, pCG<double, 4, 1>(S, bs4, 1e-10, dg, 5000), bs4), 1e-8);
+    }
+    // ---- D=2: BlockSp<double, 4> with solArray<double, 2, 2>
+    auto A4 = stencil<4>(12, 10, false);
+    solArray<double, 2, 2> b22(A4.m());
+    for (int i = 0; i < b22.len(); ++i)
+        for (int d = 0; d < 2; ++d) b22(i, d) = TinyVector<double, 2>(urand(), urand());
+    {
+        auto y = bsmv(A4, b22);
+        solArray<double, 2, 2> y2(A4.m());
+        bsmv(y2, A4, b22);
+        double e = 0.0;
+        for (size_t i = 0; i < y.size_flat(); ++i) e = std::max(e, std::abs(y.data()[i] - y2.data()[i]));
+        report("bsmv_d2_forms_agree", e, 0.0);
+    }
+    report("ilu0_pgmres_dr_d2", relres<2, 2>(A4, ez::ilu0_pGMRES_dr<double, 2, 2>(A4, b22), b22), 1e-6);
+    // ---- float matrices (widened to FP64 on the device)
+    BSM<float, 2> Af(A.m());
+    for (int i = 0; i < A.m(); ++i)
+        for (size_t k = 0; k < A.colSize(i); ++k) {
+            TinyMatrix<float, 2, 2> m;
+            for (int e = 0; e < 4; ++e) m.data()[e] = (float)A.get_data(i)[k].data()[e];
+            Af(i, A.colInd(i, (int)k)) = m;
+        }
+    solArray<float, 2> bf(A.m());
+    for (size_t i = 0; i < bf.size_flat(); ++i) bf.data()[i] = (float)b.data()[i];
+    auto xf = ez::ilu0_pGMRES_dr<float, 2>(Af, bf);
+    solArray<double, 2> xd(A.m());
+    for (size_t i = 0; i < xd.size_flat(); ++i) xd.data()[i] = xf.data()[i];
+    report("float_ilu0_pgmres_dr", relres<2, 1>(A, xd, b), 1e-4);
+    // ---- modifying a matrix invalidates its device copy
+    {
+        auto B2 = A;
+        auto y0 = bsmv(B2, b);
+        B2(0, 0)(0, 0) += 1.0;
+        auto y1 = bsmv(B2, b);
+        report("cache_invalidation", std::abs((y1(0)(0) - y0(0)(0)) - b(0)(0)), 1e-14);
+    }
+    std::printf("%s\n", failures ? "DROPIN FAILED" : "DROPIN PASSED");
+    return failures ? 1 : 0;
+}
