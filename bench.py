@@ -127,7 +127,15 @@ def main():
                 "unit": "GB/s", "frac": round(bsmv_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
                 "algorithmic_bytes": bsmv_bytes, "avg_us": round(bsmv_ms * 1e3, 2)}
     if args.bsmv_only:
-        print(json.dumps({"bsmv_us": bsmv_ms * 1e3, "bsmv_GBps": bsmv_gbps}))
+        # calibration kernel of known traffic for the FETCH_SIZE / WRITE_SIZE counters:
+        # axpy on 2^21 doubles reads 2 x 16 MiB and writes 16 MiB per launch
+        import ctypes
+        n = 1 << 21
+        xa, ya = ctx.to_device(np.ones(n)), ctx.to_device(np.ones(n))
+        for _ in range(20):
+            ctx.check(ctx.L.bsp_axpy(ctx.h, n, ctypes.c_double(0.5), xa.ptr, ya.ptr), "axpy")
+        ctx.sync()
+        print(json.dumps({"bsmv_us": bsmv_ms * 1e3, "bsmv_GBps": bsmv_gbps, "algorithmic_bytes": bsmv_bytes}))
         return
 
     t0 = time.perf_counter()
