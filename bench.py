@@ -2,21 +2,26 @@
 
 Workload (N=1): the 2D LDG-style vector Laplacian -- 512x512 quads, Q1 SWIP DG
 x 2 components -> BlockSp<double,8>, 262144 block rows, 5 blocks/row,
-coefficient lognormal(0,1) per element (seed 1), RHS N(0,1).
+coefficient lognormal(0,1) per element (seed 1), RHS N(0,1).  Synthetic data.
 
-One step = one complete FP64 ILU0-pGMRES(30, deflated restart k=10) solve to a
-true relative residual ||b - Ax||/||b|| <= 1e-10, with A, b and the ILU0
-factor already resident in HBM.  `value` is the time to solution of one step
-(seconds, lower is better).  Preconditioner setup is reported separately.
+One step = one complete solve of A x = b to a true FP64 relative residual
+||b - Ax|| / ||b|| <= 1e-10 with A, b and the preconditioner resident in HBM:
+  * the headline (`value`): mixed-precision FGMRES(30, deflated restart k=10) --
+    FP32 ILU0 eliminated by parity colours and FP32 SpMV inside, FP64 Gram-Schmidt,
+    least squares, deflation, solution update and true residual, FP64 refinement;
+  * beside it: the plain FP64 ILU0-pGMRES(30, k=10) drop-in (the reference's
+    method: natural-order ILU0, left preconditioning), measured once per run.
+`value` is the time to solution of one step in seconds (lower is better).
+Preconditioner setup is reported separately (`setup_s`).
 
 Also measured in the same run:
-  * roofline of the bsmv kernel (the HBM-bound product on this matrix):
-    algorithmic bytes per launch / average launch time (HIP events on the
-    library stream), against 8 TB/s;
+  * roofline of the bsmv kernel: algorithmic bytes per launch / average launch time
+    (HIP events on the library stream) against 8 TB/s; `traffic` is the measured HBM
+    bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/;
   * cpu_baseline: the CPU oracle (this project's restatement of the reference
     algorithm, 1 thread) on a bounded sample, scaled to the same unit.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-plain]
 """
 from __future__ import annotations
 
@@ -33,6 +38,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "time to 1e-10 FP64 rel. residual: mixed/learned vs plain FP64 GMRES; bsmv GB/s"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bsmv_dg2d_veclap_512x512.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def bsmv_algorithmic_bytes(n, bs, nnzb, elem=8, accumulate=False):
@@ -54,58 +64,57 @@ def measure_bsmv(bsp, ctx, A, p, reps=50):
         bsp.bsmv(ctx, A, x, y)
     ctx.record(1)
     ms = ctx.elapsed_ms(0, 1) / reps
-    nbytes = bsmv_algorithmic_bytes(p.n, p.bs, p.nnzb)
-    return ms, nbytes
+    return ms, bsmv_algorithmic_bytes(p.n, p.bs, p.nnzb)
 
 
 def cpu_baseline(p, gpu_iters, sample_iters=12):
-    """Oracle ILU0-pGMRES on 1 thread: setup + a bounded number of iterations,
-    scaled to the GPU run's iteration count."""
+    """Oracle ILU0-pGMRES(30,k=10) on 1 thread: setup + a bounded number of iterations on
+    the full matrix, scaled to the plain GPU run's iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
     t0 = time.perf_counter()
-    lu, dlu, dpiv = po.ilu0_factor(p)
+    po.ilu0_factor(p)
     t_setup = time.perf_counter() - t0
-    # time per iteration from a restart-free run of sample_iters iterations
     t0 = time.perf_counter()
     _, st = po.pgmres(p, p.b, 1e-30, po.PRE_ILU0, maxIters=sample_iters - 1, restart=30, k=10)
     t_run = time.perf_counter() - t0
-    # the pgmres call rebuilds the factor; subtract it
-    per_iter = max(t_run - t_setup, 1e-9) / max(st.iters, 1)
+    per_iter = max(t_run - t_setup, 1e-9) / max(st.iters, 1)   # pgmres rebuilds the factor
     x = np.random.default_rng(0).standard_normal(p.n * p.bs)
     t0 = time.perf_counter()
     reps = 3
     for _ in range(reps):
         po.bsmv(p, x)
     t_bsmv = (time.perf_counter() - t0) / reps
-    est = t_setup + per_iter * gpu_iters
     return {
-        "value": est,
+        "value": round(t_setup + per_iter * gpu_iters, 3),
         "unit": "s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle ILU0 factor + {st.iters} pGMRES(30,k=10) iterations on the full matrix, "
-                  f"time/iteration x {gpu_iters} GPU iterations + setup",
-        "setup_s": t_setup,
-        "s_per_iter": per_iter,
-        "bsmv_ms": t_bsmv * 1e3,
-        "bsmv_GBps": bsmv_algorithmic_bytes(p.n, p.bs, p.nnzb) / t_bsmv / 1e9,
+        "sample": f"oracle ILU0 factor + {st.iters} ILU0-pGMRES(30,k=10) iterations on the full matrix; "
+                  f"time/iteration x {gpu_iters} (the plain GPU solve's iterations) + setup",
+        "setup_s": round(t_setup, 3),
+        "s_per_iter": round(per_iter, 4),
+        "bsmv_ms": round(t_bsmv * 1e3, 3),
+        "bsmv_GBps": round(bsmv_algorithmic_bytes(p.n, p.bs, p.nnzb) / t_bsmv / 1e9, 2),
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--bsmv-only", action="store_true", help="only the bsmv roofline (for profiling)")
     ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solve path here is "
-              f"single-GPU and its multi-GPU sharding is a design note, not a measured configuration.",
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the solve path is built and "
+              f"measured on one GPU; its multi-GPU sharding is a design note, not a measured configuration.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -114,17 +123,20 @@ def main():
 
     t0 = time.perf_counter()
     p = pr.dg2d_veclap(args.nx, args.nx, seed=1)
-    t_gen = time.perf_counter() - t0
-    print(f"[bench] generated {p.name}: n={p.n} bs={p.bs} nnzb={p.nnzb} in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    colour = pr.parity_colours((args.nx, args.nx))
+    log(f"generated {p.name}: n={p.n} bs={p.bs} nnzb={p.nnzb} in {time.perf_counter() - t0:.1f}s")
     ctx = bsp.Context(0)
     A = bsp.BSR.from_problem(ctx, p)
     b = ctx.to_device(p.b)
 
     bsmv_ms, bsmv_bytes = measure_bsmv(bsp, ctx, A, p)
     bsmv_gbps = bsmv_bytes / (bsmv_ms * 1e-3) / 1e9
-    print(f"[bench] bsmv {bsmv_ms * 1e3:.1f} us/launch, {bsmv_gbps:.0f} GB/s", file=sys.stderr, flush=True)
+    log(f"bsmv {bsmv_ms * 1e3:.1f} us/launch, {bsmv_gbps:.0f} GB/s")
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.nx == 512:
+        traffic = json.load(open(TRAFFIC_FILE)).get("traffic_bytes_per_launch")
     roofline = {"kernel": "k_bsmv<8>", "bound": "hbm", "achieved": round(bsmv_gbps, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(bsmv_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(bsmv_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "algorithmic_bytes": bsmv_bytes, "avg_us": round(bsmv_ms * 1e3, 2)}
     if args.bsmv_only:
         # calibration kernel of known traffic for the FETCH_SIZE / WRITE_SIZE counters:
@@ -137,58 +149,86 @@ def main():
         ctx.sync()
         print(json.dumps({"bsmv_us": bsmv_ms * 1e3, "bsmv_GBps": bsmv_gbps, "algorithmic_bytes": bsmv_bytes}))
         return
+    if args.verbose:
+        ctx.set_option("verbose", args.verbose)
+    if args.profile_iters:
+        # a short run of each solver for kernel traces (not a benchmark step)
+        M32 = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F32)
+        _, st, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M32, maxIters=args.profile_iters, restart=30, k=10)
+        log(f"mixed: {st.iters} iterations in {st.solve_ms:.1f} ms")
+        M = bsp.ILU0(ctx, A)
+        for mode in (0, 1):
+            ctx.set_option("trsv", mode)
+            _, st, _ = bsp.pGMRES(ctx, A, b, 1e-10, M, maxIters=args.profile_iters, restart=30, k=10)
+            log(f"plain trsv={mode}: {st.iters} iterations in {st.solve_ms:.1f} ms")
+        return
 
+    # ---- mixed precision (headline) ----
     t0 = time.perf_counter()
-    M = bsp.ILU0(ctx, A)
+    M32 = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F32)
     ctx.sync()
-    setup_s = time.perf_counter() - t0
-    print(f"[bench] ILU0 setup {setup_s:.2f}s", file=sys.stderr, flush=True)
-    apply_ms = {}
-    r = ctx.to_device(np.random.default_rng(1).standard_normal(p.n * p.bs))
-    for mode in (1, 0):
-        ctx.set_option("trsv", mode)
-        M(r)
-        ctx.sync()
-        ctx.record(2)
-        for _ in range(3):
-            M(r)
-        ctx.record(3)
-        apply_ms[mode] = ctx.elapsed_ms(2, 3) / 3
-        print(f"[bench] ILU0 apply trsv_mode={mode}: {apply_ms[mode]:.3f} ms", file=sys.stderr, flush=True)
-    ctx.set_option("trsv", 1 if apply_ms[1] <= apply_ms[0] else 0)
-    ctx.set_option("verbose", 100)
+    setup_mixed = time.perf_counter() - t0
+    log(f"FP32 parity-coloured ILU0 setup {setup_mixed:.2f}s")
 
-    def step():
-        x, st, rc = bsp.pGMRES(ctx, A, b, 1e-10, M, maxIters=20000, restart=30, k=10,
-                               tol_mode=bsp.TOL_TRUE_RESIDUAL)
-        return x, st, rc
+    def mixed_step():
+        return bsp.FGMRES_mixed(ctx, A, b, 1e-10, M32, maxIters=20000, restart=30, k=10,
+                                spmv_precision=bsp.PREC_F32, inner_reduction=1e-6)
 
     for w in range(args.warmup):
-        _, st, rc = step()
-        print(f"[bench] warmup {w}: iters={st.iters} relres={st.true_relres:.2e} {st.solve_ms:.1f} ms",
-              file=sys.stderr, flush=True)
+        _, st, rc = mixed_step()
+        log(f"mixed warmup {w}: iters={st.iters} relres={st.true_relres:.2e} {st.solve_ms:.1f} ms")
     ctx.sync()
     t0 = time.perf_counter()
     stats = []
     for s in range(args.steps):
-        _, st, rc = step()
+        _, st, rc = mixed_step()
         stats.append(st.as_dict())
-        print(f"[bench] step {s}: iters={st.iters} relres={st.true_relres:.2e} {st.solve_ms:.1f} ms",
-              file=sys.stderr, flush=True)
+        log(f"mixed step {s}: iters={st.iters} relres={st.true_relres:.2e} {st.solve_ms:.1f} ms")
     ctx.sync()
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
-    iters = stats[-1]["iters"]
     ok = all(s["true_relres"] <= 1e-10 for s in stats)
+
+    # ---- plain FP64 drop-in (the reference's method) ----
+    plain = None
+    if not args.skip_plain:
+        t0 = time.perf_counter()
+        M = bsp.ILU0(ctx, A)
+        ctx.sync()
+        setup_plain = time.perf_counter() - t0
+        apply_ms = {}
+        r = ctx.to_device(np.random.default_rng(1).standard_normal(p.n * p.bs))
+        for mode in (1, 0):
+            ctx.set_option("trsv", mode)
+            M(r)
+            ctx.sync()
+            ctx.record(2)
+            for _ in range(3):
+                M(r)
+            ctx.record(3)
+            apply_ms[mode] = ctx.elapsed_ms(2, 3) / 3
+            log(f"natural ILU0 apply trsv_mode={mode}: {apply_ms[mode]:.3f} ms")
+        ctx.set_option("trsv", 1 if apply_ms[1] <= apply_ms[0] else 0)
+        t0 = time.perf_counter()
+        _, stp, rcp = bsp.pGMRES(ctx, A, b, 1e-10, M, maxIters=20000, restart=30, k=10,
+                                 tol_mode=bsp.TOL_TRUE_RESIDUAL)
+        ctx.sync()
+        t_plain = time.perf_counter() - t0
+        log(f"plain FP64 ILU0-pGMRES: iters={stp.iters} relres={stp.true_relres:.2e} {t_plain:.2f} s")
+        plain = {"time_s": round(t_plain, 4), "iters": stp.iters, "restarts": stp.restarts,
+                 "true_relres": stp.true_relres, "converged": bool(stp.true_relres <= 1e-10),
+                 "setup_s": round(setup_plain, 3),
+                 "ilu0_apply_ms": {"syncfree": round(apply_ms[1], 4), "level_launch": round(apply_ms[0], 4)}}
 
     cpu = None
     if not args.skip_cpu:
-        cpu = cpu_baseline(p, iters)
-        print(f"[bench] cpu baseline {cpu['value']:.1f}s (est.)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(p, plain["iters"] if plain else stats[-1]["iters"])
+        log(f"cpu baseline {cpu['value']:.1f}s (est.)")
 
+    value = ms_per_step / 1e3
     out = {
         "metric": METRIC,
-        "value": round(ms_per_step / 1e3, 6),
+        "value": round(value, 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -199,15 +239,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"dg2d_veclap {args.nx}x{args.nx} Q1-SWIP-DG x D=2 (BlockSp<double,8>), "
-                               f"ILU0-pGMRES(30,k=10) FP64 to true relres 1e-10",
-                   "n_block_rows": p.n, "bs": p.bs, "nnzb": p.nnzb, "method": "plain_fp64_ilu0_pgmres_dr"},
+        "config": {"workload": f"dg2d_veclap {args.nx}x{args.nx} Q1-SWIP-DG x D=2 (BlockSp<double,8>), solve to "
+                               f"true FP64 relres 1e-10",
+                   "n_block_rows": p.n, "bs": p.bs, "nnzb": p.nnzb,
+                   "method": "mixed FGMRES(30,k=10): FP32 parity-coloured ILU0 + FP32 SpMV, FP64 MGS/LS/deflation/"
+                             "update/residual, FP64 refinement"},
         "converged": ok,
-        "iters": iters,
+        "iters": stats[-1]["iters"],
         "restarts": stats[-1]["restarts"],
         "true_relres": stats[-1]["true_relres"],
-        "setup_s": round(setup_s, 3),
-        "ilu0_apply_ms": {"syncfree": round(apply_ms[1], 4), "level_launch": round(apply_ms[0], 4)},
+        "setup_s": round(setup_mixed, 3),
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": round(plain["time_s"] / value, 3) if plain else None,
         "bsmv_GBps": round(bsmv_gbps, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -89,6 +89,9 @@ _SIGS = {
     "bsp_precond_download_factor": [_VP, _VP, _VP, _VP, _VP],
     "bsp_pgmres": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Stats)],
     "bsp_pcg": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.POINTER(Stats)],
+    "bsp_precond_create_ilu0_ex": [_VP, _VP, _VP, C.c_int, C.POINTER(_VP)],
+    "bsp_fgmres_mixed": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                         C.POINTER(Stats)],
 }
 
 
@@ -336,6 +339,24 @@ class ILU0(_Precond):
         self.h = h
 
 
+PREC_F64 = 0
+PREC_F32 = 1
+
+
+class ILU0Ex(_Precond):
+    """ILU0 with an elimination order given by a colouring and FP64/FP32 storage
+    (bsp_precond_create_ilu0_ex).  colour=None is the reference's natural order."""
+
+    def __init__(self, ctx, A, colour=None, precision=PREC_F64):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        col = None if colour is None else np.ascontiguousarray(colour, np.int32)
+        self._colour = col
+        ctx.check(ctx.L.bsp_precond_create_ilu0_ex(ctx.h, A.h, None if col is None else _ptr(col), int(precision),
+                                                   C.byref(h)), "ILU0Ex")
+        self.h = h
+
+
 class IChol0(_Precond):
     """preConditioners::IChol0(A) (blocksp_preconditioners.hpp:364-538)."""
 
@@ -353,6 +374,17 @@ def pGMRES(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=500, restart=-1, k=-1, 
     rc = ctx.L.bsp_pgmres(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(restart), int(k),
                           int(tol_mode), C.byref(st))
     ctx.check(rc, "pgmres")
+    return x, st, rc
+
+
+def FGMRES_mixed(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=20000, restart=30, k=10, spmv_precision=PREC_F32,
+                 inner_reduction=1e-6, x=None):
+    """Mixed-precision flexible GMRES-DR with FP64 refinement (bsp_fgmres_mixed)."""
+    x = x if x is not None else ctx.empty(b.shape)
+    st = Stats()
+    rc = ctx.L.bsp_fgmres_mixed(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(restart), int(k),
+                                int(spmv_precision), float(inner_reduction), C.byref(st))
+    ctx.check(rc, "fgmres_mixed")
     return x, st, rc
 
 

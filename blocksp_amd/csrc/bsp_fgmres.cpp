@@ -1,0 +1,320 @@
+// bsp_fgmres.cpp -- mixed-precision flexible GMRES with deflated restart and FP64
+// iterative refinement.
+//
+// Outer loop (FP64): r = b - A x with the FP64 matrix; stop when ||r|| <= tol ||b||.
+// Inner solve of A e = r: right-preconditioned flexible GMRES (Saad, "A flexible
+// inner-outer preconditioned GMRES", 1993) with Morgan's deflated restart:
+//   z_j = P(v_j)            preconditioner in FP32 (or FP64 for an FP64 M), stored FP32
+//   w   = A z_j             SpMV with FP32 values and FP32 accumulation (or FP64)
+//   MGS against V (FP64), H column, v_{j+1} = w / h_{j+1,j}  (FP64)
+//   least squares on H (FP64, host), x += Z y with FP64 accumulation
+//   restart: harmonic Ritz subspace of H (FP64) -> V' = V P_{k+1}, Z' = Z P_k,
+//            H' = P_{k+1}^T H P_k (A Z' = V' H' is preserved).
+// The learned correction, when attached, adds to z_j after the FP32 preconditioner.
+#include <cmath>
+#include <cstdio>
+#include <deque>
+#include <limits>
+
+#include "bsp_dense.hpp"
+#include "bsp_internal.hpp"
+
+namespace bsp {
+
+int ensure_values_f32(bsp_ctx* ctx, const bsp_bsr* A) {
+    if (A->val32.p) return BSP_OK;
+    const int64_t cnt = A->nnzb * (int64_t)A->bs * A->bs;
+    BSP_HIP(A->val32.alloc(sizeof(float) * std::max<int64_t>(cnt, 1)));
+    return launch_d2f(ctx, cnt, A->val.as<double>(), A->val32.as<float>());
+}
+
+namespace {
+
+template <class T>
+struct Pool {
+    std::deque<DevMem> bufs;
+    std::vector<T*> free_;
+    size_t len = 0;
+    int take(bsp_ctx* ctx, T*& p) {
+        if (!free_.empty()) {
+            p = free_.back();
+            free_.pop_back();
+            return BSP_OK;
+        }
+        bufs.emplace_back();
+        BSP_HIP(bufs.back().alloc(sizeof(T) * std::max<size_t>(len, 1)));
+        p = bufs.back().template as<T>();
+        return BSP_OK;
+    }
+    void give(T* p) { free_.push_back(p); }
+};
+
+struct Pinned {
+    double* p = nullptr;
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+struct InnerStats {
+    int iters = 0, restarts = 0;
+    double res = 0.0, beta = 0.0;
+    bool stagnated = false;
+};
+
+// One inner FGMRES-DR pass on A e = r for the current x (x += e on return).
+int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double reduction,
+                int maxIters, int restart, int k, bool spmv_f32, InnerStats& st, double* r_work, double* t64) {
+    const int n = A->n_rows, bs = A->bs;
+    const int64_t len = (int64_t)n * bs;
+    const double eps = std::numeric_limits<double>::epsilon();
+    const int m_max = restart > 0 ? restart : maxIters + 1;
+    Pool<double> V;
+    Pool<float> Z;
+    V.len = Z.len = (size_t)len;
+    DevMem hcol, ydev, Pdev, Pkdev;
+    BSP_HIP(hcol.alloc(sizeof(double) * (m_max + 2)));
+    BSP_HIP(ydev.alloc(sizeof(double) * (m_max + 2)));
+    Pinned hpin, ypin, ppin;
+    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
+    BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
+    if (restart > 0) {
+        BSP_HIP(Pdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
+        BSP_HIP(Pkdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
+        BSP_HIP(hipHostMalloc((void**)&ppin.p, sizeof(double) * 2 * (m_max + 1) * (k + 2), hipHostMallocDefault));
+    }
+    double* sc = ctx->scalars.as<double>();
+    // r = b - A x (FP64), beta = ||r||, V[0] = r / beta
+    int rc = launch_residual(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x, r_work);
+    if (rc) return rc;
+    double beta = 0.0;
+    rc = reduce(ctx, 1, len, r_work, r_work, SLOT_TMP0, &beta);
+    if (rc) return rc;
+    st = InnerStats{};
+    st.beta = beta;
+    st.res = beta;
+    if (!(beta > 0.0) || !std::isfinite(beta)) return std::isfinite(beta) ? BSP_OK : fail(ctx, BSP_E_NUMERIC, "residual not finite");
+    std::vector<double*> Vb;
+    std::vector<float*> Zb;
+    double* v0 = nullptr;
+    rc = V.take(ctx, v0);
+    if (rc) return rc;
+    rc = launch_scale_div(ctx, len, r_work, sc + SLOT_TMP0, v0);
+    if (rc) return rc;
+    Vb.push_back(v0);
+    const double zero_thresh = std::min(beta * eps, eps);
+    const double target = beta * reduction;
+    std::vector<double> H, c_r, y, res_vec, res_hist;
+    int Hm = 0, Hn = 0;
+    auto H_resize = [&](int m, int nn) {
+        std::vector<double> Hnew((size_t)m * nn, 0.0);
+        for (int i = 0; i < std::min(m, Hm); ++i)
+            for (int j = 0; j < std::min(nn, Hn); ++j) Hnew[(size_t)i * nn + j] = H[(size_t)i * Hn + j];
+        H.swap(Hnew);
+        Hm = m;
+        Hn = nn;
+    };
+    const bool f32_pre = M->f32;
+    double* hd = hcol.as<double>();
+    bool restarted = false;
+    double res = beta;
+    int iter = 0, tot = 0;
+    double* w = r_work;   // reuse: r is not needed after V[0] is formed
+    while (res > target) {
+        H_resize(iter + 2, iter + 1);
+        // z = P(v_iter) in reduced precision
+        float* z = nullptr;
+        rc = Z.take(ctx, z);
+        if (rc) return rc;
+        if (f32_pre) {
+            rc = launch_d2f(ctx, len, Vb[iter], z);
+            if (!rc) rc = precond_apply_f32(ctx, M, z);
+        } else {
+            rc = launch_copy(ctx, len, Vb[iter], t64);
+            if (!rc) rc = bsp_precond_apply(ctx, M, t64);
+            if (!rc) rc = launch_d2f(ctx, len, t64, z);
+        }
+        if (rc) return rc;
+        Zb.push_back(z);
+        // w = A z
+        if (spmv_f32) {
+            rc = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
+        } else {
+            rc = launch_f2d(ctx, len, z, t64);
+            if (!rc) rc = launch_bsmv(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), t64, w, 0);
+        }
+        if (rc) return rc;
+        // MGS (FP64)
+        for (int j = 0; j <= iter; ++j) {
+            rc = launch_mgs_step(ctx, len, w, j ? Vb[j - 1] : nullptr, j ? hd + (j - 1) : nullptr, Vb[j], hd + j);
+            if (rc) return rc;
+        }
+        rc = launch_mgs_step(ctx, len, w, Vb[iter], hd + iter, nullptr, hd + iter + 1);
+        if (rc) return rc;
+        double* vn = nullptr;
+        rc = V.take(ctx, vn);
+        if (rc) return rc;
+        rc = launch_scale_div(ctx, len, w, hd + iter + 1, vn);
+        if (rc) return rc;
+        BSP_HIP(hipMemcpyAsync(hpin.p, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
+        BSP_HIP(hipStreamSynchronize(ctx->stream));
+        for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hpin.p[j];
+        const double hlast = hpin.p[iter + 1];
+        const bool breakdown = std::fabs(hlast) < zero_thresh || !std::isfinite(hlast);
+        if (!breakdown) Vb.push_back(vn);
+        else V.give(vn);
+        const int m = iter + 2, nn = iter + 1;
+        std::vector<double> c(m, 0.0);
+        if (!restarted) c[0] = beta;
+        else
+            for (int i = 0; i < (int)c_r.size() && i < m; ++i) c[i] = c_r[i];
+        y.assign(nn, 0.0);
+        dense::lstsq(m, nn, H.data(), c.data(), y.data());
+        res_vec.assign(m, 0.0);
+        double r2 = 0.0;
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < nn; ++j) s += -1.0 * H[(size_t)i * nn + j] * y[j];
+            res_vec[i] = s + c[i];
+            r2 += res_vec[i] * res_vec[i];
+        }
+        res = std::sqrt(r2);
+        ++iter;
+        ++tot;
+        if (ctx->verbose > 0 && tot % ctx->verbose == 0) {
+            std::fprintf(stderr, "[fgmres] iter %d res %.3e (target %.3e) restarts %d\n", tot, res, target, st.restarts);
+            std::fflush(stderr);
+        }
+        if (tot >= maxIters || breakdown || !std::isfinite(res)) break;
+        if (iter == restart && res > target) {
+            res_hist.push_back(res);
+            const size_t hsz = res_hist.size();
+            if (hsz > 8 && res > 0.5 * res_hist[hsz - 9]) {
+                st.stagnated = true;
+                break;
+            }
+            // x += Z y
+            for (int i = 0; i < nn; ++i) ypin.p[i] = y[i];
+            BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
+            rc = launch_combine_add_f32(ctx, len, nn, Zb.data(), ydev.as<double>(), x);
+            if (rc) return rc;
+            ++st.restarts;
+            std::vector<double> P, Hnew;
+            const int kout = dense::deflate(nn, H.data(), res_vec.data(), k, P, Hnew, c_r);
+            BSP_CHECK(kout >= 0, BSP_E_NUMERIC, "flexible deflated restart: small eigenproblem failed");
+            const int kp = kout + 1;
+            BSP_CHECK((int)Vb.size() == m && (int)Zb.size() == nn, BSP_E_NUMERIC, "basis size mismatch");
+            // P (m x kp) and P_k (nn x kout: first kout columns, first nn rows)
+            double* Pk_host = ppin.p + (size_t)m * kp;
+            for (size_t t = 0; t < P.size(); ++t) ppin.p[t] = P[t];
+            for (int i = 0; i < nn; ++i)
+                for (int j = 0; j < kout; ++j) Pk_host[(size_t)i * kout + j] = P[(size_t)i * kp + j];
+            BSP_HIP(hipMemcpyAsync(Pdev.p, ppin.p, sizeof(double) * P.size(), hipMemcpyHostToDevice, ctx->stream));
+            BSP_HIP(hipMemcpyAsync(Pkdev.p, Pk_host, sizeof(double) * nn * kout, hipMemcpyHostToDevice, ctx->stream));
+            std::vector<double*> Vn(kp);
+            std::vector<float*> Zn(kout);
+            for (int j = 0; j < kp; ++j)
+                if ((rc = V.take(ctx, Vn[j]))) return rc;
+            for (int j = 0; j < kout; ++j)
+                if ((rc = Z.take(ctx, Zn[j]))) return rc;
+            rc = launch_basis_rotate(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
+            if (!rc && kout > 0)
+                rc = launch_basis_rotate_f32(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
+            if (rc) return rc;
+            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            for (double* p : Vb) V.give(p);
+            for (float* p : Zb) Z.give(p);
+            Vb = Vn;
+            Zb = Zn;
+            H = Hnew;
+            Hm = kp;
+            Hn = kout;
+            iter = kout;
+            restarted = true;
+            y.clear();
+        }
+    }
+    if (!y.empty()) {
+        for (size_t i = 0; i < y.size(); ++i) ypin.p[i] = y[i];
+        BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
+        rc = launch_combine_add_f32(ctx, len, (int)y.size(), Zb.data(), ydev.as<double>(), x);
+        if (rc) return rc;
+    }
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    st.iters = tot;
+    st.res = res;
+    return BSP_OK;
+}
+
+}  // namespace
+
+}  // namespace bsp
+
+using namespace bsp;
+
+extern "C" {
+
+int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
+                     int maxIters, int restart, int k, int spmv_precision, double inner_reduction, bsp_stats* stats) {
+    if (!ctx || !A || !M || !b || !x) return BSP_E_INVALID;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "FGMRES needs a square matrix");
+    BSP_CHECK(M->n == A->n_rows && M->bs == A->bs, BSP_E_INVALID, "preconditioner does not match the matrix");
+    BSP_CHECK(b != x, BSP_E_INVALID, "b and x must not alias");
+    BSP_CHECK(restart <= 62, BSP_E_UNSUPPORTED, "FGMRES restart length is limited to 62");
+    BSP_CHECK(spmv_precision == BSP_PREC_F64 || spmv_precision == BSP_PREC_F32, BSP_E_INVALID, "bad SpMV precision");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int64_t len = (int64_t)A->n_rows * A->bs;
+    const bool spmv_f32 = spmv_precision == BSP_PREC_F32;
+    if (!(inner_reduction > 0.0 && inner_reduction < 1.0)) inner_reduction = 1e-6;
+    int rc;
+    if (spmv_f32 && (rc = ensure_values_f32(ctx, A))) return rc;
+    float ms = 0.f;
+    BSP_HIP(hipEventRecord(ctx->events[12], ctx->stream));
+    BSP_HIP(hipMemsetAsync(x, 0, sizeof(double) * len, ctx->stream));
+    DevMem rw, t64;
+    BSP_HIP(rw.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    BSP_HIP(t64.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    double bnorm = 0.0;
+    rc = reduce(ctx, 1, len, b, b, SLOT_TMP2, &bnorm);
+    if (rc) return rc;
+    int total = 0, restarts = 0, passes = 0;
+    double rr = 1.0, res0 = 0.0, res_est = 0.0;
+    for (; passes < 100 && total < maxIters; ++passes) {
+        // the pass reduces the current residual by `inner_reduction` (or to the target)
+        const double pass_red = std::max(inner_reduction, 0.5 * tol / std::max(rr, 1e-300));
+        InnerStats st;
+        rc = fgmres_pass(ctx, A, M, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k, spmv_f32, st,
+                         rw.as<double>(), t64.as<double>());
+        if (rc) return rc;
+        if (passes == 0) res0 = st.beta;
+        total += st.iters;
+        restarts += st.restarts;
+        res_est = st.res;
+        rc = true_relres_ext(ctx, A, b, x, rw.as<double>(), bnorm, rr);
+        if (rc) return rc;
+        if (ctx->verbose > 0) {
+            std::fprintf(stderr, "[fgmres] refinement pass %d: %d iterations, true relres %.3e\n", passes, st.iters, rr);
+            std::fflush(stderr);
+        }
+        if (rr <= tol || !std::isfinite(rr) || st.iters == 0) break;
+    }
+    rc = check_sweep_errors_ext(ctx);
+    if (rc) return rc;
+    BSP_HIP(hipEventRecord(ctx->events[13], ctx->stream));
+    BSP_HIP(hipEventSynchronize(ctx->events[13]));
+    BSP_HIP(hipEventElapsedTime(&ms, ctx->events[12], ctx->events[13]));
+    const bool conv = rr <= tol;
+    if (stats) {
+        stats->iters = total;
+        stats->restarts = restarts;
+        stats->converged = conv ? 1 : 0;
+        stats->precond_applies = total;
+        stats->res0 = res0;
+        stats->res_est = res_est;
+        stats->true_relres = rr;
+        stats->solve_ms = ms;
+    }
+    return conv ? BSP_OK : BSP_NOT_CONVERGED;
+}
+
+}  // extern "C"

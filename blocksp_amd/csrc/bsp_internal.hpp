@@ -107,6 +107,7 @@ struct bsp_bsr {
     std::vector<double> h_val;         // host copy of the values (factorisations start here)
     // lazily built transpose (for bsmv_transpose)
     std::unique_ptr<bsp_bsr> transposed;
+    mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
 };
 
 struct bsp_precond {
@@ -121,6 +122,9 @@ struct bsp_precond {
     bsp::DevMem dinv_upper;           // [n][bs][bs] applied after the backward sweep / SOR / Diag
     bsp::DevMem work0, work1;         // vectors for SOR ping-pong
     uint32_t epoch = 0;
+    bool f32 = false;                 // factor, diagonal inverses and applies in FP32
+    bsp::DevMem work32;               // FP32 staging vector for applies on FP64 vectors
+    std::vector<int32_t> elim_order;  // elimination order of the factorisation (ILU0)
     bsp_host_precond_fn host_fn = nullptr;   // BSP_PRE_HOST
     void* host_user = nullptr;
     // host copy of the factor in the reference layout (for parity download)
@@ -173,6 +177,27 @@ int launch_residual(const bsp_ctx* ctx, int n, int bs, const int* rp, const int*
                     const double* b, const double* x, double* r);
 int launch_copy(const bsp_ctx* ctx, int64_t len, const double* x, double* y);
 int launch_fill_u32(const bsp_ctx* ctx, int64_t len, uint32_t v, uint32_t* p);
+// ---- reduced precision (bsp_mixed.hip) ----
+// y (double) = A32 x32 with FP32 accumulation; y (float) variant
+int launch_bsmv_f32(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
+                    double* y);
+int launch_bsmv_f32f(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
+                     float* y);
+int launch_d2f(const bsp_ctx* ctx, int64_t n, const double* a, float* b);
+int launch_f2d(const bsp_ctx* ctx, int64_t n, const float* a, double* b);
+int launch_combine_add_f32(const bsp_ctx* ctx, int64_t len, int m, const float* const* Z, const double* c, double* x);
+int launch_basis_rotate_f32(const bsp_ctx* ctx, int64_t len, int m, int kp, const float* const* Z, const double* P,
+                            float* const* Zout);
+// FP32 copy of a matrix's values (built on first use, on the device)
+int ensure_values_f32(bsp_ctx* ctx, const bsp_bsr* A);
+// r <- M^{-1} r on an FP32 vector (FP32 preconditioners only)
+int precond_apply_f32(bsp_ctx* ctx, const bsp_precond* M, float* r);
+
+// ||b - A x|| / bnorm (FP64, on the device) and the sync-free timeout check (bsp_krylov.cpp)
+int true_relres_ext(bsp_ctx* ctx, const bsp_bsr* A, const double* b, const double* x, double* work, double bnorm,
+                    double& out);
+int check_sweep_errors_ext(bsp_ctx* ctx);
+
 // pCG updates: x += alpha p; r -= alpha Ap with alpha = num/den from device scalars
 int launch_cg_update_xr(const bsp_ctx* ctx, int64_t len, const double* num, const double* den,
                         const double* p, const double* Ap, double* x, double* r);

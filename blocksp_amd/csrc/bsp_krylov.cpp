@@ -465,3 +465,14 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
 }
 
 }  // extern "C"
+
+namespace bsp {
+
+int true_relres_ext(bsp_ctx* ctx, const bsp_bsr* A, const double* b, const double* x, double* work, double bnorm,
+                    double& out) {
+    return true_relres(ctx, A, b, x, work, bnorm, out);
+}
+
+int check_sweep_errors_ext(bsp_ctx* ctx) { return check_sweep_errors(ctx); }
+
+}  // namespace bsp

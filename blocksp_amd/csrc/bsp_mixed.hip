@@ -1,0 +1,227 @@
+// bsp_mixed.hip -- reduced-precision kernels of the mixed-precision solve path (gfx950).
+//
+//  * k_bsmv_lp<BS, AT, XT, YT>: y = A x with A in AT (float), x in XT, FP32
+//    accumulation, y stored as YT (float, or double for the FP64 Krylov basis);
+//    same lane mapping as the FP64 bsmv (BS lanes per block row, lane r owns row r).
+//  * conversions double <-> float, and the FP64 combination x += sum_j c_j Z_j with
+//    FP32 basis vectors Z_j (the solution update of flexible GMRES).
+#include "bsp_internal.hpp"
+
+#include <algorithm>
+
+namespace bsp {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+template <int BS, typename T>
+__device__ __forceinline__ void load_vec(const T* __restrict__ src, T (&a)[BS]) {
+    if constexpr (std::is_same_v<T, float> && BS % 4 == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+        for (int c = 0; c < BS / 4; ++c) {
+            const float4 t = s4[c];
+            a[4 * c] = t.x;
+            a[4 * c + 1] = t.y;
+            a[4 * c + 2] = t.z;
+            a[4 * c + 3] = t.w;
+        }
+    } else if constexpr (std::is_same_v<T, double> && BS % 2 == 0) {
+        const double2* s2 = reinterpret_cast<const double2*>(src);
+#pragma unroll
+        for (int c = 0; c < BS / 2; ++c) {
+            const double2 t = s2[c];
+            a[2 * c] = t.x;
+            a[2 * c + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) a[c] = src[c];
+    }
+}
+
+template <int BS, typename AT, typename XT, typename YT>
+__global__ __launch_bounds__(kBlock) void k_bsmv_lp(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                    const AT* __restrict__ val, const XT* __restrict__ x,
+                                                    YT* __restrict__ y) {
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    float acc = 0.0f;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    int p = p0;
+    for (; p + 1 < p1; p += 2) {
+        const int c0 = ci[p], c1 = ci[p + 1];
+        AT a0[BS], a1[BS];
+        XT x0[BS], x1[BS];
+        load_vec<BS, AT>(val + ((size_t)p * BS + r) * BS, a0);
+        load_vec<BS, AT>(val + ((size_t)(p + 1) * BS + r) * BS, a1);
+        load_vec<BS, XT>(x + (size_t)c0 * BS, x0);
+        load_vec<BS, XT>(x + (size_t)c1 * BS, x1);
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            s0 = fmaf((float)a0[c], (float)x0[c], s0);
+            s1 = fmaf((float)a1[c], (float)x1[c], s1);
+        }
+        acc += s0;
+        acc += s1;
+    }
+    if (p < p1) {
+        const int c0 = ci[p];
+        AT a0[BS];
+        XT x0[BS];
+        load_vec<BS, AT>(val + ((size_t)p * BS + r) * BS, a0);
+        load_vec<BS, XT>(x + (size_t)c0 * BS, x0);
+        float s0 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) s0 = fmaf((float)a0[c], (float)x0[c], s0);
+        acc += s0;
+    }
+    y[(size_t)i * BS + r] = (YT)acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_f2d(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+constexpr int kMaxBasis = 64;
+struct FPtrs {
+    const float* q[kMaxBasis];
+};
+struct FOut {
+    float* q[kMaxBasis];
+};
+
+// x += sum_j c_j Z_j (FP64 accumulation of FP32 vectors)
+__global__ __launch_bounds__(kBlock) void k_combine_add_f32(int64_t len, int m, FPtrs Z, const double* __restrict__ c,
+                                                            double* __restrict__ x) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= len) return;
+    double acc = x[e];
+    for (int j = 0; j < m; ++j) acc += c[j] * (double)Z.q[j][e];
+    x[e] = acc;
+}
+
+// Zout_j = sum_i Z_i P_ij (FP64 accumulation, FP32 storage), m, kp <= 64
+__global__ __launch_bounds__(kBlock) void k_basis_rotate_f32(int64_t len, int m, int kp, FPtrs Z,
+                                                             const double* __restrict__ P, FOut Out) {
+    __shared__ double sP[kMaxBasis * 16];
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int j0 = 0; j0 < kp; j0 += 16) {
+        const int jn = min(16, kp - j0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < m * jn; t += kBlock) sP[t] = P[(t / jn) * kp + j0 + (t % jn)];
+        __syncthreads();
+        if (e < len) {
+            double acc[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double zv = (double)Z.q[i][e];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j < jn) acc[j] = fma(zv, sP[i * jn + j], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < jn) Out.q[j0 + j][e] = (float)acc[j];
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
+
+template <typename AT, typename XT, typename YT>
+int bsmv_lp_dispatch(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const AT* val, const XT* x, YT* y) {
+    auto launch = [&](auto bsc) {
+        constexpr int BS = decltype(bsc)::value;
+        constexpr int RPW = kWave / BS;
+        const int64_t waves = (n + RPW - 1) / RPW;
+        hipLaunchKernelGGL((k_bsmv_lp<BS, AT, XT, YT>), dim3(grid_for(waves * kWave)), dim3(kBlock), 0, ctx->stream, n,
+                           rp, ci, val, x, y);
+    };
+    switch (bs) {
+        case 1: launch(std::integral_constant<int, 1>{}); break;
+        case 2: launch(std::integral_constant<int, 2>{}); break;
+        case 3: launch(std::integral_constant<int, 3>{}); break;
+        case 4: launch(std::integral_constant<int, 4>{}); break;
+        case 5: launch(std::integral_constant<int, 5>{}); break;
+        case 6: launch(std::integral_constant<int, 6>{}); break;
+        case 7: launch(std::integral_constant<int, 7>{}); break;
+        case 8: launch(std::integral_constant<int, 8>{}); break;
+        case 16: launch(std::integral_constant<int, 16>{}); break;
+        case 32: launch(std::integral_constant<int, 32>{}); break;
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "reduced-precision bsmv: unsupported block size");
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("bsmv_lp launch: ") + hipGetErrorString(e));
+    return BSP_OK;
+}
+
+}  // namespace
+
+int launch_bsmv_f32(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
+                    double* y) {
+    if (n <= 0) return BSP_OK;
+    return bsmv_lp_dispatch<float, float, double>(ctx, n, bs, rp, ci, val, x, y);
+}
+
+int launch_bsmv_f32f(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
+                     float* y) {
+    if (n <= 0) return BSP_OK;
+    return bsmv_lp_dispatch<float, float, float>(ctx, n, bs, rp, ci, val, x, y);
+}
+
+int launch_d2f(const bsp_ctx* ctx, int64_t n, const double* a, float* b) {
+    if (n <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, a, b);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+}
+
+int launch_f2d(const bsp_ctx* ctx, int64_t n, const float* a, double* b) {
+    if (n <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, a, b);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+}
+
+int launch_combine_add_f32(const bsp_ctx* ctx, int64_t len, int m, const float* const* Z, const double* c, double* x) {
+    for (int j0 = 0; j0 < m; j0 += kMaxBasis) {
+        const int mm = std::min(kMaxBasis, m - j0);
+        FPtrs t{};
+        for (int j = 0; j < mm; ++j) t.q[j] = Z[j0 + j];
+        hipLaunchKernelGGL(k_combine_add_f32, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, mm, t, c + j0, x);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+    }
+    return BSP_OK;
+}
+
+int launch_basis_rotate_f32(const bsp_ctx* ctx, int64_t len, int m, int kp, const float* const* Z, const double* P,
+                            float* const* Zout) {
+    if (m > kMaxBasis || kp > kMaxBasis) return fail(ctx, BSP_E_UNSUPPORTED, "basis larger than 64 vectors");
+    FPtrs t{};
+    FOut o{};
+    for (int i = 0; i < m; ++i) t.q[i] = Z[i];
+    for (int j = 0; j < kp; ++j) o.q[j] = Zout[j];
+    hipLaunchKernelGGL(k_basis_rotate_f32, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, m, kp, t, P, o);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+}
+
+}  // namespace bsp

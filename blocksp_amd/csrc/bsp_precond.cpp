@@ -28,31 +28,24 @@ int find_col(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, int
     return -1;
 }
 
-// Level schedule of a sweep.  lower: row i waits on columns < i (processed in
-// increasing level); otherwise on columns > i.  Each level is padded to a
-// multiple of the rows one wavefront handles.
-void build_schedule(Sweep& S, int n, int bs, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
-                    bool lower) {
+// Level schedule of a sweep.  Rows are visited in `topo` (a topological order of
+// the dependencies); row i waits on the columns c of its row with dep(i, c):
+// level(i) = 1 + max level(c).  Each level is padded to a multiple of the rows
+// one wavefront handles; rows within a level are in ascending index order.
+template <class Dep>
+void build_schedule_topo(Sweep& S, int n, int bs, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                         const std::vector<int32_t>& topo, Dep dep) {
     S.n = n;
     S.bs = bs;
     S.level_of.assign(n, 0);
     int nlev = 0;
-    if (lower) {
-        for (int i = 0; i < n; ++i) {
-            int l = 0;
-            for (int p = rp[i]; p < rp[i + 1]; ++p)
-                if (ci[p] < i) l = std::max(l, S.level_of[ci[p]] + 1);
-            S.level_of[i] = l;
-            nlev = std::max(nlev, l + 1);
-        }
-    } else {
-        for (int i = n - 1; i >= 0; --i) {
-            int l = 0;
-            for (int p = rp[i]; p < rp[i + 1]; ++p)
-                if (ci[p] > i) l = std::max(l, S.level_of[ci[p]] + 1);
-            S.level_of[i] = l;
-            nlev = std::max(nlev, l + 1);
-        }
+    for (int t = 0; t < n; ++t) {
+        const int i = topo[t];
+        int l = 0;
+        for (int p = rp[i]; p < rp[i + 1]; ++p)
+            if (dep(i, ci[p])) l = std::max(l, S.level_of[ci[p]] + 1);
+        S.level_of[i] = l;
+        nlev = std::max(nlev, l + 1);
     }
     S.nlevels = nlev;
     const int rpw = std::max(1, kWaveRows / bs);
@@ -71,23 +64,45 @@ void build_schedule(Sweep& S, int n, int bs, const std::vector<int32_t>& rp, con
     (void)hipMemset(S.ticket.p, 0, sizeof(unsigned long long));
 }
 
+// Natural-order schedule: lower waits on columns < i, upper on columns > i.
+void build_schedule(Sweep& S, int n, int bs, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                    bool lower) {
+    std::vector<int32_t> topo(n);
+    for (int i = 0; i < n; ++i) topo[i] = lower ? i : n - 1 - i;
+    if (lower) build_schedule_topo(S, n, bs, rp, ci, topo, [](int i, int c) { return c < i; });
+    else build_schedule_topo(S, n, bs, rp, ci, topo, [](int i, int c) { return c > i; });
+}
+
+hipError_t upload_f32(DevMem& m, const std::vector<double>& v) {
+    std::vector<float> f(v.begin(), v.end());
+    hipError_t e = m.alloc(sizeof(float) * std::max<size_t>(f.size(), 1));
+    if (e != hipSuccess || f.empty()) return e;
+    return hipMemcpy(m.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice);
+}
+
 hipError_t upload_structure(Sweep& S, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
-                            const std::vector<double>& val, int bs) {
+                            const std::vector<double>& val, int bs, bool f32 = false) {
     const size_t nnz = ci.size();
     S.nnzb = (int64_t)nnz;
     hipError_t e;
     if ((e = S.rp.alloc(sizeof(int) * rp.size())) != hipSuccess) return e;
     if ((e = S.ci.alloc(sizeof(int) * std::max<size_t>(nnz, 1))) != hipSuccess) return e;
-    if ((e = S.val.alloc(sizeof(double) * std::max<size_t>(nnz * bs * bs, 1))) != hipSuccess) return e;
     if ((e = hipMemcpy(S.rp.p, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if (nnz) {
-        if ((e = hipMemcpy(S.ci.p, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return e;
-        if ((e = hipMemcpy(S.val.p, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    if (nnz && (e = hipMemcpy(S.ci.p, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if (f32) {
+        std::vector<float> fv(val.begin(), val.end());
+        if ((e = S.val.alloc(sizeof(float) * std::max<size_t>(fv.size(), 1))) != hipSuccess) return e;
+        if (nnz && (e = hipMemcpy(S.val.p, fv.data(), sizeof(float) * fv.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return e;
+    } else {
+        if ((e = S.val.alloc(sizeof(double) * std::max<size_t>(val.size(), 1))) != hipSuccess) return e;
+        if (nnz && (e = hipMemcpy(S.val.p, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return e;
     }
+    S.f32 = f32;
     S.drp = S.rp.as<int>();
     S.dci = S.ci.as<int>();
-    S.dval = S.val.as<double>();
+    S.dval = S.val.p;
     return hipSuccess;
 }
 
@@ -224,7 +239,14 @@ int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int
     return BSP_OK;
 }
 
-int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+}  // extern "C"
+
+namespace {
+
+// Block ILU(0) in a given elimination order (pos[i] = step at which row i is
+// eliminated; natural order pos[i] = i is the reference's, preconditioners.hpp:274-291).
+// L holds the blocks (i, j) with pos[j] < pos[i], U those with pos[j] > pos[i].
+int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool f32, bsp_precond** out) {
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
@@ -233,8 +255,16 @@ int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
     const size_t b2 = (size_t)bs * bs;
     const auto& rp = A->h_rp;
     const auto& ci = A->h_ci;
+    // elimination order: colour by colour, natural order within a colour
+    std::vector<int32_t> elim(n), pos(n);
+    for (int i = 0; i < n; ++i) elim[i] = i;
+    if (colour) {
+        for (int i = 0; i < n; ++i) BSP_CHECK(colour[i] >= 0, BSP_E_INVALID, "colours must be non-negative");
+        std::stable_sort(elim.begin(), elim.end(), [&](int a, int b) { return colour[a] < colour[b]; });
+    }
+    for (int t = 0; t < n; ++t) pos[elim[t]] = t;
     std::vector<double> LU(A->h_val);
-    // rows holding a block in column c (ascending), from A's pattern
+    // rows holding a block in column c, from A's pattern
     std::vector<int32_t> cptr(n + 1, 0), crow, cpos;
     for (int p = 0; p < (int)ci.size(); ++p) cptr[ci[p] + 1]++;
     for (int c = 0; c < n; ++c) cptr[c + 1] += cptr[c];
@@ -249,24 +279,25 @@ int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
             }
     }
     std::vector<double> inv(b2), tmp(b2), prod(b2);
-    for (int i = 0; i < n; ++i) {
+    for (int t = 0; t < n; ++t) {
+        const int i = elim[t];
         const int pd = find_col(rp, ci, i, i);
         BSP_CHECK(pd >= 0, BSP_E_NUMERIC, "ILU0: missing diagonal block in row " + std::to_string(i));
         BSP_CHECK(dense::invert(bs, LU.data() + pd * b2, inv.data()), BSP_E_NUMERIC,
                   "ILU0: singular pivot block in row " + std::to_string(i));
         for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
-            if (crow[q] <= i) continue;
+            if (pos[crow[q]] <= t) continue;
             double* lki = LU.data() + (size_t)cpos[q] * b2;
             std::memcpy(tmp.data(), lki, sizeof(double) * b2);
             dense::matmul(bs, tmp.data(), inv.data(), lki);
         }
         for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
             const int k = crow[q];
-            if (k <= i) continue;
+            if (pos[k] <= t) continue;
             const double* lki = LU.data() + (size_t)cpos[q] * b2;
             for (int p = rp[k]; p < rp[k + 1]; ++p) {
                 const int j = ci[p];
-                if (j <= i) continue;
+                if (pos[j] <= t) continue;
                 const int pij = find_col(rp, ci, i, j);
                 if (pij < 0) continue;
                 dense::matmul(bs, lki, LU.data() + (size_t)pij * b2, prod.data());
@@ -280,26 +311,62 @@ int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
     M->kind = BSP_PRE_ILU0;
     M->n = n;
     M->bs = bs;
+    M->f32 = f32;
     std::vector<double> dinv(n * b2);
     for (int i = 0; i < n; ++i) {
         const int pd = find_col(rp, ci, i, i);
         BSP_CHECK(dense::invert(bs, LU.data() + pd * b2, dinv.data() + i * b2), BSP_E_NUMERIC,
                   "ILU0: singular U block in row " + std::to_string(i));
     }
-    std::vector<int32_t> lrp, lci, urp, uci;
+    // strict parts by elimination position, each row in its stored order
+    std::vector<int32_t> lrp(n + 1, 0), lci, urp(n + 1, 0), uci;
     std::vector<double> lv, uv;
-    split_strict(n, bs, rp, ci, LU, true, lrp, lci, lv);
-    split_strict(n, bs, rp, ci, LU, false, urp, uci, uv);
-    BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs));
-    BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs));
-    build_schedule(M->lower, n, bs, lrp, lci, true);
-    build_schedule(M->upper, n, bs, urp, uci, false);
-    BSP_HIP(upload(M->dinv_upper, dinv));
+    for (int i = 0; i < n; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            const int c = ci[p];
+            if (pos[c] < pos[i]) {
+                lci.push_back(c);
+                lv.insert(lv.end(), LU.begin() + p * b2, LU.begin() + (p + 1) * b2);
+            } else if (pos[c] > pos[i]) {
+                uci.push_back(c);
+                uv.insert(uv.end(), LU.begin() + p * b2, LU.begin() + (p + 1) * b2);
+            }
+        }
+        lrp[i + 1] = (int32_t)lci.size();
+        urp[i + 1] = (int32_t)uci.size();
+    }
+    BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs, f32));
+    BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs, f32));
+    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
+    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    if (f32) {
+        BSP_HIP(upload_f32(M->dinv_upper, dinv));
+        BSP_HIP(M->work32.alloc(sizeof(float) * std::max<size_t>((size_t)n * bs, 1)));
+    } else {
+        BSP_HIP(upload(M->dinv_upper, dinv));
+    }
+    M->elim_order = elim;
     M->f_rp = rp;
     M->f_ci = ci;
     M->f_val.swap(LU);
     *out = M.release();
     return BSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+    return create_ilu0_impl(ctx, A, nullptr, false, out);
+}
+
+int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision,
+                               bsp_precond** out) {
+    if (!ctx) return BSP_E_INVALID;
+    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32, BSP_E_INVALID, "precision must be F64 or F32");
+    return create_ilu0_impl(ctx, A, colour, precision == BSP_PREC_F32, out);
 }
 
 int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
@@ -446,6 +513,13 @@ int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* Mc, double* r) {
             return BSP_OK;
         }
         case BSP_PRE_ILU0: {
+            if (M->f32) {
+                float* w = M->work32.as<float>();
+                int rc = launch_d2f(ctx, len, r, w);
+                if (!rc) rc = precond_apply_f32(ctx, M, w);
+                if (!rc) rc = launch_f2d(ctx, len, w, r);
+                return rc;
+            }
             int rc = launch_trsv(ctx, M->lower, nullptr, r, ++M->epoch);
             if (rc) return rc;
             return launch_trsv(ctx, M->upper, M->dinv_upper.as<double>(), r, ++M->epoch);
@@ -510,3 +584,20 @@ int bsp_precond_download_factor(bsp_ctx* ctx, const bsp_precond* M, int32_t* row
 }
 
 }  // extern "C"
+
+namespace bsp {
+
+int precond_apply_f32(bsp_ctx* ctx, const bsp_precond* Mc, float* r) {
+    auto* M = const_cast<bsp_precond*>(Mc);
+    if (!M->f32) return fail(ctx, BSP_E_INVALID, "FP32 apply needs an FP32 preconditioner");
+    switch (M->kind) {
+        case BSP_PRE_ILU0: {
+            int rc = launch_trsv(ctx, M->lower, nullptr, r, ++M->epoch);
+            if (rc) return rc;
+            return launch_trsv(ctx, M->upper, M->dinv_upper.p, r, ++M->epoch);
+        }
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "FP32 apply is implemented for ILU0");
+    }
+}
+
+}  // namespace bsp

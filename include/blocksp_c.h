@@ -128,6 +128,17 @@ int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int
                            bsp_precond** out);
 /* ILU0(A) (preconditioners.hpp:246-357): factor + diagonal-block inverses. */
 int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
+/* ILU0 with a chosen elimination order and storage precision (no reference
+ * counterpart: the reference eliminates in natural order in the matrix's type).
+ * colour (nullable, n_rows entries >= 0): rows are eliminated colour by colour, in
+ * natural order within a colour, i.e. ILU0 of P A P^T for that permutation; with a
+ * colouring in which no two rows of one colour are coupled, each triangular sweep
+ * has at most (number of colours) levels.  NULL = natural order (the reference's).
+ * precision: BSP_PREC_F64 or BSP_PREC_F32 (factor, diagonal inverses and applies). */
+#define BSP_PREC_F64 0
+#define BSP_PREC_F32 1
+int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision,
+                               bsp_precond** out);
 /* IChol0(A) (preconditioners.hpp:364-538), A symmetric positive definite. */
 int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
 /* A user preconditioner (the reference's PreCond concept: any object with
@@ -176,6 +187,18 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
 /* pCG(A, b, tol, M, maxIters) (solvers.hpp:51-105). */
 int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
             double tol, int maxIters, int tol_mode, bsp_stats* stats);
+
+/* Mixed-precision flexible GMRES(restart) with deflated restart k and FP64 iterative
+ * refinement (no reference counterpart; it solves the same systems as pGMRES).  The
+ * preconditioner M is applied in its own precision (an FP32 ILU0 from
+ * bsp_precond_create_ilu0_ex applies in FP32); the Krylov SpMV uses FP32 values with
+ * FP32 accumulation when spmv_precision == BSP_PREC_F32; Gram-Schmidt, least squares,
+ * deflation, solution update and the true residual b - Ax are FP64.  Each refinement
+ * pass reduces the current true residual by inner_reduction (<= 0: 1e-6); the solve
+ * stops when ||b - Ax|| <= tol ||b||.  x_dev is output (initial guess 0). */
+int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
+                     double tol, int maxIters, int restart, int k, int spmv_precision, double inner_reduction,
+                     bsp_stats* stats);
 
 #ifdef __cplusplus
 }

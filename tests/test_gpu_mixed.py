@@ -1,0 +1,104 @@
+"""Coloured / FP32 ILU0 and the mixed-precision FGMRES on the GPU.
+
+Pins:
+  * coloured ILU0 = the oracle's natural-order ILU0 of P A P^T (P: colour-major
+    permutation), mapped back: relative 2-norm difference <= 1e-10 (FP64);
+  * FP32 storage: within 1e-5 relative of the FP64 coloured apply;
+  * parity colouring of a 2D grid gives 3-level sweeps (a closed form);
+  * FGMRES_mixed reaches the true FP64 relative residual it is asked for.
+"""
+import numpy as np
+import pytest
+
+import blocksp_amd as bsp
+import pyoracle as po
+from blocksp_amd import problems as pr
+
+pytestmark = pytest.mark.gpu
+
+
+def permuted(p, perm):
+    """Problem with rows/columns renumbered: new row t = old row perm[t], each row's
+    blocks in ascending new column order."""
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    rp, ci, vals = [0], [], []
+    for old in perm:
+        cols = p.colidx[p.rowptr[old]:p.rowptr[old + 1]]
+        v = p.vals[p.rowptr[old]:p.rowptr[old + 1]]
+        nc = inv[cols]
+        o = np.argsort(nc, kind="stable")
+        ci.extend(nc[o])
+        vals.append(v[o])
+        rp.append(len(ci))
+    b = p.b.reshape(p.n, p.bs)[perm].reshape(-1)
+    return pr.Problem(p.n, p.bs, p.B, p.D, np.array(rp, np.int32), np.array(ci, np.int32), np.concatenate(vals), b)
+
+
+def test_ilu0_ex_natural_equals_ilu0(ctx):
+    p = pr.dg2d_veclap(10, 9, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    r = np.random.default_rng(0).standard_normal(p.n * p.bs)
+    a, b = ctx.to_device(r), ctx.to_device(r)
+    bsp.ILU0(ctx, A)(a)
+    bsp.ILU0Ex(ctx, A, None, bsp.PREC_F64)(b)
+    assert np.array_equal(a.numpy(), b.numpy())
+
+
+@pytest.mark.parametrize("case", ["dg2d", "dg3d"])
+def test_coloured_ilu0_matches_oracle_on_permuted_matrix(ctx, case):
+    if case == "dg2d":
+        dims = (12, 10)
+        p = pr.dg2d_veclap(*dims, seed=1)
+    else:
+        dims = (5, 4, 6)
+        p = pr.dg3d_lap(*dims, seed=2)
+    colour = pr.parity_colours(dims)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F64)
+    nl_lo, _ = M.levels(True)
+    nl_up, _ = M.levels(False)
+    assert nl_lo == len(dims) + 1 and nl_up == len(dims) + 1
+    perm = np.argsort(colour, kind="stable")
+    q = permuted(p, perm)
+    lu, dlu, dpiv = po.ilu0_factor(q)
+    r = np.random.default_rng(1).standard_normal(p.n * p.bs)
+    zq = po.ilu0_apply(q, lu, dlu, dpiv, r.reshape(p.n, p.bs)[perm].reshape(-1))
+    zref = np.empty_like(zq.reshape(p.n, p.bs))
+    zref[perm] = zq.reshape(p.n, p.bs)
+    rd = ctx.to_device(r)
+    M(rd)
+    assert np.linalg.norm(rd.numpy() - zref.reshape(-1)) <= 1e-10 * np.linalg.norm(zref)
+    # FP32 storage and arithmetic
+    M32 = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F32)
+    rd32 = ctx.to_device(r)
+    M32(rd32)
+    assert np.linalg.norm(rd32.numpy() - zref.reshape(-1)) <= 1e-5 * np.linalg.norm(zref)
+
+
+@pytest.mark.parametrize("spmv", [bsp.PREC_F32, bsp.PREC_F64])
+@pytest.mark.parametrize("prec", [bsp.PREC_F32, bsp.PREC_F64])
+def test_fgmres_mixed_reaches_fp64_residual(ctx, spmv, prec):
+    print(f"fgmres spmv={spmv} precond={prec}", flush=True)
+    dims = (32, 24)
+    p = pr.dg2d_veclap(*dims, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.ILU0Ex(ctx, A, pr.parity_colours(dims), prec)
+    xd, st, rc = bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, maxIters=5000, restart=30, k=10,
+                                  spmv_precision=spmv)
+    x = xd.numpy()
+    relres = np.linalg.norm(p.b - po.bsmv(p, x)) / np.linalg.norm(p.b)
+    assert rc == 0 and st.converged == 1 and relres <= 1e-10, (rc, relres, st.iters)
+    assert abs(st.true_relres - relres) <= 1e-2 * relres + 1e-16
+    # forward error against the FP64 oracle solution
+    xo, sto = po.pgmres(p, p.b, 1e-12, po.PRE_ILU0, maxIters=5000, restart=30, k=10)
+    assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
+def test_fgmres_mixed_natural_fp32(ctx):
+    p = pr.stencil2d_b2(24, 24, seed=0)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.ILU0Ex(ctx, A, None, bsp.PREC_F32)
+    xd, st, rc = bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, maxIters=5000, restart=30, k=10)
+    relres = np.linalg.norm(p.b - po.bsmv(p, xd.numpy())) / np.linalg.norm(p.b)
+    assert rc == 0 and relres <= 1e-10
