@@ -152,7 +152,9 @@ def main():
     if args.verbose:
         ctx.set_option("verbose", args.verbose)
     if args.profile_iters:
-        # a short run of each solver for kernel traces (not a benchmark step)
+        # a short run of each solver for kernel traces (not a benchmark step); sweeps are
+        # launched level by level (no graph replay) so every kernel is traced directly
+        ctx.set_option("sweep_graphs", 0)
         M32 = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F32)
         _, st, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M32, maxIters=args.profile_iters, restart=30, k=10)
         log(f"mixed: {st.iters} iterations in {st.solve_ms:.1f} ms")

@@ -90,6 +90,8 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "sweep_wg_per_cu") {
         BSP_CHECK(value >= 1 && value <= 64, BSP_E_INVALID, "sweep_wg_per_cu must be in [1, 64]");
         ctx->sweep_wg_per_cu = (int)value;
+    } else if (k == "sweep_graphs") {
+        ctx->sweep_graphs = value != 0;
     } else if (k == "verbose") {
         ctx->verbose = (int)std::max<int64_t>(0, value);
     } else if (k == "spin_limit") {

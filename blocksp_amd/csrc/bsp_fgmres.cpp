@@ -145,16 +145,10 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         }
         if (rc) return rc;
         // MGS (FP64)
-        for (int j = 0; j <= iter; ++j) {
-            rc = launch_mgs_step(ctx, len, w, j ? Vb[j - 1] : nullptr, j ? hd + (j - 1) : nullptr, Vb[j], hd + j);
-            if (rc) return rc;
-        }
-        rc = launch_mgs_step(ctx, len, w, Vb[iter], hd + iter, nullptr, hd + iter + 1);
-        if (rc) return rc;
         double* vn = nullptr;
         rc = V.take(ctx, vn);
         if (rc) return rc;
-        rc = launch_scale_div(ctx, len, w, hd + iter + 1, vn);
+        rc = launch_arnoldi_mgs(ctx, len, w, Vb.data(), iter, hd, vn, ctx->red_partials.as<double>());
         if (rc) return rc;
         BSP_HIP(hipMemcpyAsync(hpin.p, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
         BSP_HIP(hipStreamSynchronize(ctx->stream));

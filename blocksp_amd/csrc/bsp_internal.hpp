@@ -72,6 +72,19 @@ struct Sweep {
     int nlevels = 0;
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
+    // level-launch sweeps replayed from captured HIP graphs, keyed by their operands
+    struct Graph {
+        uint64_t key[8] = {};
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+    Sweep() = default;
+    Sweep(const Sweep&) = delete;
+    Sweep& operator=(const Sweep&) = delete;
+    ~Sweep() {
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
 };
 
 }  // namespace bsp
@@ -83,10 +96,11 @@ struct bsp_ctx {
     mutable std::string last_error;
     int num_cus = 256;
     // options
-    int trsv_mode = 1;               // 0 level launches, 1 sync-free single launch
+    int trsv_mode = 0;               // 0 level launches (default: measured faster), 1 sync-free single launch
     int64_t spin_limit = int64_t(1) << 22;
     int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
+    int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
     // reduction scratch
     bsp::DevMem red_partials;        // double [kMaxRedBlocks]
     bsp::DevMem red_counter;         // uint32 [kRedSlots]
@@ -94,7 +108,7 @@ struct bsp_ctx {
     bsp::DevMem err_flag;            // uint32 [1], sync-free wait timeouts
     double* pinned = nullptr;        // host pinned [kScalarSlots]
     hipEvent_t events[16] = {};
-    static constexpr int kMaxRedBlocks = 1024;
+    static constexpr int kMaxRedBlocks = 4096;
     static constexpr int kScalarSlots = 4096;
 };
 
@@ -164,6 +178,10 @@ int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* 
 // (qnext == nullptr) sqrt(dot(v, v)).
 int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qprev, const double* h_prev,
                     const double* qnext, double* h_out);
+// Whole MGS of one Arnoldi step (bsp_mgs.hip): w against Q[0..j]; writes hcol[0..j+1]
+// (device) and qnew = w / hcol[j+1].  parts: 2 x 1024 doubles of scratch.
+int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int j, double* hcol,
+                       double* qnew, double* parts);
 // q = v / h[0]
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
 // x += sum_i c[i] * Q[i]  (c on device, m vectors)

@@ -266,44 +266,109 @@ __device__ __forceinline__ void grid_finish(double part, double* partials, unsig
     }
 }
 
+// Streaming reductions: a block handles tiles of kTile consecutive elements, each
+// thread two 16-byte loads per operand issued before use (vector path when every
+// operand is 16-byte aligned; scalar path otherwise and for the tail tile).
+constexpr int kTile = kBlock * 4;
+
 template <int KIND>
+__device__ __forceinline__ double red_op(double s, double a, double b) {
+    if constexpr (KIND == 0) return fma(a, b, s);
+    else if constexpr (KIND == 1) return fma(a, a, s);
+    else return fmax(s, fabs(a));
+}
+
+template <int KIND, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_reduce(int64_t len, const double* __restrict__ x,
                                                    const double* __restrict__ y, double* partials,
                                                    unsigned* counter, double* out) {
     __shared__ double sh[kBlock / kWave];
-    double s = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
-        if (KIND == 0) s = fma(x[i], y[i], s);
-        else if (KIND == 1) s = fma(x[i], x[i], s);
-        else s = fmax(s, fabs(x[i]));
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t base = (int64_t)blockIdx.x * kTile; base < len; base += (int64_t)gridDim.x * kTile) {
+        if (VEC && base + kTile <= len) {
+            const double2* x2 = reinterpret_cast<const double2*>(x + base);
+            const double2 a0 = x2[threadIdx.x], a1 = x2[threadIdx.x + kBlock];
+            double2 b0 = a0, b1 = a1;
+            if constexpr (KIND == 0) {
+                const double2* y2 = reinterpret_cast<const double2*>(y + base);
+                b0 = y2[threadIdx.x];
+                b1 = y2[threadIdx.x + kBlock];
+            }
+            s0 = red_op<KIND>(s0, a0.x, b0.x);
+            s1 = red_op<KIND>(s1, a0.y, b0.y);
+            s0 = red_op<KIND>(s0, a1.x, b1.x);
+            s1 = red_op<KIND>(s1, a1.y, b1.y);
+        } else {
+            for (int64_t i = base + threadIdx.x; i < len && i < base + kTile; i += kBlock)
+                s0 = red_op<KIND>(s0, x[i], KIND == 0 ? y[i] : x[i]);
+        }
     }
+    double s = KIND == 2 ? fmax(s0, s1) : s0 + s1;
     s = block_reduce<KIND>(s, sh);
     grid_finish<KIND>(s, partials, counter, out, sh);
 }
 
 // v -= h_prev * qprev (if qprev), then out = dot(qnext, v) or ||v||.
+template <bool VEC>
 __global__ __launch_bounds__(kBlock) void k_mgs_step(int64_t len, double* __restrict__ v,
                                                      const double* __restrict__ qprev,
                                                      const double* __restrict__ h_prev,
                                                      const double* __restrict__ qnext, double* partials,
                                                      unsigned* counter, double* out) {
     __shared__ double sh[kBlock / kWave];
-    const double h = qprev ? *h_prev : 0.0;
-    double s = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
-        double vi = v[i];
-        if (qprev) {
-            vi = vi + (-h) * qprev[i];
-            v[i] = vi;
+    const double mh = qprev ? -(*h_prev) : 0.0;
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t base = (int64_t)blockIdx.x * kTile; base < len; base += (int64_t)gridDim.x * kTile) {
+        if (VEC && base + kTile <= len) {
+            double2* v2 = reinterpret_cast<double2*>(v + base);
+            double2 a0 = v2[threadIdx.x], a1 = v2[threadIdx.x + kBlock];
+            double2 p0{0.0, 0.0}, p1{0.0, 0.0}, q0{0.0, 0.0}, q1{0.0, 0.0};
+            if (qprev) {
+                const double2* p2 = reinterpret_cast<const double2*>(qprev + base);
+                p0 = p2[threadIdx.x];
+                p1 = p2[threadIdx.x + kBlock];
+            }
+            if (qnext) {
+                const double2* q2 = reinterpret_cast<const double2*>(qnext + base);
+                q0 = q2[threadIdx.x];
+                q1 = q2[threadIdx.x + kBlock];
+            }
+            if (qprev) {
+                a0.x = a0.x + mh * p0.x;
+                a0.y = a0.y + mh * p0.y;
+                a1.x = a1.x + mh * p1.x;
+                a1.y = a1.y + mh * p1.y;
+                v2[threadIdx.x] = a0;
+                v2[threadIdx.x + kBlock] = a1;
+            }
+            if (qnext) {
+                s0 = fma(q0.x, a0.x, s0);
+                s1 = fma(q0.y, a0.y, s1);
+                s0 = fma(q1.x, a1.x, s0);
+                s1 = fma(q1.y, a1.y, s1);
+            } else {
+                s0 = fma(a0.x, a0.x, s0);
+                s1 = fma(a0.y, a0.y, s1);
+                s0 = fma(a1.x, a1.x, s0);
+                s1 = fma(a1.y, a1.y, s1);
+            }
+        } else {
+            for (int64_t i = base + threadIdx.x; i < len && i < base + kTile; i += kBlock) {
+                double vi = v[i];
+                if (qprev) {
+                    vi = vi + mh * qprev[i];
+                    v[i] = vi;
+                }
+                s0 = qnext ? fma(qnext[i], vi, s0) : fma(vi, vi, s0);
+            }
         }
-        s = qnext ? fma(qnext[i], vi, s) : fma(vi, vi, s);
     }
-    s = block_reduce<0>(s, sh);
+    double s = block_reduce<0>(s0 + s1, sh);
     if (qnext) grid_finish<0>(s, partials, counter, out, sh);
     else grid_finish<1>(s, partials, counter, out, sh);
 }
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 __global__ __launch_bounds__(kBlock) void k_axpy(int64_t len, double a, const double* __restrict__ x,
                                                  double* __restrict__ y) {
@@ -424,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(int64_t len, const doubl
 inline unsigned grid_for(int64_t threads) { return (unsigned)((threads + kBlock - 1) / kBlock); }
 
 int reduce_blocks(int64_t len) {
-    int64_t b = (len + kBlock * 8 - 1) / (kBlock * 8);
+    int64_t b = (len + kTile - 1) / kTile;
     return (int)std::max<int64_t>(1, std::min<int64_t>(b, bsp_ctx::kMaxRedBlocks));
 }
 
@@ -528,11 +593,14 @@ int reduce(const bsp_ctx* ctx, int kind, int64_t len, const double* x, const dou
     const int nb = reduce_blocks(len);
     double* part = ctx->red_partials.as<double>();
     unsigned* cnt = ctx->red_counter.as<unsigned>();
+    const bool vec = aligned16(x) && (kind != 0 || aligned16(y));
+#define BSP_RED(K, V) hipLaunchKernelGGL((k_reduce<K, V>), dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out)
     switch (kind) {
-        case 0: hipLaunchKernelGGL(k_reduce<0>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out); break;
-        case 1: hipLaunchKernelGGL(k_reduce<1>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out); break;
-        default: hipLaunchKernelGGL(k_reduce<2>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, x, y, part, cnt, out); break;
+        case 0: if (vec) BSP_RED(0, true); else BSP_RED(0, false); break;
+        case 1: if (vec) BSP_RED(1, true); else BSP_RED(1, false); break;
+        default: if (vec) BSP_RED(2, true); else BSP_RED(2, false); break;
     }
+#undef BSP_RED
     BSP_LAUNCH_CHECK();
     if (host_out) {
         BSP_HIP(hipMemcpyAsync(ctx->pinned, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -559,8 +627,13 @@ int launch_scal(const bsp_ctx* ctx, int64_t len, double a, double* x) {
 int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qprev, const double* h_prev,
                     const double* qnext, double* h_out) {
     const int nb = reduce_blocks(len);
-    hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kBlock), 0, ctx->stream, len, v, qprev, h_prev, qnext,
-                       ctx->red_partials.as<double>(), ctx->red_counter.as<unsigned>(), h_out);
+    const bool vec = aligned16(v) && (!qprev || aligned16(qprev)) && (!qnext || aligned16(qnext));
+    if (vec)
+        hipLaunchKernelGGL(k_mgs_step<true>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, v, qprev, h_prev, qnext,
+                           ctx->red_partials.as<double>(), ctx->red_counter.as<unsigned>(), h_out);
+    else
+        hipLaunchKernelGGL(k_mgs_step<false>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, v, qprev, h_prev, qnext,
+                           ctx->red_partials.as<double>(), ctx->red_counter.as<unsigned>(), h_out);
     BSP_LAUNCH_CHECK();
     return BSP_OK;
 }

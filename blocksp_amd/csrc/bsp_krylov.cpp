@@ -147,17 +147,10 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
         rc = bsp_precond_apply(ctx, M, v);
         ++applies;
         if (rc) return rc;
-        for (int j = 0; j <= iter; ++j) {
-            rc = launch_mgs_step(ctx, len, v, j ? Q[j - 1] : nullptr, j ? hcol_dev + (j - 1) : nullptr, Q[j],
-                                 hcol_dev + j);
-            if (rc) return rc;
-        }
-        rc = launch_mgs_step(ctx, len, v, Q[iter], hcol_dev + iter, nullptr, hcol_dev + iter + 1);
-        if (rc) return rc;
         double* qn = nullptr;
         rc = take(qn);
         if (rc) return rc;
-        rc = launch_scale_div(ctx, len, v, hcol_dev + iter + 1, qn);
+        rc = launch_arnoldi_mgs(ctx, len, v, Q.data(), iter, hcol_dev, qn, ctx->red_partials.as<double>());
         if (rc) return rc;
         BSP_HIP(hipMemcpyAsync(hpin.p, hcol_dev, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
         BSP_HIP(hipStreamSynchronize(ctx->stream));

@@ -1,0 +1,150 @@
+// bsp_mgs.hip -- modified Gram-Schmidt of one Arnoldi step without grid-wide fences.
+//
+// For w against q_0..q_j (one_arnoldi, blocksp_solvers.hpp:127-135):
+//   step s = 0..j+1 (one kernel each):
+//     if s > 0: h_{s-1} = sum of the previous step's block partials;  w -= h_{s-1} q_{s-1}
+//     partial_s[block] = sum over the block's elements of q_s w   (s <= j)   or   w w   (s = j+1)
+//   scale kernel: h_{j+1} = sqrt(|sum partial_{j+1}|);  q_{j+1} = w / h_{j+1}
+// Every block of a kernel sums the previous partials itself, in one fixed order
+// (thread t takes partials t, t+256, ..., then a fixed tree), so all blocks use the
+// bitwise-identical h, and block 0 records it in hcol for the host.  The kernel
+// boundary orders the partials; no device-scope fence or atomic is needed.
+#include "bsp_internal.hpp"
+
+#include <algorithm>
+
+namespace bsp {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kTile = kBlock * 4;            // elements per block per tile (2 x double2 per thread)
+constexpr int kMaxParts = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[w] = v;
+    __syncthreads();
+    const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// sum of nparts partials, identical in every block
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nparts, double* sh) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) s += part[b];
+    return block_sum(s, sh);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kBlock) void k_mgs2(int64_t len, double* __restrict__ v, const double* __restrict__ qprev,
+                                                 const double* __restrict__ qnext, const double* __restrict__ pin,
+                                                 double* __restrict__ pout, int nparts, double* __restrict__ hprev_out) {
+    __shared__ double sh[kBlock / kWave];
+    double mh = 0.0;
+    if (qprev) {
+        const double h = sum_partials(pin, nparts, sh);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *hprev_out = h;
+        mh = -h;
+    }
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t base = (int64_t)blockIdx.x * kTile; base < len; base += (int64_t)gridDim.x * kTile) {
+        if (VEC && base + kTile <= len) {
+            double2* v2 = reinterpret_cast<double2*>(v + base);
+            double2 a0 = v2[threadIdx.x], a1 = v2[threadIdx.x + kBlock];
+            double2 p0{0.0, 0.0}, p1{0.0, 0.0}, q0{0.0, 0.0}, q1{0.0, 0.0};
+            if (qprev) {
+                const double2* p2 = reinterpret_cast<const double2*>(qprev + base);
+                p0 = p2[threadIdx.x];
+                p1 = p2[threadIdx.x + kBlock];
+            }
+            if (qnext) {
+                const double2* q2 = reinterpret_cast<const double2*>(qnext + base);
+                q0 = q2[threadIdx.x];
+                q1 = q2[threadIdx.x + kBlock];
+            }
+            if (qprev) {
+                a0.x = a0.x + mh * p0.x;
+                a0.y = a0.y + mh * p0.y;
+                a1.x = a1.x + mh * p1.x;
+                a1.y = a1.y + mh * p1.y;
+                v2[threadIdx.x] = a0;
+                v2[threadIdx.x + kBlock] = a1;
+            }
+            if (qnext) {
+                s0 = fma(q0.x, a0.x, s0);
+                s1 = fma(q0.y, a0.y, s1);
+                s0 = fma(q1.x, a1.x, s0);
+                s1 = fma(q1.y, a1.y, s1);
+            } else {
+                s0 = fma(a0.x, a0.x, s0);
+                s1 = fma(a0.y, a0.y, s1);
+                s0 = fma(a1.x, a1.x, s0);
+                s1 = fma(a1.y, a1.y, s1);
+            }
+        } else {
+            for (int64_t i = base + threadIdx.x; i < len && i < base + kTile; i += kBlock) {
+                double vi = v[i];
+                if (qprev) {
+                    vi = vi + mh * qprev[i];
+                    v[i] = vi;
+                }
+                s0 = qnext ? fma(qnext[i], vi, s0) : fma(vi, vi, s0);
+            }
+        }
+    }
+    const double s = block_sum(s0 + s1, sh);
+    if (threadIdx.x == 0) pout[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mgs_scale(int64_t len, const double* __restrict__ v,
+                                                      const double* __restrict__ pin, int nparts,
+                                                      double* __restrict__ q, double* __restrict__ hout) {
+    __shared__ double sh[kBlock / kWave];
+    const double h = sqrt(fabs(sum_partials(pin, nparts, sh)));
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (int64_t)gridDim.x * kBlock)
+        q[i] = v[i] / h;
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+int mgs_parts(int64_t len) {
+    const int64_t b = (len + 2 * kTile - 1) / (2 * kTile);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, kMaxParts));
+}
+
+int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int j, double* hcol,
+                       double* qnew, double* parts) {
+    const int nb = mgs_parts(len);
+    double* P[2] = {parts, parts + kMaxParts};
+    bool vec = aligned16(w);
+    for (int s = 0; s <= j; ++s) vec = vec && aligned16(Q[s]);
+    for (int s = 0; s <= j + 1; ++s) {
+        const double* qprev = s ? Q[s - 1] : nullptr;
+        const double* qnext = s <= j ? Q[s] : nullptr;
+        const double* pin = P[(s + 1) & 1];
+        double* pout = P[s & 1];
+        double* hprev = s ? hcol + (s - 1) : nullptr;
+        if (vec)
+            hipLaunchKernelGGL(k_mgs2<true>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qprev, qnext, pin, pout, nb,
+                               hprev);
+        else
+            hipLaunchKernelGGL(k_mgs2<false>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qprev, qnext, pin, pout,
+                               nb, hprev);
+    }
+    const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_mgs_scale, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, P[(j + 1) & 1], nb, qnew,
+                       hcol + j + 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("mgs launch: ") + hipGetErrorString(e));
+    return BSP_OK;
+}
+
+}  // namespace bsp

@@ -24,6 +24,7 @@
 #include "bsp_internal.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace bsp {
 
@@ -32,6 +33,8 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kDirectLevels = 8;     // up to this many levels: direct launches
+constexpr size_t kMaxGraphs = 8;     // captured sweep graphs kept per schedule
 
 template <int BS, typename VT>
 __device__ __forceinline__ void load_row(const VT* __restrict__ src, VT (&a)[BS]) {
@@ -217,11 +220,46 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
         hipLaunchKernelGGL((k_sweep_syncfree<BS, VT, SOR>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, S.nslots);
         S.tickets_used += (unsigned long long)(nchunks + grid);
     } else {
-        for (int l = 0; l < S.nlevels; ++l) {
-            const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
-            if (cnt <= 0) continue;
-            hipLaunchKernelGGL((k_sweep_level<BS, VT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, ctx->stream,
-                               a, s0, cnt);
+        auto launch_levels = [&](hipStream_t st) {
+            for (int l = 0; l < S.nlevels; ++l) {
+                const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
+                if (cnt <= 0) continue;
+                hipLaunchKernelGGL((k_sweep_level<BS, VT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, st, a,
+                                   s0, cnt);
+            }
+        };
+        if (S.nlevels <= kDirectLevels || !ctx->sweep_graphs) {
+            launch_levels(ctx->stream);
+        } else {
+            // many levels: replay one captured graph instead of nlevels host launches
+            uint64_t key[8] = {(uint64_t)(uintptr_t)a.x, (uint64_t)(uintptr_t)a.dinv, (uint64_t)(uintptr_t)a.b,
+                               (uint64_t)(uintptr_t)a.xo, 0, (uint64_t)a.forward, (uint64_t)(uintptr_t)a.val, 1};
+            std::memcpy(&key[4], &a.w, sizeof(double));
+            Sweep::Graph* hit = nullptr;
+            for (auto& g : S.graphs)
+                if (std::equal(key, key + 8, g.key)) hit = &g;
+            if (!hit) {
+                hipGraph_t graph = nullptr;
+                hipGraphExec_t exec = nullptr;
+                hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    launch_levels(ctx->stream);
+                    e = hipStreamEndCapture(ctx->stream, &graph);
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (graph) (void)hipGraphDestroy(graph);
+                if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("sweep graph: ") + hipGetErrorString(e));
+                if (S.graphs.size() >= kMaxGraphs) {
+                    (void)hipGraphExecDestroy(S.graphs.front().exec);
+                    S.graphs.erase(S.graphs.begin());
+                }
+                S.graphs.push_back({});
+                std::copy(key, key + 8, S.graphs.back().key);
+                S.graphs.back().exec = exec;
+                hit = &S.graphs.back();
+            }
+            hipError_t e = hipGraphLaunch(hit->exec, ctx->stream);
+            if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("sweep graph launch: ") + hipGetErrorString(e));
         }
     }
     hipError_t e = hipGetLastError();

@@ -167,7 +167,7 @@ def test_ilu0_parity(ctx, case, mode):
     z = rd.numpy()
     zref = po.ilu0_apply(p, lu, dlu, dpiv, r)
     assert np.linalg.norm(z - zref) <= 1e-10 * np.linalg.norm(zref)
-    ctx.set_option("trsv", 1)
+    ctx.set_option("trsv", 0)
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -187,7 +187,7 @@ def test_ichol0_parity(ctx, case, mode):
     M(rd)
     zref = po.ichol0_apply(p.n, p.bs, lrp, lci, lv, r)
     assert np.linalg.norm(rd.numpy() - zref) <= 1e-10 * np.linalg.norm(zref)
-    ctx.set_option("trsv", 1)
+    ctx.set_option("trsv", 0)
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -204,7 +204,7 @@ def test_sor_parity(ctx, typ, mode):
         M(rd)
         zref = po.sor_apply(p, r, w, nu, typ)
         assert np.linalg.norm(rd.numpy() - zref) <= 1e-10 * np.linalg.norm(zref)
-    ctx.set_option("trsv", 1)
+    ctx.set_option("trsv", 0)
 
 
 def test_diag_parity(ctx):
