@@ -90,6 +90,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "sweep_wg_per_cu") {
         BSP_CHECK(value >= 1 && value <= 64, BSP_E_INVALID, "sweep_wg_per_cu must be in [1, 64]");
         ctx->sweep_wg_per_cu = (int)value;
+    } else if (k == "bsmv_variant") {
+        BSP_CHECK(value >= 0 && value <= 2, BSP_E_INVALID, "bsmv_variant must be 0, 1 or 2");
+        ctx->bsmv_variant = (int)value;
     } else if (k == "sweep_graphs") {
         ctx->sweep_graphs = value != 0;
     } else if (k == "verbose") {

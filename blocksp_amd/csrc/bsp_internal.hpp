@@ -101,6 +101,8 @@ struct bsp_ctx {
     int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
+    int bsmv_variant = 0;            // 0: lane = block row r, two blocks per step; 1: non-temporal matrix
+                                     // loads (measured 2.2x slower); 2: coalesced 16-byte chunks per lane
     // reduction scratch
     bsp::DevMem red_partials;        // double [kMaxRedBlocks]
     bsp::DevMem red_counter;         // uint32 [kRedSlots]

@@ -311,3 +311,21 @@ def test_full_size_bsmv_properties(ctx):
     assert np.max(np.abs(lhs - rhs)) <= 1e-12 * np.max(np.abs(rhs))
     # against the numpy product (independent order)
     assert np.max(np.abs(y1.numpy() - p.bsmv(x1))) <= 1e-12 * np.max(np.abs(p.bsmv(np.abs(x1))))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("bs", [2, 3, 4, 8])
+def test_bsmv_variants(ctx, bs, variant):
+    ctx.set_option("bsmv_variant", variant)
+    try:
+        n = 257
+        rp, ci, v = random_bsr(n, bs, density_blocks=9, seed=40 + bs)
+        A = bsp.BSR(ctx, n, n, bs, rp, ci, v)
+        x = np.random.default_rng(3).standard_normal(n * bs)
+        yd = ctx.zeros(n * bs)
+        bsp.bsmv(ctx, A, ctx.to_device(x), yd)
+        yref = po.bsmv_arrays(n, bs, rp, ci, v, x)
+        absax = po.bsmv_arrays(n, bs, rp, ci, np.abs(v), np.abs(x))
+        check_bsmv(yd.numpy(), yref, absax)
+    finally:
+        ctx.set_option("bsmv_variant", 0)
