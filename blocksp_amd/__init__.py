@@ -39,7 +39,8 @@ class BspError(RuntimeError):
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
                 ("precond_applies", C.c_int), ("res0", C.c_double), ("res_est", C.c_double),
-                ("true_relres", C.c_double), ("setup_ms", C.c_double), ("solve_ms", C.c_double)]
+                ("true_relres", C.c_double), ("setup_ms", C.c_double), ("solve_ms", C.c_double),
+                ("fallback", C.c_int), ("reserved", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -92,6 +93,23 @@ _SIGS = {
     "bsp_precond_create_ilu0_ex": [_VP, _VP, _VP, C.c_int, C.POINTER(_VP)],
     "bsp_fgmres_mixed": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                          C.POINTER(Stats)],
+    "bsp_fgmres_fallback": [_VP, _VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                            C.c_double, C.POINTER(Stats)],
+    "bsp_mpnn_create": [_VP, C.c_int, C.c_uint64, C.POINTER(_VP)],
+    "bsp_mpnn_destroy": [_VP],
+    "bsp_mpnn_num_params": [_VP, C.POINTER(C.c_int64)],
+    "bsp_mpnn_get_params": [_VP, _VP, _VP],
+    "bsp_mpnn_set_params": [_VP, _VP, _VP],
+    "bsp_mpnn_get_grads": [_VP, _VP, _VP],
+    "bsp_mpnn_graph_create": [_VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_mpnn_graph_destroy": [_VP],
+    "bsp_mpnn_infer": [_VP, _VP, _VP, _VP, _VP],
+    "bsp_mpnn_loss_grad": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_double)],
+    "bsp_mpnn_adam_step": [_VP, _VP, C.c_float],
+    "bsp_mpnn_train": [_VP, _VP, C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_float, _VP],
+    "bsp_precond_create_learned": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(_VP)],
+    "bsp_ctx_record_basis": [_VP, _VP, C.c_int64, C.c_int, C.c_int],
+    "bsp_ctx_recorded": [_VP, C.POINTER(C.c_int)],
 }
 
 
@@ -357,6 +375,26 @@ class ILU0Ex(_Precond):
         self.h = h
 
 
+_HOST_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_int64, C.c_void_p)
+
+
+class HostPrecond(_Precond):
+    """User preconditioner on the host (bsp_precond_create_host): fn(r) overwrites the
+    numpy view r with M^{-1} r, as a reference PreCond callable does (tests.hpp:113)."""
+
+    def __init__(self, ctx, A, fn):
+        self.ctx, self.A = ctx, A
+
+        def tramp(ptr, length, user):
+            fn(np.ctypeslib.as_array(ptr, shape=(length,)))
+
+        self._cb = _HOST_FN(tramp)
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_precond_create_host(ctx.h, A.h, C.cast(self._cb, C.c_void_p), None, C.byref(h)),
+                  "HostPrecond")
+        self.h = h
+
+
 class IChol0(_Precond):
     """preConditioners::IChol0(A) (blocksp_preconditioners.hpp:364-538)."""
 
@@ -378,14 +416,126 @@ def pGMRES(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=500, restart=-1, k=-1, 
 
 
 def FGMRES_mixed(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=20000, restart=30, k=10, spmv_precision=PREC_F32,
-                 inner_reduction=1e-6, x=None):
-    """Mixed-precision flexible GMRES-DR with FP64 refinement (bsp_fgmres_mixed)."""
+                 inner_reduction=1e-6, x=None, fallback=None):
+    """Mixed-precision flexible GMRES-DR with FP64 refinement (bsp_fgmres_mixed, or
+    bsp_fgmres_fallback when a fallback preconditioner is given)."""
     x = x if x is not None else ctx.empty(b.shape)
     st = Stats()
-    rc = ctx.L.bsp_fgmres_mixed(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(restart), int(k),
-                                int(spmv_precision), float(inner_reduction), C.byref(st))
+    if fallback is None:
+        rc = ctx.L.bsp_fgmres_mixed(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(restart), int(k),
+                                    int(spmv_precision), float(inner_reduction), C.byref(st))
+    else:
+        rc = ctx.L.bsp_fgmres_fallback(ctx.h, A.h, M.h, fallback.h, b.ptr, x.ptr, float(tol), int(maxIters),
+                                       int(restart), int(k), int(spmv_precision), float(inner_reduction), C.byref(st))
     ctx.check(rc, "fgmres_mixed")
     return x, st, rc
+
+
+class MPNN:
+    """Learned ILU0 correction network (bsp_mpnn_*): 3 message-passing layers, hidden 64."""
+
+    def __init__(self, ctx, bs=8, seed=0):
+        self.ctx, self.bs = ctx, int(bs)
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_mpnn_create(ctx.h, self.bs, C.c_uint64(seed), C.byref(h)), "mpnn_create")
+        self.h = h
+        cnt = C.c_int64()
+        ctx.check(ctx.L.bsp_mpnn_num_params(self.h, C.byref(cnt)), "mpnn_num_params")
+        self.nparams = cnt.value
+
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        self.ctx.check(self.ctx.L.bsp_mpnn_get_params(self.ctx.h, self.h, _ptr(out)), "mpnn_get_params")
+        return out
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert th.size == self.nparams
+        self.ctx.check(self.ctx.L.bsp_mpnn_set_params(self.ctx.h, self.h, _ptr(th)), "mpnn_set_params")
+
+    def grads(self):
+        out = np.empty(self.nparams, np.float32)
+        self.ctx.check(self.ctx.L.bsp_mpnn_get_grads(self.ctx.h, self.h, _ptr(out)), "mpnn_get_grads")
+        return out
+
+    def infer(self, graph):
+        n, E = graph.n, graph.E
+        dD = np.empty((n, self.bs * self.bs), np.float32)
+        g = np.empty(max(E, 1), np.float32)
+        self.ctx.check(self.ctx.L.bsp_mpnn_infer(self.ctx.h, self.h, graph.h, _ptr(dD), _ptr(g)), "mpnn_infer")
+        return dD, g[:E]
+
+    def loss_grad(self, graph, Z: DeviceArray, K):
+        L = C.c_double()
+        self.ctx.check(self.ctx.L.bsp_mpnn_loss_grad(self.ctx.h, self.h, graph.h, Z.ptr, int(K), C.byref(L)),
+                       "mpnn_loss_grad")
+        return L.value
+
+    def adam_step(self, lr):
+        self.ctx.check(self.ctx.L.bsp_mpnn_adam_step(self.ctx.h, self.h, float(lr)), "mpnn_adam_step")
+
+    def train(self, graphs, Zs, Ks, epochs, batch=16, lr=1e-3):
+        ng = len(graphs)
+        gh = (C.c_void_p * ng)(*[g.h.value for g in graphs])
+        zp = (C.c_void_p * ng)(*[z.ptr.value for z in Zs])
+        ks = (C.c_int * ng)(*[int(k) for k in Ks])
+        out = np.zeros(max(int(epochs), 1))
+        self.ctx.check(self.ctx.L.bsp_mpnn_train(self.ctx.h, self.h, ng, gh, zp, ks, int(epochs), int(batch),
+                                                 float(lr), _ptr(out)), "mpnn_train")
+        return out[:int(epochs)]
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.L.bsp_mpnn_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class MPNNGraph:
+    """Block graph of A with the ILU0 factor of M to correct (bsp_mpnn_graph_create)."""
+
+    def __init__(self, ctx, A, M):
+        self.ctx, self.A, self.M = ctx, A, M
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_mpnn_graph_create(ctx.h, A.h, M.h, C.byref(h)), "mpnn_graph_create")
+        self.h = h
+        self.n = A.n_rows
+        self.E = int(A.nnzb) - A.n_rows
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.L.bsp_mpnn_graph_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class LearnedILU0(_Precond):
+    """ILU0 (elimination order from `colour`) corrected by a trained MPNN (bsp_precond_create_learned)."""
+
+    def __init__(self, ctx, A, net: MPNN, colour=None, precision=PREC_F64):
+        self.ctx, self.A = ctx, A
+        h = C.c_void_p()
+        col = None if colour is None else np.ascontiguousarray(colour, np.int32)
+        self._colour = col
+        ctx.check(ctx.L.bsp_precond_create_learned(ctx.h, A.h, net.h, None if col is None else _ptr(col),
+                                                   int(precision), C.byref(h)), "LearnedILU0")
+        self.h = h
+
+
+def record_basis(ctx, buf: DeviceArray, length, cap, every=1):
+    """Record Krylov basis vectors of the following pGMRES solves into buf (cap x length doubles)."""
+    ctx.check(ctx.L.bsp_ctx_record_basis(ctx.h, None if buf is None else buf.ptr, int(length), int(cap),
+                                         int(every)), "record_basis")
+
+
+def recorded(ctx):
+    c = C.c_int()
+    ctx.check(ctx.L.bsp_ctx_recorded(ctx.h, C.byref(c)), "recorded")
+    return c.value
 
 
 def pCG(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=1000, tol_mode=TOL_REFERENCE, x=None):

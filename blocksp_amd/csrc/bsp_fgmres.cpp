@@ -10,7 +10,6 @@
 //   least squares on H (FP64, host), x += Z y with FP64 accumulation
 //   restart: harmonic Ritz subspace of H (FP64) -> V' = V P_{k+1}, Z' = Z P_k,
 //            H' = P_{k+1}^T H P_k (A Z' = V' H' is preserved).
-// The learned correction, when attached, adds to z_j after the FP32 preconditioner.
 #include <cmath>
 #include <cstdio>
 #include <deque>
@@ -60,6 +59,7 @@ struct InnerStats {
     int iters = 0, restarts = 0;
     double res = 0.0, beta = 0.0;
     bool stagnated = false;
+    bool nonfinite = false;   // the pass met a non-finite value; its last update was dropped
 };
 
 // One inner FGMRES-DR pass on A e = r for the current x (x += e on return).
@@ -179,7 +179,12 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             std::fprintf(stderr, "[fgmres] iter %d res %.3e (target %.3e) restarts %d\n", tot, res, target, st.restarts);
             std::fflush(stderr);
         }
-        if (tot >= maxIters || breakdown || !std::isfinite(res)) break;
+        if (!std::isfinite(res)) {
+            st.nonfinite = true;
+            y.clear();
+            break;
+        }
+        if (tot >= maxIters || breakdown) break;
         if (iter == restart && res > target) {
             res_hist.push_back(res);
             const size_t hsz = res_hist.size();
@@ -250,9 +255,18 @@ extern "C" {
 
 int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
                      int maxIters, int restart, int k, int spmv_precision, double inner_reduction, bsp_stats* stats) {
+    return bsp_fgmres_fallback(ctx, A, M, nullptr, b, x, tol, maxIters, restart, k, spmv_precision, inner_reduction,
+                               stats);
+}
+
+int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const bsp_precond* M_fallback,
+                        const double* b, double* x, double tol, int maxIters, int restart, int k, int spmv_precision,
+                        double inner_reduction, bsp_stats* stats) {
     if (!ctx || !A || !M || !b || !x) return BSP_E_INVALID;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "FGMRES needs a square matrix");
     BSP_CHECK(M->n == A->n_rows && M->bs == A->bs, BSP_E_INVALID, "preconditioner does not match the matrix");
+    BSP_CHECK(!M_fallback || (M_fallback->n == A->n_rows && M_fallback->bs == A->bs), BSP_E_INVALID,
+              "fallback preconditioner does not match the matrix");
     BSP_CHECK(b != x, BSP_E_INVALID, "b and x must not alias");
     BSP_CHECK(restart <= 62, BSP_E_UNSUPPORTED, "FGMRES restart length is limited to 62");
     BSP_CHECK(spmv_precision == BSP_PREC_F64 || spmv_precision == BSP_PREC_F32, BSP_E_INVALID, "bad SpMV precision");
@@ -273,11 +287,14 @@ int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const
     if (rc) return rc;
     int total = 0, restarts = 0, passes = 0;
     double rr = 1.0, res0 = 0.0, res_est = 0.0;
+    const bsp_precond* cur = M;
+    bool switched = false;
     for (; passes < 100 && total < maxIters; ++passes) {
+        const double rr_before = rr;
         // the pass reduces the current residual by `inner_reduction` (or to the target)
         const double pass_red = std::max(inner_reduction, 0.5 * tol / std::max(rr, 1e-300));
         InnerStats st;
-        rc = fgmres_pass(ctx, A, M, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k, spmv_f32, st,
+        rc = fgmres_pass(ctx, A, cur, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k, spmv_f32, st,
                          rw.as<double>(), t64.as<double>());
         if (rc) return rc;
         if (passes == 0) res0 = st.beta;
@@ -287,8 +304,23 @@ int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const
         rc = true_relres_ext(ctx, A, b, x, rw.as<double>(), bnorm, rr);
         if (rc) return rc;
         if (ctx->verbose > 0) {
-            std::fprintf(stderr, "[fgmres] refinement pass %d: %d iterations, true relres %.3e\n", passes, st.iters, rr);
+            std::fprintf(stderr, "[fgmres] refinement pass %d: %d iterations, true relres %.3e%s\n", passes, st.iters,
+                         rr, st.stagnated ? " (stagnated)" : "");
             std::fflush(stderr);
+        }
+        const bool bad = st.stagnated || st.nonfinite || !std::isfinite(rr) || rr > rr_before;
+        if (bad && M_fallback && !switched) {
+            switched = true;
+            cur = M_fallback;
+            if (!std::isfinite(rr)) {
+                BSP_HIP(hipMemsetAsync(x, 0, sizeof(double) * len, ctx->stream));
+                rr = 1.0;
+            }
+            if (ctx->verbose > 0) {
+                std::fprintf(stderr, "[fgmres] switching to the fallback preconditioner\n");
+                std::fflush(stderr);
+            }
+            continue;
         }
         if (rr <= tol || !std::isfinite(rr) || st.iters == 0) break;
     }
@@ -307,6 +339,7 @@ int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const
         stats->res_est = res_est;
         stats->true_relres = rr;
         stats->solve_ms = ms;
+        stats->fallback = switched ? 1 : 0;
     }
     return conv ? BSP_OK : BSP_NOT_CONVERGED;
 }

@@ -109,6 +109,10 @@ struct bsp_ctx {
     bsp::DevMem scalars;             // double [kScalarSlots]
     bsp::DevMem err_flag;            // uint32 [1], sync-free wait timeouts
     double* pinned = nullptr;        // host pinned [kScalarSlots]
+    // Krylov basis recorder (bsp_ctx_record_basis)
+    double* rec_buf = nullptr;
+    int rec_cap = 0, rec_every = 1, rec_count = 0;
+    int64_t rec_seen = 0, rec_len = -1;
     hipEvent_t events[16] = {};
     static constexpr int kMaxRedBlocks = 4096;
     static constexpr int kScalarSlots = 4096;
@@ -212,6 +216,14 @@ int launch_basis_rotate_f32(const bsp_ctx* ctx, int64_t len, int m, int kp, cons
 int ensure_values_f32(bsp_ctx* ctx, const bsp_bsr* A);
 // r <- M^{-1} r on an FP32 vector (FP32 preconditioners only)
 int precond_apply_f32(bsp_ctx* ctx, const bsp_precond* M, float* r);
+
+// Block ILU(0) factor in A's block layout, eliminating rows colour by colour (natural order
+// when colour == nullptr), and the device preconditioner built from such a factor
+// (bsp_precond.cpp).  ilu0_build takes LU's storage.
+int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std::vector<int32_t>& elim,
+                     std::vector<int32_t>& pos, std::vector<double>& LU);
+int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim, const std::vector<int32_t>& pos,
+               std::vector<double>& LU, bool f32, bsp_precond** out);
 
 // ||b - A x|| / bnorm (FP64, on the device) and the sync-free timeout check (bsp_krylov.cpp)
 int true_relres_ext(bsp_ctx* ctx, const bsp_bsr* A, const double* b, const double* x, double* work, double bnorm,

@@ -142,6 +142,12 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     while (res > tolerance) {
         // ---- one_arnoldi (solvers.hpp:109-136) ----
         H_resize(iter + 2, iter + 1);
+        if (ctx->rec_buf && ctx->rec_len == len && ctx->rec_count < ctx->rec_cap &&
+            ctx->rec_seen++ % ctx->rec_every == 0) {
+            BSP_HIP(hipMemcpyAsync(ctx->rec_buf + (size_t)ctx->rec_count * len, Q[iter], sizeof(double) * len,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+            ++ctx->rec_count;
+        }
         rc = launch_bsmv(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), Q[iter], v, 0);
         if (rc) return rc;
         rc = bsp_precond_apply(ctx, M, v);
@@ -352,6 +358,7 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
         stats->res_est = res;
         stats->true_relres = rr;
         stats->solve_ms = ms;
+        stats->fallback = 0;
     }
     return conv ? BSP_OK : BSP_NOT_CONVERGED;
 }
@@ -453,6 +460,7 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
         stats->res_est = res;
         stats->true_relres = rr;
         stats->solve_ms = ms;
+        stats->fallback = 0;
     }
     return conv ? BSP_OK : BSP_NOT_CONVERGED;
 }

@@ -241,29 +241,28 @@ int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int
 
 }  // extern "C"
 
-namespace {
+namespace bsp {
 
 // Block ILU(0) in a given elimination order (pos[i] = step at which row i is
 // eliminated; natural order pos[i] = i is the reference's, preconditioners.hpp:274-291).
-// L holds the blocks (i, j) with pos[j] < pos[i], U those with pos[j] > pos[i].
-int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool f32, bsp_precond** out) {
-    if (!ctx || !A || !out) return BSP_E_INVALID;
-    *out = nullptr;
+// The factor is returned in A's block layout (LU[p] for A's block p).
+int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std::vector<int32_t>& elim,
+                     std::vector<int32_t>& pos, std::vector<double>& LU) {
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
-    BSP_HIP(hipSetDevice(ctx->device));
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
     const auto& rp = A->h_rp;
     const auto& ci = A->h_ci;
     // elimination order: colour by colour, natural order within a colour
-    std::vector<int32_t> elim(n), pos(n);
+    elim.assign(n, 0);
+    pos.assign(n, 0);
     for (int i = 0; i < n; ++i) elim[i] = i;
     if (colour) {
         for (int i = 0; i < n; ++i) BSP_CHECK(colour[i] >= 0, BSP_E_INVALID, "colours must be non-negative");
         std::stable_sort(elim.begin(), elim.end(), [&](int a, int b) { return colour[a] < colour[b]; });
     }
     for (int t = 0; t < n; ++t) pos[elim[t]] = t;
-    std::vector<double> LU(A->h_val);
+    LU = A->h_val;
     // rows holding a block in column c, from A's pattern
     std::vector<int32_t> cptr(n + 1, 0), crow, cpos;
     for (int p = 0; p < (int)ci.size(); ++p) cptr[ci[p] + 1]++;
@@ -306,6 +305,18 @@ int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool
             }
         }
     }
+    return BSP_OK;
+}
+
+// Device preconditioner from an ILU(0) factor in A's block layout: L holds the blocks
+// (i, j) with pos[j] < pos[i], U those with pos[j] > pos[i] and the diagonal.
+int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim, const std::vector<int32_t>& pos,
+               std::vector<double>& LU, bool f32, bsp_precond** out) {
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    const auto& rp = A->h_rp;
+    const auto& ci = A->h_ci;
     auto M = std::make_unique<bsp_precond>();
     M->ctx = ctx;
     M->kind = BSP_PRE_ILU0;
@@ -352,6 +363,21 @@ int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool
     M->f_val.swap(LU);
     *out = M.release();
     return BSP_OK;
+}
+
+}  // namespace bsp
+
+namespace {
+
+int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool f32, bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_HIP(hipSetDevice(ctx->device));
+    std::vector<int32_t> elim, pos;
+    std::vector<double> LU;
+    int rc = ilu0_factor_host(ctx, A, colour, elim, pos, LU);
+    if (rc) return rc;
+    return ilu0_build(ctx, A, elim, pos, LU, f32, out);
 }
 
 }  // namespace

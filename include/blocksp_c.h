@@ -171,6 +171,8 @@ typedef struct {
     double true_relres; /* ||b - A x||_2 / ||b||_2 of the returned x */
     double setup_ms;    /* not filled by the solvers (preconditioners are built separately) */
     double solve_ms;    /* wall time of the solve on the context stream */
+    int fallback;       /* bsp_fgmres_fallback: 1 when the fallback preconditioner took over */
+    int reserved;
 } bsp_stats;
 
 /* Stopping rule: 0 = the reference's (pGMRES: least-squares residual estimate of the
@@ -199,6 +201,69 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
 int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
                      double tol, int maxIters, int restart, int k, int spmv_precision, double inner_reduction,
                      bsp_stats* stats);
+
+/* As bsp_fgmres_mixed, with a fallback preconditioner (no reference counterpart).  When a
+ * refinement pass with M stagnates (8 restarts without halving the residual estimate),
+ * produces a non-finite value, or leaves the true residual larger than before, the pass's
+ * update is discarded if it was not finite and the remaining passes use M_fallback
+ * (stats->fallback = 1).  M_fallback may be NULL (then this is bsp_fgmres_mixed). */
+int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const bsp_precond* M_fallback,
+                        const double* b_dev, double* x_dev, double tol, int maxIters, int restart, int k,
+                        int spmv_precision, double inner_reduction, bsp_stats* stats);
+
+/* ---- Learned ILU0 correction (no reference counterpart) ----
+ * A 3-layer message-passing network (hidden width 64) over the block graph of A predicts,
+ * for each block row i, a correction dD_i of the diagonal factor block and, for each
+ * off-diagonal block (i, j), a scaling g_ij of the factor block:
+ *     F_ii = LU_ii + 0.1 ||A_ii||_F dD_i,     F_ij = LU_ij (1 + 0.5 tanh g_ij),
+ * where LU is the block ILU(0) factor.  The corrected factor is applied by the same
+ * level-scheduled sweeps as ILU0, so a learned apply costs what an ILU0 apply costs.
+ * Training minimises mean_k ||L'U' z_k - A z_k||^2 / ||A z_k||^2 over sample vectors z_k
+ * (recorded Krylov basis vectors of FP64 ILU0-pGMRES solves) with Adam; forward, backward
+ * and optimiser run as HIP kernels in FP32.  Requires block size <= 8 and a structurally
+ * symmetric block pattern with every diagonal block present.  Parameters are FP32, in the
+ * order Wn[64][bs*bs] bn[64] We[64][bs*bs+1] be[64] {Wm[64][128] bm[64] Wu[64][128] bu[64]}x3
+ * Wd[bs*bs][64] bd[bs*bs] wg[128] bg[1] (row-major, y = W x + b). */
+typedef struct bsp_mpnn bsp_mpnn;
+typedef struct bsp_mpnn_graph bsp_mpnn_graph;
+
+/* Network for block size bs; trunk weights uniform(+-1/sqrt(fan_in)) from `seed`, heads zero
+ * (so an untrained network reproduces plain ILU0). */
+int bsp_mpnn_create(bsp_ctx* ctx, int bs, uint64_t seed, bsp_mpnn** out);
+int bsp_mpnn_destroy(bsp_mpnn* net);
+int bsp_mpnn_num_params(const bsp_mpnn* net, int64_t* count);
+int bsp_mpnn_get_params(bsp_ctx* ctx, const bsp_mpnn* net, float* host);
+/* Sets the parameters and resets the Adam state. */
+int bsp_mpnn_set_params(bsp_ctx* ctx, bsp_mpnn* net, const float* host);
+/* Gradient of the last bsp_mpnn_loss_grad call. */
+int bsp_mpnn_get_grads(bsp_ctx* ctx, const bsp_mpnn* net, float* host);
+/* Graph of A with the ILU0 factor of M (an ILU0 preconditioner of A, any elimination order)
+ * as the factor to correct.  A must outlive the graph. */
+int bsp_mpnn_graph_create(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, bsp_mpnn_graph** out);
+int bsp_mpnn_graph_destroy(bsp_mpnn_graph* g);
+/* Network outputs on a graph: dD (n x bs*bs) and g (one per off-diagonal block, in CSR order). */
+int bsp_mpnn_infer(bsp_ctx* ctx, const bsp_mpnn* net, bsp_mpnn_graph* g, float* dD_host, float* g_host);
+/* Loss over K sample vectors Z_dev (K x n*bs, FP64, contiguous) and its gradient (kept in
+ * the network for bsp_mpnn_adam_step / bsp_mpnn_get_grads). */
+int bsp_mpnn_loss_grad(bsp_ctx* ctx, bsp_mpnn* net, bsp_mpnn_graph* g, const double* Z_dev, int K, double* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with the last gradient. */
+int bsp_mpnn_adam_step(bsp_ctx* ctx, bsp_mpnn* net, float lr);
+/* `epochs` passes over the graphs; each pass takes minibatches of `batch` vectors from each
+ * graph's sample set in turn (one Adam step per minibatch).  epoch_loss[e] (may be NULL)
+ * receives the mean minibatch loss of pass e. */
+int bsp_mpnn_train(bsp_ctx* ctx, bsp_mpnn* net, int ngraphs, bsp_mpnn_graph* const* graphs,
+                   const double* const* Z_dev, const int* K, int epochs, int batch, float lr, double* epoch_loss);
+/* ILU0 of A in the elimination order given by `colour` (NULL: natural), corrected by the
+ * network's inference on A's graph, as a preconditioner of the given precision. */
+int bsp_precond_create_learned(bsp_ctx* ctx, const bsp_bsr* A, const bsp_mpnn* net, const int32_t* colour,
+                               int precision, bsp_precond** out);
+
+/* Records the Krylov basis vectors of the following bsp_pgmres solves of length `len`
+ * (= n * bs; solves of another length are not recorded) on this context into buf_dev
+ * (capacity `cap` vectors; every `every`-th basis vector is kept).  cap = 0 stops recording.
+ * bsp_ctx_recorded gives the number recorded so far. */
+int bsp_ctx_record_basis(bsp_ctx* ctx, double* buf_dev, int64_t len, int cap, int every);
+int bsp_ctx_recorded(const bsp_ctx* ctx, int* count);
 
 #ifdef __cplusplus
 }
