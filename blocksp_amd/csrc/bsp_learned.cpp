@@ -452,8 +452,8 @@ int bsp_mpnn_loss_grad(bsp_ctx* ctx, bsp_mpnn* net, bsp_mpnn_graph* g, const dou
         if ((rc = launch_d2f(ctx, len, g->r64.as<double>(), r32 + k * len))) return rc;
     }
     if ((rc = graph_forward(ctx, net, g))) return rc;
-    if ((rc = assemble(ctx, g->nnzb, b2, g->slot_row.as<int>(), g->slot_edge.as<int>(), g->lu32.as<float>(),
-                       g->s32.as<float>(), g->dD.as<float>(), g->gl.as<float>(), kAlpha, kBeta, g->F.as<float>())))
+    if ((rc = assemble(ctx, g->nnzb, bs, g->slot_row.as<int>(), g->slot_edge.as<int>(), g->lu32.as<float>(),
+                       g->dD.as<float>(), g->gl.as<float>(), kAlpha, kBeta, g->F.as<float>())))
         return rc;
     const int* rp = A->rp.as<int>();
     const int* ci = A->ci.as<int>();
@@ -487,8 +487,8 @@ int bsp_mpnn_loss_grad(bsp_ctx* ctx, bsp_mpnn* net, bsp_mpnn_graph* g, const dou
     if ((rc = grad_F(ctx, g->nnzb, bs, K, len, g->slot_row.as<int>(), ci, lo, g->gy.as<float>(), g->u.as<float>(),
                      g->gu.as<float>(), z32, g->gF.as<float>())))
         return rc;
-    if ((rc = grad_heads(ctx, n, g->E, b2, g->diag_slot.as<int>(), g->e_slot.as<int>(), g->s32.as<float>(),
-                         g->lu32.as<float>(), g->gF.as<float>(), g->gl.as<float>(), kAlpha, kBeta,
+    if ((rc = grad_heads(ctx, n, g->E, bs, g->diag_slot.as<int>(), g->e_slot.as<int>(), g->lu32.as<float>(),
+                         g->dD.as<float>(), g->gF.as<float>(), g->gl.as<float>(), kAlpha, kBeta,
                          g->g_dD.as<float>(), g->g_gl.as<float>())))
         return rc;
     if ((rc = graph_backward(ctx, net, g))) return rc;
@@ -563,18 +563,26 @@ int bsp_precond_create_learned(bsp_ctx* ctx, const bsp_bsr* A, const bsp_mpnn* n
     if ((rc = bsp_mpnn_graph_create(ctx, A, &base, &graw))) return rc;
     std::unique_ptr<bsp_mpnn_graph> g(graw);
     LU.swap(base.f_val);
-    const int n = A->n_rows, b2 = A->bs * A->bs;
-    std::vector<float> dD((size_t)n * b2), gl(std::max(g->E, 1)), s32(n);
+    const int n = A->n_rows, bs = A->bs, b2 = bs * bs;
+    std::vector<float> dD((size_t)n * b2), gl(std::max(g->E, 1));
     if ((rc = bsp_mpnn_infer(ctx, net, g.get(), dD.data(), gl.data()))) return rc;
-    BSP_HIP(hipMemcpy(s32.data(), g->s32.p, sizeof(float) * n, hipMemcpyDeviceToHost));
     const auto& rp = A->h_rp;
     const auto& ci = A->h_ci;
     int e = 0;
+    std::vector<double> T(b2), L0(b2);
     for (int i = 0; i < n; ++i)
         for (int p = rp[i]; p < rp[i + 1]; ++p) {
             double* f = LU.data() + (size_t)p * b2;
             if (ci[p] == i) {
-                for (int q = 0; q < b2; ++q) f[q] += (double)kAlpha * s32[i] * dD[(size_t)i * b2 + q];
+                // F_ii = LU_ii (I + alpha tanh(D_i))
+                for (int q = 0; q < b2; ++q) T[q] = std::tanh((double)dD[(size_t)i * b2 + q]);
+                std::copy(f, f + b2, L0.begin());
+                for (int r = 0; r < bs; ++r)
+                    for (int c = 0; c < bs; ++c) {
+                        double acc = 0.0;
+                        for (int k = 0; k < bs; ++k) acc += L0[r * bs + k] * T[k * bs + c];
+                        f[r * bs + c] = L0[r * bs + c] + (double)kAlpha * acc;
+                    }
             } else {
                 const double sc = 1.0 + (double)kBeta * std::tanh((double)gl[e++]);
                 for (int q = 0; q < b2; ++q) f[q] *= sc;

@@ -202,17 +202,24 @@ __global__ void k_edge_feat(int E, int b2, const int* __restrict__ esrc, const i
     ef2[e] = (float)(log(sj) - log(si));
 }
 
-__global__ void k_assemble(int64_t total, int b2, const int* __restrict__ slot_row, const int* __restrict__ slot_edge,
-                           const float* __restrict__ lu, const float* __restrict__ s32, const float* __restrict__ dD,
-                           const float* __restrict__ gl, float alpha, float beta, float* __restrict__ F) {
+// F_ii = LU_ii (I + alpha tanh(D_i)),  F_e = LU_e (1 + beta tanh(g_e))
+__global__ void k_assemble(int64_t total, int bs, const int* __restrict__ slot_row, const int* __restrict__ slot_edge,
+                           const float* __restrict__ lu, const float* __restrict__ dD, const float* __restrict__ gl,
+                           float alpha, float beta, float* __restrict__ F) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= total) return;
+    const int b2 = bs * bs;
     const int64_t p = t / b2;
     const int q = (int)(t - p * b2);
     const int e = slot_edge[p];
     if (e < 0) {
         const int i = slot_row[p];
-        F[t] = lu[t] + alpha * s32[i] * dD[(size_t)i * b2 + q];
+        const int r = q / bs, c = q - r * bs;
+        const float* L = lu + (size_t)p * b2 + r * bs;
+        const float* D = dD + (size_t)i * b2 + c;
+        float acc = 0.0f;
+        for (int k = 0; k < bs; ++k) acc = fmaf(L[k], tanhf(D[k * bs]), acc);
+        F[t] = fmaf(alpha, acc, lu[t]);
     } else {
         F[t] = lu[t] * (1.0f + beta * tanhf(gl[e]));
     }
@@ -322,12 +329,20 @@ __global__ void k_grad_F(int64_t total, int bs, int K, int64_t len, const int* _
     gF[t] = acc;
 }
 
-__global__ void k_grad_heads_d(int n, int b2, const int* __restrict__ diag_slot, const float* __restrict__ s32,
-                               const float* __restrict__ gF, float alpha, float* __restrict__ g_dD) {
+// g_dD[i][r][c] = alpha (1 - tanh(D_i[r][c])^2) (LU_ii^T gF_ii)[r][c]
+__global__ void k_grad_heads_d(int n, int bs, const int* __restrict__ diag_slot, const float* __restrict__ lu,
+                               const float* __restrict__ dD, const float* __restrict__ gF, float alpha,
+                               float* __restrict__ g_dD) {
+    const int b2 = bs * bs;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * b2) return;
     const int i = (int)(t / b2), q = (int)(t - (int64_t)i * b2);
-    g_dD[t] = alpha * s32[i] * gF[(size_t)diag_slot[i] * b2 + q];
+    const int r = q / bs, c = q - r * bs;
+    const size_t base = (size_t)diag_slot[i] * b2;
+    float acc = 0.0f;
+    for (int k = 0; k < bs; ++k) acc = fmaf(lu[base + k * bs + r], gF[base + k * bs + c], acc);
+    const float th = tanhf(dD[t]);
+    g_dD[t] = alpha * (1.0f - th * th) * acc;
 }
 
 __global__ void k_grad_heads_g(int E, int b2, const int* __restrict__ eslot, const float* __restrict__ lu,
@@ -433,12 +448,12 @@ int features(const bsp_ctx* ctx, int n, int E, int b2, const int* diag_slot, con
     return BSP_OK;
 }
 
-int assemble(const bsp_ctx* ctx, int64_t nnzb, int b2, const int* slot_row, const int* slot_edge, const float* lu,
-             const float* s32, const float* dD, const float* gl, float alpha, float beta, float* F) {
-    const int64_t total = nnzb * b2;
+int assemble(const bsp_ctx* ctx, int64_t nnzb, int bs, const int* slot_row, const int* slot_edge, const float* lu,
+             const float* dD, const float* gl, float alpha, float beta, float* F) {
+    const int64_t total = nnzb * bs * bs;
     if (total <= 0) return BSP_OK;
-    hipLaunchKernelGGL(k_assemble, dim3(blocks_for(total)), dim3(kBlock), 0, ctx->stream, total, b2, slot_row,
-                       slot_edge, lu, s32, dD, gl, alpha, beta, F);
+    hipLaunchKernelGGL(k_assemble, dim3(blocks_for(total)), dim3(kBlock), 0, ctx->stream, total, bs, slot_row,
+                       slot_edge, lu, dD, gl, alpha, beta, F);
     BSP_HIP(hipGetLastError());
     return BSP_OK;
 }
@@ -484,11 +499,11 @@ int grad_F(const bsp_ctx* ctx, int64_t nnzb, int bs, int K, int64_t len, const i
     return BSP_OK;
 }
 
-int grad_heads(const bsp_ctx* ctx, int n, int E, int b2, const int* diag_slot, const int* eslot, const float* s32,
-               const float* lu, const float* gF, const float* gl, float alpha, float beta, float* g_dD,
-               float* g_gl) {
-    hipLaunchKernelGGL(k_grad_heads_d, dim3(blocks_for((int64_t)n * b2)), dim3(kBlock), 0, ctx->stream, n, b2,
-                       diag_slot, s32, gF, alpha, g_dD);
+int grad_heads(const bsp_ctx* ctx, int n, int E, int bs, const int* diag_slot, const int* eslot, const float* lu,
+               const float* dD, const float* gF, const float* gl, float alpha, float beta, float* g_dD, float* g_gl) {
+    const int b2 = bs * bs;
+    hipLaunchKernelGGL(k_grad_heads_d, dim3(blocks_for((int64_t)n * b2)), dim3(kBlock), 0, ctx->stream, n, bs,
+                       diag_slot, lu, dD, gF, alpha, g_dD);
     if (E > 0)
         hipLaunchKernelGGL(k_grad_heads_g, dim3(blocks_for(E)), dim3(kBlock), 0, ctx->stream, E, b2, eslot, lu, gF, gl,
                            beta, g_gl);

@@ -8,8 +8,10 @@ namespace mpnn {
 
 constexpr int kHidden = 64;
 constexpr int kLayers = 3;
-constexpr float kAlpha = 0.1f;   // diagonal correction: F_ii = LU_ii + kAlpha s_i dD_i
-constexpr float kBeta = 0.5f;    // off-diagonal scaling: F_ij = LU_ij (1 + kBeta tanh g_ij)
+// diagonal correction F_ii = LU_ii (I + kAlpha tanh(D_i)); with |entries| <= kAlpha and
+// block size <= 8, ||kAlpha tanh(D_i)||_2 <= 0.4, so F_ii stays invertible
+constexpr float kAlpha = 0.05f;
+constexpr float kBeta = 0.25f;   // off-diagonal scaling: F_ij = LU_ij (1 + kBeta tanh g_ij)
 constexpr int kMaxPartialBlocks = 1024;
 
 // One input part of a dense layer: row r reads x[(idx ? idx[r] : r) * stride + 0 .. dim-1].
@@ -44,8 +46,8 @@ int linear_bwd_w(const bsp_ctx* ctx, int R, int out, const Part& p0, const Part*
 int segsum(const bsp_ctx* ctx, int n, const int* ptr, const int* idx, const float* src, float* dst, bool accumulate);
 int features(const bsp_ctx* ctx, int n, int E, int b2, const int* diag_slot, const int* esrc, const int* edst,
              const int* eslot, const double* val, double* sd, float* s32, float* x, float* ef, float* ef2);
-int assemble(const bsp_ctx* ctx, int64_t nnzb, int b2, const int* slot_row, const int* slot_edge, const float* lu,
-             const float* s32, const float* dD, const float* gl, float alpha, float beta, float* F);
+int assemble(const bsp_ctx* ctx, int64_t nnzb, int bs, const int* slot_row, const int* slot_edge, const float* lu,
+             const float* dD, const float* gl, float alpha, float beta, float* F);
 int factor_prod(const bsp_ctx* ctx, bool upper, int n, int bs, int K, const int* rp, const int* ci,
                 const unsigned char* lower, const float* F, const float* X, const float* add, float* Y);
 int loss_sums(const bsp_ctx* ctx, int64_t len, int K, int nblk, const float* y, const float* r, double* out);
@@ -54,8 +56,8 @@ int grad_u(const bsp_ctx* ctx, int n, int bs, int K, const int* rp, const int* c
            const unsigned char* lower, const float* F, const float* gy, float* gu);
 int grad_F(const bsp_ctx* ctx, int64_t nnzb, int bs, int K, int64_t len, const int* slot_row, const int* ci,
            const unsigned char* lower, const float* gy, const float* u, const float* gu, const float* z, float* gF);
-int grad_heads(const bsp_ctx* ctx, int n, int E, int b2, const int* diag_slot, const int* eslot, const float* s32,
-               const float* lu, const float* gF, const float* gl, float alpha, float beta, float* g_dD, float* g_gl);
+int grad_heads(const bsp_ctx* ctx, int n, int E, int bs, const int* diag_slot, const int* eslot, const float* lu,
+               const float* dD, const float* gF, const float* gl, float alpha, float beta, float* g_dD, float* g_gl);
 int adam(const bsp_ctx* ctx, int64_t n, float* theta, const float* g, float* m, float* v, float lr, float b1,
          float b2, float eps, float c1, float c2);
 

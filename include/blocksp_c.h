@@ -213,10 +213,11 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
 
 /* ---- Learned ILU0 correction (no reference counterpart) ----
  * A 3-layer message-passing network (hidden width 64) over the block graph of A predicts,
- * for each block row i, a correction dD_i of the diagonal factor block and, for each
- * off-diagonal block (i, j), a scaling g_ij of the factor block:
- *     F_ii = LU_ii + 0.1 ||A_ii||_F dD_i,     F_ij = LU_ij (1 + 0.5 tanh g_ij),
- * where LU is the block ILU(0) factor.  The corrected factor is applied by the same
+ * for each block row i, a correction D_i (bs x bs) of the diagonal factor block and, for
+ * each off-diagonal block (i, j), a scaling g_ij of the factor block:
+ *     F_ii = LU_ii (I + 0.05 tanh(D_i)),     F_ij = LU_ij (1 + 0.25 tanh g_ij),
+ * where LU is the block ILU(0) factor (tanh elementwise; the bound keeps every F_ii
+ * invertible with ||F_ii^-1|| <= ||LU_ii^-1|| / 0.6).  The corrected factor is applied by the same
  * level-scheduled sweeps as ILU0, so a learned apply costs what an ILU0 apply costs.
  * Training minimises mean_k ||L'U' z_k - A z_k||^2 / ||A z_k||^2 over sample vectors z_k
  * (recorded Krylov basis vectors of FP64 ILU0-pGMRES solves) with Adam; forward, backward

@@ -17,7 +17,10 @@ Model (hidden width H = 64, three message-passing layers, block size B <= 8):
            h   <- h + relu(Wu_l [h_i ; a_i] + bu_l)
   heads:   dD_i = Wd h_i + bd (B*B values),  g_ij = wg . [h_i ; h_j] + bg
 Correction of an ILU0 factor LU (stored in A's block pattern):
-  F_ii = LU_ii + ALPHA s_i reshape(dD_i),   F_ij = LU_ij (1 + BETA tanh g_ij)
+  F_ii = LU_ii (I + ALPHA tanh(D_i)),   F_ij = LU_ij (1 + BETA tanh g_ij)
+with D_i = dD_i as a B x B matrix (row-major).  Every entry of ALPHA tanh(D_i) is at
+most ALPHA in magnitude, so ||ALPHA tanh(D_i)||_2 <= B ALPHA = 0.4 for B = 8: the
+corrected diagonal block stays invertible with ||F_ii^-1|| <= ||LU_ii^-1|| / 0.6.
 Loss over sample vectors z_k (r_k = A z_k):
   U' = blocks of F at or right of the diagonal in elimination order, L' = I + the rest,
   loss = mean_k ||L' U' z_k - r_k||^2 / ||r_k||^2
@@ -29,8 +32,8 @@ import numpy as np
 
 H = 64
 NLAYERS = 3
-ALPHA = 0.1
-BETA = 0.5
+ALPHA = 0.05
+BETA = 0.25
 
 
 def param_shapes(bs: int):
@@ -139,7 +142,8 @@ def forward(theta, g: Graph):
 
 def corrected_factor(g: Graph, dD, gl):
     F = g.lu.copy()
-    F[g.diag_slot] += ALPHA * g.s[:, None, None] * dD.reshape(g.n, g.bs, g.bs)
+    Ld = g.lu[g.diag_slot]
+    F[g.diag_slot] += ALPHA * np.einsum("nij,njk->nik", Ld, np.tanh(dD.reshape(g.n, g.bs, g.bs)))
     F[g.e_slot] *= (1.0 + BETA * np.tanh(gl))[:, None, None]
     return F
 
@@ -190,7 +194,9 @@ def loss_and_grad(theta, g: Graph, Z):
     gF[lo] = np.einsum("kpi,kpj->pij", gyb[:, g.rows[lo], :], ub[:, g.colidx[lo], :])
     gF[up] = np.einsum("kpi,kpj->pij", gu[:, g.rows[up], :], zb[:, g.colidx[up], :])
     # to the network outputs
-    g_dD = ALPHA * g.s[:, None] * gF[g.diag_slot].reshape(n, -1)
+    T = np.tanh(dD.reshape(n, bs, bs))
+    gT = ALPHA * np.einsum("nki,nkj->nij", g.lu[g.diag_slot], gF[g.diag_slot])
+    g_dD = (gT * (1.0 - T ** 2)).reshape(n, -1)
     th = np.tanh(gl)
     g_g = BETA * (1.0 - th ** 2) * (g.lu[g.e_slot] * gF[g.e_slot]).sum((1, 2))
     G = {k: np.zeros_like(v) for k, v in P.items()}

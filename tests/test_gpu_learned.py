@@ -159,18 +159,31 @@ def test_nonfinite_preconditioner_falls_back(ctx):
     assert np.isfinite(x.numpy()).all()
 
 
-def test_garbage_network_falls_back(ctx):
+def test_non_finite_network_is_refused(ctx):
+    p = _grf(4, 10)
+    A = bsp.BSR.from_problem(ctx, p)
+    net = bsp.MPNN(ctx, 8, seed=0)
+    th = net.params()
+    th[-1] = np.nan                          # the edge head's bias
+    net.set_params(th)
+    with pytest.raises(bsp.BspError):
+        bsp.LearnedILU0(ctx, A, net)
+
+
+def test_saturated_network_still_converges(ctx):
+    """Heads driven to their bounds everywhere (the worst a trained network can output):
+    the corrected factor stays invertible and FGMRES, with the ILU0 fallback armed, reaches
+    the FP64 target."""
     p = _grf(6, 10)
     A = bsp.BSR.from_problem(ctx, p)
     b = ctx.to_device(p.b)
     net = bsp.MPNN(ctx, 8, seed=0)
     rng = np.random.default_rng(0)
     P = mr.unpack(net.params().astype(np.float64), 8)
-    P["bd"][:] = rng.uniform(-300, 300, P["bd"].shape)   # diagonal blocks swamped by noise
-    P["bg"][:] = -40.0                                   # off-diagonal factor blocks scaled by 0.5
+    P["bd"][:] = rng.choice([-300.0, 300.0], P["bd"].shape)
+    P["wg"][:] = rng.choice([-100.0, 100.0], P["wg"].shape)
     net.set_params(mr.pack(P, 8).astype(np.float32))
     M0 = bsp.ILU0(ctx, A)
     Ml = bsp.LearnedILU0(ctx, A, net)
     x, st, rc = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Ml, spmv_precision=bsp.PREC_F64, fallback=M0)
-    assert rc == 0 and st.true_relres <= 1e-10
-    assert st.fallback == 1, st.as_dict()
+    assert rc == 0 and st.true_relres <= 1e-10, st.as_dict()

@@ -44,6 +44,18 @@ def test_param_count_bs8():
     assert mr.num_params(8) == 62209
 
 
+def test_diagonal_correction_stays_invertible():
+    # worst case: every entry of tanh(D) at +-1 -> ||ALPHA T||_2 <= 8 ALPHA < 1
+    g, p = _graph()
+    rng = np.random.default_rng(0)
+    dD = 1e3 * rng.choice([-1.0, 1.0], size=(g.n, 64))
+    F = mr.corrected_factor(g, dD, np.zeros(len(g.e_slot)))
+    for i in range(g.n):
+        U = g.lu[g.diag_slot[i]]
+        Fi = F[g.diag_slot[i]]
+        assert np.linalg.norm(np.linalg.inv(Fi), 2) <= np.linalg.norm(np.linalg.inv(U), 2) / (1 - 8 * mr.ALPHA) + 1e-9
+
+
 def test_zero_heads_reproduce_ilu0_factor():
     g, p = _graph()
     th = mr.init_params(8, 0)
@@ -139,7 +151,7 @@ def test_gradient_torch_autograd():
     gl = (torch.cat([h[src], h[dst]], 1) @ P["wg"].T)[:, 0] + P["bg"][0]
     F = t(g.lu).clone()
     di, ei = t(g.diag_slot), t(g.e_slot)
-    F = F.index_add(0, di, mr.ALPHA * t(g.s)[:, None, None] * dD.reshape(-1, 8, 8))
+    F = F.index_add(0, di, mr.ALPHA * torch.bmm(t(g.lu)[di], torch.tanh(dD.reshape(-1, 8, 8))))
     scale = torch.ones(len(g.colidx), dtype=torch.float64).index_copy(0, ei, 1 + mr.BETA * torch.tanh(gl))
     F = F * scale[:, None, None]
     # dense L', U' from F, then the loss
