@@ -110,6 +110,12 @@ _SIGS = {
     "bsp_precond_create_learned": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(_VP)],
     "bsp_ctx_record_basis": [_VP, _VP, C.c_int64, C.c_int, C.c_int],
     "bsp_ctx_recorded": [_VP, C.POINTER(C.c_int)],
+    "bsp_batch_create": [_VP, _VP, C.c_int, _VP, C.c_int, C.POINTER(_VP)],
+    "bsp_batch_destroy": [_VP],
+    "bsp_batch_set_tiles": [_VP, _VP, C.c_int, C.c_int],
+    "bsp_batch_autotune": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "bsp_batch_pcg": [_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, _VP, _VP, C.POINTER(C.c_double)],
+    "bsp_batch_info": [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
 }
 
 
@@ -545,3 +551,63 @@ def pCG(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=1000, tol_mode=TOL_REFEREN
     rc = ctx.L.bsp_pcg(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(tol_mode), C.byref(st))
     ctx.check(rc, "pcg")
     return x, st, rc
+
+
+class Batch:
+    """Independent systems (diagonal blocks of the block-diagonal A) solved by IChol0-pCG in
+    one persistent launch (bsp_batch_*).  sys_rowptr: first block row of each system, plus n."""
+
+    def __init__(self, ctx, A: BSR, sys_rowptr, nrhs):
+        self.ctx, self.A, self.nrhs = ctx, A, int(nrhs)
+        self.sys_rowptr = np.ascontiguousarray(sys_rowptr, np.int32)
+        self.nsys = len(self.sys_rowptr) - 1
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_batch_create(ctx.h, A.h, self.nsys, _ptr(self.sys_rowptr), self.nrhs, C.byref(h)),
+                  "batch_create")
+        self.h = h
+
+    def set_tiles(self, wg, rhs_per_item):
+        self.ctx.check(self.ctx.L.bsp_batch_set_tiles(self.ctx.h, self.h, int(wg), int(rhs_per_item)), "set_tiles")
+
+    def autotune(self, B: DeviceArray, X: DeviceArray, probe_iters=8):
+        wg, r = C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.L.bsp_batch_autotune(self.ctx.h, self.h, B.ptr, X.ptr, int(probe_iters), C.byref(wg),
+                                                     C.byref(r)), "autotune")
+        return wg.value, r.value
+
+    def info(self):
+        wg, r, ni = C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.L.bsp_batch_info(self.h, C.byref(wg), C.byref(r), C.byref(ni)), "batch_info")
+        return wg.value, r.value, ni.value
+
+    def pcg(self, B: DeviceArray, X: DeviceArray, tol, maxIters=5000, tol_mode=TOL_REFERENCE):
+        its = np.zeros(self.nsys * self.nrhs, np.int32)
+        rel = np.zeros(self.nsys * self.nrhs)
+        ms = C.c_double()
+        rc = self.ctx.L.bsp_batch_pcg(self.ctx.h, self.h, B.ptr, X.ptr, float(tol), int(maxIters), int(tol_mode),
+                                      _ptr(its), _ptr(rel), C.byref(ms))
+        self.ctx.check(rc, "batch_pcg")
+        return its.reshape(self.nsys, self.nrhs), rel.reshape(self.nsys, self.nrhs), ms.value, rc
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.L.bsp_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def block_diagonal(problems):
+    """Concatenate independent problems into one block-diagonal (rowptr, colidx, vals, sys_rowptr)."""
+    rps, cis, vs, sys = [], [], [], [0]
+    off_r, off_nz = 0, 0
+    for p in problems:
+        rps.append(np.asarray(p.rowptr[:-1], np.int64) + off_nz)
+        cis.append(np.asarray(p.colidx, np.int64) + off_r)
+        vs.append(p.vals)
+        off_r += p.n
+        off_nz += len(p.colidx)
+        sys.append(off_r)
+    rowptr = np.concatenate(rps + [np.array([off_nz])]).astype(np.int32)
+    return rowptr, np.concatenate(cis).astype(np.int32), np.concatenate(vs), np.array(sys, np.int32)

@@ -1,0 +1,351 @@
+// bsp_batch.cpp -- batched IChol0-pCG: set-up, autotuner and the graph-captured
+// persistent launch (kernels in bsp_batch.hip).
+//
+// Per system the method is the reference's pCG (blocksp_solvers.hpp:51-105) with
+// IChol0 (blocksp_preconditioners.hpp:364-538).  The batch is one block-diagonal
+// matrix, so the IChol0 factor of the whole matrix is the set of per-system factors
+// and its level sets restricted to a system are that system's level sets.
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <numeric>
+#include <tuple>
+
+#include "bsp_batch.hpp"
+#include "bsp_internal.hpp"
+
+using namespace bsp;
+
+struct bsp_batch {
+    bsp_ctx* ctx = nullptr;
+    const bsp_bsr* A = nullptr;
+    int nsys = 0, nrhs = 0, bs = 0;
+    int64_t nrows = 0;
+    std::vector<int32_t> sys_row0;
+    bsp_precond* M = nullptr;
+    DevMem d_sys_row0, frows, flev, flev_base, brows, blev, blev_base;
+    DevMem Bi, Xi, Rv, P, Z, AP, iters, res, work;
+    DevMem item_sys, item_rhs0;
+    int items_R = 0, nitems = 0;
+    int wg = 256, R = 1;
+    bool tuned = false;
+    struct Cached {
+        std::tuple<const double*, double*, double, int, int, int, int> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Cached> graphs;
+    ~bsp_batch() {
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (M) bsp_precond_destroy(M);
+    }
+};
+
+namespace {
+
+// Autotuner cache: (device, bs, nrhs, nsys, rows) -> (workgroup size, RHS per item)
+std::map<std::tuple<int, int, int, int, int64_t>, std::pair<int, int>>& tune_cache() {
+    static std::map<std::tuple<int, int, int, int, int64_t>, std::pair<int, int>> c;
+    return c;
+}
+
+// Level sets of a sweep restricted to each system: rows of system s sorted by level, and
+// per-system level pointers (absolute positions into `rows`).
+void level_sets(const std::vector<int>& level_of, const std::vector<int32_t>& sys_row0, std::vector<int32_t>& rows,
+                std::vector<int32_t>& lev, std::vector<int32_t>& lev_base) {
+    const int nsys = (int)sys_row0.size() - 1;
+    rows.resize(level_of.size());
+    lev.clear();
+    lev_base.assign(nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        const int a = sys_row0[s], b = sys_row0[s + 1];
+        int nl = 0;
+        for (int i = a; i < b; ++i) nl = std::max(nl, level_of[i] + 1);
+        std::vector<int32_t> cnt(nl + 1, 0);
+        for (int i = a; i < b; ++i) cnt[level_of[i] + 1]++;
+        for (int l = 0; l < nl; ++l) cnt[l + 1] += cnt[l];
+        lev_base[s] = (int32_t)lev.size();
+        for (int l = 0; l <= nl; ++l) lev.push_back(a + cnt[l]);
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (int i = a; i < b; ++i) rows[a + fill[level_of[i]]++] = i;
+    }
+    lev_base[nsys] = (int32_t)lev.size();
+}
+
+hipError_t upload_i(DevMem& m, const std::vector<int32_t>& v) {
+    hipError_t e = m.alloc(sizeof(int32_t) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(m.p, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice);
+}
+
+// Work items (system, first RHS) for R right-hand sides per item, largest system first.
+int build_items(bsp_ctx* ctx, bsp_batch* b, int R) {
+    if (b->items_R == R) return BSP_OK;
+    std::vector<int32_t> order(b->nsys);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return b->sys_row0[x + 1] - b->sys_row0[x] > b->sys_row0[y + 1] - b->sys_row0[y];
+    });
+    std::vector<int32_t> isys, ir0;
+    for (int s : order)
+        for (int k0 = 0; k0 < b->nrhs; k0 += R) {
+            isys.push_back(s);
+            ir0.push_back(k0);
+        }
+    BSP_HIP(upload_i(b->item_sys, isys));
+    BSP_HIP(upload_i(b->item_rhs0, ir0));
+    b->nitems = (int)isys.size();
+    b->items_R = R;
+    return BSP_OK;
+}
+
+batch::Args make_args(bsp_batch* b, double tol, int maxIters, int mode) {
+    const bsp_precond* M = b->M;
+    batch::Args a;
+    a.nitems = b->nitems;
+    a.item_sys = b->item_sys.as<int>();
+    a.item_rhs0 = b->item_rhs0.as<int>();
+    a.sys_row0 = b->d_sys_row0.as<int>();
+    a.rp = b->A->rp.as<int>();
+    a.ci = b->A->ci.as<int>();
+    a.val = b->A->val.as<double>();
+    a.lrp = M->lower.drp;
+    a.lci = M->lower.dci;
+    a.lval = static_cast<const double*>(M->lower.dval);
+    a.dinvL = M->dinv_lower.as<double>();
+    a.urp = M->upper.drp;
+    a.uci = M->upper.dci;
+    a.uval = static_cast<const double*>(M->upper.dval);
+    a.dinvU = M->dinv_upper.as<double>();
+    a.frows = b->frows.as<int>();
+    a.flev = b->flev.as<int>();
+    a.flev_base = b->flev_base.as<int>();
+    a.brows = b->brows.as<int>();
+    a.blev = b->blev.as<int>();
+    a.blev_base = b->blev_base.as<int>();
+    a.nrhs = b->nrhs;
+    a.B = b->Bi.as<double>();
+    a.X = b->Xi.as<double>();
+    a.Rv = b->Rv.as<double>();
+    a.P = b->P.as<double>();
+    a.Z = b->Z.as<double>();
+    a.AP = b->AP.as<double>();
+    a.tol = tol;
+    a.maxIters = maxIters;
+    a.mode = mode;
+    a.iters = b->iters.as<int>();
+    a.res = b->res.as<double>();
+    a.work = b->work.as<unsigned>();
+    return a;
+}
+
+// Persistent grid: every resident workgroup slot, at most one per work item.
+int grid_size(const bsp_ctx* ctx, const bsp_batch* b) {
+    return std::max(1, std::min(b->nitems, batch::max_resident_blocks(ctx, b->bs, b->R, b->wg)));
+}
+
+// The whole solve as stream work: B -> interleaved, counter = 0, persistent pCG, X back.
+// Only kernel launches (it is captured into a graph); items and grid are prepared before.
+int enqueue(bsp_ctx* ctx, bsp_batch* b, int grid, const double* B, double* X, double tol, int maxIters, int mode) {
+    int rc;
+    const int64_t len = b->nrows * b->bs;
+    if ((rc = batch::launch_interleave(ctx, len, b->nrhs, B, b->Bi.as<double>(), true))) return rc;
+    if ((rc = launch_fill_u32(ctx, 1, 0u, b->work.as<uint32_t>()))) return rc;
+    batch::Args a = make_args(b, tol, maxIters, mode);
+    if ((rc = batch::launch_pcg(ctx, b->bs, b->R, b->wg, grid, a))) return rc;
+    return batch::launch_interleave(ctx, len, b->nrhs, b->Xi.as<double>(), X, false);
+}
+
+double time_solve(bsp_ctx* ctx, bsp_batch* b, const double* B, double* X, int iters, int& rc) {
+    rc = build_items(ctx, b, b->R);
+    if (rc) return 0.0;
+    const int grid = grid_size(ctx, b);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventRecord(ctx->events[10], ctx->stream);
+    rc = enqueue(ctx, b, grid, B, X, 0.0, iters, 1);
+    (void)hipEventRecord(ctx->events[11], ctx->stream);
+    (void)hipEventSynchronize(ctx->events[11]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->events[10], ctx->events[11]);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsp_batch_create(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
+                     bsp_batch** out) {
+    if (!ctx || !A || !sys_rowptr || !out || nsys <= 0 || nrhs <= 0) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "batched systems must be square");
+    BSP_CHECK(sys_rowptr[0] == 0 && sys_rowptr[nsys] == A->n_rows, BSP_E_INVALID,
+              "system row pointers must cover the matrix");
+    BSP_CHECK(batch::supported(A->bs, 1), BSP_E_UNSUPPORTED, "batched pCG supports block sizes 1, 2, 4 and 8");
+    BSP_HIP(hipSetDevice(ctx->device));
+    auto b = std::make_unique<bsp_batch>();
+    b->ctx = ctx;
+    b->A = A;
+    b->nsys = nsys;
+    b->nrhs = nrhs;
+    b->bs = A->bs;
+    b->nrows = A->n_rows;
+    b->sys_row0.assign(sys_rowptr, sys_rowptr + nsys + 1);
+    for (int s = 0; s < nsys; ++s) {
+        BSP_CHECK(sys_rowptr[s + 1] >= sys_rowptr[s], BSP_E_INVALID, "system row pointers must be non-decreasing");
+        for (int i = sys_rowptr[s]; i < sys_rowptr[s + 1]; ++i)
+            for (int p = A->h_rp[i]; p < A->h_rp[i + 1]; ++p)
+                BSP_CHECK(A->h_ci[p] >= sys_rowptr[s] && A->h_ci[p] < sys_rowptr[s + 1], BSP_E_INVALID,
+                          "block row " + std::to_string(i) + " couples system " + std::to_string(s) +
+                              " to another system");
+    }
+    int rc = bsp_precond_create_ichol0(ctx, A, &b->M);
+    if (rc) return rc;
+    std::vector<int32_t> rows, lev, base;
+    level_sets(b->M->lower.level_of, b->sys_row0, rows, lev, base);
+    BSP_HIP(upload_i(b->frows, rows));
+    BSP_HIP(upload_i(b->flev, lev));
+    BSP_HIP(upload_i(b->flev_base, base));
+    level_sets(b->M->upper.level_of, b->sys_row0, rows, lev, base);
+    BSP_HIP(upload_i(b->brows, rows));
+    BSP_HIP(upload_i(b->blev, lev));
+    BSP_HIP(upload_i(b->blev_base, base));
+    BSP_HIP(upload_i(b->d_sys_row0, b->sys_row0));
+    const size_t vec = sizeof(double) * std::max<size_t>((size_t)b->nrows * b->bs * nrhs, 1);
+    BSP_HIP(b->Bi.alloc(vec));
+    BSP_HIP(b->Xi.alloc(vec));
+    BSP_HIP(b->Rv.alloc(vec));
+    BSP_HIP(b->P.alloc(vec));
+    BSP_HIP(b->Z.alloc(vec));
+    BSP_HIP(b->AP.alloc(vec));
+    BSP_HIP(b->iters.alloc(sizeof(int) * (size_t)nsys * nrhs));
+    BSP_HIP(b->res.alloc(sizeof(double) * 2 * (size_t)nsys * nrhs));
+    BSP_HIP(b->work.alloc(sizeof(unsigned)));
+    b->R = nrhs % 4 == 0 ? 4 : nrhs % 2 == 0 ? 2 : 1;
+    *out = b.release();
+    return BSP_OK;
+}
+
+int bsp_batch_destroy(bsp_batch* b) {
+    delete b;
+    return BSP_OK;
+}
+
+int bsp_batch_set_tiles(bsp_ctx* ctx, bsp_batch* b, int wg_size, int rhs_per_item) {
+    if (!ctx || !b) return BSP_E_INVALID;
+    BSP_CHECK(wg_size % 64 == 0 && wg_size >= 64 && wg_size <= 1024, BSP_E_INVALID,
+              "workgroup size must be a multiple of 64 in [64, 1024]");
+    BSP_CHECK(batch::supported(b->bs, rhs_per_item) && b->nrhs % rhs_per_item == 0, BSP_E_INVALID,
+              "RHS per item must be 1, 2 or 4 and divide the number of right-hand sides");
+    b->wg = wg_size;
+    b->R = rhs_per_item;
+    b->tuned = true;
+    return BSP_OK;
+}
+
+int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, int probe_iters,
+                       int* wg_size, int* rhs_per_item) {
+    if (!ctx || !b || !B_dev || !X_dev) return BSP_E_INVALID;
+    BSP_HIP(hipSetDevice(ctx->device));
+    const auto key = std::make_tuple(ctx->device, b->bs, b->nrhs, b->nsys, b->nrows);
+    auto& cache = tune_cache();
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        probe_iters = std::max(1, probe_iters);
+        double best = 1e300;
+        std::pair<int, int> choice{b->wg, b->R};
+        for (int R : {1, 2, 4}) {
+            if (b->nrhs % R || !batch::supported(b->bs, R)) continue;
+            for (int wg : {128, 256, 512, 1024}) {
+                b->wg = wg;
+                b->R = R;
+                int rc = BSP_OK;
+                time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);   // warm-up (code load, caches)
+                if (rc) return rc;
+                const double ms = time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);
+                if (rc) return rc;
+                if (ctx->verbose > 0) {
+                    std::fprintf(stderr, "[batch autotune] wg %d rhs/item %d: %.3f ms for %d iterations\n", wg, R, ms,
+                                 probe_iters);
+                    std::fflush(stderr);
+                }
+                if (ms < best) {
+                    best = ms;
+                    choice = {wg, R};
+                }
+            }
+        }
+        it = cache.emplace(key, choice).first;
+    }
+    b->wg = it->second.first;
+    b->R = it->second.second;
+    b->tuned = true;
+    if (wg_size) *wg_size = b->wg;
+    if (rhs_per_item) *rhs_per_item = b->R;
+    return BSP_OK;
+}
+
+int bsp_batch_pcg(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, double tol, int maxIters,
+                  int tol_mode, int32_t* iters_host, double* relres_host, double* solve_ms) {
+    if (!ctx || !b || !B_dev || !X_dev || maxIters < 0) return BSP_E_INVALID;
+    BSP_CHECK(tol_mode == BSP_TOL_REFERENCE || tol_mode == BSP_TOL_TRUE_RESIDUAL, BSP_E_INVALID, "bad tol_mode");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int mode = tol_mode == BSP_TOL_REFERENCE ? 0 : 1;
+    int rc = build_items(ctx, b, b->R);
+    if (rc) return rc;
+    const int grid = grid_size(ctx, b);
+    // one captured graph per (operands, tiles); replays launch the whole solve at once
+    const auto key = std::make_tuple(B_dev, X_dev, tol, maxIters, mode, b->wg, b->R);
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : b->graphs)
+        if (g.key == key) exec = g.exec;
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        BSP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue(ctx, b, grid, B_dev, X_dev, tol, maxIters, mode);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        BSP_HIP(e);
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BSP_HIP(e);
+        if (b->graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(b->graphs.front().exec);
+            b->graphs.erase(b->graphs.begin());
+        }
+        b->graphs.push_back({key, exec});
+    }
+    BSP_HIP(hipEventRecord(ctx->events[10], ctx->stream));
+    BSP_HIP(hipGraphLaunch(exec, ctx->stream));
+    BSP_HIP(hipEventRecord(ctx->events[11], ctx->stream));
+    BSP_HIP(hipEventSynchronize(ctx->events[11]));
+    float ms = 0.f;
+    BSP_HIP(hipEventElapsedTime(&ms, ctx->events[10], ctx->events[11]));
+    if (solve_ms) *solve_ms = ms;
+    const size_t cnt = (size_t)b->nsys * b->nrhs;
+    std::vector<int32_t> its(cnt);
+    std::vector<double> res(2 * cnt);
+    BSP_HIP(hipMemcpy(its.data(), b->iters.p, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+    BSP_HIP(hipMemcpy(res.data(), b->res.p, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost));
+    if (iters_host) std::copy(its.begin(), its.end(), iters_host);
+    bool conv = true;
+    for (size_t t = 0; t < cnt; ++t) {
+        const double rr = res[2 * t + 1] > 0.0 ? res[2 * t] / res[2 * t + 1] : res[2 * t];
+        if (relres_host) relres_host[t] = rr;
+        conv = conv && res[2 * t] <= tol * res[2 * t + 1];
+    }
+    return conv ? BSP_OK : BSP_NOT_CONVERGED;
+}
+
+int bsp_batch_info(const bsp_batch* b, int* wg_size, int* rhs_per_item, int* nitems) {
+    if (!b) return BSP_E_INVALID;
+    if (wg_size) *wg_size = b->wg;
+    if (rhs_per_item) *rhs_per_item = b->R;
+    if (nitems) *nitems = b->nsys * (b->nrhs / b->R);
+    return BSP_OK;
+}
+
+}  // extern "C"

@@ -266,6 +266,38 @@ int bsp_precond_create_learned(bsp_ctx* ctx, const bsp_bsr* A, const bsp_mpnn* n
 int bsp_ctx_record_basis(bsp_ctx* ctx, double* buf_dev, int64_t len, int cap, int every);
 int bsp_ctx_recorded(const bsp_ctx* ctx, int* count);
 
+/* ---- Batched IChol0-pCG over many independent systems (no reference counterpart) ----
+ * The systems are the diagonal blocks of one block-diagonal BSR matrix A: system s owns
+ * block rows [sys_rowptr[s], sys_rowptr[s+1]) and may not couple to other systems.  Each
+ * (system, right-hand side) is solved by the reference's pCG (blocksp_solvers.hpp:51-105)
+ * with the IChol0 preconditioner (blocksp_preconditioners.hpp:364-538) of that system.
+ * All systems run in one persistent kernel launch (work items largest system first, one
+ * workgroup per item, level-set sweeps separated by workgroup barriers), replayed from a
+ * captured HIP graph.  Block sizes 1, 2, 4, 8. */
+typedef struct bsp_batch bsp_batch;
+/* Builds the IChol0 factor and the per-system level sets (A must outlive the batch). */
+int bsp_batch_create(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
+                     bsp_batch** out);
+int bsp_batch_destroy(bsp_batch* b);
+/* Tiles: workgroup size (multiple of 64, <= 1024) and right-hand sides per work item
+ * (1, 2 or 4, dividing nrhs). */
+int bsp_batch_set_tiles(bsp_ctx* ctx, bsp_batch* b, int wg_size, int rhs_per_item);
+/* Times every tile choice on probe_iters iterations of the batch (B_dev/X_dev as for
+ * bsp_batch_pcg; X_dev is overwritten), keeps the fastest, and caches the choice per
+ * (device, block size, nrhs, nsys, rows) for later batches. */
+int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, int probe_iters,
+                       int* wg_size, int* rhs_per_item);
+/* Solves A_s X_{s,k} = B_{s,k} for every system s and right-hand side k.  B_dev and X_dev
+ * hold nrhs vectors of length n*bs one after another (vector k: every system's rows in
+ * order).  Stopping rule per (s, k): tol_mode 0 ||r|| <= tol ||M^{-1} b|| (the reference's),
+ * 1 ||r|| <= tol ||b|| (r the recurrence residual).  iters_host / relres_host (may be NULL):
+ * nsys * nrhs entries, (s, k) at s * nrhs + k; relres is ||r|| over the rule's reference
+ * norm.  solve_ms: device time of the launch.  Returns BSP_NOT_CONVERGED if any pair did
+ * not meet the rule within maxIters. */
+int bsp_batch_pcg(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, double tol, int maxIters,
+                  int tol_mode, int32_t* iters_host, double* relres_host, double* solve_ms);
+int bsp_batch_info(const bsp_batch* b, int* wg_size, int* rhs_per_item, int* nitems);
+
 #ifdef __cplusplus
 }
 #endif

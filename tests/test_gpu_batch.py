@@ -1,0 +1,111 @@
+"""Batched IChol0-pCG over independent systems in one persistent launch."""
+import numpy as np
+import pytest
+
+import blocksp_amd as bsp
+from blocksp_amd import problems as pr
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(ctx, nsys=12, lo=40, hi=1500, seed=2, nrhs=4):
+    probs = pr.batch_blocklap(nsys=nsys, lo=lo, hi=hi, seed=seed, nrhs=nrhs)
+    rp, ci, vals, sys = bsp.block_diagonal(probs)
+    n = int(sys[-1])
+    A = bsp.BSR(ctx, n, n, 4, rp, ci, vals)
+    Bh = np.stack([np.concatenate([p.meta["rhs"][k] for p in probs]) for k in range(nrhs)])
+    return probs, A, sys, Bh
+
+
+def _solution_of(X, probs, sys, s, k):
+    return X[k, sys[s] * 4:sys[s + 1] * 4]
+
+
+def test_reference_rule_matches_oracle_pcg(ctx):
+    probs, A, sys, Bh = _batch(ctx)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=2000, tol_mode=bsp.TOL_REFERENCE)
+    assert rc == 0
+    Xh = X.numpy()
+    for s, p in enumerate(probs):
+        for k in range(4):
+            b = p.meta["rhs"][k]
+            xo, st = po.pcg(p, b, 1e-10, po.PRE_ICHOL0, maxIters=2000)
+            x = _solution_of(Xh, probs, sys, s, k)
+            assert abs(int(its[s, k]) - st.iters) <= 1, (s, k, its[s, k], st.iters)
+            assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+            assert np.linalg.norm(b - p.bsmv(x)) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_true_residual_rule_and_device_pcg_agree(ctx):
+    probs, A, sys, Bh = _batch(ctx, nsys=6, seed=5)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=2000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    assert rc == 0 and (rel <= 1e-10).all()
+    Xh = X.numpy()
+    for s, p in enumerate(probs):
+        Ai = bsp.BSR.from_problem(ctx, p)
+        M = bsp.IChol0(ctx, Ai)
+        for k in range(4):
+            b = p.meta["rhs"][k]
+            x = _solution_of(Xh, probs, sys, s, k)
+            assert np.linalg.norm(b - p.bsmv(x)) <= 2e-10 * np.linalg.norm(b)
+            xd, st, _ = bsp.pCG(ctx, Ai, ctx.to_device(b), 1e-10, M, maxIters=2000)
+            assert abs(int(its[s, k]) - st.iters) <= 2
+
+
+@pytest.mark.parametrize("wg,r", [(64, 1), (256, 2), (512, 4), (1024, 4), (128, 4)])
+def test_tiles_give_the_same_solutions(ctx, wg, r):
+    probs, A, sys, Bh = _batch(ctx, nsys=8, seed=7)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    X0 = ctx.zeros(Bh.shape)
+    its0, _, _, _ = bat.pcg(B, X0, 1e-10, maxIters=2000)
+    bat.set_tiles(wg, r)
+    X1 = ctx.zeros(Bh.shape)
+    its1, _, _, rc = bat.pcg(B, X1, 1e-10, maxIters=2000)
+    assert rc == 0
+    assert np.abs(its1 - its0).max() <= 1
+    a, b = X0.numpy(), X1.numpy()
+    assert np.linalg.norm(a - b) <= 1e-9 * np.linalg.norm(a)
+
+
+def test_graph_replay_is_bitwise_repeatable_and_follows_new_rhs(ctx):
+    probs, A, sys, Bh = _batch(ctx, nsys=5, seed=9)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    bat.pcg(B, X, 1e-10)
+    x1 = X.numpy()
+    X.zero()
+    bat.pcg(B, X, 1e-10)              # replay of the captured graph
+    np.testing.assert_array_equal(X.numpy(), x1)
+    B.upload(-2.0 * Bh)               # same buffers, new values: the replay reads them
+    bat.pcg(B, X, 1e-10)
+    np.testing.assert_allclose(X.numpy(), -2.0 * x1, rtol=0, atol=1e-9 * np.abs(x1).max())
+
+
+def test_autotune_picks_a_valid_tile_and_caches_it(ctx):
+    probs, A, sys, Bh = _batch(ctx, nsys=10, seed=11)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    wg, r = bat.autotune(B, X, probe_iters=4)
+    assert wg in (128, 256, 512, 1024) and r in (1, 2, 4)
+    bat2 = bsp.Batch(ctx, A, sys, 4)
+    assert bat2.autotune(B, X, probe_iters=4) == (wg, r)
+    its, rel, ms, rc = bat2.pcg(B, X, 1e-10, maxIters=2000)
+    assert rc == 0
+
+
+def test_coupled_systems_are_rejected(ctx):
+    probs, A, sys, Bh = _batch(ctx, nsys=3, seed=1)
+    bad = sys.copy()
+    bad[1] += 1                        # moves a row of system 0 into system 1
+    with pytest.raises(bsp.BspError):
+        bsp.Batch(ctx, A, bad, 4)
