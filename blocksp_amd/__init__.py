@@ -116,6 +116,9 @@ _SIGS = {
     "bsp_batch_autotune": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsp_batch_pcg": [_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, _VP, _VP, C.POINTER(C.c_double)],
     "bsp_batch_info": [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "bsp_bsmm_rhs_bf16": [_VP, _VP, C.c_int, _VP, _VP],
+    "bsp_rhs_pack_bf16": [_VP, C.c_int64, C.c_int, C.c_int, _VP, _VP],
+    "bsp_rhs_unpack_f32": [_VP, C.c_int64, C.c_int, C.c_int, _VP, _VP],
 }
 
 
@@ -288,6 +291,20 @@ def bsmv_dim(ctx, A: BSR, D, dim, x: DeviceArray, y: DeviceArray, accumulate=Fal
 
 def bsmm_rhs(ctx, A: BSR, nrhs, X: DeviceArray, Y: DeviceArray):
     ctx.check(ctx.L.bsp_bsmm_rhs(ctx.h, A.h, int(nrhs), X.ptr, Y.ptr), "bsmm_rhs")
+
+
+def rhs_pack_bf16(ctx, rows, bs, nrhs, X: DeviceArray, Xb: DeviceArray):
+    """RHS-minor FP64 X -> block-major BF16 (uint16) Xb."""
+    ctx.check(ctx.L.bsp_rhs_pack_bf16(ctx.h, int(rows), int(bs), int(nrhs), X.ptr, Xb.ptr), "rhs_pack_bf16")
+
+
+def rhs_unpack_f32(ctx, rows, bs, nrhs, Yb: DeviceArray, Y: DeviceArray):
+    ctx.check(ctx.L.bsp_rhs_unpack_f32(ctx.h, int(rows), int(bs), int(nrhs), Yb.ptr, Y.ptr), "rhs_unpack_f32")
+
+
+def bsmm_rhs_bf16(ctx, A: BSR, nrhs, Xb: DeviceArray, Yb: DeviceArray):
+    """Yb = A Xb on the matrix cores (BF16 operands, FP32 accumulation, block-major layouts)."""
+    ctx.check(ctx.L.bsp_bsmm_rhs_bf16(ctx.h, A.h, int(nrhs), Xb.ptr, Yb.ptr), "bsmm_rhs_bf16")
 
 
 class _Precond:

@@ -246,6 +246,10 @@ int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev) {
     BSP_HIP(hipMemcpyAsync(A->val.p, vals_dev, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     BSP_HIP(hipMemcpyAsync(A->h_val.data(), vals_dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
     BSP_HIP(hipStreamSynchronize(ctx->stream));
+    // reduced-precision copies and the transpose are rebuilt from the new values on next use
+    A->val32.release();
+    A->val16.release();
+    A->transposed.reset();
     return BSP_OK;
 }
 

@@ -128,6 +128,7 @@ struct bsp_bsr {
     // lazily built transpose (for bsmv_transpose)
     std::unique_ptr<bsp_bsr> transposed;
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
+    mutable bsp::DevMem val16;         // BF16 copy of the values (MFMA block SpMM)
 };
 
 struct bsp_precond {

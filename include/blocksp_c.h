@@ -99,6 +99,16 @@ int bsp_bsmv_transpose(bsp_ctx* ctx, const bsp_bsr* A, const double* x_dev, doub
 int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X_dev, double* Y_dev);
 /* C = A * B, structure in the reference's first-touch order (blocksp_multiply.hpp:51-97). */
 int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C);
+/* BF16 block SpMM on the matrix cores (no reference counterpart): Yb = A Xb for nrhs
+ * right-hand sides (a multiple of 16), 8 x 8 blocks only, BF16 values (rounded to nearest
+ * even from A's FP64 values on first use) and BF16 right-hand sides, FP32 accumulation
+ * and results.  Block-major layouts: Xb[(col * nrhs + k) * 8 + c] (uint16 BF16 bits),
+ * Yb[(row * nrhs + k) * 8 + r] (float). */
+int bsp_bsmm_rhs_bf16(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const uint16_t* Xb_dev, float* Yb_dev);
+/* RHS-minor FP64 X[(row * bs + c) * nrhs + k] (the bsp_bsmm_rhs layout) -> block-major BF16. */
+int bsp_rhs_pack_bf16(bsp_ctx* ctx, int64_t rows, int bs, int nrhs, const double* X_dev, uint16_t* Xb_dev);
+/* Block-major FP32 -> RHS-minor FP64. */
+int bsp_rhs_unpack_f32(bsp_ctx* ctx, int64_t rows, int bs, int nrhs, const float* Yb_dev, double* Y_dev);
 
 /* ---- BLAS-1 on solArrays (blocksp_containers.hpp:858-864, 917-1012) ---- */
 int bsp_dot(bsp_ctx* ctx, int64_t len, const double* x_dev, const double* y_dev, double* out_host);
