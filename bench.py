@@ -19,9 +19,14 @@ Also measured in the same run:
     (HIP events on the library stream) against 8 TB/s; `traffic` is the measured HBM
     bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/;
   * cpu_baseline: the CPU oracle (this project's restatement of the reference
-    algorithm, 1 thread) on a bounded sample, scaled to the same unit.
+    algorithm, 1 thread) on a bounded sample, scaled to the same unit;
+  * `workloads`: the secondary configurations -- the 3D 64^3 DG Laplacian (BF16 MFMA
+    block SpMM with 32 RHS, mixed vs plain FP64 solve with forward errors), 512 batched
+    uneven IChol0-pCG systems in one persistent launch, and the learned ILU0 correction
+    on the 48^3 random-coefficient problem (trained on the GPU in the same run).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-plain]
+                       [--skip-extras] [--only bf16|batch|learned]
 """
 from __future__ import annotations
 
@@ -100,6 +105,173 @@ def cpu_baseline(p, gpu_iters, sample_iters=12):
     }
 
 
+def timed_ms(ctx, fn, reps):
+    fn()
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(reps):
+        fn()
+    ctx.record(1)
+    ctx.sync()
+    return ctx.elapsed_ms(0, 1) / reps
+
+
+def bench_bf16(bsp, pr, ctx, solve=True):
+    """3D DG Laplacian 64^3 (262144 block rows, bs 8, 7 blocks/row): BF16 MFMA block SpMM
+    with 32 N(0,1) right-hand sides against FP64, and the mixed-precision solve against
+    the plain FP64 one with forward errors against a tight FP64 solve."""
+    p = pr.dg3d_lap(64, 64, 64, seed=4)
+    A = bsp.BSR.from_problem(ctx, p)
+    nrhs = 32
+    X = np.random.default_rng(0).standard_normal((p.n * 8, nrhs))
+    Xd = ctx.to_device(X)
+    Xb = ctx.empty((p.n * nrhs * 8,), np.uint16)
+    bsp.rhs_pack_bf16(ctx, p.n, 8, nrhs, Xd, Xb)
+    Yb = ctx.empty((p.n * nrhs * 8,), np.float32)
+    Y64 = ctx.empty(X.shape)
+    t_bf = timed_ms(ctx, lambda: bsp.bsmm_rhs_bf16(ctx, A, nrhs, Xb, Yb), 20)
+    t_64 = timed_ms(ctx, lambda: bsp.bsmm_rhs(ctx, A, nrhs, Xd, Y64), 5)
+    Y = ctx.empty(X.shape)
+    bsp.rhs_unpack_f32(ctx, p.n, 8, nrhs, Yb, Y)
+    a, b64 = Y.numpy(), Y64.numpy()
+    nbytes = p.nnzb * 64 * 2 + p.n * 8 * nrhs * (2 + 4) + p.nnzb * 4 + (p.n + 1) * 4
+    flops = 2.0 * p.nnzb * 64 * nrhs
+    out = {"workload": "dg3d_lap 64^3 Q1-SWIP-DG (BlockSp<double,8>, 262144 block rows, 7 blocks/row), 32 RHS N(0,1)",
+           "bf16_spmm_us": round(t_bf * 1e3, 2), "bf16_spmm_GBps": round(nbytes / t_bf / 1e6, 1),
+           "bf16_spmm_frac_hbm": round(nbytes / t_bf / 1e6 / HBM_PEAK_GBPS, 4),
+           "bf16_spmm_TFLOPs": round(flops / t_bf / 1e9, 2), "algorithmic_bytes": nbytes,
+           "fp64_bsmm_rhs_us": round(t_64 * 1e3, 2), "speedup_vs_fp64_bsmm": round(t_64 / t_bf, 2),
+           "bf16_vs_fp64_rel_diff": float(np.linalg.norm(a - b64) / np.linalg.norm(b64))}
+    log(f"bf16 MFMA SpMM {t_bf * 1e3:.1f} us ({out['bf16_spmm_GBps']:.0f} GB/s), FP64 {t_64 * 1e3:.1f} us")
+    if not solve:
+        return out
+    b = ctx.to_device(p.b)
+    M0 = bsp.ILU0(ctx, A)
+    xr, st_r, _ = bsp.pGMRES(ctx, A, b, 1e-13, M0, maxIters=20000, restart=30, k=10, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    xref = xr.numpy()
+    fwd = lambda x: float(np.linalg.norm(x.numpy() - xref) / np.linalg.norm(xref))
+    x0, st0, _ = bsp.pGMRES(ctx, A, b, 1e-10, M0, maxIters=20000, restart=30, k=10, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    Mc = bsp.ILU0Ex(ctx, A, colour=pr.parity_colours((64, 64, 64)), precision=bsp.PREC_F32)
+    x1, st1, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mc)
+    out.update({"plain_fp64_ms": round(st0.solve_ms, 2), "plain_fp64_iters": st0.iters,
+                "plain_fp64_relres": st0.true_relres, "plain_fp64_fwd_err": fwd(x0),
+                "mixed_f32_ms": round(st1.solve_ms, 2), "mixed_f32_iters": st1.iters,
+                "mixed_f32_relres": st1.true_relres, "mixed_f32_fwd_err": fwd(x1),
+                "mixed_speedup": round(st0.solve_ms / st1.solve_ms, 2),
+                "fwd_err_reference": f"FP64 ILU0-pGMRES to true relres {st_r.true_relres:.1e}"})
+    log(f"3D mixed {st1.solve_ms:.0f} ms vs plain FP64 {st0.solve_ms:.0f} ms")
+    return out
+
+
+def bench_batch(bsp, pr, ctx, baseline_every=16):
+    """512 independent SPD block Laplacians (B=4, 1e3..5e4 block rows, 4 RHS): IChol0-pCG to
+    ||r|| <= 1e-10 ||b|| in one persistent graph-replayed launch with autotuned tiles, and
+    the same systems of a subset through one device pCG launch sequence per system."""
+    probs = pr.batch_blocklap(nsys=512, lo=1e3, hi=5e4, seed=2, nrhs=4)
+    rp, ci, vals, sysp = bsp.block_diagonal(probs)
+    n = int(sysp[-1])
+    Bh = np.stack([np.concatenate([q.meta["rhs"][k] for q in probs]) for k in range(4)])
+    A = bsp.BSR(ctx, n, n, 4, rp, ci, vals)
+    del rp, ci, vals
+    t0 = time.perf_counter()
+    bat = bsp.Batch(ctx, A, sysp, 4)
+    setup = time.perf_counter() - t0
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    wg, r = bat.autotune(B, X, probe_iters=8)
+    bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    Xh = X.numpy()
+    worst = 0.0
+    for k in range(4):
+        xk = ctx.to_device(Xh[k])
+        yk = ctx.zeros(Xh[k].shape)
+        bsp.bsmv(ctx, A, xk, yk)
+        rk = (Bh[k] - yk.numpy()).reshape(-1, 4)
+        bk = Bh[k].reshape(-1, 4)
+        rn = np.sqrt(np.add.reduceat((rk ** 2).sum(1), sysp[:-1]))
+        bn = np.sqrt(np.add.reduceat((bk ** 2).sum(1), sysp[:-1]))
+        worst = max(worst, float((rn / bn).max()))
+    sel = list(range(0, 512, baseline_every))
+    per_sys = 0.0
+    for s in sel:
+        q = probs[s]
+        Ai = bsp.BSR.from_problem(ctx, q)
+        M = bsp.IChol0(ctx, Ai)
+        for k in range(4):
+            bk = ctx.to_device(q.meta["rhs"][k])
+            ctx.sync()
+            t0 = time.perf_counter()
+            bsp.pCG(ctx, Ai, bk, 1e-10, M, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+            ctx.sync()
+            per_sys += time.perf_counter() - t0
+    sub = [probs[s] for s in sel]
+    rp2, ci2, v2, sys2 = bsp.block_diagonal(sub)
+    A2 = bsp.BSR(ctx, int(sys2[-1]), int(sys2[-1]), 4, rp2, ci2, v2)
+    bat2 = bsp.Batch(ctx, A2, sys2, 4)
+    bat2.set_tiles(wg, r)
+    B2 = ctx.to_device(np.stack([np.concatenate([q.meta["rhs"][k] for q in sub]) for k in range(4)]))
+    X2 = ctx.zeros(B2.shape)
+    bat2.pcg(B2, X2, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    _, _, ms2, _ = bat2.pcg(B2, X2, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    out = {"workload": "512 SPD block Laplacians B=4, 1e3..5e4 block rows (log-uniform), 4 RHS, IChol0-pCG to 1e-10",
+           "block_rows": n, "batched_s": round(ms / 1e3, 4), "tiles": {"wg": wg, "rhs_per_item": r},
+           "setup_s": round(setup, 2), "converged": rc == 0, "iters_max": int(its.max()),
+           "iters_mean": round(float(its.mean()), 1), "true_relres_max": worst,
+           "subset_systems": len(sel), "subset_per_system_launches_s": round(per_sys, 3),
+           "subset_batched_s": round(ms2 / 1e3, 4), "subset_speedup": round(per_sys / (ms2 / 1e3), 1)}
+    log(f"batched 512 systems {ms:.0f} ms; subset per-system {per_sys:.2f} s vs batched {ms2:.0f} ms")
+    return out
+
+
+def bench_learned(bsp, pr, ctx, train_only=False):
+    """3D random-coefficient diffusion 48^3 (seed 3): the learned ILU0 correction trained on
+    the GPU from Krylov vectors of FP64 ILU0-pGMRES solves of 16 seeded 24^3 problems
+    (2000 pairs), then FGMRES(30) with it (ILU0 fallback armed) against the same FGMRES
+    without the network and against plain FP64 ILU0-pGMRES."""
+    from blocksp_amd import learned
+    tr = [pr.dg3d_grf(24, 24, 24, seed=10 + s) for s in range(16)]
+    t0 = time.perf_counter()
+    data = learned.training_set(ctx, tr, count=250)
+    ctx.sync()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net, losses = learned.train(ctx, data, epochs=20, batch=16, lr=1e-3)
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    out = {"workload": "dg3d_grf 48^3 (110592 block rows, bs 8), lognormal coefficient, correlation 4 cells, seed 3",
+           "pairs": int(sum(d[4] for d in data)), "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+           "loss_first_epoch": round(float(losses[0]), 5), "loss_last_epoch": round(float(losses[-1]), 5)}
+    if train_only:
+        return out
+    p = pr.dg3d_grf(48, 48, 48, seed=3)
+    A = bsp.BSR.from_problem(ctx, p)
+    b = ctx.to_device(p.b)
+    M0 = bsp.ILU0(ctx, A)
+    _, stp, _ = bsp.pGMRES(ctx, A, b, 1e-10, M0, maxIters=20000, restart=30, k=10, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    t0 = time.perf_counter()
+    Ml = bsp.LearnedILU0(ctx, A, net, precision=bsp.PREC_F32)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    Mn = bsp.ILU0Ex(ctx, A, precision=bsp.PREC_F32)
+    _, stl, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Ml, fallback=Mn)
+    _, stn, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mn)
+    Mc = bsp.ILU0Ex(ctx, A, colour=pr.parity_colours((48, 48, 48)), precision=bsp.PREC_F32)
+    _, stc, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mc)
+    out.update({"plain_fp64": {"ms": round(stp.solve_ms, 1), "iters": stp.iters, "relres": stp.true_relres},
+                "learned_fgmres": {"ms": round(stl.solve_ms, 1), "iters": stl.iters, "relres": stl.true_relres,
+                                   "fallback": stl.fallback, "setup_s": round(t_setup, 3)},
+                "same_fgmres_without_net": {"ms": round(stn.solve_ms, 1), "iters": stn.iters,
+                                            "relres": stn.true_relres},
+                "coloured_fgmres_without_net": {"ms": round(stc.solve_ms, 1), "iters": stc.iters,
+                                                "relres": stc.true_relres},
+                "iteration_reduction": round(1 - stl.iters / stn.iters, 3),
+                "learned_speedup_vs_plain_fp64": round(stp.solve_ms / stl.solve_ms, 2)})
+    log(f"learned FGMRES {stl.iters} its {stl.solve_ms:.0f} ms vs without net {stn.iters} its; "
+        f"plain FP64 {stp.iters} its {stp.solve_ms:.0f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +283,9 @@ def main():
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
+    ap.add_argument("--skip-extras", action="store_true", help="skip the 3D BF16, batched and learned workloads")
+    ap.add_argument("--only", choices=["bf16", "batch", "learned"], default=None,
+                    help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the solve path is built and "
@@ -120,6 +295,16 @@ def main():
 
     import blocksp_amd as bsp
     from blocksp_amd import problems as pr
+
+    if args.only:
+        ctx = bsp.Context(0)
+        if args.verbose:
+            ctx.set_option("verbose", args.verbose)
+        fn = {"bf16": lambda: bench_bf16(bsp, pr, ctx, solve=False),
+              "batch": lambda: bench_batch(bsp, pr, ctx, baseline_every=512),
+              "learned": lambda: bench_learned(bsp, pr, ctx, train_only=True)}[args.only]
+        print(json.dumps(fn()))
+        return
 
     t0 = time.perf_counter()
     p = pr.dg2d_veclap(args.nx, args.nx, seed=1)
@@ -227,6 +412,11 @@ def main():
         cpu = cpu_baseline(p, plain["iters"] if plain else stats[-1]["iters"])
         log(f"cpu baseline {cpu['value']:.1f}s (est.)")
 
+    extras = None
+    if not args.skip_extras:
+        extras = {"dg3d_bf16_mixed": bench_bf16(bsp, pr, ctx), "batched_ichol0_pcg": bench_batch(bsp, pr, ctx),
+                  "learned_ilu0": bench_learned(bsp, pr, ctx)}
+
     value = ms_per_step / 1e3
     out = {
         "metric": METRIC,
@@ -256,6 +446,7 @@ def main():
         "bsmv_GBps": round(bsmv_gbps, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "workloads": extras,
     }
     print(json.dumps(out))
 
