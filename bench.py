@@ -298,6 +298,9 @@ def main():
 
     if args.only:
         ctx = bsp.Context(0)
+        # profiling runs launch sweeps level by level: rocprofv3's kernel tracing has crashed
+        # (host SIGSEGV inside the tool) while a sweep graph was being captured
+        ctx.set_option("sweep_graphs", 0)
         if args.verbose:
             ctx.set_option("verbose", args.verbose)
         fn = {"bf16": lambda: bench_bf16(bsp, pr, ctx, solve=False),

@@ -93,6 +93,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "bsmv_variant") {
         BSP_CHECK(value >= 0 && value <= 2, BSP_E_INVALID, "bsmv_variant must be 0, 1 or 2");
         ctx->bsmv_variant = (int)value;
+    } else if (k == "bf16_variant") {
+        BSP_CHECK(value >= 0 && value <= 3, BSP_E_INVALID, "bf16_variant must be 0, 1, 2 or 3");
+        ctx->bf16_variant = (int)value;
     } else if (k == "sweep_graphs") {
         ctx->sweep_graphs = value != 0;
     } else if (k == "verbose") {

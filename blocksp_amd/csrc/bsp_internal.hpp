@@ -103,6 +103,10 @@ struct bsp_ctx {
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
     int bsmv_variant = 0;            // 0: lane = block row r, two blocks per step; 1: non-temporal matrix
                                      // loads (measured 2.2x slower); 2: coalesced 16-byte chunks per lane
+    int bf16_variant = 0;            // BF16 MFMA SpMM on the 64^3 DG matrix, 32 RHS: 0 loads of 4 slots of a
+                                     // row pair issued before their MFMAs + streaming matrix loads / result
+                                     // stores (176 us); 1 slot by slot, streaming (193 us); 2 slot by slot,
+                                     // cached (211 us); 3 as 0 with 8 slots per batch (182 us)
     // reduction scratch
     bsp::DevMem red_partials;        // double [kMaxRedBlocks]
     bsp::DevMem red_counter;         // uint32 [kRedSlots]

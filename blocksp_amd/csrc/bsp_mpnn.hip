@@ -155,8 +155,17 @@ __global__ void k_reduce_wgrad(int nblk, int out, int in_total, const float* __r
     const int total = out * (in_total + 1);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * total + t];
+    // four independent partial sums (loads in flight), combined in a fixed order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = 0;
+    for (; b + 4 <= nblk; b += 4) {
+        s0 += partial[(size_t)b * total + t];
+        s1 += partial[(size_t)(b + 1) * total + t];
+        s2 += partial[(size_t)(b + 2) * total + t];
+        s3 += partial[(size_t)(b + 3) * total + t];
+    }
+    for (; b < nblk; ++b) s0 += partial[(size_t)b * total + t];
+    const double s = (s0 + s1) + (s2 + s3);
     const int o = t / (in_total + 1), c = t - o * (in_total + 1);
     if (c < in_total) gW[(size_t)o * in_total + c] = (float)s;
     else gb[o] = (float)s;

@@ -12,7 +12,7 @@ constexpr int kLayers = 3;
 // block size <= 8, ||kAlpha tanh(D_i)||_2 <= 0.4, so F_ii stays invertible
 constexpr float kAlpha = 0.05f;
 constexpr float kBeta = 0.25f;   // off-diagonal scaling: F_ij = LU_ij (1 + kBeta tanh g_ij)
-constexpr int kMaxPartialBlocks = 1024;
+constexpr int kMaxPartialBlocks = 256;   // workgroups (partials) of a weight-gradient reduction
 
 // One input part of a dense layer: row r reads x[(idx ? idx[r] : r) * stride + 0 .. dim-1].
 struct Part {

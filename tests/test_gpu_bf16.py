@@ -23,7 +23,7 @@ def _random_bsr(n, seed):
     rng = np.random.default_rng(seed)
     rows = []
     for i in range(n):
-        k = 0 if i == 5 else rng.integers(1, 8)         # row 5 is empty
+        k = 0 if i == 5 else (11 if i == 7 else rng.integers(1, 8))   # row 5 empty, row 7 longer than 8
         rows.append(np.sort(rng.choice(n, size=k, replace=False)))
     rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
     ci = np.concatenate(rows).astype(np.int32)
@@ -40,8 +40,17 @@ def _ref(rp, ci, Ab, Xb, n, nrhs):
     return Y
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,nrhs", [(37, 32), (36, 16), (11, 48)])
-def test_bf16_spmm_matches_numpy(ctx, n, nrhs):
+def test_bf16_spmm_matches_numpy(ctx, n, nrhs, variant):
+    ctx.set_option("bf16_variant", variant)
+    try:
+        _check_spmm(ctx, n, nrhs)
+    finally:
+        ctx.set_option("bf16_variant", 0)
+
+
+def _check_spmm(ctx, n, nrhs):
     rp, ci, vals = _random_bsr(n, seed=n + nrhs)
     A = bsp.BSR(ctx, n, n, 8, rp, ci, vals)
     X = np.random.default_rng(1).standard_normal((n * 8, nrhs))     # RHS-minor
