@@ -326,6 +326,19 @@ def main():
     roofline = {"kernel": "k_bsmv<8>", "bound": "hbm", "achieved": round(bsmv_gbps, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(bsmv_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "algorithmic_bytes": bsmv_bytes, "avg_us": round(bsmv_ms * 1e3, 2)}
+    # bsmm(A, A): symbolic pass on the host (first-touch order), numeric pass on the device
+    t0 = time.perf_counter()
+    A2 = bsp.bsmm(ctx, A, A)
+    ctx.sync()
+    t_bsmm_total = time.perf_counter() - t0
+    t_num = timed_ms(ctx, lambda: bsp.bsmm_numeric(ctx, A, A, A2), 5)
+    n_pairs = int(sum((p.rowptr[c + 1] - p.rowptr[c]) for c in p.colidx))
+    bsmm_bytes = 2 * p.nnzb * p.bs * p.bs * 8 + A2.nnzb * p.bs * p.bs * 8 + n_pairs * 8 + (A2.nnzb + 1) * 4
+    bsmm = {"nnzb_C": A2.nnzb, "pairs": n_pairs, "first_call_s": round(t_bsmm_total, 3),
+            "numeric_us": round(t_num * 1e3, 1), "numeric_GBps_unique": round(bsmm_bytes / t_num / 1e6, 1),
+            "numeric_GFLOPs": round(2.0 * n_pairs * p.bs ** 3 / t_num / 1e6, 1)}
+    log(f"bsmm(A,A): first call {t_bsmm_total:.2f}s (symbolic on host), numeric {t_num * 1e3:.0f} us")
+    del A2
     if args.bsmv_only:
         # calibration kernel of known traffic for the FETCH_SIZE / WRITE_SIZE counters:
         # axpy on 2^21 doubles reads 2 x 16 MiB and writes 16 MiB per launch
@@ -447,6 +460,7 @@ def main():
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(plain["time_s"] / value, 3) if plain else None,
         "bsmv_GBps": round(bsmv_gbps, 1),
+        "bsmm_AA": bsmm,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "workloads": extras,

@@ -72,6 +72,8 @@ _SIGS = {
     "bsp_bsmv_transpose": [_VP, _VP, _VP, _VP, C.c_int],
     "bsp_bsmm_rhs": [_VP, _VP, C.c_int, _VP, _VP],
     "bsp_bsmm": [_VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_bsmm_numeric": [_VP, _VP, _VP, _VP],
+    "bsp_bsmv_dim_transpose": [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int],
     "bsp_dot": [_VP, C.c_int64, _VP, _VP, C.POINTER(C.c_double)],
     "bsp_nrm2": [_VP, C.c_int64, _VP, C.POINTER(C.c_double)],
     "bsp_infnorm": [_VP, C.c_int64, _VP, C.POINTER(C.c_double)],
@@ -264,6 +266,16 @@ class BSR:
     def from_problem(cls, ctx, p):
         return cls(ctx, p.n, p.n, p.bs, p.rowptr, p.colidx, p.vals)
 
+    @classmethod
+    def _wrap(cls, ctx, h, keep=()):
+        """A BSR object around a handle the library created (e.g. by bsp_bsmm)."""
+        self = cls.__new__(cls)
+        self.ctx, self.h, self._keep = ctx, h, keep
+        n, m, bs, nz = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        ctx.check(ctx.L.bsp_bsr_info(h, C.byref(n), C.byref(m), C.byref(bs), C.byref(nz)), "bsr_info")
+        self.n_rows, self.n_cols, self.bs, self.nnzb = n.value, m.value, bs.value, nz.value
+        return self
+
     def download(self):
         rp = np.empty(self.n_rows + 1, np.int32)
         ci = np.empty(self.nnzb, np.int32)
@@ -287,6 +299,27 @@ def bsmv(ctx, A: BSR, x: DeviceArray, y: DeviceArray, accumulate=False):
 
 def bsmv_dim(ctx, A: BSR, D, dim, x: DeviceArray, y: DeviceArray, accumulate=False):
     ctx.check(ctx.L.bsp_bsmv_dim(ctx.h, A.h, int(D), int(dim), x.ptr, y.ptr, int(accumulate)), "bsmv_dim")
+
+
+def bsmv_transpose(ctx, A: BSR, x: DeviceArray, y: DeviceArray, accumulate=False):
+    """y (+)= A^T x  (bsmv(..., transpose=true), blocksp_multiply.hpp:196-204, 236-245)."""
+    ctx.check(ctx.L.bsp_bsmv_transpose(ctx.h, A.h, x.ptr, y.ptr, int(accumulate)), "bsmv_transpose")
+
+
+def bsmv_dim_transpose(ctx, A: BSR, D, dim, x: DeviceArray, y: DeviceArray, accumulate=False):
+    ctx.check(ctx.L.bsp_bsmv_dim_transpose(ctx.h, A.h, int(D), int(dim), x.ptr, y.ptr, int(accumulate)),
+              "bsmv_dim_transpose")
+
+
+def bsmm(ctx, A: BSR, B: BSR) -> BSR:
+    """C = A B on the device (bsmm, blocksp_multiply.hpp:51-136); C keeps A and B alive."""
+    h = C.c_void_p()
+    ctx.check(ctx.L.bsp_bsmm(ctx.h, A.h, B.h, C.byref(h)), "bsmm")
+    return BSR._wrap(ctx, h, keep=(A, B))
+
+
+def bsmm_numeric(ctx, A: BSR, B: BSR, Cm: BSR):
+    ctx.check(ctx.L.bsp_bsmm_numeric(ctx.h, A.h, B.h, Cm.h), "bsmm_numeric")
 
 
 def bsmm_rhs(ctx, A: BSR, nrhs, X: DeviceArray, Y: DeviceArray):

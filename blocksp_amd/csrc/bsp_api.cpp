@@ -247,8 +247,10 @@ int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev) {
     if (!ctx || !A || !vals_dev) return BSP_E_INVALID;
     const size_t bytes = sizeof(double) * A->nnzb * A->bs * A->bs;
     BSP_HIP(hipMemcpyAsync(A->val.p, vals_dev, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    A->h_val.resize(A->nnzb * A->bs * A->bs);
     BSP_HIP(hipMemcpyAsync(A->h_val.data(), vals_dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
     BSP_HIP(hipStreamSynchronize(ctx->stream));
+    A->h_val_stale = false;
     // reduced-precision copies and the transpose are rebuilt from the new values on next use
     A->val32.release();
     A->val16.release();
@@ -276,20 +278,10 @@ int bsp_bsmv_dim(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x
                            y, accumulate);
 }
 
-int bsp_bsmv_transpose(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int accumulate) {
-    if (!ctx || !A) return BSP_E_INVALID;
-    return fail(ctx, BSP_E_UNSUPPORTED, "bsmv_transpose is not implemented yet");
-}
-
 int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X, double* Y) {
     if (!ctx || !A) return BSP_E_INVALID;
     BSP_CHECK(nrhs >= 1 && X && Y && X != Y, BSP_E_INVALID, "bad multi-RHS arguments");
     return launch_bsmm_rhs(ctx, A->n_rows, A->bs, nrhs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), X, Y);
-}
-
-int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C) {
-    if (!ctx || !A || !B || !C) return BSP_E_INVALID;
-    return fail(ctx, BSP_E_UNSUPPORTED, "bsmm is not implemented yet");
 }
 
 // ---------------------------------------------------------------------------

@@ -7,6 +7,7 @@
 // and its level sets restricted to a system are that system's level sets.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <map>
 #include <numeric>
 #include <tuple>
@@ -254,6 +255,13 @@ int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* 
         probe_iters = std::max(1, probe_iters);
         double best = 1e300;
         std::pair<int, int> choice{b->wg, b->R};
+        // parity gate: after probe_iters iterations every candidate must reproduce the first
+        // candidate's per-(system, RHS) residual norms and ||X|| (reduction orders differ
+        // between tiles, so equal up to rounding)
+        const size_t cnt = (size_t)b->nsys * b->nrhs;
+        const int64_t xlen = b->nrows * b->bs * b->nrhs;
+        std::vector<double> ref_res, res(2 * cnt);
+        double ref_x = -1.0;
         for (int R : {1, 2, 4}) {
             if (b->nrhs % R || !batch::supported(b->bs, R)) continue;
             for (int wg : {128, 256, 512, 1024}) {
@@ -264,12 +272,24 @@ int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* 
                 if (rc) return rc;
                 const double ms = time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);
                 if (rc) return rc;
+                BSP_HIP(hipMemcpy(res.data(), b->res.p, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost));
+                double xn = 0.0;
+                if ((rc = reduce(ctx, 1, xlen, X_dev, X_dev, SLOT_TMP0, &xn))) return rc;
+                bool same = true;
+                if (ref_x < 0.0) {
+                    ref_res = res;
+                    ref_x = xn;
+                } else {
+                    same = std::fabs(xn - ref_x) <= 1e-9 * std::max(ref_x, 1e-300);
+                    for (size_t t = 0; t < cnt && same; ++t)
+                        same = std::fabs(res[2 * t] - ref_res[2 * t]) <= 1e-6 * std::max(ref_res[2 * t], 1e-300);
+                }
                 if (ctx->verbose > 0) {
-                    std::fprintf(stderr, "[batch autotune] wg %d rhs/item %d: %.3f ms for %d iterations\n", wg, R, ms,
-                                 probe_iters);
+                    std::fprintf(stderr, "[batch autotune] wg %d rhs/item %d: %.3f ms for %d iterations%s\n", wg, R, ms,
+                                 probe_iters, same ? "" : " (rejected: results differ)");
                     std::fflush(stderr);
                 }
-                if (ms < best) {
+                if (same && ms < best) {
                     best = ms;
                     choice = {wg, R};
                 }

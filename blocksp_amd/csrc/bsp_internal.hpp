@@ -128,9 +128,14 @@ struct bsp_bsr {
     int64_t nnzb = 0;
     bsp::DevMem rp, ci, val;
     std::vector<int32_t> h_rp, h_ci;   // host copy of the structure
-    std::vector<double> h_val;         // host copy of the values (factorisations start here)
+    mutable std::vector<double> h_val; // host copy of the values (factorisations start here)
+    mutable bool h_val_stale = false;  // values produced on the device (bsmm): copy back on first host use
     // lazily built transpose (for bsmv_transpose)
-    std::unique_ptr<bsp_bsr> transposed;
+    mutable std::unique_ptr<bsp_bsr> transposed;
+    // C = A B from bsp_bsmm: contributing (A block, B block) pairs per block of C
+    bsp::DevMem mm_pair_ptr, mm_pairs;
+    const bsp_bsr* mm_A = nullptr;
+    const bsp_bsr* mm_B = nullptr;
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
     mutable bsp::DevMem val16;         // BF16 copy of the values (MFMA block SpMM)
 };
@@ -229,6 +234,10 @@ int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std:
                      std::vector<int32_t>& pos, std::vector<double>& LU);
 int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim, const std::vector<int32_t>& pos,
                std::vector<double>& LU, bool f32, bsp_precond** out);
+
+// Host copy of device-produced values (bsp_bsmm) and the explicit transpose (bsp_spgemm.hip).
+int sync_host_values(bsp_ctx* ctx, const bsp_bsr* A);
+int ensure_transpose(bsp_ctx* ctx, const bsp_bsr* A);
 
 // ||b - A x|| / bnorm (FP64, on the device) and the sync-free timeout check (bsp_krylov.cpp)
 int true_relres_ext(bsp_ctx* ctx, const bsp_bsr* A, const double* b, const double* x, double* work, double bnorm,

@@ -128,6 +128,7 @@ void split_strict(int n, int bs, const std::vector<int32_t>& rp, const std::vect
 }
 
 int diag_inverses(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, std::vector<double>& dinv) {
+    if (int rc = sync_host_values(ctx, A)) return rc;
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
     dinv.assign(n * b2, 0.0);
@@ -249,6 +250,7 @@ namespace bsp {
 int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std::vector<int32_t>& elim,
                      std::vector<int32_t>& pos, std::vector<double>& LU) {
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
+    if (int rc = sync_host_values(ctx, A)) return rc;
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
     const auto& rp = A->h_rp;
@@ -400,6 +402,7 @@ int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out)
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "IChol0 needs a square matrix");
     BSP_HIP(hipSetDevice(ctx->device));
+    if (int rc = sync_host_values(ctx, A)) return rc;
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
     // lowTri (blocksp_containers.hpp:277-285): blocks with col <= row, in row order

@@ -534,7 +534,9 @@ void bsmv_device(solArray<T, B, D>& b, const BlockSp<T, M>& A, const solArray<T,
     dx.upload(x.data());
     if (accumulate) dy.upload(b.data());
     int rc;
-    if (dim_mode) {
+    if (dim_mode && transpose) {
+        rc = bsp_bsmv_dim_transpose(gpu::ctx(), dA->h, D, dim, dx.get(), dy.get(), accumulate ? 1 : 0);
+    } else if (dim_mode) {
         rc = bsp_bsmv_dim(gpu::ctx(), dA->h, D, dim, dx.get(), dy.get(), accumulate ? 1 : 0);
     } else if (transpose) {
         rc = bsp_bsmv_transpose(gpu::ctx(), dA->h, dx.get(), dy.get(), accumulate ? 1 : 0);
@@ -552,10 +554,7 @@ template <typename T, int B, int D = 1>
 solArray<T, B, D> bsmv(const BlockSp<T, B>& A, const solArray<T, B, D>& x, int dim = -1, bool transpose = false) {
     solArray<T, B, D> b(transpose ? A.n() : A.m());
     if (D == 1 && dim < 0) mult_detail::bsmv_device(b, A, x, false, transpose, 0, -1);
-    else {
-        if (transpose) throw Error(BSP_E_UNSUPPORTED, "per-dimension transpose bsmv is not supported");
-        mult_detail::bsmv_device(b, A, x, false, false, 1, dim);
-    }
+    else mult_detail::bsmv_device(b, A, x, false, transpose, 1, dim);
     return b;
 }
 
@@ -573,7 +572,7 @@ template <typename T, int B, int D = 1>
 void bsmv(solArray<T, B, D>& b, const BlockSp<T, B>& A, const solArray<T, B, D>& x, int dim = -1,
           bool transpose = false) {
     if (D == 1 && dim < 0) mult_detail::bsmv_device(b, A, x, true, transpose, 0, -1);
-    else mult_detail::bsmv_device(b, A, x, true, false, 1, dim);
+    else mult_detail::bsmv_device(b, A, x, true, transpose, 1, dim);
 }
 
 template <typename T, int B, int D>

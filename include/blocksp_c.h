@@ -91,14 +91,23 @@ int bsp_bsmv(bsp_ctx* ctx, const bsp_bsr* A, const double* x_dev, double* y_dev,
  * 252-285); A has bs = B, vectors have n*B*D entries; dim < 0 multiplies all dimensions. */
 int bsp_bsmv_dim(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x_dev, double* y_dev,
                  int accumulate);
-/* y (+)= A^T x (transpose=true in blocksp_multiply.hpp:196-204, 236-245). */
+/* y (+)= A^T x (transpose=true in blocksp_multiply.hpp:196-204, 236-245); A^T is built once
+ * per matrix (rows of A^T in ascending source row) and reused. */
 int bsp_bsmv_transpose(bsp_ctx* ctx, const bsp_bsr* A, const double* x_dev, double* y_dev,
                        int accumulate);
+/* Per-dimension transposed product: bsmv(A, x, dim, transpose=true) for BlockSp<T,B> and
+ * solArray<T,B,D> (blocksp_multiply.hpp:365-380). */
+int bsp_bsmv_dim_transpose(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x_dev, double* y_dev,
+                           int accumulate);
 /* Multi-RHS product Y = A X with nrhs right-hand sides stored RHS-minor
  * (X[(row*bs + r)*nrhs + k]); no single reference call, it is nrhs bsmv calls. */
 int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X_dev, double* Y_dev);
-/* C = A * B, structure in the reference's first-touch order (blocksp_multiply.hpp:51-97). */
+/* C = A * B (bsmm, blocksp_multiply.hpp:51-136): structure in the reference's first-touch
+ * order (symbolic pass on the host, bit-exact), values computed on the device.  A and B must
+ * outlive C for bsp_bsmm_numeric. */
 int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C);
+/* Recomputes C = A * B on the device for new values of A and B (same structures). */
+int bsp_bsmm_numeric(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr* C);
 /* BF16 block SpMM on the matrix cores (no reference counterpart): Yb = A Xb for nrhs
  * right-hand sides (a multiple of 16), 8 x 8 blocks only, BF16 values (rounded to nearest
  * even from A's FP64 values on first use) and BF16 right-hand sides, FP32 accumulation
@@ -293,8 +302,9 @@ int bsp_batch_destroy(bsp_batch* b);
  * (1, 2 or 4, dividing nrhs). */
 int bsp_batch_set_tiles(bsp_ctx* ctx, bsp_batch* b, int wg_size, int rhs_per_item);
 /* Times every tile choice on probe_iters iterations of the batch (B_dev/X_dev as for
- * bsp_batch_pcg; X_dev is overwritten), keeps the fastest, and caches the choice per
- * (device, block size, nrhs, nsys, rows) for later batches. */
+ * bsp_batch_pcg; X_dev is overwritten), keeps the fastest of those whose per-(system, RHS)
+ * residual norms and ||X|| reproduce the first candidate's (up to rounding), and caches the
+ * choice per (device, block size, nrhs, nsys, rows) for later batches. */
 int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, int probe_iters,
                        int* wg_size, int* rhs_per_item);
 /* Solves A_s X_{s,k} = B_{s,k} for every system s and right-hand side k.  B_dev and X_dev

@@ -109,3 +109,27 @@ def test_coupled_systems_are_rejected(ctx):
     bad[1] += 1                        # moves a row of system 0 into system 1
     with pytest.raises(bsp.BspError):
         bsp.Batch(ctx, A, bad, 4)
+
+
+@pytest.mark.parametrize("wg,r", [(256, 4), (1024, 2), (128, 1)])
+def test_batched_equals_one_launch_per_system_bitwise(ctx, wg, r):
+    """The batched kernel's per-item work depends only on that system and the tiles: each
+    system launched on its own gives bitwise the same solution."""
+    probs, A, sys, Bh = _batch(ctx, nsys=6, seed=13)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    bat.set_tiles(wg, r)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    its, _, _, rc = bat.pcg(B, X, 1e-10, maxIters=2000)
+    assert rc == 0
+    Xh = X.numpy()
+    for s, p in enumerate(probs):
+        As = bsp.BSR.from_problem(ctx, p)
+        one = bsp.Batch(ctx, As, np.array([0, p.n], np.int32), 4)
+        one.set_tiles(wg, r)
+        Bs = ctx.to_device(np.stack([p.meta["rhs"][k] for k in range(4)]))
+        Xs = ctx.zeros(Bs.shape)
+        its1, _, _, rc1 = one.pcg(Bs, Xs, 1e-10, maxIters=2000)
+        assert rc1 == 0
+        np.testing.assert_array_equal(its1[0], its[s])
+        np.testing.assert_array_equal(Xs.numpy(), Xh[:, sys[s] * 4:sys[s + 1] * 4])

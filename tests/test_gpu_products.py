@@ -1,0 +1,112 @@
+"""Transposed block products and block SpGEMM (bsmm) against the oracle."""
+import numpy as np
+import pytest
+
+import blocksp_amd as bsp
+from blocksp_amd import problems as pr
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_bsr(nr, nc, bs, seed, dens=0.3):
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(nr):
+        k = int(rng.integers(0, max(1, int(dens * nc)) + 1)) if i != 2 else 0   # row 2 empty
+        rows.append(rng.permutation(rng.choice(nc, size=min(k, nc), replace=False)))   # unsorted columns
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32) if rp[-1] else np.zeros(0, np.int32)
+    return rp, ci, rng.standard_normal((len(ci), bs, bs))
+
+
+def _abs_transpose_bound(nr, nc, bs, rp, ci, v, x):
+    return po.bsmv_transpose(nr, nc, bs, rp, ci, np.abs(v), np.abs(x))
+
+
+@pytest.mark.parametrize("bs", [1, 3, 8])
+def test_bsmv_transpose_matches_oracle(ctx, bs):
+    nr, nc = 23, 17
+    rp, ci, v = _rand_bsr(nr, nc, bs, seed=bs)
+    A = bsp.BSR(ctx, nr, nc, bs, rp, ci, v)
+    x = np.random.default_rng(1).standard_normal(nr * bs)
+    y0 = np.random.default_rng(2).standard_normal(nc * bs)
+    bound = _abs_transpose_bound(nr, nc, bs, rp, ci, v, x)
+    for acc in (False, True):
+        xd, yd = ctx.to_device(x), ctx.to_device(y0)
+        bsp.bsmv_transpose(ctx, A, xd, yd, accumulate=acc)
+        ref = po.bsmv_transpose(nr, nc, bs, rp, ci, v, x, y=y0 if acc else None, accumulate=acc)
+        assert np.all(np.abs(yd.numpy() - ref) <= 1e-13 * (bound + (np.abs(y0) if acc else 0)) + 1e-300)
+
+
+def test_bsmv_dim_transpose_is_the_per_dimension_adjoint(ctx):
+    # <A^T x, y>_dim == <x, A y>_dim for every dimension, and dim = -1 covers all
+    n, B, D = 19, 4, 3
+    rp, ci, v = _rand_bsr(n, n, B, seed=7)
+    A = bsp.BSR(ctx, n, n, B, rp, ci, v)
+    rng = np.random.default_rng(3)
+    x, y = rng.standard_normal(n * B * D), rng.standard_normal(n * B * D)
+    xd, yd = ctx.to_device(x), ctx.to_device(y)
+    Aty, Ax = ctx.zeros(n * B * D), ctx.zeros(n * B * D)
+    bsp.bsmv_dim_transpose(ctx, A, D, -1, yd, Aty)
+    bsp.bsmv_dim(ctx, A, D, -1, xd, Ax)
+    at = Aty.numpy().reshape(n, D, B)
+    ax = Ax.numpy().reshape(n, D, B)
+    for d in range(D):
+        lhs = np.sum(x.reshape(n, D, B)[:, d] * at[:, d])
+        rhs = np.sum(ax[:, d] * y.reshape(n, D, B)[:, d])
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1)
+    # one dimension only: the others stay zero
+    out = ctx.zeros(n * B * D)
+    bsp.bsmv_dim_transpose(ctx, A, D, 1, yd, out)
+    o = out.numpy().reshape(n, D, B)
+    np.testing.assert_allclose(o[:, 1], at[:, 1], rtol=1e-14, atol=1e-14)
+    assert not o[:, 0].any() and not o[:, 2].any()
+
+
+@pytest.mark.parametrize("bs", [2, 8])
+def test_bsmm_structure_bit_exact_values_within_tolerance(ctx, bs):
+    n, m, k = 21, 15, 18
+    arp, aci, av = _rand_bsr(n, m, bs, seed=10 + bs)
+    brp, bci, bv = _rand_bsr(m, k, bs, seed=20 + bs)
+    A = bsp.BSR(ctx, n, m, bs, arp, aci, av)
+    Bm = bsp.BSR(ctx, m, k, bs, brp, bci, bv)
+    Cm = bsp.bsmm(ctx, A, Bm)
+    crp, cci, cv = Cm.download()
+    orp, oci, ov = po.bsmm(n, bs, arp, aci, av, brp, bci, bv)
+    np.testing.assert_array_equal(crp, orp)
+    np.testing.assert_array_equal(cci, oci)
+    _, _, bound = po.bsmm(n, bs, arp, aci, np.abs(av), brp, bci, np.abs(bv))
+    assert np.all(np.abs(cv - ov) <= 1e-13 * bound + 1e-300)
+    # new values, same structures: the numeric pass only
+    av2, bv2 = -2.0 * av + 1.0, 0.5 * bv
+    av2_d, bv2_d = ctx.to_device(av2), ctx.to_device(bv2)      # kept alive across the calls
+    ctx.check(ctx.L.bsp_bsr_set_values_dev(ctx.h, A.h, av2_d.ptr), "set_values")
+    ctx.check(ctx.L.bsp_bsr_set_values_dev(ctx.h, Bm.h, bv2_d.ptr), "set_values")
+    bsp.bsmm_numeric(ctx, A, Bm, Cm)
+    _, _, ov2 = po.bsmm(n, bs, arp, aci, av2, brp, bci, bv2)
+    _, _, bound2 = po.bsmm(n, bs, arp, aci, np.abs(av2), brp, bci, np.abs(bv2))
+    assert np.all(np.abs(Cm.download()[2] - ov2) <= 1e-13 * bound2 + 1e-300)
+
+
+def test_bsmm_square_of_dg_matrix_feeds_a_preconditioner(ctx):
+    p = pr.dg2d_veclap(12, 12, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    A2 = bsp.bsmm(ctx, A, A)
+    orp, oci, ov = po.bsmm(p.n, p.bs, p.rowptr, p.colidx, p.vals, p.rowptr, p.colidx, p.vals)
+    rp, ci, v = A2.download()
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(ci, oci)
+    assert np.abs(v - ov).max() <= 1e-12 * np.abs(ov).max()
+    # the device-computed values reach the host factorisation (ILU0 of A^2)
+    M = bsp.ILU0(ctx, A2)
+    _, _, lu = M.factor()
+    q = pr.Problem(p.n, p.bs, p.B, p.D, orp, oci, ov, p.b)
+    lu_ref = po.ilu0_factor(q)[0]
+    assert np.abs(lu - lu_ref).max() <= 1e-9 * np.abs(lu_ref).max()
+    # A^T on a symmetric matrix is A
+    x = ctx.to_device(p.b)
+    y1, y2 = ctx.zeros(p.n * p.bs), ctx.zeros(p.n * p.bs)
+    bsp.bsmv(ctx, A, x, y1)
+    bsp.bsmv_transpose(ctx, A, x, y2)
+    np.testing.assert_allclose(y2.numpy(), y1.numpy(), rtol=1e-12, atol=1e-12 * np.abs(y1.numpy()).max())
