@@ -15,6 +15,24 @@ int fail(const bsp_ctx* ctx, int code, const std::string& msg) {
 
 using namespace bsp;
 
+namespace bsp {
+
+int bsmv_matrix(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int accumulate) {
+    if (ctx->bsmv_variant >= 3 && bsmv_perm_supported(A->bs)) {
+        if (!A->valp.p) {
+            BSP_HIP(A->valp.alloc(sizeof(double) * std::max<int64_t>(A->nnzb * A->bs * A->bs, 1)));
+            int rc = launch_permute_blocks(ctx, A->nnzb, A->bs, A->val.as<double>(), A->valp.as<double>());
+            if (rc) return rc;
+        }
+        return launch_bsmv_perm(ctx, A->n_rows, A->bs, A->rp.as<int>(), A->ci.as<int>(), A->valp.as<double>(), x, y,
+                                accumulate);
+    }
+    return launch_bsmv(ctx, A->n_rows, A->bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), x, y,
+                       accumulate);
+}
+
+}  // namespace bsp
+
 extern "C" {
 
 int bsp_ctx_create(int device, void* stream, bsp_ctx** out) {
@@ -91,7 +109,7 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
         BSP_CHECK(value >= 1 && value <= 64, BSP_E_INVALID, "sweep_wg_per_cu must be in [1, 64]");
         ctx->sweep_wg_per_cu = (int)value;
     } else if (k == "bsmv_variant") {
-        BSP_CHECK(value >= 0 && value <= 2, BSP_E_INVALID, "bsmv_variant must be 0, 1 or 2");
+        BSP_CHECK(value >= 0 && value <= 5, BSP_E_INVALID, "bsmv_variant must be in [0, 5]");
         ctx->bsmv_variant = (int)value;
     } else if (k == "bf16_variant") {
         BSP_CHECK(value >= 0 && value <= 3, BSP_E_INVALID, "bf16_variant must be 0, 1, 2 or 3");
@@ -254,6 +272,7 @@ int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev) {
     // reduced-precision copies and the transpose are rebuilt from the new values on next use
     A->val32.release();
     A->val16.release();
+    A->valp.release();
     A->transposed.reset();
     return BSP_OK;
 }
@@ -266,8 +285,7 @@ int bsp_bsmv(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int acc
     if (!ctx || !A) return BSP_E_INVALID;
     BSP_CHECK(x && y, BSP_E_INVALID, "null vector");
     BSP_CHECK(x != y, BSP_E_INVALID, "bsmv: x and y must not alias");
-    return launch_bsmv(ctx, A->n_rows, A->bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), x, y,
-                       accumulate);
+    return bsmv_matrix(ctx, A, x, y, accumulate);
 }
 
 int bsp_bsmv_dim(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x, double* y, int accumulate) {

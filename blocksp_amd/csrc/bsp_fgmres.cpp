@@ -141,7 +141,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             rc = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
         } else {
             rc = launch_f2d(ctx, len, z, t64);
-            if (!rc) rc = launch_bsmv(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), t64, w, 0);
+            if (!rc) rc = bsmv_matrix(ctx, A, t64, w, 0);
         }
         if (rc) return rc;
         // MGS (FP64)

@@ -101,8 +101,11 @@ struct bsp_ctx {
     int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
-    int bsmv_variant = 0;            // 0: lane = block row r, two blocks per step; 1: non-temporal matrix
-                                     // loads (measured 2.2x slower); 2: coalesced 16-byte chunks per lane
+    int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +
+                                     // XCD-aware row ranges + streaming value loads (117.6 us, 6.04 TB/s, the
+                                     // default for even bs <= 8); 4 chunk-major + XCD (126.3 us); 3 chunk-major
+                                     // (127.1 us); 0 row-major lane = row (131.2 us); 1 row-major streaming
+                                     // (2.2x slower); 2 row-major coalesced 16-byte chunks (slower)
     int bf16_variant = 0;            // BF16 MFMA SpMM on the 64^3 DG matrix, 32 RHS: 0 loads of 4 slots of a
                                      // row pair issued before their MFMAs + streaming matrix loads / result
                                      // stores (176 us); 1 slot by slot, streaming (193 us); 2 slot by slot,
@@ -138,6 +141,7 @@ struct bsp_bsr {
     const bsp_bsr* mm_B = nullptr;
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
     mutable bsp::DevMem val16;         // BF16 copy of the values (MFMA block SpMM)
+    mutable bsp::DevMem valp;          // chunk-major copy of the values (bsmv variant 3)
 };
 
 struct bsp_precond {
@@ -172,6 +176,14 @@ int launch_bsmv_dim(const bsp_ctx* ctx, int n, int B, int D, int dim, const int*
 int launch_bsmm_rhs(const bsp_ctx* ctx, int n, int bs, int nrhs, const int* rp, const int* ci,
                     const double* val, const double* X, double* Y);
 int launch_block_diag(const bsp_ctx* ctx, int n, int bs, const double* dinv, double* x);
+// y (+)= A x with the context's bsmv variant (the chunk-major copy of A's values is built on
+// first use for variants >= 3) -- the product every solver uses (bsp_api.cpp)
+int bsmv_matrix(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int accumulate);
+// chunk-major copy of the values for k_bsmv_perm (even block sizes <= 8)
+bool bsmv_perm_supported(int bs);
+int launch_permute_blocks(const bsp_ctx* ctx, int64_t nnzb, int bs, const double* val, double* valp);
+int launch_bsmv_perm(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* valp,
+                     const double* x, double* y, int accumulate);
 
 // Deterministic reductions; the result lands in device scalar slot `slot`
 // (ctx->scalars) and, if host_out != nullptr, is copied back synchronously.

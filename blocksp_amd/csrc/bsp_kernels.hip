@@ -93,6 +93,82 @@ __global__ __launch_bounds__(kBlock) void k_bsmv(int n, const int* __restrict__ 
     y[(size_t)i * BS + r] = acc;
 }
 
+typedef double d2v_p __attribute__((ext_vector_type(2)));
+
+// Same mapping and arithmetic order, reading a copy of the values whose blocks are
+// stored chunk-major: entry (r, c) of a BS x BS block sits at ((c/2) * BS + r) * 2 + c%2,
+// so the k-th 16-byte load of the BS lanes of a block row covers one contiguous
+// BS*16-byte run (in the row-major layout the same load touches BS separate rows).
+// XCD: workgroups are remapped so each of the 8 XCDs walks one contiguous range of block
+// rows (neighbouring x blocks stay in that XCD's L2).  NT: streaming loads for the values.
+template <int BS, bool XCD, bool NT>
+__global__ __launch_bounds__(kBlock) void k_bsmv_perm(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      const double* __restrict__ valp, const double* __restrict__ x,
+                                                      double* __restrict__ y, int accumulate, int nwg) {
+    static_assert(BS % 2 == 0, "chunk-major layout needs an even block size");
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    int wg = blockIdx.x;
+    if constexpr (XCD) {
+        const int per = (nwg + 7) / 8;
+        wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (wg >= nwg) return;
+    }
+    const int wave = (wg * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    double acc = accumulate ? y[(size_t)i * BS + r] : 0.0;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    auto load_row = [&](int p, double (&a)[BS]) {
+        const double2* s2 = reinterpret_cast<const double2*>(valp + (size_t)p * BS * BS) + r;
+#pragma unroll
+        for (int k = 0; k < BS / 2; ++k) {
+            double2 t;
+            if constexpr (NT) {
+                const d2v_p u = __builtin_nontemporal_load(reinterpret_cast<const d2v_p*>(s2 + k * BS));
+                t.x = u.x;
+                t.y = u.y;
+            } else {
+                t = s2[k * BS];
+            }
+            a[2 * k] = t.x;
+            a[2 * k + 1] = t.y;
+        }
+    };
+    int p = p0;
+    for (; p + 1 < p1; p += 2) {
+        const int c0 = ci[p], c1 = ci[p + 1];
+        double a0[BS], a1[BS], x0[BS], x1[BS];
+        load_row(p, a0);
+        load_row(p + 1, a1);
+        load_vec<BS>(x + (size_t)c0 * BS, x0);
+        load_vec<BS>(x + (size_t)c1 * BS, x1);
+        acc += row_dot<BS>(a0, x0);
+        acc += row_dot<BS>(a1, x1);
+    }
+    if (p < p1) {
+        const int c0 = ci[p];
+        double a0[BS], x0[BS];
+        load_row(p, a0);
+        load_vec<BS>(x + (size_t)c0 * BS, x0);
+        acc += row_dot<BS>(a0, x0);
+    }
+    y[(size_t)i * BS + r] = acc;
+}
+
+__global__ void k_permute_blocks(int64_t total, int bs, const double* __restrict__ val, double* __restrict__ valp) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int b2 = bs * bs;
+    const int64_t p = t / b2;
+    const int e = (int)(t - p * b2);
+    const int r = e / bs, c = e - r * bs;
+    valp[p * b2 + ((c / 2) * bs + r) * 2 + (c & 1)] = val[t];
+}
+
 // Same mapping; the matrix values (read exactly once) are streamed with
 // non-temporal loads so that they do not evict x from the caches, and four
 // blocks of a row are loaded before any is used (more bytes in flight).
@@ -674,6 +750,50 @@ int launch_bsmv(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci,
         const int64_t waves = (n + rpw - 1) / rpw;
         hipLaunchKernelGGL(k_bsmv_generic, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, ctx->stream, n, bs, rp, ci,
                            val, x, y, accumulate);
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+bool bsmv_perm_supported(int bs) { return bs == 2 || bs == 4 || bs == 6 || bs == 8; }
+
+int launch_permute_blocks(const bsp_ctx* ctx, int64_t nnzb, int bs, const double* val, double* valp) {
+    const int64_t total = nnzb * bs * bs;
+    if (total <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_permute_blocks, dim3(grid_for(total)), dim3(kBlock), 0, ctx->stream, total, bs, val, valp);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+template <int BS>
+void launch_perm_bs(hipStream_t st, int mode, int n, const int* rp, const int* ci, const double* valp,
+                    const double* x, double* y, int accumulate) {
+    constexpr int rpw = kWave / BS;
+    const int64_t waves = (n + rpw - 1) / rpw;
+    const int nwg = (int)grid_for(waves * kWave);
+    const int grid8 = (nwg + 7) / 8 * 8;
+    if (mode == 1)
+        hipLaunchKernelGGL((k_bsmv_perm<BS, true, false>), dim3(grid8), dim3(kBlock), 0, st, n, rp, ci, valp, x, y,
+                           accumulate, nwg);
+    else if (mode == 2)
+        hipLaunchKernelGGL((k_bsmv_perm<BS, true, true>), dim3(grid8), dim3(kBlock), 0, st, n, rp, ci, valp, x, y,
+                           accumulate, nwg);
+    else
+        hipLaunchKernelGGL((k_bsmv_perm<BS, false, false>), dim3(nwg), dim3(kBlock), 0, st, n, rp, ci, valp, x, y,
+                           accumulate, nwg);
+}
+
+int launch_bsmv_perm(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* valp,
+                     const double* x, double* y, int accumulate) {
+    if (n <= 0) return BSP_OK;
+    // variant 3: plain order; 4: XCD-aware row ranges; 5: XCD-aware + streaming value loads
+    const int mode = ctx->bsmv_variant == 4 ? 1 : ctx->bsmv_variant == 5 ? 2 : 0;
+    switch (bs) {
+        case 2: launch_perm_bs<2>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
+        case 4: launch_perm_bs<4>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
+        case 6: launch_perm_bs<6>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
+        case 8: launch_perm_bs<8>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "chunk-major bsmv needs block size 2, 4, 6 or 8");
     }
     BSP_LAUNCH_CHECK();
     return BSP_OK;

@@ -148,7 +148,7 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
                                    hipMemcpyDeviceToDevice, ctx->stream));
             ++ctx->rec_count;
         }
-        rc = launch_bsmv(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), Q[iter], v, 0);
+        rc = bsmv_matrix(ctx, A, Q[iter], v, 0);
         if (rc) return rc;
         rc = bsp_precond_apply(ctx, M, v);
         ++applies;
@@ -413,7 +413,7 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
         int iter = 0;
         // rz in SLOT 0.. : s0 = rz, s1 = pAp, s3 = rz_new
         while (res > tolerance) {
-            rc = launch_bsmv(ctx, n, bs, rp, ci, av, p, Ap, 0);
+            rc = bsmv_matrix(ctx, A, p, Ap, 0);
             if (rc) return rc;
             rc = reduce(ctx, 0, len, r, z, SLOT_TMP0, nullptr);
             if (rc) return rc;

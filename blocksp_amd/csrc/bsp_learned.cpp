@@ -446,7 +446,7 @@ int bsp_mpnn_loss_grad(bsp_ctx* ctx, bsp_mpnn* net, bsp_mpnn_graph* g, const dou
     float* r32 = g->r32.as<float>();
     if ((rc = launch_d2f(ctx, len * K, Z_dev, z32))) return rc;
     for (int k = 0; k < K; ++k) {
-        if ((rc = launch_bsmv(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), Z_dev + k * len,
+        if ((rc = bsmv_matrix(ctx, A, Z_dev + k * len,
                               g->r64.as<double>(), 0)))
             return rc;
         if ((rc = launch_d2f(ctx, len, g->r64.as<double>(), r32 + k * len))) return rc;

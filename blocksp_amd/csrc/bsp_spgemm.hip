@@ -211,6 +211,7 @@ int bsp_bsmm_numeric(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr* 
     C->h_val_stale = true;
     C->val32.release();
     C->val16.release();
+    C->valp.release();
     C->transposed.reset();
     return BSP_OK;
 }

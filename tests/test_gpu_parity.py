@@ -313,7 +313,7 @@ def test_full_size_bsmv_properties(ctx):
     assert np.max(np.abs(y1.numpy() - p.bsmv(x1))) <= 1e-12 * np.max(np.abs(p.bsmv(np.abs(x1))))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("bs", [2, 3, 4, 8])
 def test_bsmv_variants(ctx, bs, variant):
     ctx.set_option("bsmv_variant", variant)
@@ -328,4 +328,4 @@ def test_bsmv_variants(ctx, bs, variant):
         absax = po.bsmv_arrays(n, bs, rp, ci, np.abs(v), np.abs(x))
         check_bsmv(yd.numpy(), yref, absax)
     finally:
-        ctx.set_option("bsmv_variant", 0)
+        ctx.set_option("bsmv_variant", 5)   # the default
