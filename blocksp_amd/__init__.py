@@ -73,6 +73,8 @@ _SIGS = {
     "bsp_bsmm_rhs": [_VP, _VP, C.c_int, _VP, _VP],
     "bsp_bsmm": [_VP, _VP, _VP, C.POINTER(_VP)],
     "bsp_bsmm_numeric": [_VP, _VP, _VP, _VP],
+    "bsp_pgmres_ritz": [_VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "bsp_arnoldi_eigenvalues": [_VP, _VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int, C.POINTER(C.c_int)],
     "bsp_bsmv_dim_transpose": [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int],
     "bsp_dot": [_VP, C.c_int64, _VP, _VP, C.POINTER(C.c_double)],
     "bsp_nrm2": [_VP, C.c_int64, _VP, C.POINTER(C.c_double)],
@@ -592,6 +594,24 @@ def recorded(ctx):
     c = C.c_int()
     ctx.check(ctx.L.bsp_ctx_recorded(ctx.h, C.byref(c)), "recorded")
     return c.value
+
+
+def pgmres_ritz(ctx, cap=256):
+    """(Harmonic) Ritz values of the last pGMRES run on ctx: (complex array, harmonic flag)."""
+    re, im = np.zeros(cap), np.zeros(cap)
+    cnt, harm = C.c_int(), C.c_int()
+    ctx.check(ctx.L.bsp_pgmres_ritz(ctx.h, _ptr(re), _ptr(im), cap, C.byref(cnt), C.byref(harm)), "pgmres_ritz")
+    k = min(cnt.value, cap)
+    return re[:k] + 1j * im[:k], bool(harm.value)
+
+
+def arnoldi_eigenvalues(ctx, A: BSR, iters=30, M=None, harmonic=False):
+    """arnoldi_eigenvalues (blocksp_solvers.hpp:298-333) on the device: Ritz values as a complex array."""
+    re, im = np.zeros(iters), np.zeros(iters)
+    cnt = C.c_int()
+    ctx.check(ctx.L.bsp_arnoldi_eigenvalues(ctx.h, A.h, None if M is None else M.h, int(iters), int(harmonic),
+                                            _ptr(re), _ptr(im), int(iters), C.byref(cnt)), "arnoldi_eigenvalues")
+    return re[:cnt.value] + 1j * im[:cnt.value]
 
 
 def pCG(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=1000, tol_mode=TOL_REFERENCE, x=None):

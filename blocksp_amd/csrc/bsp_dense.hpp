@@ -379,6 +379,27 @@ inline int invariant_subspace(int n, const double* A, int k, std::vector<double>
     return kout;
 }
 
+// Ritz values of an Arnoldi Hessenberg H (m x n row-major, m = n + 1 or m = n): the
+// eigenvalues of its square part H_n, or with `harmonic` the harmonic Ritz values, the
+// eigenvalues of H_n + h_{n+1,n}^2 H_n^{-T} e_n e_n^T (dense_solvers.hpp:919-965).
+inline bool ritz_values(int m, int n, const double* H, bool harmonic, std::vector<cplx>& ev) {
+    std::vector<double> Ht((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) Ht[(size_t)i * n + j] = H[(size_t)i * n + j];
+    if (harmonic && m > n) {
+        const double h_mn = H[(size_t)n * n + n - 1];
+        std::vector<double> w(n), e(n, 0.0);
+        e[n - 1] = 1.0;
+        if (!solve_transposed(n, Ht.data(), e.data(), w.data())) return false;
+        for (int i = 0; i < n; ++i) Ht[(size_t)i * n + n - 1] += h_mn * h_mn * w[i];
+    }
+    Schur S;
+    if (!S.compute(n, Ht.data())) return false;
+    ev.resize(n);
+    for (int i = 0; i < n; ++i) ev[i] = S.t(i, i);
+    return true;
+}
+
 // Deflated restart of GMRES (Morgan, GMRES-DR; deflated_restart.hpp:38-269).
 // In: H (m x n, m = n+1, row-major), res_vec (m), k.  Out: P (m x kp, row-major,
 // kp = kout+1) such that Q_new = Q P, Hnew (kp x kout), c_r (kp).  Returns kout or -1.

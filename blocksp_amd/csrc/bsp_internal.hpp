@@ -116,6 +116,10 @@ struct bsp_ctx {
     bsp::DevMem scalars;             // double [kScalarSlots]
     bsp::DevMem err_flag;            // uint32 [1], sync-free wait timeouts
     double* pinned = nullptr;        // host pinned [kScalarSlots]
+    // final Hessenberg of the last bsp_pgmres (m x n row-major) for the Ritz-value diagnostics
+    std::vector<double> last_H;
+    int last_Hm = 0, last_Hn = 0;
+    bool last_restarted = false;
     // Krylov basis recorder (bsp_ctx_record_basis)
     double* rec_buf = nullptr;
     int rec_cap = 0, rec_every = 1, rec_count = 0;

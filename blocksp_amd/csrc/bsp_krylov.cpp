@@ -273,6 +273,10 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     iters_out = tot;
     res_out = res;
     converged = res <= tolerance;
+    ctx->last_H = H;
+    ctx->last_Hm = Hm;
+    ctx->last_Hn = Hn;
+    ctx->last_restarted = restarted;
     return check_sweep_errors(ctx);
 }
 

@@ -597,6 +597,57 @@ namespace dense {
 /// pivoting, CholeskyFactor = Cholesky (SPD blocks).
 template <typename T, int N> struct LUFactor { static constexpr int kind = 0; };
 template <typename T, int N> struct CholeskyFactor { static constexpr int kind = 1; };
+
+/// Eigenvalue set of an Arnoldi Hessenberg (dense_solvers.hpp:909-1021): real and
+/// imaginary parts, magnitudes and the same summary statistics and printouts.
+template <typename T>
+struct Eigenvalues_Arr {
+    std::vector<double> WR, WI, WM;
+    bool harmonic_Ritz = false;
+    Eigenvalues_Arr() {}
+    Eigenvalues_Arr(std::vector<double> re, std::vector<double> im, bool harmonic)
+        : WR(std::move(re)), WI(std::move(im)), harmonic_Ritz(harmonic) {
+        WM.resize(WR.size());
+        for (size_t i = 0; i < WR.size(); ++i) WM[i] = std::sqrt(WR[i] * WR[i] + WI[i] * WI[i]);
+    }
+    const std::vector<double>& real() const { return WR; }
+    const std::vector<double>& imag() const { return WI; }
+    const std::vector<double>& mag() const { return WM; }
+    static double vmin(const std::vector<double>& v) { return *std::min_element(v.begin(), v.end()); }
+    static double vmax(const std::vector<double>& v) { return *std::max_element(v.begin(), v.end()); }
+    double min_real() const { return vmin(WR); }
+    double max_real() const { return vmax(WR); }
+    double min_imag() const { return vmin(WI); }
+    double max_imag() const { return vmax(WI); }
+    double min_mag() const { return vmin(WM); }
+    double max_mag() const { return vmax(WM); }
+    int arg_min_mag() const { return (int)(std::min_element(WM.begin(), WM.end()) - WM.begin()); }
+    int arg_max_mag() const { return (int)(std::max_element(WM.begin(), WM.end()) - WM.begin()); }
+    double cond_number() const { return max_mag() / min_mag(); }
+    void printEigVals() const {
+        auto pr = [](const std::vector<double>& v) {
+            for (double e : v) std::cout << e << ' ';
+            std::cout << std::endl;
+        };
+        if (harmonic_Ritz) std::cout << "Harmonic Ritz values" << std::endl;
+        std::cout << "Eigenvalues - real component" << std::endl;
+        pr(WR);
+        std::cout << "Eigenvalues - imaginary component" << std::endl;
+        pr(WI);
+        std::cout << "Eigenvalues - magnitude" << std::endl;
+        pr(WM);
+    }
+    void printEigInfo() const {
+        if (WR.empty()) return;
+        if (harmonic_Ritz) std::cout << "Harmonic Ritz info" << std::endl;
+        else std::cout << "Eigenvalue info" << std::endl;
+        std::cout << "min/max real eig " << min_real() << ' ' << max_real() << std::endl;
+        std::cout << "min/max img  eig " << min_imag() << ' ' << max_imag() << std::endl;
+        std::cout << "smallest magn eig " << min_mag() << std::endl;
+        std::cout << "largest magn eig  " << max_mag() << std::endl;
+        std::cout << "ratio/cond " << cond_number() << std::endl;
+    }
+};
 }  // namespace dense
 
 namespace preConditioners {
@@ -777,8 +828,49 @@ solArray<T, B, D> pGMRES(const BlockSp<T, B * D>& A, const solArray<T, B, D>& b,
         std::cout << "pGMRES iters " << st.iters << std::endl;
         if (st.iters > maxIters) std::cout << "maximum pGMRES iterations reached" << std::endl;
     }
-    if (printEigInfo) std::cout << "printEigInfo: Ritz-value diagnostics are not computed on the device" << std::endl;
+    if (printEigInfo) {
+        // eigenvalues of the final Hessenberg's square part; harmonic Ritz values if restarted
+        int cnt = 0, harm = 0;
+        gpu::check(gpu::ctx(), bsp_pgmres_ritz(gpu::ctx(), nullptr, nullptr, 0, &cnt, &harm), "pgmres_ritz");
+        std::vector<double> re(cnt), im(cnt);
+        gpu::check(gpu::ctx(), bsp_pgmres_ritz(gpu::ctx(), re.data(), im.data(), cnt, &cnt, &harm), "pgmres_ritz");
+        dense::Eigenvalues_Arr<T>(std::move(re), std::move(im), harm != 0).printEigInfo();
+    }
     return x;
+}
+
+/// Arnoldi eigenvalue estimates of M^{-1} A (blocksp_solvers.hpp:298-333): `iters` Arnoldi
+/// steps from q_0 = e_0 on the device; (harmonic) Ritz values of the Hessenberg.
+template <typename T, int M, typename PreCond = preConditioners::NoPre<T, M, 1>>
+dense::Eigenvalues_Arr<T> arnoldi_eigenvalues(const BlockSp<T, M>& A, int iters = 30,
+                                              const PreCond* preCond = nullptr, bool printEigInfo = false,
+                                              bool printEigVals = false, bool harmonic_Ritz = false) {
+    auto dA = gpu::device_matrix(A);
+    gpu::DPrecond wrapper;
+    bsp_precond* h = nullptr;
+    solver_detail::HostPrecondThunk<T, M, 1, PreCond> thunk{preCond};
+    if (preCond) {
+        if constexpr (solver_detail::DevicePrecond<PreCond>) h = preCond->handle();
+        if (!h && !std::is_same_v<PreCond, preConditioners::NoPre<T, M, 1>>) {
+            gpu::check(gpu::ctx(),
+                       bsp_precond_create_host(gpu::ctx(), dA->h, &solver_detail::HostPrecondThunk<T, M, 1, PreCond>::call,
+                                               &thunk, &wrapper.h),
+                       "host preconditioner");
+            h = wrapper.h;
+        }
+    }
+    std::vector<double> re(iters), im(iters);
+    int cnt = 0;
+    gpu::check(gpu::ctx(),
+               bsp_arnoldi_eigenvalues(gpu::ctx(), dA->h, h, iters, harmonic_Ritz ? 1 : 0, re.data(), im.data(), iters,
+                                       &cnt),
+               "arnoldi_eigenvalues");
+    re.resize(cnt);
+    im.resize(cnt);
+    dense::Eigenvalues_Arr<T> eig(std::move(re), std::move(im), harmonic_Ritz);
+    if (printEigInfo) eig.printEigInfo();
+    if (printEigVals) eig.printEigVals();
+    return eig;
 }
 
 /// pGMRES with block SOR preconditioner / LU block inversion (solvers.hpp:344-391).

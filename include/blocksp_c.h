@@ -209,6 +209,16 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
 int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
             double tol, int maxIters, int tol_mode, bsp_stats* stats);
 
+/* Ritz-value diagnostics (printEigInfo, blocksp_solvers.hpp:272-288): eigenvalues of the
+ * square part of the last bsp_pgmres run's Hessenberg on this context -- harmonic Ritz values
+ * (*harmonic = 1) when that run restarted.  Up to cap values into re/im; *count = all. */
+int bsp_pgmres_ritz(bsp_ctx* ctx, double* re, double* im, int cap, int* count, int* harmonic);
+/* arnoldi_eigenvalues(A, iters, preCond, ..., harmonic_Ritz) (blocksp_solvers.hpp:298-333):
+ * `iters` Arnoldi steps of M^{-1} A (M may be NULL) from q_0 = e_0 on the device, then the
+ * (harmonic) Ritz values of the Hessenberg.  Fewer values when an invariant subspace is found. */
+int bsp_arnoldi_eigenvalues(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, int iters, int harmonic,
+                            double* re, double* im, int cap, int* count);
+
 /* Mixed-precision flexible GMRES(restart) with deflated restart k and FP64 iterative
  * refinement (no reference counterpart; it solves the same systems as pGMRES).  The
  * preconditioner M is applied in its own precision (an FP32 ILU0 from

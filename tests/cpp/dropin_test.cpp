@@ -182,6 +182,27 @@ int main() {
         auto y1 = bsmv(B2, b);
         report("cache_invalidation", std::abs((y1(0)(0) - y0(0)(0)) - b(0)(0)), 1e-14);
     }
+    // ---- Arnoldi eigenvalues (blocksp_solvers.hpp:298-333): the SPD stencil's components
+    // decouple and its eigenvalues are 4.5 - 2 cos(pi j / 9) - 2 cos(pi k / 9).  From e_0 (a
+    // corner cell) the Krylov space is the x<->y-symmetric part of component 0 (36 dimensions),
+    // so 20 steps stay inside it: the Ritz values lie in [lmin, lmax] and the largest converges.
+    {
+        auto S = stencil<2>(8, 8, true);
+        auto eig = arnoldi_eigenvalues<double, 2>(S, 20);
+        const double lmax = 4.5 + 4.0 * std::cos(M_PI / 9.0), lmin = 4.5 - 4.0 * std::cos(M_PI / 9.0);
+        report("arnoldi_ritz_count", std::abs((double)eig.real().size() - 20.0), 0.0);
+        report("arnoldi_max_eig", std::abs(eig.max_real() - lmax), 1e-5);   // 5.9e-7 after 20 steps
+        report("arnoldi_in_spectrum", std::max(0.0, std::max(lmin - eig.min_real(), eig.max_real() - lmax)), 1e-12);
+        report("arnoldi_imag", std::max(std::abs(eig.min_imag()), std::abs(eig.max_imag())), 1e-10);
+        preConditioners::Diag<dense::LUFactor<double, 2>, double, 2> Md(S);
+        auto eigp = arnoldi_eigenvalues<double, 2>(S, 20, &Md, true, false, true);
+        report("arnoldi_diag_harmonic_positive", eigp.min_real() > 0.0 ? 0.0 : 1.0, 0.0);
+        preConditioners::ILU0<double, 2> Mi(S);
+        solArray<double, 2> bs(S.m());
+        for (size_t i = 0; i < bs.size_flat(); ++i) bs.data()[i] = urand();
+        auto xs = pGMRES<double, 2, 1>(S, bs, 1e-10, Mi, 500, 10, 4, false, true);   // restarted: harmonic info
+        report("pgmres_with_eig_info", relres<2, 1>(S, xs, bs), 1e-8);
+    }
     std::printf("%s\n", failures ? "DROPIN FAILED" : "DROPIN PASSED");
     return failures ? 1 : 0;
 }
