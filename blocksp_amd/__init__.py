@@ -74,6 +74,8 @@ _SIGS = {
     "bsp_bsmm": [_VP, _VP, _VP, C.POINTER(_VP)],
     "bsp_bsmm_numeric": [_VP, _VP, _VP, _VP],
     "bsp_pgmres_ritz": [_VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "bsp_bsr_save": [_VP, _VP, C.c_char_p],
+    "bsp_bsr_load": [_VP, C.c_char_p, C.POINTER(_VP)],
     "bsp_arnoldi_eigenvalues": [_VP, _VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int, C.POINTER(C.c_int)],
     "bsp_bsmv_dim_transpose": [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int],
     "bsp_dot": [_VP, C.c_int64, _VP, _VP, C.POINTER(C.c_double)],
@@ -277,6 +279,16 @@ class BSR:
         ctx.check(ctx.L.bsp_bsr_info(h, C.byref(n), C.byref(m), C.byref(bs), C.byref(nz)), "bsr_info")
         self.n_rows, self.n_cols, self.bs, self.nnzb = n.value, m.value, bs.value, nz.value
         return self
+
+    def save(self, path):
+        """Write the matrix to a BSR cache file (blocksp_amd.bsrio layout)."""
+        self.ctx.check(self.ctx.L.bsp_bsr_save(self.ctx.h, self.h, str(path).encode()), "bsr_save")
+
+    @classmethod
+    def load(cls, ctx, path):
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_bsr_load(ctx.h, str(path).encode(), C.byref(h)), "bsr_load")
+        return cls._wrap(ctx, h)
 
     def download(self):
         rp = np.empty(self.n_rows + 1, np.int32)

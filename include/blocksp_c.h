@@ -99,6 +99,11 @@ int bsp_bsmv_transpose(bsp_ctx* ctx, const bsp_bsr* A, const double* x_dev, doub
  * solArray<T,B,D> (blocksp_multiply.hpp:365-380). */
 int bsp_bsmv_dim_transpose(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x_dev, double* y_dev,
                            int accumulate);
+/* BSR binary cache files (no reference counterpart; the reference reads .mtx text): magic
+ * "BSPBSR1\0", int32 n_rows, n_cols, bs, 0, int64 nnzb, int32 rowptr[n_rows+1], int32
+ * colidx[nnzb] (stored order), float64 values[nnzb*bs*bs] (row-major blocks), little-endian. */
+int bsp_bsr_save(bsp_ctx* ctx, const bsp_bsr* A, const char* path);
+int bsp_bsr_load(bsp_ctx* ctx, const char* path, bsp_bsr** out);
 /* Multi-RHS product Y = A X with nrhs right-hand sides stored RHS-minor
  * (X[(row*bs + r)*nrhs + k]); no single reference call, it is nrhs bsmv calls. */
 int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X_dev, double* Y_dev);
