@@ -75,6 +75,7 @@ _SIGS = {
     "bsp_bsmm_numeric": [_VP, _VP, _VP, _VP],
     "bsp_pgmres_ritz": [_VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsp_bsr_save": [_VP, _VP, C.c_char_p],
+    "bsp_orthogonalize": [_VP, C.c_int64, _VP, C.c_int, _VP, _VP, _VP],
     "bsp_bsr_load": [_VP, C.c_char_p, C.POINTER(_VP)],
     "bsp_arnoldi_eigenvalues": [_VP, _VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int, C.POINTER(C.c_int)],
     "bsp_bsmv_dim_transpose": [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int],
@@ -606,6 +607,16 @@ def recorded(ctx):
     c = C.c_int()
     ctx.check(ctx.L.bsp_ctx_recorded(ctx.h, C.byref(c)), "recorded")
     return c.value
+
+
+def orthogonalize(ctx, Q, w: DeviceArray, qnew: DeviceArray):
+    """Classical Gram-Schmidt step with conditional reorthogonalisation (bsp_orthogonalize).
+    Q: list of orthonormal DeviceArrays.  Returns the m + 1 coefficients (last: ||w||)."""
+    m = len(Q)
+    ptrs = (C.c_void_p * m)(*[q.ptr.value for q in Q])
+    h = np.zeros(m + 1)
+    ctx.check(ctx.L.bsp_orthogonalize(ctx.h, int(w.size), ptrs, m, w.ptr, _ptr(h), qnew.ptr), "orthogonalize")
+    return h
 
 
 def pgmres_ritz(ctx, cap=256):

@@ -1,0 +1,245 @@
+// bsp_cgs.hip -- classical Gram-Schmidt with conditional reorthogonalisation (FP64) for
+// the flexible GMRES of the mixed-precision solver.
+//
+// For w against the orthonormal q_0..q_{m-1} (device pointer array Q):
+//   A : h1 = Q^T w and ||w||^2                     (reads the m vectors and w once)
+//   B : w <- w - Q h1 and ||w||^2                  (reads the m vectors and w, writes w)
+//   if ||w_new|| < 0.01 ||w_old|| (cancellation; the Daniel-Gragg-Kaufman-Stewart test with
+//   a threshold chosen for FP64) the pass is repeated once (A2, B2) and h = h1 + h2;
+//   q_m = w / ||w||.
+// The decision is taken on the device (a flag written by the reduction after B); A2 and B2
+// are always launched and return at once when the flag is clear, so the host never waits
+// inside a step.  Per step this reads 2m + 4 vectors (m + 1 with MGS ... 4m + 4), in about
+// nine launches instead of m + 2.  All sums are per-block partials added in a fixed order.
+#include "bsp_internal.hpp"
+
+#include <algorithm>
+
+namespace bsp {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kGroup = 8;          // dot products per thread in the multi-dot kernel
+constexpr int kMaxParts = 512;     // element blocks of a pass
+constexpr int kMaxBasis = 64;      // basis vectors (FGMRES restart <= 62)
+// Reorthogonalise when ||w_new|| < 0.01 ||w_old||: one classical pass leaves components of
+// relative size ~eps ||w_old|| / ||w_new|| along the basis, so below this ratio orthogonality
+// would degrade past ~1e-14; above it the second pass only costs bandwidth.
+constexpr double kReorth2 = 1e-4;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Block sums of N values; thread 0 of the block ends with the totals in v.
+template <int N>
+__device__ __forceinline__ void block_sums(double (&v)[N], double (*sh)[N]) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
+}
+
+// Multi-dot: part[s][b] = sum over block b's elements of q_s w  (s = 8 g + k, grid.y = g),
+// and (g == 0) part[m][b] = sum of w^2.  Skipped when *flag == 0 (gate != nullptr).
+__global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* __restrict__ w,
+                                                     const double* const* __restrict__ Q, int m,
+                                                     double* __restrict__ part, int nparts,
+                                                     const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    __shared__ double sh[kBlock / kWave][kGroup + 1];
+    const int g = blockIdx.y;
+    const int s0 = g * kGroup;
+    const int ns = min(kGroup, m - s0);
+    double acc[kGroup + 1];
+#pragma unroll
+    for (int k = 0; k <= kGroup; ++k) acc[k] = 0.0;
+    const double* q[kGroup];
+#pragma unroll
+    for (int k = 0; k < kGroup; ++k) q[k] = k < ns ? Q[s0 + k] : nullptr;
+    // pairs of elements with 16-byte loads (len even; vectors come from hipMalloc), then a tail
+    const int64_t npair = len / 2;
+    const double2* w2 = reinterpret_cast<const double2*>(w);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npair; i += (int64_t)gridDim.x * kBlock) {
+        const double2 wi = w2[i];
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k)
+            if (k < ns) {
+                const double2 qi = reinterpret_cast<const double2*>(q[k])[i];
+                acc[k] = fma(qi.y, wi.y, fma(qi.x, wi.x, acc[k]));
+            }
+        if (g == 0) acc[kGroup] = fma(wi.y, wi.y, fma(wi.x, wi.x, acc[kGroup]));
+    }
+    if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t i = len - 1;
+        const double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k)
+            if (k < ns) acc[k] = fma(q[k][i], wi, acc[k]);
+        if (g == 0) acc[kGroup] = fma(wi, wi, acc[kGroup]);
+    }
+    block_sums<kGroup + 1>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k)
+            if (k < ns) part[(size_t)(s0 + k) * nparts + blockIdx.x] = acc[k];
+        if (g == 0) part[(size_t)m * nparts + blockIdx.x] = acc[kGroup];
+    }
+}
+
+// w <- w - sum_s h[s] q_s; part_norm[b] = block sum of the new w^2.
+__global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __restrict__ w,
+                                                       const double* const* __restrict__ Q, int m,
+                                                       const double* __restrict__ h, double* __restrict__ part_norm,
+                                                       const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    __shared__ double sh[kBlock / kWave][1];
+    __shared__ const double* qs[kMaxBasis];
+    __shared__ double hs[kMaxBasis];
+    for (int s = threadIdx.x; s < m; s += kBlock) {
+        qs[s] = Q[s];
+        hs[s] = h[s];
+    }
+    __syncthreads();
+    double acc[1] = {0.0};
+    const int64_t npair = len / 2;
+    double2* w2 = reinterpret_cast<double2*>(w);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npair; i += (int64_t)gridDim.x * kBlock) {
+        double2 wi = w2[i];
+        for (int s = 0; s < m; ++s) {
+            const double2 qi = reinterpret_cast<const double2*>(qs[s])[i];
+            wi.x = fma(-hs[s], qi.x, wi.x);
+            wi.y = fma(-hs[s], qi.y, wi.y);
+        }
+        w2[i] = wi;
+        acc[0] = fma(wi.y, wi.y, fma(wi.x, wi.x, acc[0]));
+    }
+    if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t i = len - 1;
+        double wi = w[i];
+        for (int s = 0; s < m; ++s) wi = fma(-hs[s], qs[s][i], wi);
+        w[i] = wi;
+        acc[0] = fma(wi, wi, acc[0]);
+    }
+    block_sums<1>(acc, sh);
+    if (threadIdx.x == 0) part_norm[blockIdx.x] = acc[0];
+}
+
+// Fixed-order sums of the partials.
+//   stage 0 (after A):  h1[s] = sum part[s];  wn2 = sum part[m]
+//   stage 1 (after B):  n2 = sum part_norm; flag = (n2 < wn2 / 2); if !flag: hcol = h1, norm
+//   stage 2 (after A2): h2[s] = sum part[s]; hcol[s] = h1[s] + h2[s]      (flag set only)
+//   stage 3 (after B2): hcol[m] = norm = sqrt(sum part_norm)            (flag set only)
+__global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, const double* __restrict__ part,
+                                                       const double* __restrict__ part_norm, int nparts,
+                                                       double* __restrict__ st, int* __restrict__ flag,
+                                                       double* __restrict__ hcol) {
+    // st: [0, m) h1, [m, 2m) h2, 2m wn2, 2m + 1 norm
+    if ((stage == 2 || stage == 3) && *flag == 0) return;
+    __shared__ double sh[kBlock / kWave][1];
+    const int t = blockIdx.x;   // one block per output
+    double acc[1] = {0.0};
+    const double* src = (stage == 0 || stage == 2) ? part + (size_t)t * nparts : part_norm;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) acc[0] += src[b];
+    block_sums<1>(acc, sh);
+    if (threadIdx.x != 0) return;
+    const double v = acc[0];
+    if (stage == 0) {
+        if (t < m) st[t] = v;
+        else st[2 * m] = v;
+    } else if (stage == 1) {
+        const bool reorth = v < kReorth2 * st[2 * m];
+        *flag = reorth ? 1 : 0;
+        if (!reorth) {
+            const double nrm = sqrt(fabs(v));
+            for (int s = 0; s < m; ++s) hcol[s] = st[s];
+            hcol[m] = nrm;
+            st[2 * m + 1] = nrm;
+        }
+    } else if (stage == 2) {
+        st[m + t] = v;
+        hcol[t] = st[t] + v;
+    } else {
+        const double nrm = sqrt(fabs(v));
+        hcol[m] = nrm;
+        st[2 * m + 1] = nrm;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cgs_scale(int64_t len, const double* __restrict__ w,
+                                                      const double* __restrict__ st, int m,
+                                                      double* __restrict__ q) {
+    const double h = st[2 * m + 1];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (int64_t)gridDim.x * kBlock)
+        q[i] = w[i] / h;
+}
+
+}  // namespace
+
+size_t cgs_scratch_doubles(int mmax) { return (size_t)(mmax + 1) * kMaxParts + kMaxParts + 2 * mmax + 4; }
+
+// scratch: cgs_scratch_doubles(m) doubles; flag: one int (device)
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Qd, int m, double* hcol,
+                       double* qnew, double* scratch, int* flag) {
+    if (m < 1 || m > kMaxBasis) return fail(ctx, BSP_E_INVALID, "cgs: basis size out of range");
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((len + 8 * kBlock - 1) / (8 * kBlock), kMaxParts));
+    double* part = scratch;
+    double* part_norm = scratch + (size_t)(m + 1) * kMaxParts;
+    double* st = part_norm + kMaxParts;
+    const int ng = (m + kGroup - 1) / kGroup;
+    const int mm = std::max(m, 1);
+    hipLaunchKernelGGL(k_cgs_dots, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, part, part_norm, nb, st, flag,
+                       hcol);
+    hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, st, part_norm,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, part, part_norm, nb, st, flag, hcol);
+    // second pass, only when the device flagged cancellation
+    hipLaunchKernelGGL(k_cgs_dots, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb, flag);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(mm), dim3(kBlock), 0, ctx->stream, 2, m, part, part_norm, nb, st, flag, hcol);
+    hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, st + m, part_norm, flag);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, part, part_norm, nb, st, flag, hcol);
+    const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_cgs_scale, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, st, m, qnew);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("cgs launch: ") + hipGetErrorString(e));
+    return BSP_OK;
+}
+
+}  // namespace bsp
+
+using namespace bsp;
+
+extern "C" {
+
+int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs, int m, double* w_dev,
+                      double* h_host, double* qnew_dev) {
+    if (!ctx || !Q_dev_ptrs || !w_dev || !qnew_dev || len <= 0) return BSP_E_INVALID;
+    BSP_CHECK(m >= 1 && m <= 64, BSP_E_INVALID, "orthogonalize: 1 <= m <= 64 vectors");
+    BSP_HIP(hipSetDevice(ctx->device));
+    DevMem qd, scratch, flag, hcol;
+    BSP_HIP(qd.alloc(sizeof(double*) * m));
+    BSP_HIP(scratch.alloc(sizeof(double) * cgs_scratch_doubles(m)));
+    BSP_HIP(flag.alloc(sizeof(int)));
+    BSP_HIP(hcol.alloc(sizeof(double) * (m + 1)));
+    BSP_HIP(hipMemcpy(qd.p, Q_dev_ptrs, sizeof(double*) * m, hipMemcpyHostToDevice));
+    int rc = launch_arnoldi_cgs(ctx, len, w_dev, qd.as<const double*>(), m, hcol.as<double>(), qnew_dev,
+                                scratch.as<double>(), flag.as<int>());
+    if (rc) return rc;
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    if (h_host) BSP_HIP(hipMemcpy(h_host, hcol.p, sizeof(double) * (m + 1), hipMemcpyDeviceToHost));
+    return BSP_OK;
+}
+
+}  // extern "C"

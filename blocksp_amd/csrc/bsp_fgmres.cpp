@@ -78,6 +78,21 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     Pinned hpin, ypin, ppin;
     BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
     BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
+    // classical Gram-Schmidt (default): device pointer array of the basis, scratch, flag
+    const bool use_cgs = ctx->fgmres_ortho == 0;
+    DevMem qptr_dev, cgs_scratch, cgs_flag;
+    struct PinnedPtrs {
+        const double** p = nullptr;
+        ~PinnedPtrs() {
+            if (p) (void)hipHostFree(p);
+        }
+    } qptr_host;
+    if (use_cgs) {
+        BSP_HIP(qptr_dev.alloc(sizeof(double*) * (m_max + 2)));
+        BSP_HIP(cgs_scratch.alloc(sizeof(double) * cgs_scratch_doubles(m_max + 1)));
+        BSP_HIP(cgs_flag.alloc(sizeof(int)));
+        BSP_HIP(hipHostMalloc((void**)&qptr_host.p, sizeof(double*) * (m_max + 2), hipHostMallocDefault));
+    }
     if (restart > 0) {
         BSP_HIP(Pdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
         BSP_HIP(Pkdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
@@ -148,7 +163,16 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         double* vn = nullptr;
         rc = V.take(ctx, vn);
         if (rc) return rc;
-        rc = launch_arnoldi_mgs(ctx, len, w, Vb.data(), iter, hd, vn, ctx->red_partials.as<double>());
+        if (use_cgs) {
+            // the previous iteration's copies finished at its host synchronisation
+            for (int j = 0; j <= iter; ++j) qptr_host.p[j] = Vb[j];
+            BSP_HIP(hipMemcpyAsync(qptr_dev.p, qptr_host.p, sizeof(double*) * (iter + 1), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            rc = launch_arnoldi_cgs(ctx, len, w, qptr_dev.as<const double*>(), iter + 1, hd, vn,
+                                    cgs_scratch.as<double>(), cgs_flag.as<int>());
+        } else {
+            rc = launch_arnoldi_mgs(ctx, len, w, Vb.data(), iter, hd, vn, ctx->red_partials.as<double>());
+        }
         if (rc) return rc;
         BSP_HIP(hipMemcpyAsync(hpin.p, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
         BSP_HIP(hipStreamSynchronize(ctx->stream));

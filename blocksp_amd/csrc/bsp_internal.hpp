@@ -106,6 +106,8 @@ struct bsp_ctx {
                                      // default for even bs <= 8); 4 chunk-major + XCD (126.3 us); 3 chunk-major
                                      // (127.1 us); 0 row-major lane = row (131.2 us); 1 row-major streaming
                                      // (2.2x slower); 2 row-major coalesced 16-byte chunks (slower)
+    int fgmres_ortho = 0;            // mixed FGMRES orthogonalisation: 0 classical Gram-Schmidt with conditional
+                                     // reorthogonalisation, 1 modified Gram-Schmidt (the reference's)
     int bf16_variant = 0;            // BF16 MFMA SpMM on the 64^3 DG matrix, 32 RHS: 0 loads of 4 slots of a
                                      // row pair issued before their MFMAs + streaming matrix loads / result
                                      // stores (176 us); 1 slot by slot, streaming (193 us); 2 slot by slot,
@@ -214,6 +216,13 @@ int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qp
 // (device) and qnew = w / hcol[j+1].  parts: 2 x 1024 doubles of scratch.
 int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int j, double* hcol,
                        double* qnew, double* parts);
+// Classical Gram-Schmidt with one conditional reorthogonalisation (bsp_cgs.hip): w against
+// the m (<= 64) orthonormal vectors whose device pointers are in Qd (a device array);
+// hcol[0..m] (device) and qnew = w / hcol[m].  scratch: cgs_scratch_doubles(m) doubles,
+// flag: one int, both device.
+size_t cgs_scratch_doubles(int mmax);
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Qd, int m, double* hcol,
+                       double* qnew, double* scratch, int* flag);
 // q = v / h[0]
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
 // x += sum_i c[i] * Q[i]  (c on device, m vectors)

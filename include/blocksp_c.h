@@ -214,6 +214,15 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
 int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
             double tol, int maxIters, int tol_mode, bsp_stats* stats);
 
+/* Orthogonalisation step of the mixed-precision FGMRES (no reference counterpart; the
+ * reference's one_arnoldi uses modified Gram-Schmidt, blocksp_solvers.hpp:127-135): w against
+ * the m (<= 64) orthonormal device vectors Q_dev_ptrs[0..m-1] (a host array of device
+ * pointers) by classical Gram-Schmidt, repeated once when ||w|| drops below 0.01 of its
+ * value (cancellation).  h_host[0..m-1] = coefficients, h_host[m] = ||w||; qnew = w / ||w||;
+ * w_dev is overwritten with the orthogonalised w. */
+int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs, int m, double* w_dev,
+                      double* h_host, double* qnew_dev);
+
 /* Ritz-value diagnostics (printEigInfo, blocksp_solvers.hpp:272-288): eigenvalues of the
  * square part of the last bsp_pgmres run's Hessenberg on this context -- harmonic Ritz values
  * (*harmonic = 1) when that run restarted.  Up to cap values into re/im; *count = all. */

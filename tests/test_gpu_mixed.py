@@ -102,3 +102,54 @@ def test_fgmres_mixed_natural_fp32(ctx):
     xd, st, rc = bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, maxIters=5000, restart=30, k=10)
     relres = np.linalg.norm(p.b - po.bsmv(p, xd.numpy())) / np.linalg.norm(p.b)
     assert rc == 0 and relres <= 1e-10
+
+
+def _orthonormal(n, m, seed):
+    q, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((n, m)))
+    return q.T.copy()
+
+
+@pytest.mark.parametrize("m", [1, 7, 8, 9, 31])
+def test_orthogonalize_matches_numpy(ctx, m):
+    n = 5000
+    Qh = _orthonormal(n, m, m)
+    w = np.random.default_rng(100 + m).standard_normal(n)
+    Q = [ctx.to_device(q) for q in Qh]
+    wd, qn = ctx.to_device(w), ctx.zeros(n)
+    h = bsp.orthogonalize(ctx, Q, wd, qn)
+    href = Qh @ w
+    r = w - Qh.T @ href
+    np.testing.assert_allclose(h[:m], href, rtol=0, atol=1e-12 * np.linalg.norm(w))
+    assert abs(h[m] - np.linalg.norm(r)) <= 1e-12 * np.linalg.norm(w)
+    np.testing.assert_allclose(qn.numpy(), r / np.linalg.norm(r), rtol=0, atol=1e-12)
+
+
+def test_orthogonalize_reorthogonalises_under_cancellation(ctx):
+    # w almost in span(Q): one classical pass leaves O(eps ||w|| / ||r||) components along Q;
+    # the second pass (taken because ||r|| << ||w||) removes them
+    n, m = 20000, 12
+    Qh = _orthonormal(n, m, 3)
+    rng = np.random.default_rng(4)
+    w = Qh.T @ rng.standard_normal(m) + 1e-9 * rng.standard_normal(n)
+    Q = [ctx.to_device(q) for q in Qh]
+    wd, qn = ctx.to_device(w), ctx.zeros(n)
+    h = bsp.orthogonalize(ctx, Q, wd, qn)
+    q = qn.numpy()
+    assert np.abs(Qh @ q).max() <= 1e-12
+    assert abs(np.linalg.norm(q) - 1.0) <= 1e-12
+    np.testing.assert_allclose(h[:m], Qh @ w, rtol=0, atol=1e-12 * np.linalg.norm(w))
+
+
+@pytest.mark.parametrize("ortho", [0, 1])
+def test_fgmres_mixed_with_both_orthogonalisations(ctx, ortho):
+    p = pr.dg2d_veclap(48, 48, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    b = ctx.to_device(p.b)
+    M = bsp.ILU0Ex(ctx, A, colour=pr.parity_colours((48, 48)), precision=bsp.PREC_F32)
+    ctx.set_option("fgmres_ortho", ortho)
+    try:
+        x, st, rc = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M)
+    finally:
+        ctx.set_option("fgmres_ortho", 0)
+    assert rc == 0 and st.true_relres <= 1e-10
+    assert np.linalg.norm(p.b - p.bsmv(x.numpy())) <= 1e-10 * np.linalg.norm(p.b)
