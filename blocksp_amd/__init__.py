@@ -72,6 +72,7 @@ _SIGS = {
     "bsp_bsmv_transpose": [_VP, _VP, _VP, _VP, C.c_int],
     "bsp_bsmm_rhs": [_VP, _VP, C.c_int, _VP, _VP],
     "bsp_bsmm": [_VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_bsmm_trans_a": [_VP, _VP, _VP, C.POINTER(_VP)],
     "bsp_bsmm_numeric": [_VP, _VP, _VP, _VP],
     "bsp_pgmres_ritz": [_VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsp_bsr_save": [_VP, _VP, C.c_char_p],
@@ -326,10 +327,14 @@ def bsmv_dim_transpose(ctx, A: BSR, D, dim, x: DeviceArray, y: DeviceArray, accu
               "bsmv_dim_transpose")
 
 
-def bsmm(ctx, A: BSR, B: BSR) -> BSR:
-    """C = A B on the device (bsmm, blocksp_multiply.hpp:51-136); C keeps A and B alive."""
+def bsmm(ctx, A: BSR, B: BSR, Atran=False) -> BSR:
+    """C = A B, or A^T B with Atran, on the device (bsmm, blocksp_multiply.hpp:51-136);
+    C keeps A and B alive."""
     h = C.c_void_p()
-    ctx.check(ctx.L.bsp_bsmm(ctx.h, A.h, B.h, C.byref(h)), "bsmm")
+    if Atran:
+        ctx.check(ctx.L.bsp_bsmm_trans_a(ctx.h, A.h, B.h, C.byref(h)), "bsmm_trans_a")
+    else:
+        ctx.check(ctx.L.bsp_bsmm(ctx.h, A.h, B.h, C.byref(h)), "bsmm")
     return BSR._wrap(ctx, h, keep=(A, B))
 
 

@@ -141,10 +141,11 @@ struct bsp_bsr {
     mutable bool h_val_stale = false;  // values produced on the device (bsmm): copy back on first host use
     // lazily built transpose (for bsmv_transpose)
     mutable std::unique_ptr<bsp_bsr> transposed;
-    // C = A B from bsp_bsmm: contributing (A block, B block) pairs per block of C
+    // C = A B (or A^T B) from bsp_bsmm: contributing (A block, B block) pairs per block of C
     bsp::DevMem mm_pair_ptr, mm_pairs;
     const bsp_bsr* mm_A = nullptr;
     const bsp_bsr* mm_B = nullptr;
+    bool mm_Atran = false;             // C = A^T B: the pairs index A's explicit transpose
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
     mutable bsp::DevMem val16;         // BF16 copy of the values (MFMA block SpMM)
     mutable bsp::DevMem valp;          // chunk-major copy of the values (bsmv variant 3)

@@ -1,4 +1,4 @@
-// bsp_spgemm.hip -- block sparse matrix-matrix product C = A B (bsmm,
+// bsp_spgemm.hip -- block sparse matrix-matrix products C = A B and C = A^T B (bsmm,
 // blocksp_multiply.hpp:51-136) and the transposed products A^T x.
 //
 // Symbolic pass (host, once per structure): row i of C lists the block columns j in the
@@ -86,6 +86,88 @@ int ensure_transpose(bsp_ctx* ctx, const bsp_bsr* A) {
     return BSP_OK;
 }
 
+// Symbolic pass and first numeric pass of C = L B, where L = A (C = A B) or L = A^T held as
+// A's explicit transpose (C = A^T B).  Row j of A^T lists the rows i of A holding column j in
+// ascending i, so the first-touch walk over L visits C's blocks in the order the reference's
+// transposed product inserts them (blocksp_multiply.hpp:99-134) and adds the pair products
+// in the same i order.
+int bsmm_build(bsp_ctx* ctx, const bsp_bsr* L, const bsp_bsr* owner, bool transA, const bsp_bsr* B,
+               bsp_bsr** out) {
+    const int n = L->n_rows, m = B->n_cols, bs = L->bs;
+    // symbolic: first-touch columns per row and the contributing pairs per C block
+    std::vector<int32_t> crp(n + 1, 0), cci, slot_of(m, -1), touched;
+    std::vector<int32_t> pcount;   // pairs per C block
+    for (int i = 0; i < n; ++i) {
+        touched.clear();
+        for (int pa = L->h_rp[i]; pa < L->h_rp[i + 1]; ++pa) {
+            const int k = L->h_ci[pa];
+            for (int pb = B->h_rp[k]; pb < B->h_rp[k + 1]; ++pb) {
+                const int j = B->h_ci[pb];
+                if (slot_of[j] < 0) {
+                    slot_of[j] = (int32_t)cci.size();
+                    cci.push_back(j);
+                    pcount.push_back(0);
+                    touched.push_back(j);
+                }
+                pcount[slot_of[j]]++;
+            }
+        }
+        for (int j : touched) slot_of[j] = -1;
+        crp[i + 1] = (int32_t)cci.size();
+    }
+    const int64_t nnzc = (int64_t)cci.size();
+    std::vector<int32_t> pptr(nnzc + 1, 0);
+    for (int64_t t = 0; t < nnzc; ++t) pptr[t + 1] = pptr[t] + pcount[t];
+    std::vector<int32_t> pairs(2 * (size_t)pptr[nnzc]);
+    {
+        std::vector<int32_t> fill(pptr.begin(), pptr.end() - 1);
+        for (int i = 0; i < n; ++i) {
+            for (int t = crp[i]; t < crp[i + 1]; ++t) slot_of[cci[t]] = t;
+            for (int pa = L->h_rp[i]; pa < L->h_rp[i + 1]; ++pa) {
+                const int k = L->h_ci[pa];
+                for (int pb = B->h_rp[k]; pb < B->h_rp[k + 1]; ++pb) {
+                    const int q = fill[slot_of[B->h_ci[pb]]]++;
+                    pairs[2 * (size_t)q] = pa;
+                    pairs[2 * (size_t)q + 1] = pb;
+                }
+            }
+            for (int t = crp[i]; t < crp[i + 1]; ++t) slot_of[cci[t]] = -1;
+        }
+    }
+    // C with zero values, then the numeric pass on the device
+    const size_t b2 = (size_t)bs * bs;
+    auto C = std::make_unique<bsp_bsr>();
+    C->ctx = ctx;
+    C->n_rows = n;
+    C->n_cols = m;
+    C->bs = bs;
+    C->nnzb = nnzc;
+    C->h_rp = crp;
+    C->h_ci = cci;
+    BSP_HIP(C->rp.alloc(sizeof(int32_t) * (n + 1)));
+    BSP_HIP(C->ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnzc, 1)));
+    BSP_HIP(C->val.alloc(sizeof(double) * std::max<size_t>(nnzc * b2, 1)));
+    BSP_HIP(hipMemcpy(C->rp.p, crp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+    if (nnzc) BSP_HIP(hipMemcpy(C->ci.p, cci.data(), sizeof(int32_t) * nnzc, hipMemcpyHostToDevice));
+    BSP_HIP(C->mm_pair_ptr.alloc(sizeof(int32_t) * (nnzc + 1)));
+    BSP_HIP(C->mm_pairs.alloc(sizeof(int32_t) * std::max<size_t>(pairs.size(), 2)));
+    BSP_HIP(hipMemcpy(C->mm_pair_ptr.p, pptr.data(), sizeof(int32_t) * (nnzc + 1), hipMemcpyHostToDevice));
+    if (!pairs.empty())
+        BSP_HIP(hipMemcpy(C->mm_pairs.p, pairs.data(), sizeof(int32_t) * pairs.size(), hipMemcpyHostToDevice));
+    C->mm_A = owner;
+    C->mm_B = B;
+    C->mm_Atran = transA;
+    C->h_val_stale = true;
+    bsp_bsr* raw = C.release();
+    int rc = bsp_bsmm_numeric(ctx, owner, B, raw);
+    if (rc) {
+        bsp_bsr_destroy(raw);
+        return rc;
+    }
+    *out = raw;
+    return BSP_OK;
+}
+
 }  // namespace bsp
 
 using namespace bsp;
@@ -122,90 +204,37 @@ int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** out) {
     BSP_CHECK(A->n_cols == B->n_rows, BSP_E_INVALID, "bsmm: inner dimensions differ");
     BSP_CHECK(A->bs == B->bs, BSP_E_INVALID, "bsmm: block sizes differ");
     BSP_HIP(hipSetDevice(ctx->device));
-    const int n = A->n_rows, m = B->n_cols, bs = A->bs;
-    // symbolic: first-touch columns per row and the contributing pairs per C block
-    std::vector<int32_t> crp(n + 1, 0), cci, slot_of(m, -1), touched;
-    std::vector<int32_t> pcount;   // pairs per C block
-    for (int i = 0; i < n; ++i) {
-        touched.clear();
-        for (int pa = A->h_rp[i]; pa < A->h_rp[i + 1]; ++pa) {
-            const int k = A->h_ci[pa];
-            for (int pb = B->h_rp[k]; pb < B->h_rp[k + 1]; ++pb) {
-                const int j = B->h_ci[pb];
-                if (slot_of[j] < 0) {
-                    slot_of[j] = (int32_t)cci.size();
-                    cci.push_back(j);
-                    pcount.push_back(0);
-                    touched.push_back(j);
-                }
-                pcount[slot_of[j]]++;
-            }
-        }
-        for (int j : touched) slot_of[j] = -1;
-        crp[i + 1] = (int32_t)cci.size();
-    }
-    const int64_t nnzc = (int64_t)cci.size();
-    std::vector<int32_t> pptr(nnzc + 1, 0);
-    for (int64_t t = 0; t < nnzc; ++t) pptr[t + 1] = pptr[t] + pcount[t];
-    std::vector<int32_t> pairs(2 * (size_t)pptr[nnzc]);
-    {
-        std::vector<int32_t> fill(pptr.begin(), pptr.end() - 1);
-        for (int i = 0; i < n; ++i) {
-            for (int t = crp[i]; t < crp[i + 1]; ++t) slot_of[cci[t]] = t;
-            for (int pa = A->h_rp[i]; pa < A->h_rp[i + 1]; ++pa) {
-                const int k = A->h_ci[pa];
-                for (int pb = B->h_rp[k]; pb < B->h_rp[k + 1]; ++pb) {
-                    const int q = fill[slot_of[B->h_ci[pb]]]++;
-                    pairs[2 * (size_t)q] = pa;
-                    pairs[2 * (size_t)q + 1] = pb;
-                }
-            }
-            for (int t = crp[i]; t < crp[i + 1]; ++t) slot_of[cci[t]] = -1;
-        }
-    }
-    // C with zero values, then the numeric pass on the device
-    const size_t b2 = (size_t)bs * bs;
-    auto C = std::make_unique<bsp_bsr>();
-    C->ctx = ctx;
-    C->n_rows = n;
-    C->n_cols = m;
-    C->bs = bs;
-    C->nnzb = nnzc;
-    C->h_rp = crp;
-    C->h_ci = cci;
-    BSP_HIP(C->rp.alloc(sizeof(int32_t) * (n + 1)));
-    BSP_HIP(C->ci.alloc(sizeof(int32_t) * std::max<int64_t>(nnzc, 1)));
-    BSP_HIP(C->val.alloc(sizeof(double) * std::max<size_t>(nnzc * b2, 1)));
-    BSP_HIP(hipMemcpy(C->rp.p, crp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
-    if (nnzc) BSP_HIP(hipMemcpy(C->ci.p, cci.data(), sizeof(int32_t) * nnzc, hipMemcpyHostToDevice));
-    BSP_HIP(C->mm_pair_ptr.alloc(sizeof(int32_t) * (nnzc + 1)));
-    BSP_HIP(C->mm_pairs.alloc(sizeof(int32_t) * std::max<size_t>(pairs.size(), 2)));
-    BSP_HIP(hipMemcpy(C->mm_pair_ptr.p, pptr.data(), sizeof(int32_t) * (nnzc + 1), hipMemcpyHostToDevice));
-    if (!pairs.empty())
-        BSP_HIP(hipMemcpy(C->mm_pairs.p, pairs.data(), sizeof(int32_t) * pairs.size(), hipMemcpyHostToDevice));
-    C->mm_A = A;
-    C->mm_B = B;
-    C->h_val_stale = true;
-    bsp_bsr* raw = C.release();
-    int rc = bsp_bsmm_numeric(ctx, A, B, raw);
-    if (rc) {
-        bsp_bsr_destroy(raw);
-        return rc;
-    }
-    *out = raw;
-    return BSP_OK;
+    return bsmm_build(ctx, A, A, false, B, out);
+}
+
+int bsp_bsmm_trans_a(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** out) {
+    if (!ctx || !A || !B || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == B->n_rows, BSP_E_INVALID, "bsmm_trans_a: A and B have different row counts");
+    BSP_CHECK(A->bs == B->bs, BSP_E_INVALID, "bsmm_trans_a: block sizes differ");
+    BSP_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_transpose(ctx, A);
+    if (rc) return rc;
+    return bsmm_build(ctx, A->transposed.get(), A, true, B, out);
 }
 
 int bsp_bsmm_numeric(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr* C) {
     if (!ctx || !A || !B || !C) return BSP_E_INVALID;
     BSP_CHECK(C->mm_A == A && C->mm_B == B && C->mm_pair_ptr.p, BSP_E_INVALID,
-              "bsmm_numeric: C was not created by bsp_bsmm(A, B)");
+              "bsmm_numeric: C was not created by bsp_bsmm(A, B) or bsp_bsmm_trans_a(A, B)");
     BSP_HIP(hipSetDevice(ctx->device));
+    const bsp_bsr* L = A;
+    if (C->mm_Atran) {
+        // A's transpose is rebuilt from its current values when they changed
+        int rc = ensure_transpose(ctx, A);
+        if (rc) return rc;
+        L = A->transposed.get();
+    }
     const int64_t total = C->nnzb * C->bs * C->bs;
     if (total > 0) {
         hipLaunchKernelGGL(k_spgemm_numeric, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, C->nnzb, C->bs, C->mm_pair_ptr.as<int>(), C->mm_pairs.as<int2>(),
-                           A->val.as<double>(), B->val.as<double>(), C->val.as<double>());
+                           L->val.as<double>(), B->val.as<double>(), C->val.as<double>());
         BSP_HIP(hipGetLastError());
     }
     C->h_val_stale = true;

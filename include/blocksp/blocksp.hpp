@@ -216,6 +216,71 @@ public:
                 if (m_colIndex[im][in] - i <= im) low(im, m_colIndex[im][in]) = m_data[im][in];
         return low;
     }
+    // upTri (containers.hpp:287-296)
+    CSR upTri(int i = 0) const {
+        CSR up(m_m, m_n);
+        for (int im = 0; im < m_m; ++im)
+            for (size_t in = 0; in < m_colIndex[im].size(); ++in)
+                if (m_colIndex[im][in] - i >= im) up(im, m_colIndex[im][in]) = m_data[im][in];
+        return up;
+    }
+    // transpose (containers.hpp:264-274): the structure is transposed and each element copied
+    // as it is (conjugated for complex scalars); bs_tr also transposes the blocks.
+    CSR transpose() const {
+        CSR tr(m_n, m_m);
+        for (int im = 0; im < m_m; ++im)
+            for (size_t in = 0; in < m_colIndex[im].size(); ++in) {
+                if constexpr (is_complex_v<T>) tr(m_colIndex[im][in], im) = std::conj(m_data[im][in]);
+                else tr(m_colIndex[im][in], im) = m_data[im][in];
+            }
+        return tr;
+    }
+    // arithmetic (containers.hpp:302-353): blocks of the other matrix are added in its row
+    // order, inserting the ones this matrix lacks at the end of the row
+    CSR& operator+=(const CSR& o) {
+        assert(o.m_m == m_m && o.m_n == m_n);
+        for (int im = 0; im < m_m; ++im)
+            for (size_t in = 0; in < o.m_colIndex[im].size(); ++in) (*this)(im, o.m_colIndex[im][in]) += o.m_data[im][in];
+        return *this;
+    }
+    CSR& operator-=(const CSR& o) {
+        assert(o.m_m == m_m && o.m_n == m_n);
+        for (int im = 0; im < m_m; ++im)
+            for (size_t in = 0; in < o.m_colIndex[im].size(); ++in) (*this)(im, o.m_colIndex[im][in]) -= o.m_data[im][in];
+        return *this;
+    }
+    CSR operator+(const CSR& o) const { CSR r = *this; return r += o; }
+    CSR operator-(const CSR& o) const { CSR r = *this; return r -= o; }
+    // scaling by a scalar (or, for scalar elements, by an element)
+    template <typename S>
+        requires requires(T t, S s) { t *= s; }
+    CSR& operator*=(const S& s) {
+        touch();
+        for (auto& row : m_data)
+            for (auto& e : row) e *= s;
+        return *this;
+    }
+    template <typename S>
+        requires requires(T t, S s) { t /= s; }
+    CSR& operator/=(const S& s) {
+        touch();
+        for (auto& row : m_data)
+            for (auto& e : row) e /= s;
+        return *this;
+    }
+    // addLarger / subtractLarger (containers.hpp:359-378): rows of a taller matrix, my rows only
+    CSR& addLarger(const CSR& o) {
+        assert(o.m_m >= m_m && o.m_n == m_n);
+        for (int im = 0; im < m_m; ++im)
+            for (size_t in = 0; in < o.m_colIndex[im].size(); ++in) (*this)(im, o.m_colIndex[im][in]) += o.m_data[im][in];
+        return *this;
+    }
+    CSR& subtractLarger(const CSR& o) {
+        assert(o.m_m >= m_m && o.m_n == m_n);
+        for (int im = 0; im < m_m; ++im)
+            for (size_t in = 0; in < o.m_colIndex[im].size(); ++in) (*this)(im, o.m_colIndex[im][in]) -= o.m_data[im][in];
+        return *this;
+    }
     // rowIndicesByCol (containers.hpp:410-417)
     std::vector<std::unordered_set<int>> rowIndicesByCol() const {
         std::vector<std::unordered_set<int>> ind(m_n);
@@ -241,6 +306,17 @@ public:
     std::uint64_t id() const { return m_id; }
     std::uint64_t version() const { return m_version; }
 };
+
+/// Scalar products of a sparse matrix (containers.hpp:498-504).
+template <typename T, typename S>
+    requires requires(T t, S s) { t *= s; }
+CSR<T> operator*(const S& s, CSR<T> A) { return A *= s; }
+template <typename T, typename S>
+    requires requires(T t, S s) { t *= s; }
+CSR<T> operator*(CSR<T> A, const S& s) { return A *= s; }
+template <typename T, typename S>
+    requires requires(T t, S s) { t /= s; }
+CSR<T> operator/(CSR<T> A, const S& s) { return A /= s; }
 
 }  // namespace Sp
 
@@ -371,6 +447,37 @@ inline TinyVector<T, B * D> tvConcat(const TinyVector<TinyVector<T, B>, D>& tv) 
     for (int dim = 0; dim < D; ++dim)
         for (int ib = 0; ib < B; ++ib) cat(dim * B + ib) = tv(dim)(ib);
     return cat;
+}
+/// trace (blocksp_utility.hpp:350-380).
+template <typename T, int B>
+inline T trace(const TinyMatrix<T, B, B>& A) {
+    T t = T(0);
+    for (int i = 0; i < B; ++i) t += A(i, i);
+    return t;
+}
+template <typename T, int B>
+inline T trace(const BlockSp<T, B>& A) {
+    T t = T(0);
+    const int end = std::min(A.m(), A.n());
+    for (int ie = 0; ie < end; ++ie)
+        for (int ib = 0; ib < B; ++ib) t += A(ie, ie)(ib, ib);
+    return t;
+}
+/// frobeniusNorm (blocksp_utility.hpp:328-336, 383-388): sqrt(trace(A^* A)), the square root
+/// of the sum of squared magnitudes of the stored entries.
+template <typename T, int M, int N>
+inline ScalarType_t<T> frobeniusNorm(const TinyMatrix<T, M, N>& A) {
+    ScalarType_t<T> s = 0;
+    for (const T& e : A.a) s += std::norm(e);
+    return std::sqrt(s);
+}
+template <typename T, int B>
+inline ScalarType_t<T> frobeniusNorm(const BlockSp<T, B>& A) {
+    ScalarType_t<T> s = 0;
+    for (int im = 0; im < A.m(); ++im)
+        for (const auto& blk : A.get_data(im))
+            for (const T& e : blk.a) s += std::norm(e);
+    return std::sqrt(s);
 }
 /// setFromGlobal (blocksp_utility.hpp:614-639).
 template <typename T, int B>
@@ -586,6 +693,102 @@ solArray<T, B, D> operator*(const BlockSp<T, B * D>& A, const solArray<T, B, D>&
     solArray<T, B, D> b(A.m());
     mult_detail::bsmv_device(b, A, x, false, false, 0, -1);
     return b;
+}
+
+// ===========================================================================
+// bs_id, bs_tr, bsmm (blocksp_multiply.hpp:21-159, 334-363)
+// ===========================================================================
+
+namespace mult_detail {
+
+/// Identity of size blocks (multiply.hpp:21-28).
+template <typename T, int M>
+BlockSp<T, M> blkSpIdent(int size) {
+    BlockSp<T, M> I(size);
+    for (int im = 0; im < size; ++im) {
+        auto& blk = I(im, im);
+        for (int i = 0; i < M; ++i) blk(i, i) = T(1);
+    }
+    return I;
+}
+
+/// Transpose with transposed blocks (multiply.hpp:31-44): row j lists the rows i holding a
+/// block in column j in ascending i.
+template <typename T, int M>
+BlockSp<T, M> blkSpTranspose(const BlockSp<T, M>& A) {
+    BlockSp<T, M> tr(A.n(), A.m());
+    for (int im = 0; im < A.m(); ++im)
+        for (size_t in = 0; in < A.colSize(im); ++in) {
+            const int j = A.colInd(im, (int)in);
+            const auto& src = A.get_data(im)[in];
+            auto& dst = tr(j, im);
+            for (int r = 0; r < M; ++r)
+                for (int c = 0; c < M; ++c) dst(c, r) = src(r, c);
+        }
+    return tr;
+}
+
+/// C = A B or A^T B on the device (multiply.hpp:51-136): C's rows list their blocks in the
+/// reference's first-touch order; the product runs in FP64.
+template <typename T, int M>
+BlockSp<T, M> blkSpMatMat(const BlockSp<T, M>& A, const BlockSp<T, M>& B, bool Atran = false) {
+    if (!Atran) assert(A.n() == B.m());
+    else assert(A.m() == B.m());
+    auto dA = gpu::device_matrix(A);
+    auto dB = gpu::device_matrix(B);
+    bsp_bsr* h = nullptr;
+    const int rc = Atran ? bsp_bsmm_trans_a(gpu::ctx(), dA->h, dB->h, &h) : bsp_bsmm(gpu::ctx(), dA->h, dB->h, &h);
+    gpu::check(gpu::ctx(), rc, "bsmm");
+    std::unique_ptr<bsp_bsr, int (*)(bsp_bsr*)> guard(h, bsp_bsr_destroy);
+    int nr = 0, nc = 0, bs = 0;
+    int64_t nz = 0;
+    gpu::check(gpu::ctx(), bsp_bsr_info(h, &nr, &nc, &bs, &nz), "bsr_info");
+    std::vector<int32_t> rp(nr + 1), ci(std::max<int64_t>(nz, 1));
+    std::vector<double> v(std::max<size_t>((size_t)nz * M * M, 1));
+    gpu::check(gpu::ctx(), bsp_bsr_download(gpu::ctx(), h, rp.data(), ci.data(), v.data()), "bsr_download");
+    BlockSp<T, M> C(nr, nc);
+    for (int i = 0; i < nr; ++i) {
+        C.reserve(i, rp[i + 1] - rp[i]);
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            auto& blk = C(i, ci[p]);
+            for (int e = 0; e < M * M; ++e) blk.data()[e] = static_cast<T>(v[(size_t)p * M * M + e]);
+        }
+    }
+    return C;
+}
+
+/// D = A B C or A^T B C (multiply.hpp:138-147).
+template <typename T, int M>
+BlockSp<T, M> blkSpMatMat(const BlockSp<T, M>& A, const BlockSp<T, M>& B, const BlockSp<T, M>& C,
+                          bool Atran = false) {
+    return blkSpMatMat<T, M>(A, blkSpMatMat<T, M>(B, C), Atran);
+}
+
+/// E = A B C D or A^T B C D (multiply.hpp:149-159).
+template <typename T, int M>
+BlockSp<T, M> blkSpMatMat(const BlockSp<T, M>& A, const BlockSp<T, M>& B, const BlockSp<T, M>& C,
+                          const BlockSp<T, M>& D, bool Atran = false) {
+    return blkSpMatMat<T, M>(A, blkSpMatMat<T, M>(B, C, D), Atran);
+}
+
+}  // namespace mult_detail
+
+template <typename T, int M>
+BlockSp<T, M> bs_id(int size) { return mult_detail::blkSpIdent<T, M>(size); }
+template <typename T, int M>
+BlockSp<T, M> bs_tr(const BlockSp<T, M>& A) { return mult_detail::blkSpTranspose<T, M>(A); }
+template <typename T, int M>
+BlockSp<T, M> bsmm(const BlockSp<T, M>& A, const BlockSp<T, M>& B, bool Atran = false) {
+    return mult_detail::blkSpMatMat<T, M>(A, B, Atran);
+}
+template <typename T, int M>
+BlockSp<T, M> bsmm(const BlockSp<T, M>& A, const BlockSp<T, M>& B, const BlockSp<T, M>& C, bool Atran = false) {
+    return mult_detail::blkSpMatMat<T, M>(A, B, C, Atran);
+}
+template <typename T, int M>
+BlockSp<T, M> bsmm(const BlockSp<T, M>& A, const BlockSp<T, M>& B, const BlockSp<T, M>& C, const BlockSp<T, M>& D,
+                   bool Atran = false) {
+    return mult_detail::blkSpMatMat<T, M>(A, B, C, D, Atran);
 }
 
 // ===========================================================================

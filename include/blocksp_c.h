@@ -111,7 +111,11 @@ int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X_dev, 
  * order (symbolic pass on the host, bit-exact), values computed on the device.  A and B must
  * outlive C for bsp_bsmm_numeric. */
 int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C);
-/* Recomputes C = A * B on the device for new values of A and B (same structures). */
+/* C = A^T * B (bsmm with Atran = true, blocksp_multiply.hpp:99-134): row j of C in the
+ * order the reference inserts it (rows i of A holding column j ascending, then row i of B in
+ * its stored order); A^T is built once from A's values and rebuilt when they change. */
+int bsp_bsmm_trans_a(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C);
+/* Recomputes C = A * B (or A^T * B) on the device for new values of A and B (same structures). */
 int bsp_bsmm_numeric(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr* C);
 /* BF16 block SpMM on the matrix cores (no reference counterpart): Yb = A Xb for nrhs
  * right-hand sides (a multiple of 16), 8 x 8 blocks only, BF16 values (rounded to nearest

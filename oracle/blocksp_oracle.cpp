@@ -1274,4 +1274,46 @@ int orc_bsmm(int nA, int bs, const int* a_rowptr, const int* a_colidx, const dou
     return nz;
 }
 
+int orc_bsmm_transA(int nA, int mA, int bs, const int* a_rowptr, const int* a_colidx, const double* a_vals,
+                    const int* b_rowptr, const int* b_colidx, const double* b_vals,
+                    int* c_rowptr, int* c_colidx, double* c_vals, int cap) {
+    // multiply.hpp:99-134: for each row i of A, each block A(i, j) and each block B(i, c),
+    // C(j, c) += A(i, j)^T B(i, c); C(j, c) is inserted on first touch.
+    const int b2 = bs * bs;
+    std::vector<std::vector<int>> cols(mA);
+    std::vector<std::vector<double>> vals(mA);
+    std::vector<double> at(b2), mm(b2);
+    for (int i = 0; i < nA; ++i)
+        for (int pa = a_rowptr[i]; pa < a_rowptr[i + 1]; ++pa) {
+            const int j = a_colidx[pa];
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) at[r * bs + c] = a_vals[(size_t)pa * b2 + c * bs + r];
+            for (int pb = b_rowptr[i]; pb < b_rowptr[i + 1]; ++pb) {
+                const int c = b_colidx[pb];
+                auto& rc = cols[j];
+                size_t t = (size_t)(std::find(rc.begin(), rc.end(), c) - rc.begin());
+                if (t == rc.size()) {
+                    rc.push_back(c);
+                    vals[j].resize(vals[j].size() + b2, 0.0);
+                }
+                matmat(bs, at.data(), b_vals + (size_t)pb * b2, mm.data());
+                double* cv = vals[j].data() + t * b2;
+                for (int e = 0; e < b2; ++e) cv[e] += mm[e];
+            }
+        }
+    int nz = 0;
+    for (int j = 0; j < mA; ++j) nz += (int)cols[j].size();
+    if (nz > cap) return -nz;
+    int pos = 0;
+    c_rowptr[0] = 0;
+    for (int j = 0; j < mA; ++j) {
+        for (size_t t = 0; t < cols[j].size(); ++t, ++pos) {
+            c_colidx[pos] = cols[j][t];
+            std::memcpy(c_vals + (size_t)pos * b2, vals[j].data() + t * b2, sizeof(double) * b2);
+        }
+        c_rowptr[j + 1] = pos;
+    }
+    return nz;
+}
+
 }  // extern "C"

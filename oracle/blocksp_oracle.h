@@ -128,6 +128,13 @@ int orc_bsmm(int nA, int bs, const int* a_rowptr, const int* a_colidx, const dou
              const int* b_rowptr, const int* b_colidx, const double* b_vals,
              int* c_rowptr, int* c_colidx, double* c_vals, int cap);
 
+/* bsmm with Atran: C = A^T * B for A (nA x mA blocks) and B (nA rows), C(j, c) inserted on
+ * first touch while walking A's rows in order (multiply.hpp:99-134).  Returns nnzb of C,
+ * or -(needed) if cap is too small. */
+int orc_bsmm_transA(int nA, int mA, int bs, const int* a_rowptr, const int* a_colidx, const double* a_vals,
+                    const int* b_rowptr, const int* b_colidx, const double* b_vals,
+                    int* c_rowptr, int* c_colidx, double* c_vals, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,7 +43,7 @@ def lib():
             build()
         L = C.CDLL(_LIB)
         for name in ("orc_pcg", "orc_pgmres", "orc_harmonic_subspace", "orc_ichol0_factor",
-                     "orc_levels", "orc_bsmm"):
+                     "orc_levels", "orc_bsmm", "orc_bsmm_transA"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -248,5 +248,18 @@ def bsmm(pa_n, bs, arp, aci, av, brp, bci, bv):
     nz = lib().orc_bsmm(pa_n, bs, _p(_i32(arp), C.c_int), _p(_i32(aci), C.c_int), _p(_f64(av), C.c_double),
                         _p(_i32(brp), C.c_int), _p(_i32(bci), C.c_int), _p(_f64(bv), C.c_double),
                         _p(crp, C.c_int), _p(cci, C.c_int), _p(cv, C.c_double), cap)
+    assert nz >= 0
+    return crp, cci[:nz].copy(), cv[:nz].copy()
+
+
+def bsmm_transA(n, m, bs, arp, aci, av, brp, bci, bv):
+    """C = A^T B (bsmm with Atran, blocksp_multiply.hpp:99-134); A is n x m blocks, B has n rows."""
+    cap = int(sum((arp[i + 1] - arp[i]) * (brp[i + 1] - brp[i]) for i in range(n)))
+    crp = np.zeros(m + 1, np.int32)
+    cci = np.zeros(max(cap, 1), np.int32)
+    cv = np.zeros((max(cap, 1), bs, bs))
+    nz = lib().orc_bsmm_transA(n, m, bs, _p(_i32(arp), C.c_int), _p(_i32(aci), C.c_int), _p(_f64(av), C.c_double),
+                               _p(_i32(brp), C.c_int), _p(_i32(bci), C.c_int), _p(_f64(bv), C.c_double),
+                               _p(crp, C.c_int), _p(cci, C.c_int), _p(cv, C.c_double), cap)
     assert nz >= 0
     return crp, cci[:nz].copy(), cv[:nz].copy()

@@ -203,6 +203,78 @@ int main() {
         auto xs = pGMRES<double, 2, 1>(S, bs, 1e-10, Mi, 500, 10, 4, false, true);   // restarted: harmonic info
         report("pgmres_with_eig_info", relres<2, 1>(S, xs, bs), 1e-8);
     }
+    // ---- bsmm / bs_id / bs_tr and matrix arithmetic, as in the reference's pseudo-heat test
+    // (blocksp_tests.hpp:270-285): ASym = I - delta A^T A is SPD, then solved with pCG.
+    {
+        constexpr int BD = 4;
+        const int l = 40;
+        BSM<double, BD> R(l, l);
+        for (int i = 0; i < l; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const int j = (int)((urand() + 1.0) * 0.5 * l) % l;
+                auto& blk = R(i, j);
+                for (int e = 0; e < BD * BD; ++e) blk.data()[e] += urand();
+            }
+        auto dense = [&](const BSM<double, BD>& M) {
+            std::vector<double> d((size_t)M.m() * BD * M.n() * BD, 0.0);
+            const size_t ld = (size_t)M.n() * BD;
+            for (int i = 0; i < M.m(); ++i)
+                for (size_t k = 0; k < M.colSize(i); ++k) {
+                    const int j = M.colInd(i, (int)k);
+                    for (int r = 0; r < BD; ++r)
+                        for (int c = 0; c < BD; ++c) d[(i * BD + r) * ld + j * BD + c] += M.get_data(i)[k](r, c);
+                }
+            return d;
+        };
+        const auto Rd = dense(R);
+        const int N = l * BD;
+        auto RtR = BlockSp::bsmm(R, R, true);
+        auto RR = BlockSp::bsmm(R, R);
+        auto RtR2 = BlockSp::bsmm(BlockSp::bs_tr(R), R);   // same product through an explicit transpose
+        const auto RtRd = dense(RtR), RRd = dense(RR), RtR2d = dense(RtR2);
+        double e1 = 0.0, e2 = 0.0, e3 = 0.0, scale = 0.0;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                double t = 0.0, q = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    t += Rd[(size_t)k * N + i] * Rd[(size_t)k * N + j];
+                    q += Rd[(size_t)i * N + k] * Rd[(size_t)k * N + j];
+                }
+                scale = std::max(scale, std::abs(t));
+                e1 = std::max(e1, std::abs(RtRd[(size_t)i * N + j] - t));
+                e2 = std::max(e2, std::abs(RRd[(size_t)i * N + j] - q));
+                e3 = std::max(e3, std::abs(RtR2d[(size_t)i * N + j] - t));
+            }
+        report("bsmm_AtA_dense", e1 / scale, 1e-13);
+        report("bsmm_AA_dense", e2 / scale, 1e-13);
+        report("bsmm_trA_A_dense", e3 / scale, 1e-13);
+        bool same_order = RtR.m() == RtR2.m();
+        for (int i = 0; same_order && i < RtR.m(); ++i) same_order = RtR.get_colIndex(i) == RtR2.get_colIndex(i);
+        report("bsmm_AtA_structure_equals_bs_tr_path", same_order ? 0.0 : 1.0, 0.0);
+        report("frobenius_norm_is_sqrt_trace_AtA",
+               std::abs(BlockSp::util::frobeniusNorm(R) - std::sqrt(BlockSp::util::trace(RtR))) /
+                   BlockSp::util::frobeniusNorm(R), 1e-14);
+        const double delta = 1.0 / (2.0 * BlockSp::util::trace(RtR));   // keeps I - delta A^T A SPD
+        auto ASym{BlockSp::bs_id<double, BD>(l) - delta * BlockSp::bsmm(R, R, true)};
+        ASym.shrink_to_fit();
+        double esym = 0.0;
+        const auto Sd = dense(ASym);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j)
+                esym = std::max(esym, std::abs(Sd[(size_t)i * N + j] - ((i == j ? 1.0 : 0.0) - delta * RtRd[(size_t)i * N + j])));
+        report("pseudo_heat_assembly", esym, 1e-15);
+        solArray<double, BD> bh(l);
+        for (size_t i = 0; i < bh.size_flat(); ++i) bh.data()[i] = urand();
+        preConditioners::ILU0<double, BD> Mh(ASym);
+        auto xh = pCG<double, BD, 1>(ASym, bh, 1e-12, Mh, 500);
+        report("pseudo_heat_pcg", relres<BD, 1>(ASym, xh, bh), 1e-10);
+        auto Up = ASym.upTri(), Lo = ASym.lowTri(-1);
+        auto Sum = Lo + Up;   // strict lower + upper = the whole matrix
+        double esplit = 0.0;
+        const auto Sumd = dense(Sum);
+        for (size_t i = 0; i < Sd.size(); ++i) esplit = std::max(esplit, std::abs(Sumd[i] - Sd[i]));
+        report("upTri_plus_strict_lowTri", esplit, 0.0);
+    }
     std::printf("%s\n", failures ? "DROPIN FAILED" : "DROPIN PASSED");
     return failures ? 1 : 0;
 }

@@ -110,3 +110,52 @@ def test_bsmm_square_of_dg_matrix_feeds_a_preconditioner(ctx):
     bsp.bsmv(ctx, A, x, y1)
     bsp.bsmv_transpose(ctx, A, x, y2)
     np.testing.assert_allclose(y2.numpy(), y1.numpy(), rtol=1e-12, atol=1e-12 * np.abs(y1.numpy()).max())
+
+
+@pytest.mark.parametrize("bs", [3, 8])
+def test_bsmm_transA_structure_bit_exact_values_within_tolerance(ctx, bs):
+    n, m, k = 19, 13, 16
+    arp, aci, av = _rand_bsr(n, m, bs, seed=50 + bs)
+    brp, bci, bv = _rand_bsr(n, k, bs, seed=60 + bs)
+    A = bsp.BSR(ctx, n, m, bs, arp, aci, av)
+    Bm = bsp.BSR(ctx, n, k, bs, brp, bci, bv)
+    Cm = bsp.bsmm(ctx, A, Bm, Atran=True)
+    assert (Cm.n_rows, Cm.n_cols) == (m, k)
+    crp, cci, cv = Cm.download()
+    orp, oci, ov = po.bsmm_transA(n, m, bs, arp, aci, av, brp, bci, bv)
+    np.testing.assert_array_equal(crp, orp)
+    np.testing.assert_array_equal(cci, oci)
+    _, _, bound = po.bsmm_transA(n, m, bs, arp, aci, np.abs(av), brp, bci, np.abs(bv))
+    assert np.all(np.abs(cv - ov) <= 1e-13 * bound + 1e-300)
+    # new values of A: the numeric pass rebuilds A^T from them
+    av2 = 3.0 * av - 0.5
+    av2_d = ctx.to_device(av2)
+    ctx.check(ctx.L.bsp_bsr_set_values_dev(ctx.h, A.h, av2_d.ptr), "set_values")
+    bsp.bsmm_numeric(ctx, A, Bm, Cm)
+    _, _, ov2 = po.bsmm_transA(n, m, bs, arp, aci, av2, brp, bci, bv)
+    _, _, bound2 = po.bsmm_transA(n, m, bs, arp, aci, np.abs(av2), brp, bci, np.abs(bv))
+    assert np.all(np.abs(Cm.download()[2] - ov2) <= 1e-13 * bound2 + 1e-300)
+
+
+def test_bsmm_transA_of_dg_matrix_is_its_square_and_gives_frobenius_norm(ctx):
+    # the DG matrix is symmetric, so A^T A = A A; trace(A^T A) = ||A||_F^2 (utility.hpp:383-388)
+    p = pr.dg2d_veclap(10, 10, seed=2)
+    A = bsp.BSR.from_problem(ctx, p)
+    AtA = bsp.bsmm(ctx, A, A, Atran=True)
+    rp, ci, v = AtA.download()
+    orp, oci, ov = po.bsmm_transA(p.n, p.n, p.bs, p.rowptr, p.colidx, p.vals, p.rowptr, p.colidx, p.vals)
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(ci, oci)
+    assert np.abs(v - ov).max() <= 1e-12 * np.abs(ov).max()
+    tr = sum(np.trace(v[q]) for i in range(p.n) for q in range(rp[i], rp[i + 1]) if ci[q] == i)
+    fro2 = float(np.sum(p.vals * p.vals))
+    assert abs(tr - fro2) <= 1e-12 * fro2
+
+
+def test_bsmm_transA_rejects_mismatched_rows(ctx):
+    rp, ci, v = _rand_bsr(5, 4, 2, seed=1)
+    rp2, ci2, v2 = _rand_bsr(6, 4, 2, seed=2)
+    A = bsp.BSR(ctx, 5, 4, 2, rp, ci, v)
+    Bm = bsp.BSR(ctx, 6, 4, 2, rp2, ci2, v2)
+    with pytest.raises(Exception):
+        bsp.bsmm(ctx, A, Bm, Atran=True)

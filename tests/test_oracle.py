@@ -466,3 +466,65 @@ def test_bsmm_first_touch_order_and_values():
         assert list(cci[crp[i]:crp[i + 1]]) == order
     Ad, Bd = dense_of(n, bs, rpa, cia, va), dense_of(n, bs, rpb, cib, vb)
     assert np.allclose(dense_of(n, bs, crp, cci, cv), Ad @ Bd, atol=1e-12)
+
+
+def _rect_bsr(n, m, bs, seed, maxblk=4):
+    rng = np.random.default_rng(seed)
+    rows = [list(rng.permutation(rng.choice(m, size=int(rng.integers(0, maxblk + 1)), replace=False)))
+            for _ in range(n)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    ci = np.array([c for r in rows for c in r], np.int32)
+    return rp, ci, rng.standard_normal((len(ci), bs, bs))
+
+
+def _dense_rect(n, m, bs, rp, ci, v):
+    A = np.zeros((n * bs, m * bs))
+    for i in range(n):
+        for p in range(rp[i], rp[i + 1]):
+            A[i * bs:(i + 1) * bs, ci[p] * bs:(ci[p] + 1) * bs] += v[p]
+    return A
+
+
+def test_bsmm_transA_hand_example_order():
+    # A (3 x 2 blocks, bs 1): row 0 = {1: 2}, row 1 = {0: 3, 1: 5}, row 2 = {0: 7}
+    # B (3 x 3 blocks):        row 0 = {2: 1}, row 1 = {1: 1, 0: 2}, row 2 = {2: 4}
+    # row 0 of C = A^T B is touched by i = 1 (cols 1, 0 of B row 1) then i = 2 (col 2):
+    #   order [1, 0, 2], values [3, 6, 28]; row 1 by i = 0 (col 2) then i = 1 (cols 1, 0):
+    #   order [2, 1, 0], values [2, 5, 10]
+    arp, aci, av = np.array([0, 1, 3, 4], np.int32), np.array([1, 0, 1, 0], np.int32), np.array([2., 3., 5., 7.])
+    brp, bci, bv = np.array([0, 1, 3, 4], np.int32), np.array([2, 1, 0, 2], np.int32), np.array([1., 1., 2., 4.])
+    crp, cci, cv = po.bsmm_transA(3, 2, 1, arp, aci, av.reshape(-1, 1, 1), brp, bci, bv.reshape(-1, 1, 1))
+    assert list(crp) == [0, 3, 6]
+    assert list(cci) == [1, 0, 2, 2, 1, 0]
+    assert list(cv.ravel()) == [3., 6., 28., 2., 5., 10.]
+
+
+@pytest.mark.parametrize("bs", [1, 3])
+def test_bsmm_transA_dense_and_explicit_transpose(bs):
+    n, m, k = 11, 7, 9
+    arp, aci, av = _rect_bsr(n, m, bs, seed=31 + bs)
+    brp, bci, bv = _rect_bsr(n, k, bs, seed=41 + bs)
+    crp, cci, cv = po.bsmm_transA(n, m, bs, arp, aci, av, brp, bci, bv)
+    Ad, Bd = _dense_rect(n, m, bs, arp, aci, av), _dense_rect(n, k, bs, brp, bci, bv)
+    np.testing.assert_allclose(_dense_rect(m, k, bs, crp, cci, cv), Ad.T @ Bd, atol=1e-12)
+    # the same product through an explicit transpose (rows i ascending) and the plain bsmm
+    tr = [[] for _ in range(m)]
+    for i in range(n):
+        for p in range(arp[i], arp[i + 1]):
+            tr[aci[p]].append((i, av[p].T))
+    trp = np.concatenate([[0], np.cumsum([len(r) for r in tr])]).astype(np.int32)
+    tci = np.array([i for r in tr for i, _ in r], np.int32)
+    tv = np.array([b for r in tr for _, b in r]).reshape(-1, bs, bs)
+    orp, oci, ov = po.bsmm(m, bs, trp, tci, tv, brp, bci, bv)
+    np.testing.assert_array_equal(crp, orp)
+    np.testing.assert_array_equal(cci, oci)
+    np.testing.assert_array_equal(cv, ov)
+
+
+def test_bsmm_transA_trace_is_squared_frobenius_norm():
+    # frobeniusNorm(A) = sqrt(trace(bsmm(A, A, true))) (utility.hpp:383-388)
+    n, bs = 13, 4
+    rp, ci, v = rand_bsr(n, bs, seed=5)
+    crp, cci, cv = po.bsmm_transA(n, n, bs, rp, ci, v, rp, ci, v)
+    tr = sum(np.trace(cv[p]) for i in range(n) for p in range(crp[i], crp[i + 1]) if cci[p] == i)
+    assert abs(tr - np.sum(v * v)) <= 1e-12 * np.sum(v * v)
