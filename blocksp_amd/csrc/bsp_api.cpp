@@ -114,6 +114,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "fgmres_ortho") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_ortho must be 0 (CGS2) or 1 (MGS)");
         ctx->fgmres_ortho = (int)value;
+    } else if (k == "fgmres_basis_f32") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_basis_f32 must be 0 or 1");
+        ctx->fgmres_basis_f32 = (int)value;
     } else if (k == "bf16_variant") {
         BSP_CHECK(value >= 0 && value <= 3, BSP_E_INVALID, "bf16_variant must be 0, 1, 2 or 3");
         ctx->bf16_variant = (int)value;

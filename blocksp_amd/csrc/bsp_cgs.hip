@@ -1,5 +1,7 @@
-// bsp_cgs.hip -- classical Gram-Schmidt with conditional reorthogonalisation (FP64) for
-// the flexible GMRES of the mixed-precision solver.
+// bsp_cgs.hip -- classical Gram-Schmidt with conditional reorthogonalisation (FP64
+// arithmetic) for the flexible GMRES of the mixed-precision solver.  The basis is stored in
+// FP64 or in FP32 (half the bytes per pass; each q_m is rounded to FP32 once, after it is
+// orthogonalised and normalised in FP64).
 //
 // For w against the orthonormal q_0..q_{m-1} (device pointer array Q):
 //   A : h1 = Q^T w and ||w||^2                     (reads the m vectors and w once)
@@ -50,12 +52,50 @@ __device__ __forceinline__ void block_sums(double (&v)[N], double (*sh)[N]) {
         for (int k = 0; k < N; ++k) v[k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
 }
 
+// Element groups: a thread handles N consecutive elements per step with 16-byte loads of the
+// basis (N = 2 FP64 or 4 FP32 values) and N/2 16-byte loads of w.  Vectors come from hipMalloc
+// and the library's pools (256-byte aligned); the len % N tail is done by one thread.
+template <typename QT> struct QVec;
+template <> struct QVec<double> { static constexpr int N = 2; };
+template <> struct QVec<float> { static constexpr int N = 4; };
+
+template <int N>
+__device__ __forceinline__ void load_w(const double* w, int64_t i, double (&o)[N]) {
+    const double2* p = reinterpret_cast<const double2*>(w) + i * (N / 2);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+        const double2 v = p[k];
+        o[2 * k] = v.x;
+        o[2 * k + 1] = v.y;
+    }
+}
+template <int N>
+__device__ __forceinline__ void store_w(double* w, int64_t i, const double (&o)[N]) {
+    double2* p = reinterpret_cast<double2*>(w) + i * (N / 2);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) p[k] = make_double2(o[2 * k], o[2 * k + 1]);
+}
+__device__ __forceinline__ void load_q(const double* q, int64_t i, double (&o)[2]) {
+    const double2 v = reinterpret_cast<const double2*>(q)[i];
+    o[0] = v.x;
+    o[1] = v.y;
+}
+__device__ __forceinline__ void load_q(const float* q, int64_t i, double (&o)[4]) {
+    const float4 v = reinterpret_cast<const float4*>(q)[i];
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+
 // Multi-dot: part[s][b] = sum over block b's elements of q_s w  (s = 8 g + k, grid.y = g),
 // and (g == 0) part[m][b] = sum of w^2.  Skipped when *flag == 0 (gate != nullptr).
+template <typename QT>
 __global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* __restrict__ w,
-                                                     const double* const* __restrict__ Q, int m,
+                                                     const QT* const* __restrict__ Q, int m,
                                                      double* __restrict__ part, int nparts,
                                                      const int* __restrict__ gate) {
+    constexpr int N = QVec<QT>::N;
     if (gate && *gate == 0) return;
     __shared__ double sh[kBlock / kWave][kGroup + 1];
     const int g = blockIdx.y;
@@ -64,30 +104,33 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* 
     double acc[kGroup + 1];
 #pragma unroll
     for (int k = 0; k <= kGroup; ++k) acc[k] = 0.0;
-    const double* q[kGroup];
+    const QT* q[kGroup];
 #pragma unroll
     for (int k = 0; k < kGroup; ++k) q[k] = k < ns ? Q[s0 + k] : nullptr;
-    // pairs of elements with 16-byte loads (len even; vectors come from hipMalloc), then a tail
-    const int64_t npair = len / 2;
-    const double2* w2 = reinterpret_cast<const double2*>(w);
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npair; i += (int64_t)gridDim.x * kBlock) {
-        const double2 wi = w2[i];
+    const int64_t nvec = len / N;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * kBlock) {
+        double wi[N];
+        load_w<N>(w, i, wi);
 #pragma unroll
         for (int k = 0; k < kGroup; ++k)
             if (k < ns) {
-                const double2 qi = reinterpret_cast<const double2*>(q[k])[i];
-                acc[k] = fma(qi.y, wi.y, fma(qi.x, wi.x, acc[k]));
-            }
-        if (g == 0) acc[kGroup] = fma(wi.y, wi.y, fma(wi.x, wi.x, acc[kGroup]));
-    }
-    if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-        const int64_t i = len - 1;
-        const double wi = w[i];
+                double qi[N];
+                load_q(q[k], i, qi);
 #pragma unroll
-        for (int k = 0; k < kGroup; ++k)
-            if (k < ns) acc[k] = fma(q[k][i], wi, acc[k]);
-        if (g == 0) acc[kGroup] = fma(wi, wi, acc[kGroup]);
+                for (int e = 0; e < N; ++e) acc[k] = fma(qi[e], wi[e], acc[k]);
+            }
+        if (g == 0)
+#pragma unroll
+            for (int e = 0; e < N; ++e) acc[kGroup] = fma(wi[e], wi[e], acc[kGroup]);
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int64_t i = nvec * N; i < len; ++i) {
+            const double wi = w[i];
+#pragma unroll
+            for (int k = 0; k < kGroup; ++k)
+                if (k < ns) acc[k] = fma((double)q[k][i], wi, acc[k]);
+            if (g == 0) acc[kGroup] = fma(wi, wi, acc[kGroup]);
+        }
     block_sums<kGroup + 1>(acc, sh);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -98,13 +141,15 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* 
 }
 
 // w <- w - sum_s h[s] q_s; part_norm[b] = block sum of the new w^2.
+template <typename QT>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __restrict__ w,
-                                                       const double* const* __restrict__ Q, int m,
+                                                       const QT* const* __restrict__ Q, int m,
                                                        const double* __restrict__ h, double* __restrict__ part_norm,
                                                        const int* __restrict__ gate) {
+    constexpr int N = QVec<QT>::N;
     if (gate && *gate == 0) return;
     __shared__ double sh[kBlock / kWave][1];
-    __shared__ const double* qs[kMaxBasis];
+    __shared__ const QT* qs[kMaxBasis];
     __shared__ double hs[kMaxBasis];
     for (int s = threadIdx.x; s < m; s += kBlock) {
         qs[s] = Q[s];
@@ -112,32 +157,34 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __re
     }
     __syncthreads();
     double acc[1] = {0.0};
-    const int64_t npair = len / 2;
-    double2* w2 = reinterpret_cast<double2*>(w);
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < npair; i += (int64_t)gridDim.x * kBlock) {
-        double2 wi = w2[i];
+    const int64_t nvec = len / N;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * kBlock) {
+        double wi[N];
+        load_w<N>(w, i, wi);
         for (int s = 0; s < m; ++s) {
-            const double2 qi = reinterpret_cast<const double2*>(qs[s])[i];
-            wi.x = fma(-hs[s], qi.x, wi.x);
-            wi.y = fma(-hs[s], qi.y, wi.y);
+            double qi[N];
+            load_q(qs[s], i, qi);
+#pragma unroll
+            for (int e = 0; e < N; ++e) wi[e] = fma(-hs[s], qi[e], wi[e]);
         }
-        w2[i] = wi;
-        acc[0] = fma(wi.y, wi.y, fma(wi.x, wi.x, acc[0]));
+        store_w<N>(w, i, wi);
+#pragma unroll
+        for (int e = 0; e < N; ++e) acc[0] = fma(wi[e], wi[e], acc[0]);
     }
-    if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-        const int64_t i = len - 1;
-        double wi = w[i];
-        for (int s = 0; s < m; ++s) wi = fma(-hs[s], qs[s][i], wi);
-        w[i] = wi;
-        acc[0] = fma(wi, wi, acc[0]);
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int64_t i = nvec * N; i < len; ++i) {
+            double wi = w[i];
+            for (int s = 0; s < m; ++s) wi = fma(-hs[s], (double)qs[s][i], wi);
+            w[i] = wi;
+            acc[0] = fma(wi, wi, acc[0]);
+        }
     block_sums<1>(acc, sh);
     if (threadIdx.x == 0) part_norm[blockIdx.x] = acc[0];
 }
 
 // Fixed-order sums of the partials.
 //   stage 0 (after A):  h1[s] = sum part[s];  wn2 = sum part[m]
-//   stage 1 (after B):  n2 = sum part_norm; flag = (n2 < wn2 / 2); if !flag: hcol = h1, norm
+//   stage 1 (after B):  n2 = sum part_norm; flag = (n2 < 1e-4 wn2); if !flag: hcol = h1, norm
 //   stage 2 (after A2): h2[s] = sum part[s]; hcol[s] = h1[s] + h2[s]      (flag set only)
 //   stage 3 (after B2): hcol[m] = norm = sqrt(sum part_norm)            (flag set only)
 __global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, const double* __restrict__ part,
@@ -176,12 +223,45 @@ __global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, const d
     }
 }
 
+template <typename QT>
 __global__ __launch_bounds__(kBlock) void k_cgs_scale(int64_t len, const double* __restrict__ w,
                                                       const double* __restrict__ st, int m,
-                                                      double* __restrict__ q) {
+                                                      QT* __restrict__ q) {
     const double h = st[2 * m + 1];
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (int64_t)gridDim.x * kBlock)
-        q[i] = w[i] / h;
+        q[i] = (QT)(w[i] / h);
+}
+
+template <typename QT>
+int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Qd, int m, double* hcol, QT* qnew,
+                double* scratch, int* flag) {
+    if (m < 1 || m > kMaxBasis) return fail(ctx, BSP_E_INVALID, "cgs: basis size out of range");
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((len + 8 * kBlock - 1) / (8 * kBlock), kMaxParts));
+    double* part = scratch;
+    double* part_norm = scratch + (size_t)(m + 1) * kMaxParts;
+    double* st = part_norm + kMaxParts;
+    const int ng = (m + kGroup - 1) / kGroup;
+    const int mm = std::max(m, 1);
+    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, part, part_norm, nb, st, flag,
+                       hcol);
+    auto update = [&](const double* h, const int* gate) {
+        hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, h, part_norm,
+                           gate);
+    };
+    update(st, nullptr);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, part, part_norm, nb, st, flag, hcol);
+    // second pass, only when the device flagged cancellation
+    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb, flag);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(mm), dim3(kBlock), 0, ctx->stream, 2, m, part, part_norm, nb, st, flag, hcol);
+    update(st + m, flag);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, part, part_norm, nb, st, flag, hcol);
+    const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_cgs_scale<QT>, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, st, m, qnew);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("cgs launch: ") + hipGetErrorString(e));
+    return BSP_OK;
 }
 
 }  // namespace
@@ -191,30 +271,13 @@ size_t cgs_scratch_doubles(int mmax) { return (size_t)(mmax + 1) * kMaxParts + k
 // scratch: cgs_scratch_doubles(m) doubles; flag: one int (device)
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Qd, int m, double* hcol,
                        double* qnew, double* scratch, int* flag) {
-    if (m < 1 || m > kMaxBasis) return fail(ctx, BSP_E_INVALID, "cgs: basis size out of range");
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((len + 8 * kBlock - 1) / (8 * kBlock), kMaxParts));
-    double* part = scratch;
-    double* part_norm = scratch + (size_t)(m + 1) * kMaxParts;
-    double* st = part_norm + kMaxParts;
-    const int ng = (m + kGroup - 1) / kGroup;
-    const int mm = std::max(m, 1);
-    hipLaunchKernelGGL(k_cgs_dots, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb,
-                       (const int*)nullptr);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, part, part_norm, nb, st, flag,
-                       hcol);
-    hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, st, part_norm,
-                       (const int*)nullptr);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, part, part_norm, nb, st, flag, hcol);
-    // second pass, only when the device flagged cancellation
-    hipLaunchKernelGGL(k_cgs_dots, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb, flag);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(mm), dim3(kBlock), 0, ctx->stream, 2, m, part, part_norm, nb, st, flag, hcol);
-    hipLaunchKernelGGL(k_cgs_update, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, st + m, part_norm, flag);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, part, part_norm, nb, st, flag, hcol);
-    const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(k_cgs_scale, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, st, m, qnew);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("cgs launch: ") + hipGetErrorString(e));
-    return BSP_OK;
+    return arnoldi_cgs<double>(ctx, len, w, Qd, m, hcol, qnew, scratch, flag);
+}
+
+// The same step with the basis stored in FP32 (FP64 arithmetic; q_m is rounded to FP32 once).
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Qd, int m, double* hcol,
+                       float* qnew, double* scratch, int* flag) {
+    return arnoldi_cgs<float>(ctx, len, w, Qd, m, hcol, qnew, scratch, flag);
 }
 
 }  // namespace bsp

@@ -6,7 +6,8 @@
 // inner-outer preconditioned GMRES", 1993) with Morgan's deflated restart:
 //   z_j = P(v_j)            preconditioner in FP32 (or FP64 for an FP64 M), stored FP32
 //   w   = A z_j             SpMV with FP32 values and FP32 accumulation (or FP64)
-//   MGS against V (FP64), H column, v_{j+1} = w / h_{j+1,j}  (FP64)
+//   Gram-Schmidt against V (FP64 arithmetic; V stored in FP64, or in FP32 with the
+//   fgmres_basis_f32 option), H column, v_{j+1} = w / h_{j+1,j}
 //   least squares on H (FP64, host), x += Z y with FP64 accumulation
 //   restart: harmonic Ritz subspace of H (FP64) -> V' = V P_{k+1}, Z' = Z P_k,
 //            H' = P_{k+1}^T H P_k (A Z' = V' H' is preserved).
@@ -14,6 +15,7 @@
 #include <cstdio>
 #include <deque>
 #include <limits>
+#include <type_traits>
 
 #include "bsp_dense.hpp"
 #include "bsp_internal.hpp"
@@ -62,36 +64,39 @@ struct InnerStats {
     bool nonfinite = false;   // the pass met a non-finite value; its last update was dropped
 };
 
-// One inner FGMRES-DR pass on A e = r for the current x (x += e on return).
+// One inner FGMRES-DR pass on A e = r for the current x (x += e on return).  VT is the
+// storage type of the Arnoldi basis V (FP64, or FP32 with FP64 Gram-Schmidt arithmetic).
+template <typename VT>
 int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double reduction,
                 int maxIters, int restart, int k, bool spmv_f32, InnerStats& st, double* r_work, double* t64) {
     const int n = A->n_rows, bs = A->bs;
     const int64_t len = (int64_t)n * bs;
     const double eps = std::numeric_limits<double>::epsilon();
     const int m_max = restart > 0 ? restart : maxIters + 1;
-    Pool<double> V;
+    constexpr bool vf32 = std::is_same_v<VT, float>;
+    Pool<VT> V;
     Pool<float> Z;
     V.len = Z.len = (size_t)len;
     DevMem hcol, ydev, Pdev, Pkdev;
-    BSP_HIP(hcol.alloc(sizeof(double) * (m_max + 2)));
+    BSP_HIP(hcol.alloc(sizeof(double) * 2 * (m_max + 2)));
     BSP_HIP(ydev.alloc(sizeof(double) * (m_max + 2)));
     Pinned hpin, ypin, ppin;
-    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
+    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * 2 * (m_max + 2), hipHostMallocDefault));
     BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
     // classical Gram-Schmidt (default): device pointer array of the basis, scratch, flag
-    const bool use_cgs = ctx->fgmres_ortho == 0;
+    const bool use_cgs = vf32 || ctx->fgmres_ortho == 0;
     DevMem qptr_dev, cgs_scratch, cgs_flag;
     struct PinnedPtrs {
-        const double** p = nullptr;
+        const VT** p = nullptr;
         ~PinnedPtrs() {
             if (p) (void)hipHostFree(p);
         }
     } qptr_host;
     if (use_cgs) {
-        BSP_HIP(qptr_dev.alloc(sizeof(double*) * (m_max + 2)));
+        BSP_HIP(qptr_dev.alloc(sizeof(VT*) * (m_max + 2)));
         BSP_HIP(cgs_scratch.alloc(sizeof(double) * cgs_scratch_doubles(m_max + 1)));
         BSP_HIP(cgs_flag.alloc(sizeof(int)));
-        BSP_HIP(hipHostMalloc((void**)&qptr_host.p, sizeof(double*) * (m_max + 2), hipHostMallocDefault));
+        BSP_HIP(hipHostMalloc((void**)&qptr_host.p, sizeof(VT*) * (m_max + 2), hipHostMallocDefault));
     }
     if (restart > 0) {
         BSP_HIP(Pdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
@@ -109,9 +114,9 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     st.beta = beta;
     st.res = beta;
     if (!(beta > 0.0) || !std::isfinite(beta)) return std::isfinite(beta) ? BSP_OK : fail(ctx, BSP_E_NUMERIC, "residual not finite");
-    std::vector<double*> Vb;
+    std::vector<VT*> Vb;
     std::vector<float*> Zb;
-    double* v0 = nullptr;
+    VT* v0 = nullptr;
     rc = V.take(ctx, v0);
     if (rc) return rc;
     rc = launch_scale_div(ctx, len, r_work, sc + SLOT_TMP0, v0);
@@ -130,57 +135,98 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         Hn = nn;
     };
     const bool f32_pre = M->f32;
-    double* hd = hcol.as<double>();
+    const size_t hslot = (size_t)m_max + 2;   // H column slots: step j uses slot j % 2
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (hipEvent_t& x : ev.e) BSP_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
     bool restarted = false;
     double res = beta;
     int iter = 0, tot = 0;
     double* w = r_work;   // reuse: r is not needed after V[0] is formed
-    while (res > target) {
-        H_resize(iter + 2, iter + 1);
-        // z = P(v_iter) in reduced precision
+    // One Arnoldi step for column `col`, enqueued without waiting: z = P(V[col]), w = A z,
+    // Gram-Schmidt of w against V[0..col], q = w / h -> V[col + 1] (appended to Vb at once),
+    // and the H column copied to host slot `slot` (event ev.e[slot]).
+    struct Step {
         float* z = nullptr;
-        rc = Z.take(ctx, z);
-        if (rc) return rc;
+        VT* vn = nullptr;
+        int slot = 0;
+    };
+    auto enqueue_step = [&](int col, int slot, Step& s) -> int {
+        s.slot = slot;
+        int r2 = Z.take(ctx, s.z);
+        if (r2) return r2;
+        float* z = s.z;
         if (f32_pre) {
-            rc = launch_d2f(ctx, len, Vb[iter], z);
-            if (!rc) rc = precond_apply_f32(ctx, M, z);
+            if constexpr (vf32) {
+                BSP_HIP(hipMemcpyAsync(z, Vb[col], sizeof(float) * len, hipMemcpyDeviceToDevice, ctx->stream));
+            } else {
+                r2 = launch_d2f(ctx, len, Vb[col], z);
+            }
+            if (!r2) r2 = precond_apply_f32(ctx, M, z);
         } else {
-            rc = launch_copy(ctx, len, Vb[iter], t64);
-            if (!rc) rc = bsp_precond_apply(ctx, M, t64);
-            if (!rc) rc = launch_d2f(ctx, len, t64, z);
+            if constexpr (vf32) r2 = launch_f2d(ctx, len, Vb[col], t64);
+            else r2 = launch_copy(ctx, len, Vb[col], t64);
+            if (!r2) r2 = bsp_precond_apply(ctx, M, t64);
+            if (!r2) r2 = launch_d2f(ctx, len, t64, z);
         }
-        if (rc) return rc;
-        Zb.push_back(z);
+        if (r2) return r2;
         // w = A z
         if (spmv_f32) {
-            rc = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
+            r2 = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
         } else {
-            rc = launch_f2d(ctx, len, z, t64);
-            if (!rc) rc = bsmv_matrix(ctx, A, t64, w, 0);
+            r2 = launch_f2d(ctx, len, z, t64);
+            if (!r2) r2 = bsmv_matrix(ctx, A, t64, w, 0);
         }
-        if (rc) return rc;
-        // MGS (FP64)
-        double* vn = nullptr;
-        rc = V.take(ctx, vn);
-        if (rc) return rc;
+        if (r2) return r2;
+        r2 = V.take(ctx, s.vn);
+        if (r2) return r2;
+        double* hd = hcol.as<double>() + slot * hslot;
         if (use_cgs) {
-            // the previous iteration's copies finished at its host synchronisation
-            for (int j = 0; j <= iter; ++j) qptr_host.p[j] = Vb[j];
-            BSP_HIP(hipMemcpyAsync(qptr_dev.p, qptr_host.p, sizeof(double*) * (iter + 1), hipMemcpyHostToDevice,
+            // entries 0..col only grow within a cycle (a restart synchronises before it
+            // replaces them), so a copy still pending from the previous step reads the same values
+            for (int j = 0; j <= col; ++j) qptr_host.p[j] = Vb[j];
+            BSP_HIP(hipMemcpyAsync(qptr_dev.p, qptr_host.p, sizeof(VT*) * (col + 1), hipMemcpyHostToDevice,
                                    ctx->stream));
-            rc = launch_arnoldi_cgs(ctx, len, w, qptr_dev.as<const double*>(), iter + 1, hd, vn,
+            r2 = launch_arnoldi_cgs(ctx, len, w, qptr_dev.as<const VT*>(), col + 1, hd, s.vn,
                                     cgs_scratch.as<double>(), cgs_flag.as<int>());
         } else {
-            rc = launch_arnoldi_mgs(ctx, len, w, Vb.data(), iter, hd, vn, ctx->red_partials.as<double>());
+            if constexpr (!vf32)
+                r2 = launch_arnoldi_mgs(ctx, len, w, Vb.data(), col, hd, s.vn, ctx->red_partials.as<double>());
         }
+        if (r2) return r2;
+        BSP_HIP(hipMemcpyAsync(hpin.p + slot * hslot, hd, sizeof(double) * (col + 2), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        BSP_HIP(hipEventRecord(ev.e[slot], ctx->stream));
+        Vb.push_back(s.vn);
+        return BSP_OK;
+    };
+    // The step after the current one is enqueued before the host waits for the current H
+    // column, so the device works on step j + 1 while the host solves the least-squares
+    // problem of step j.  A step enqueued past the end of the pass (convergence, breakdown,
+    // the iteration limit) is dropped unused; none is enqueued past a restart.
+    Step cur, nxt;
+    if (res > target) {
+        rc = enqueue_step(iter, 0, cur);
         if (rc) return rc;
-        BSP_HIP(hipMemcpyAsync(hpin.p, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
-        BSP_HIP(hipStreamSynchronize(ctx->stream));
-        for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hpin.p[j];
-        const double hlast = hpin.p[iter + 1];
+    }
+    while (res > target) {
+        const bool spec = (restart <= 0 || iter + 1 < restart) && tot + 1 < maxIters;
+        if (spec) {
+            rc = enqueue_step(iter + 1, cur.slot ^ 1, nxt);
+            if (rc) return rc;
+        }
+        BSP_HIP(hipEventSynchronize(ev.e[cur.slot]));
+        H_resize(iter + 2, iter + 1);
+        const double* hp = hpin.p + cur.slot * hslot;
+        for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hp[j];
+        const double hlast = hp[iter + 1];
         const bool breakdown = std::fabs(hlast) < zero_thresh || !std::isfinite(hlast);
-        if (!breakdown) Vb.push_back(vn);
-        else V.give(vn);
+        Zb.push_back(cur.z);
         const int m = iter + 2, nn = iter + 1;
         std::vector<double> c(m, 0.0);
         if (!restarted) c[0] = beta;
@@ -234,18 +280,19 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
                 for (int j = 0; j < kout; ++j) Pk_host[(size_t)i * kout + j] = P[(size_t)i * kp + j];
             BSP_HIP(hipMemcpyAsync(Pdev.p, ppin.p, sizeof(double) * P.size(), hipMemcpyHostToDevice, ctx->stream));
             BSP_HIP(hipMemcpyAsync(Pkdev.p, Pk_host, sizeof(double) * nn * kout, hipMemcpyHostToDevice, ctx->stream));
-            std::vector<double*> Vn(kp);
+            std::vector<VT*> Vn(kp);
             std::vector<float*> Zn(kout);
             for (int j = 0; j < kp; ++j)
                 if ((rc = V.take(ctx, Vn[j]))) return rc;
             for (int j = 0; j < kout; ++j)
                 if ((rc = Z.take(ctx, Zn[j]))) return rc;
-            rc = launch_basis_rotate(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
+            if constexpr (vf32) rc = launch_basis_rotate_f32(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
+            else rc = launch_basis_rotate(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
             if (!rc && kout > 0)
                 rc = launch_basis_rotate_f32(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
             if (rc) return rc;
             BSP_HIP(hipStreamSynchronize(ctx->stream));
-            for (double* p : Vb) V.give(p);
+            for (VT* p : Vb) V.give(p);
             for (float* p : Zb) Z.give(p);
             Vb = Vn;
             Zb = Zn;
@@ -255,7 +302,13 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             iter = kout;
             restarted = true;
             y.clear();
+            rc = enqueue_step(iter, 0, cur);
+            if (rc) return rc;
+            continue;
         }
+        if (res <= target) break;
+        BSP_CHECK(spec, BSP_E_NUMERIC, "fgmres: no step enqueued");
+        cur = nxt;
     }
     if (!y.empty()) {
         for (size_t i = 0; i < y.size(); ++i) ypin.p[i] = y[i];
@@ -318,8 +371,12 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
         // the pass reduces the current residual by `inner_reduction` (or to the target)
         const double pass_red = std::max(inner_reduction, 0.5 * tol / std::max(rr, 1e-300));
         InnerStats st;
-        rc = fgmres_pass(ctx, A, cur, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k, spmv_f32, st,
-                         rw.as<double>(), t64.as<double>());
+        if (ctx->fgmres_basis_f32)
+            rc = fgmres_pass<float>(ctx, A, cur, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k,
+                                    spmv_f32, st, rw.as<double>(), t64.as<double>());
+        else
+            rc = fgmres_pass<double>(ctx, A, cur, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k,
+                                     spmv_f32, st, rw.as<double>(), t64.as<double>());
         if (rc) return rc;
         if (passes == 0) res0 = st.beta;
         total += st.iters;

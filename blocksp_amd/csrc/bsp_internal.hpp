@@ -108,6 +108,8 @@ struct bsp_ctx {
                                      // (2.2x slower); 2 row-major coalesced 16-byte chunks (slower)
     int fgmres_ortho = 0;            // mixed FGMRES orthogonalisation: 0 classical Gram-Schmidt with conditional
                                      // reorthogonalisation, 1 modified Gram-Schmidt (the reference's)
+    int fgmres_basis_f32 = 0;        // mixed FGMRES Arnoldi basis storage: 0 FP64, 1 FP32 (FP64 arithmetic,
+                                     // classical Gram-Schmidt only)
     int bf16_variant = 0;            // BF16 MFMA SpMM on the 64^3 DG matrix, 32 RHS: 0 loads of 4 slots of a
                                      // row pair issued before their MFMAs + streaming matrix loads / result
                                      // stores (176 us); 1 slot by slot, streaming (193 us); 2 slot by slot,
@@ -224,8 +226,11 @@ int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double*
 size_t cgs_scratch_doubles(int mmax);
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Qd, int m, double* hcol,
                        double* qnew, double* scratch, int* flag);
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Qd, int m, double* hcol,
+                       float* qnew, double* scratch, int* flag);
 // q = v / h[0]
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
+int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, float* q);
 // x += sum_i c[i] * Q[i]  (c on device, m vectors)
 int launch_combine_add(const bsp_ctx* ctx, int64_t len, int m, const double* const* Q, const double* c,
                        double* x);

@@ -561,10 +561,11 @@ __global__ __launch_bounds__(kBlock) void k_scal(int64_t len, double a, double* 
     if (i < len) x[i] *= a;
 }
 
+template <typename QT>
 __global__ __launch_bounds__(kBlock) void k_scale_div(int64_t len, const double* __restrict__ v,
-                                                      const double* __restrict__ h, double* __restrict__ q) {
+                                                      const double* __restrict__ h, QT* __restrict__ q) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < len) q[i] = v[i] / *h;
+    if (i < len) q[i] = (QT)(v[i] / *h);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(int64_t len, const double* __restrict__ x, double* __restrict__ y) {
@@ -877,7 +878,13 @@ int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qp
 }
 
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q) {
-    hipLaunchKernelGGL(k_scale_div, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, v, h, q);
+    hipLaunchKernelGGL(k_scale_div<double>, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, v, h, q);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, float* q) {
+    hipLaunchKernelGGL(k_scale_div<float>, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, v, h, q);
     BSP_LAUNCH_CHECK();
     return BSP_OK;
 }

@@ -153,3 +153,31 @@ def test_fgmres_mixed_with_both_orthogonalisations(ctx, ortho):
         ctx.set_option("fgmres_ortho", 0)
     assert rc == 0 and st.true_relres <= 1e-10
     assert np.linalg.norm(p.b - p.bsmv(x.numpy())) <= 1e-10 * np.linalg.norm(p.b)
+
+
+@pytest.mark.parametrize("prec", [bsp.PREC_F32, bsp.PREC_F64])
+def test_fgmres_mixed_fp32_basis_reaches_fp64_residual(ctx, prec):
+    # FP32 storage of the Arnoldi basis (FP64 Gram-Schmidt arithmetic); the FP64 outer
+    # refinement still drives the true residual to the FP64 tolerance
+    dims = (48, 48)
+    p = pr.dg2d_veclap(*dims, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    b = ctx.to_device(p.b)
+    M = bsp.ILU0Ex(ctx, A, colour=pr.parity_colours(dims), precision=prec)
+    x64, st64, rc64 = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M)
+    ctx.set_option("fgmres_basis_f32", 1)
+    try:
+        x, st, rc = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M)
+    finally:
+        ctx.set_option("fgmres_basis_f32", 0)
+    relres = np.linalg.norm(p.b - p.bsmv(x.numpy())) / np.linalg.norm(p.b)
+    assert rc == 0 and st.converged == 1 and relres <= 1e-10, (rc, relres, st.iters)
+    assert abs(st.true_relres - relres) <= 1e-2 * relres + 1e-16
+    assert rc64 == 0 and st.iters <= 1.3 * st64.iters + 30, (st.iters, st64.iters)
+    xo = x64.numpy()
+    assert np.linalg.norm(x.numpy() - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
+def test_fgmres_basis_option_is_checked(ctx):
+    with pytest.raises(Exception):
+        ctx.set_option("fgmres_basis_f32", 2)
