@@ -3,16 +3,18 @@
 // FP64 or in FP32 (half the bytes per pass; each q_m is rounded to FP32 once, after it is
 // orthogonalised and normalised in FP64).
 //
-// For w against the orthonormal q_0..q_{m-1} (device pointer array Q):
+// For w against the orthonormal q_0..q_{m-1} (device pointers, passed by value):
 //   A : h1 = Q^T w and ||w||^2                     (reads the m vectors and w once)
 //   B : w <- w - Q h1 and ||w||^2                  (reads the m vectors and w, writes w)
 //   if ||w_new|| < 0.01 ||w_old|| (cancellation; the Daniel-Gragg-Kaufman-Stewart test with
 //   a threshold chosen for FP64) the pass is repeated once (A2, B2) and h = h1 + h2;
-//   q_m = w / ||w||.
-// The decision is taken on the device (a flag written by the reduction after B); A2 and B2
-// are always launched and return at once when the flag is clear, so the host never waits
-// inside a step.  Per step this reads 2m + 4 vectors (m + 1 with MGS ... 4m + 4), in about
-// nine launches instead of m + 2.  All sums are per-block partials added in a fixed order.
+//   q_m = w / ||w|| (and, on request, an FP32 copy of q_m for the next preconditioner apply).
+// The decision is taken on the device (a flag written after B); A2 and B2 are always
+// launched and return at once when the flag is clear, so the host never waits inside a step.
+// Each pass is followed by a small kernel that adds its per-workgroup partials in a fixed
+// order; a step is nine launches and reads 2m + 4 vectors (modified Gram-Schmidt: m + 2
+// launches, 4m + 4 vectors).  (Folding those sums into the last workgroup of each pass, with a
+// device-scope fence per workgroup, was measured 20% slower per step.)
 #include "bsp_internal.hpp"
 
 #include <algorithm>
@@ -30,6 +32,11 @@ constexpr int kMaxBasis = 64;      // basis vectors (FGMRES restart <= 62)
 // relative size ~eps ||w_old|| / ||w_new|| along the basis, so below this ratio orthogonality
 // would degrade past ~1e-14; above it the second pass only costs bandwidth.
 constexpr double kReorth2 = 1e-4;
+
+template <typename QT>
+struct QPtrs {
+    const QT* q[kMaxBasis];
+};
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -88,13 +95,19 @@ __device__ __forceinline__ void load_q(const float* q, int64_t i, double (&o)[4]
     o[3] = v.w;
 }
 
-// Multi-dot: part[s][b] = sum over block b's elements of q_s w  (s = 8 g + k, grid.y = g),
-// and (g == 0) part[m][b] = sum of w^2.  Skipped when *flag == 0 (gate != nullptr).
+// Scratch of one step: part[(m + 1) * kMaxParts], part_norm[kMaxParts], st[2m + 2]
+// (h1, h2, ||w_old||^2, ||w_new||).
+struct Scratch {
+    double* part;
+    double* part_norm;
+    double* st;
+};
+
+// Pass A or A2 (gated): part[s][b] = sum over block b's elements of q_s w (s = 8 g + k,
+// grid.y = g) and, for g == 0, part[m][b] = sum of w^2.
 template <typename QT>
-__global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* __restrict__ w,
-                                                     const QT* const* __restrict__ Q, int m,
-                                                     double* __restrict__ part, int nparts,
-                                                     const int* __restrict__ gate) {
+__global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* __restrict__ w, QPtrs<QT> Q, int m,
+                                                     Scratch sc, int nparts, const int* __restrict__ gate) {
     constexpr int N = QVec<QT>::N;
     if (gate && *gate == 0) return;
     __shared__ double sh[kBlock / kWave][kGroup + 1];
@@ -106,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* 
     for (int k = 0; k <= kGroup; ++k) acc[k] = 0.0;
     const QT* q[kGroup];
 #pragma unroll
-    for (int k = 0; k < kGroup; ++k) q[k] = k < ns ? Q[s0 + k] : nullptr;
+    for (int k = 0; k < kGroup; ++k) q[k] = k < ns ? Q.q[s0 + k] : nullptr;
     const int64_t nvec = len / N;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * kBlock) {
         double wi[N];
@@ -135,15 +148,14 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* 
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < kGroup; ++k)
-            if (k < ns) part[(size_t)(s0 + k) * nparts + blockIdx.x] = acc[k];
-        if (g == 0) part[(size_t)m * nparts + blockIdx.x] = acc[kGroup];
+            if (k < ns) sc.part[(size_t)(s0 + k) * nparts + blockIdx.x] = acc[k];
+        if (g == 0) sc.part[(size_t)m * nparts + blockIdx.x] = acc[kGroup];
     }
 }
 
-// w <- w - sum_s h[s] q_s; part_norm[b] = block sum of the new w^2.
+// Pass B or B2 (gated): w <- w - sum_s h[s] q_s; part_norm[b] = block sum of the new w^2.
 template <typename QT>
-__global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __restrict__ w,
-                                                       const QT* const* __restrict__ Q, int m,
+__global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __restrict__ w, QPtrs<QT> Q, int m,
                                                        const double* __restrict__ h, double* __restrict__ part_norm,
                                                        const int* __restrict__ gate) {
     constexpr int N = QVec<QT>::N;
@@ -152,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __re
     __shared__ const QT* qs[kMaxBasis];
     __shared__ double hs[kMaxBasis];
     for (int s = threadIdx.x; s < m; s += kBlock) {
-        qs[s] = Q[s];
+        qs[s] = Q.q[s];
         hs[s] = h[s];
     }
     __syncthreads();
@@ -187,16 +199,15 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __re
 //   stage 1 (after B):  n2 = sum part_norm; flag = (n2 < 1e-4 wn2); if !flag: hcol = h1, norm
 //   stage 2 (after A2): h2[s] = sum part[s]; hcol[s] = h1[s] + h2[s]      (flag set only)
 //   stage 3 (after B2): hcol[m] = norm = sqrt(sum part_norm)            (flag set only)
-__global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, const double* __restrict__ part,
-                                                       const double* __restrict__ part_norm, int nparts,
-                                                       double* __restrict__ st, int* __restrict__ flag,
-                                                       double* __restrict__ hcol) {
+__global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, Scratch sc, int nparts,
+                                                       int* __restrict__ flag, double* __restrict__ hcol) {
     // st: [0, m) h1, [m, 2m) h2, 2m wn2, 2m + 1 norm
     if ((stage == 2 || stage == 3) && *flag == 0) return;
     __shared__ double sh[kBlock / kWave][1];
+    double* st = sc.st;
     const int t = blockIdx.x;   // one block per output
     double acc[1] = {0.0};
-    const double* src = (stage == 0 || stage == 2) ? part + (size_t)t * nparts : part_norm;
+    const double* src = (stage == 0 || stage == 2) ? sc.part + (size_t)t * nparts : sc.part_norm;
     for (int b = threadIdx.x; b < nparts; b += kBlock) acc[0] += src[b];
     block_sums<1>(acc, sh);
     if (threadIdx.x != 0) return;
@@ -223,42 +234,53 @@ __global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, const d
     }
 }
 
+// q = w / ||w|| in the basis precision; z (optional) = FP32 copy of q, the input of the next
+// step's FP32 preconditioner apply.
 template <typename QT>
 __global__ __launch_bounds__(kBlock) void k_cgs_scale(int64_t len, const double* __restrict__ w,
                                                       const double* __restrict__ st, int m,
-                                                      QT* __restrict__ q) {
+                                                      QT* __restrict__ q, float* __restrict__ z) {
     const double h = st[2 * m + 1];
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (int64_t)gridDim.x * kBlock)
-        q[i] = (QT)(w[i] / h);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (int64_t)gridDim.x * kBlock) {
+        const QT v = (QT)(w[i] / h);
+        q[i] = v;
+        if (z) z[i] = (float)v;
+    }
+}
+
+Scratch carve(double* scratch, int m) {
+    Scratch sc;
+    sc.part = scratch;
+    sc.part_norm = sc.part + (size_t)(m + 1) * kMaxParts;
+    sc.st = sc.part_norm + kMaxParts;
+    return sc;
 }
 
 template <typename QT>
-int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Qd, int m, double* hcol, QT* qnew,
-                double* scratch, int* flag) {
+int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, int m, double* hcol, QT* qnew,
+                float* znext, double* scratch, int* flag) {
     if (m < 1 || m > kMaxBasis) return fail(ctx, BSP_E_INVALID, "cgs: basis size out of range");
+    QPtrs<QT> qp{};
+    for (int s = 0; s < m; ++s) qp.q[s] = Q[s];
     const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((len + 8 * kBlock - 1) / (8 * kBlock), kMaxParts));
-    double* part = scratch;
-    double* part_norm = scratch + (size_t)(m + 1) * kMaxParts;
-    double* st = part_norm + kMaxParts;
+    const Scratch sc = carve(scratch, m);
     const int ng = (m + kGroup - 1) / kGroup;
     const int mm = std::max(m, 1);
-    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb,
+    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc, nb,
                        (const int*)nullptr);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, part, part_norm, nb, st, flag,
-                       hcol);
-    auto update = [&](const double* h, const int* gate) {
-        hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, h, part_norm,
-                           gate);
-    };
-    update(st, nullptr);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, part, part_norm, nb, st, flag, hcol);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, sc, nb, flag, hcol);
+    hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st, sc.part_norm,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, sc, nb, flag, hcol);
     // second pass, only when the device flagged cancellation
-    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, Qd, m, part, nb, flag);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(mm), dim3(kBlock), 0, ctx->stream, 2, m, part, part_norm, nb, st, flag, hcol);
-    update(st + m, flag);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, part, part_norm, nb, st, flag, hcol);
+    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc, nb,
+                       (const int*)flag);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(mm), dim3(kBlock), 0, ctx->stream, 2, m, sc, nb, flag, hcol);
+    hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st + m,
+                       sc.part_norm, (const int*)flag);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, sc, nb, flag, hcol);
     const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(k_cgs_scale<QT>, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, st, m, qnew);
+    hipLaunchKernelGGL(k_cgs_scale<QT>, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, sc.st, m, qnew, znext);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("cgs launch: ") + hipGetErrorString(e));
     return BSP_OK;
@@ -266,18 +288,18 @@ int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Qd,
 
 }  // namespace
 
-size_t cgs_scratch_doubles(int mmax) { return (size_t)(mmax + 1) * kMaxParts + kMaxParts + 2 * mmax + 4; }
+size_t cgs_scratch_doubles() { return (size_t)(kMaxBasis + 1) * kMaxParts + kMaxParts + 2 * kMaxBasis + 2; }
 
-// scratch: cgs_scratch_doubles(m) doubles; flag: one int (device)
-int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Qd, int m, double* hcol,
-                       double* qnew, double* scratch, int* flag) {
-    return arnoldi_cgs<double>(ctx, len, w, Qd, m, hcol, qnew, scratch, flag);
+// Q: host array of m device pointers; scratch: cgs_scratch_doubles() doubles; flag: one int.
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
+                       double* qnew, float* znext, double* scratch, int* flag) {
+    return arnoldi_cgs<double>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag);
 }
 
 // The same step with the basis stored in FP32 (FP64 arithmetic; q_m is rounded to FP32 once).
-int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Qd, int m, double* hcol,
-                       float* qnew, double* scratch, int* flag) {
-    return arnoldi_cgs<float>(ctx, len, w, Qd, m, hcol, qnew, scratch, flag);
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,
+                       float* qnew, float* znext, double* scratch, int* flag) {
+    return arnoldi_cgs<float>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag);
 }
 
 }  // namespace bsp
@@ -291,14 +313,12 @@ int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs
     if (!ctx || !Q_dev_ptrs || !w_dev || !qnew_dev || len <= 0) return BSP_E_INVALID;
     BSP_CHECK(m >= 1 && m <= 64, BSP_E_INVALID, "orthogonalize: 1 <= m <= 64 vectors");
     BSP_HIP(hipSetDevice(ctx->device));
-    DevMem qd, scratch, flag, hcol;
-    BSP_HIP(qd.alloc(sizeof(double*) * m));
-    BSP_HIP(scratch.alloc(sizeof(double) * cgs_scratch_doubles(m)));
+    DevMem scratch, flag, hcol;
+    BSP_HIP(scratch.alloc(sizeof(double) * cgs_scratch_doubles()));
     BSP_HIP(flag.alloc(sizeof(int)));
     BSP_HIP(hcol.alloc(sizeof(double) * (m + 1)));
-    BSP_HIP(hipMemcpy(qd.p, Q_dev_ptrs, sizeof(double*) * m, hipMemcpyHostToDevice));
-    int rc = launch_arnoldi_cgs(ctx, len, w_dev, qd.as<const double*>(), m, hcol.as<double>(), qnew_dev,
-                                scratch.as<double>(), flag.as<int>());
+    int rc = launch_arnoldi_cgs(ctx, len, w_dev, Q_dev_ptrs, m, hcol.as<double>(), qnew_dev, nullptr, scratch.as<double>(),
+                            flag.as<int>());
     if (rc) return rc;
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     if (h_host) BSP_HIP(hipMemcpy(h_host, hcol.p, sizeof(double) * (m + 1), hipMemcpyDeviceToHost));

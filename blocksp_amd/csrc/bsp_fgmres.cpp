@@ -83,20 +83,12 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     Pinned hpin, ypin, ppin;
     BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * 2 * (m_max + 2), hipHostMallocDefault));
     BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
-    // classical Gram-Schmidt (default): device pointer array of the basis, scratch, flag
+    // classical Gram-Schmidt (default): scratch and the reorthogonalisation flag
     const bool use_cgs = vf32 || ctx->fgmres_ortho == 0;
-    DevMem qptr_dev, cgs_scratch, cgs_flag;
-    struct PinnedPtrs {
-        const VT** p = nullptr;
-        ~PinnedPtrs() {
-            if (p) (void)hipHostFree(p);
-        }
-    } qptr_host;
+    DevMem cgs_scratch, cgs_flag;
     if (use_cgs) {
-        BSP_HIP(qptr_dev.alloc(sizeof(VT*) * (m_max + 2)));
-        BSP_HIP(cgs_scratch.alloc(sizeof(double) * cgs_scratch_doubles(m_max + 1)));
+        BSP_HIP(cgs_scratch.alloc(sizeof(double) * cgs_scratch_doubles()));
         BSP_HIP(cgs_flag.alloc(sizeof(int)));
-        BSP_HIP(hipHostMalloc((void**)&qptr_host.p, sizeof(VT*) * (m_max + 2), hipHostMallocDefault));
     }
     if (restart > 0) {
         BSP_HIP(Pdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
@@ -151,17 +143,23 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     // One Arnoldi step for column `col`, enqueued without waiting: z = P(V[col]), w = A z,
     // Gram-Schmidt of w against V[0..col], q = w / h -> V[col + 1] (appended to Vb at once),
     // and the H column copied to host slot `slot` (event ev.e[slot]).
+    // With an FP32 preconditioner the Gram-Schmidt step also writes the FP32 copy of V[col + 1]
+    // (znext), which the next step preconditions in place instead of converting V[col + 1].
     struct Step {
         float* z = nullptr;
+        float* znext = nullptr;
         VT* vn = nullptr;
         int slot = 0;
     };
-    auto enqueue_step = [&](int col, int slot, Step& s) -> int {
+    auto enqueue_step = [&](int col, int slot, Step& s, float* zin) -> int {
         s.slot = slot;
-        int r2 = Z.take(ctx, s.z);
-        if (r2) return r2;
+        int r2 = BSP_OK;
+        if (zin) s.z = zin;
+        else if ((r2 = Z.take(ctx, s.z))) return r2;
         float* z = s.z;
-        if (f32_pre) {
+        if (f32_pre && zin) {
+            r2 = precond_apply_f32(ctx, M, z);
+        } else if (f32_pre) {
             if constexpr (vf32) {
                 BSP_HIP(hipMemcpyAsync(z, Vb[col], sizeof(float) * len, hipMemcpyDeviceToDevice, ctx->stream));
             } else {
@@ -185,15 +183,12 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         if (r2) return r2;
         r2 = V.take(ctx, s.vn);
         if (r2) return r2;
+        s.znext = nullptr;
         double* hd = hcol.as<double>() + slot * hslot;
         if (use_cgs) {
-            // entries 0..col only grow within a cycle (a restart synchronises before it
-            // replaces them), so a copy still pending from the previous step reads the same values
-            for (int j = 0; j <= col; ++j) qptr_host.p[j] = Vb[j];
-            BSP_HIP(hipMemcpyAsync(qptr_dev.p, qptr_host.p, sizeof(VT*) * (col + 1), hipMemcpyHostToDevice,
-                                   ctx->stream));
-            r2 = launch_arnoldi_cgs(ctx, len, w, qptr_dev.as<const VT*>(), col + 1, hd, s.vn,
-                                    cgs_scratch.as<double>(), cgs_flag.as<int>());
+            if (f32_pre && (r2 = Z.take(ctx, s.znext))) return r2;
+            r2 = launch_arnoldi_cgs(ctx, len, w, Vb.data(), col + 1, hd, s.vn, s.znext, cgs_scratch.as<double>(),
+                                    cgs_flag.as<int>());
         } else {
             if constexpr (!vf32)
                 r2 = launch_arnoldi_mgs(ctx, len, w, Vb.data(), col, hd, s.vn, ctx->red_partials.as<double>());
@@ -211,13 +206,13 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     // the iteration limit) is dropped unused; none is enqueued past a restart.
     Step cur, nxt;
     if (res > target) {
-        rc = enqueue_step(iter, 0, cur);
+        rc = enqueue_step(iter, 0, cur, nullptr);
         if (rc) return rc;
     }
     while (res > target) {
         const bool spec = (restart <= 0 || iter + 1 < restart) && tot + 1 < maxIters;
         if (spec) {
-            rc = enqueue_step(iter + 1, cur.slot ^ 1, nxt);
+            rc = enqueue_step(iter + 1, cur.slot ^ 1, nxt, cur.znext);
             if (rc) return rc;
         }
         BSP_HIP(hipEventSynchronize(ev.e[cur.slot]));
@@ -294,6 +289,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             BSP_HIP(hipStreamSynchronize(ctx->stream));
             for (VT* p : Vb) V.give(p);
             for (float* p : Zb) Z.give(p);
+            if (cur.znext) Z.give(cur.znext);   // an FP32 copy of a basis vector the rotation replaced
             Vb = Vn;
             Zb = Zn;
             H = Hnew;
@@ -302,7 +298,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             iter = kout;
             restarted = true;
             y.clear();
-            rc = enqueue_step(iter, 0, cur);
+            rc = enqueue_step(iter, 0, cur, nullptr);
             if (rc) return rc;
             continue;
         }

@@ -220,14 +220,14 @@ int launch_mgs_step(const bsp_ctx* ctx, int64_t len, double* v, const double* qp
 int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int j, double* hcol,
                        double* qnew, double* parts);
 // Classical Gram-Schmidt with one conditional reorthogonalisation (bsp_cgs.hip): w against
-// the m (<= 64) orthonormal vectors whose device pointers are in Qd (a device array);
-// hcol[0..m] (device) and qnew = w / hcol[m].  scratch: cgs_scratch_doubles(m) doubles,
-// flag: one int, both device.
-size_t cgs_scratch_doubles(int mmax);
-int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Qd, int m, double* hcol,
-                       double* qnew, double* scratch, int* flag);
-int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Qd, int m, double* hcol,
-                       float* qnew, double* scratch, int* flag);
+// the m (<= 64) orthonormal device vectors Q[0..m-1] (a host array of device pointers);
+// hcol[0..m] (device), qnew = w / hcol[m] and, if znext is not null, znext = FP32 copy of
+// qnew.  scratch: cgs_scratch_doubles() device doubles; flag: one device int.
+size_t cgs_scratch_doubles();
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
+                       double* qnew, float* znext, double* scratch, int* flag);
+int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,
+                       float* qnew, float* znext, double* scratch, int* flag);
 // q = v / h[0]
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, float* q);
