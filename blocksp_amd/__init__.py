@@ -435,6 +435,7 @@ class ILU0(_Precond):
 
 PREC_F64 = 0
 PREC_F32 = 1
+PREC_BF16 = 2   # ILU0 factor stored in BF16, applied in FP32
 
 
 class ILU0Ex(_Precond):

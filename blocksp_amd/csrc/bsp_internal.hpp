@@ -62,6 +62,8 @@ struct Sweep {
     const int* dci = nullptr;    // (SOR) the matrix's own arrays
     const void* dval = nullptr;
     bool f32 = false;            // values / vectors in FP32 instead of FP64
+    bool bf16 = false;           // values (and the diagonal inverses applied with them) stored in
+                                 // BF16, vectors and arithmetic in FP32 (f32 is set too)
     unsigned long long tickets_used = 0;   // ticket counter value at the next sync-free launch
     // schedule: rows sorted by level, each level padded with -1 to a multiple of
     // the rows handled by one wavefront, so a wavefront never holds two levels.
@@ -166,6 +168,7 @@ struct bsp_precond {
     bsp::DevMem work0, work1;         // vectors for SOR ping-pong
     uint32_t epoch = 0;
     bool f32 = false;                 // factor, diagonal inverses and applies in FP32
+    bool bf16 = false;                // factor and diagonal inverses stored in BF16 (applies in FP32)
     bsp::DevMem work32;               // FP32 staging vector for applies on FP64 vectors
     std::vector<int32_t> elim_order;  // elimination order of the factorisation (ILU0)
     bsp_host_precond_fn host_fn = nullptr;   // BSP_PRE_HOST
@@ -263,8 +266,9 @@ int precond_apply_f32(bsp_ctx* ctx, const bsp_precond* M, float* r);
 // (bsp_precond.cpp).  ilu0_build takes LU's storage.
 int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std::vector<int32_t>& elim,
                      std::vector<int32_t>& pos, std::vector<double>& LU);
+// precision: BSP_PREC_F64, BSP_PREC_F32 or BSP_PREC_BF16 (storage of factor and diagonal inverses).
 int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim, const std::vector<int32_t>& pos,
-               std::vector<double>& LU, bool f32, bsp_precond** out);
+               std::vector<double>& LU, int precision, bsp_precond** out);
 
 // Host copy of device-produced values (bsp_bsmm) and the explicit transpose (bsp_spgemm.hip).
 int sync_host_values(bsp_ctx* ctx, const bsp_bsr* A);

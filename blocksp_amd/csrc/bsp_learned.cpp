@@ -544,7 +544,8 @@ int bsp_precond_create_learned(bsp_ctx* ctx, const bsp_bsr* A, const bsp_mpnn* n
                                int precision, bsp_precond** out) {
     if (!ctx || !A || !net || !out) return BSP_E_INVALID;
     *out = nullptr;
-    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32, BSP_E_INVALID, "bad precision");
+    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
+              "bad precision");
     BSP_CHECK(net->bs == A->bs, BSP_E_INVALID, "network and matrix block sizes differ");
     BSP_HIP(hipSetDevice(ctx->device));
     std::vector<int32_t> elim, pos;
@@ -589,7 +590,7 @@ int bsp_precond_create_learned(bsp_ctx* ctx, const bsp_bsr* A, const bsp_mpnn* n
             }
         }
     for (double v : LU) BSP_CHECK(std::isfinite(v), BSP_E_NUMERIC, "learned factor correction is not finite");
-    return ilu0_build(ctx, A, elim, pos, LU, precision == BSP_PREC_F32, out);
+    return ilu0_build(ctx, A, elim, pos, LU, precision, out);
 }
 
 int bsp_ctx_record_basis(bsp_ctx* ctx, double* buf_dev, int64_t len, int cap, int every) {

@@ -80,8 +80,28 @@ hipError_t upload_f32(DevMem& m, const std::vector<double>& v) {
     return hipMemcpy(m.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice);
 }
 
+// FP64 -> BF16 bits: to FP32 (round to nearest), then the upper 16 bits rounded to nearest
+// even; NaN stays a quiet NaN.
+uint16_t to_bf16(double v) {
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+hipError_t upload_bf16(DevMem& m, const std::vector<double>& v) {
+    std::vector<uint16_t> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = to_bf16(v[i]);
+    hipError_t e = m.alloc(sizeof(uint16_t) * std::max<size_t>(h.size(), 1));
+    if (e != hipSuccess || h.empty()) return e;
+    return hipMemcpy(m.p, h.data(), sizeof(uint16_t) * h.size(), hipMemcpyHostToDevice);
+}
+
 hipError_t upload_structure(Sweep& S, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
-                            const std::vector<double>& val, int bs, bool f32 = false) {
+                            const std::vector<double>& val, int bs, int precision = BSP_PREC_F64) {
+    const bool f32 = precision != BSP_PREC_F64;
     const size_t nnz = ci.size();
     S.nnzb = (int64_t)nnz;
     hipError_t e;
@@ -89,7 +109,9 @@ hipError_t upload_structure(Sweep& S, const std::vector<int32_t>& rp, const std:
     if ((e = S.ci.alloc(sizeof(int) * std::max<size_t>(nnz, 1))) != hipSuccess) return e;
     if ((e = hipMemcpy(S.rp.p, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if (nnz && (e = hipMemcpy(S.ci.p, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if (f32) {
+    if (precision == BSP_PREC_BF16) {
+        if ((e = upload_bf16(S.val, val)) != hipSuccess) return e;
+    } else if (f32) {
         std::vector<float> fv(val.begin(), val.end());
         if ((e = S.val.alloc(sizeof(float) * std::max<size_t>(fv.size(), 1))) != hipSuccess) return e;
         if (nnz && (e = hipMemcpy(S.val.p, fv.data(), sizeof(float) * fv.size(), hipMemcpyHostToDevice)) != hipSuccess)
@@ -100,6 +122,7 @@ hipError_t upload_structure(Sweep& S, const std::vector<int32_t>& rp, const std:
             return e;
     }
     S.f32 = f32;
+    S.bf16 = precision == BSP_PREC_BF16;
     S.drp = S.rp.as<int>();
     S.dci = S.ci.as<int>();
     S.dval = S.val.p;
@@ -313,7 +336,10 @@ int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std:
 // Device preconditioner from an ILU(0) factor in A's block layout: L holds the blocks
 // (i, j) with pos[j] < pos[i], U those with pos[j] > pos[i] and the diagonal.
 int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim, const std::vector<int32_t>& pos,
-               std::vector<double>& LU, bool f32, bsp_precond** out) {
+               std::vector<double>& LU, int precision, bsp_precond** out) {
+    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
+              "ILU0 precision must be F64, F32 or BF16");
+    const bool f32 = precision != BSP_PREC_F64;
     BSP_HIP(hipSetDevice(ctx->device));
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
@@ -325,6 +351,7 @@ int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
     M->n = n;
     M->bs = bs;
     M->f32 = f32;
+    M->bf16 = precision == BSP_PREC_BF16;
     std::vector<double> dinv(n * b2);
     for (int i = 0; i < n; ++i) {
         const int pd = find_col(rp, ci, i, i);
@@ -348,13 +375,14 @@ int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
         lrp[i + 1] = (int32_t)lci.size();
         urp[i + 1] = (int32_t)uci.size();
     }
-    BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs, f32));
-    BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs, f32));
+    BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs, precision));
+    BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs, precision));
     std::vector<int32_t> rev(elim.rbegin(), elim.rend());
     build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
     build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
     if (f32) {
-        BSP_HIP(upload_f32(M->dinv_upper, dinv));
+        if (M->bf16) BSP_HIP(upload_bf16(M->dinv_upper, dinv));
+        else BSP_HIP(upload_f32(M->dinv_upper, dinv));
         BSP_HIP(M->work32.alloc(sizeof(float) * std::max<size_t>((size_t)n * bs, 1)));
     } else {
         BSP_HIP(upload(M->dinv_upper, dinv));
@@ -371,7 +399,7 @@ int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
 
 namespace {
 
-int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool f32, bsp_precond** out) {
+int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision, bsp_precond** out) {
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_HIP(hipSetDevice(ctx->device));
@@ -379,7 +407,7 @@ int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool
     std::vector<double> LU;
     int rc = ilu0_factor_host(ctx, A, colour, elim, pos, LU);
     if (rc) return rc;
-    return ilu0_build(ctx, A, elim, pos, LU, f32, out);
+    return ilu0_build(ctx, A, elim, pos, LU, precision, out);
 }
 
 }  // namespace
@@ -387,14 +415,15 @@ int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bool
 extern "C" {
 
 int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
-    return create_ilu0_impl(ctx, A, nullptr, false, out);
+    return create_ilu0_impl(ctx, A, nullptr, BSP_PREC_F64, out);
 }
 
 int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision,
                                bsp_precond** out) {
     if (!ctx) return BSP_E_INVALID;
-    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32, BSP_E_INVALID, "precision must be F64 or F32");
-    return create_ilu0_impl(ctx, A, colour, precision == BSP_PREC_F32, out);
+    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
+              "precision must be F64, F32 or BF16");
+    return create_ilu0_impl(ctx, A, colour, precision, out);
 }
 
 int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {

@@ -62,6 +62,43 @@ __device__ __forceinline__ void load_row(const VT* __restrict__ src, VT (&a)[BS]
     }
 }
 
+// BF16 matrix storage (bit patterns in uint16), widened to FP32: a BF16 value is the upper
+// half of the FP32 with the same bits.
+struct bf16_t {
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
+// One matrix row of BS values of storage type MT into arithmetic type VT.
+template <int BS, typename MT, typename VT>
+__device__ __forceinline__ void load_mrow(const MT* __restrict__ src, VT (&a)[BS]) {
+    if constexpr (std::is_same_v<MT, VT>) {
+        load_row<BS, VT>(src, a);
+    } else {
+        static_assert(std::is_same_v<MT, bf16_t> && std::is_same_v<VT, float>, "BF16 rows widen to FP32");
+        if constexpr (BS % 8 == 0) {
+            const uint4* s8 = reinterpret_cast<const uint4*>(src);
+#pragma unroll
+            for (int c = 0; c < BS / 8; ++c) {
+                const uint4 t = s8[c];
+                a[8 * c + 0] = bf16_lo(t.x);
+                a[8 * c + 1] = bf16_hi(t.x);
+                a[8 * c + 2] = bf16_lo(t.y);
+                a[8 * c + 3] = bf16_hi(t.y);
+                a[8 * c + 4] = bf16_lo(t.z);
+                a[8 * c + 5] = bf16_hi(t.z);
+                a[8 * c + 6] = bf16_lo(t.w);
+                a[8 * c + 7] = bf16_hi(t.w);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) a[c] = __uint_as_float((uint32_t)src[c].bits << 16);
+        }
+    }
+}
+
 template <int BS, typename VT>
 __device__ __forceinline__ VT row_dot(const VT (&a)[BS], const VT (&x)[BS]) {
     VT s = VT(0);
@@ -104,11 +141,12 @@ struct SweepArgs {
 };
 
 // One chunk of slots: [slot_begin, slot_end) with kWavesPerBlock * RPW slots per block.
-template <int BS, typename VT, bool SF, bool SOR>
+// Factor values and diagonal inverses are stored as MT, vectors and arithmetic are VT.
+template <int BS, typename VT, typename MT, bool SF, bool SOR>
 __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, int slot_end) {
     constexpr int RPW = kWave / BS;
-    const VT* __restrict__ val = static_cast<const VT*>(a.val);
-    const VT* __restrict__ dinv = static_cast<const VT*>(a.dinv);
+    const MT* __restrict__ val = static_cast<const MT*>(a.val);
+    const MT* __restrict__ dinv = static_cast<const MT*>(a.dinv);
     VT* x = static_cast<VT*>(a.x);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
@@ -141,7 +179,7 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
                 if (c == i) continue;
                 const bool dep = a.forward ? c < i : c > i;
                 VT av[BS], xv[BS];
-                load_row<BS, VT>(val + ((size_t)p * BS + r) * BS, av);
+                load_mrow<BS, MT, VT>(val + ((size_t)p * BS + r) * BS, av);
                 if (dep) {
                     load_row<BS, VT>(x + (size_t)c * BS, xv);
                 } else if (xo) {
@@ -158,7 +196,7 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
             for (int p = p0; p < p1; ++p) {
                 const int c = a.ci[p];
                 VT av[BS], xv[BS];
-                load_row<BS, VT>(val + ((size_t)p * BS + r) * BS, av);
+                load_mrow<BS, MT, VT>(val + ((size_t)p * BS + r) * BS, av);
                 load_row<BS, VT>(x + (size_t)c * BS, xv);
                 s -= row_dot<BS, VT>(av, xv);
             }
@@ -168,7 +206,7 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
     if (dinv) {
         const int base = (g < RPW ? g : 0) * BS;
         VT d[BS];
-        if (active) load_row<BS, VT>(dinv + ((size_t)i * BS + r) * BS, d);
+        if (active) load_mrow<BS, MT, VT>(dinv + ((size_t)i * BS + r) * BS, d);
         out = VT(0);
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
@@ -187,13 +225,13 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
     }
 }
 
-template <int BS, typename VT, bool SOR>
+template <int BS, typename VT, typename MT, bool SOR>
 __global__ __launch_bounds__(kBlock) void k_sweep_level(SweepArgs a, int slot0, int count) {
     constexpr int SPB = kWavesPerBlock * (kWave / BS);
-    sweep_chunk<BS, VT, false, SOR>(a, slot0 + blockIdx.x * SPB, slot0 + count);
+    sweep_chunk<BS, VT, MT, false, SOR>(a, slot0 + blockIdx.x * SPB, slot0 + count);
 }
 
-template <int BS, typename VT, bool SOR>
+template <int BS, typename VT, typename MT, bool SOR>
 __global__ __launch_bounds__(kBlock) void k_sweep_syncfree(SweepArgs a, int nslots) {
     constexpr int SPB = kWavesPerBlock * (kWave / BS);
     __shared__ long long s_chunk;
@@ -205,11 +243,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep_syncfree(SweepArgs a, int nslo
         const long long chunk = s_chunk;
         __syncthreads();
         if (chunk >= a.nchunks) break;
-        sweep_chunk<BS, VT, true, SOR>(a, (int)chunk * SPB, nslots);
+        sweep_chunk<BS, VT, MT, true, SOR>(a, (int)chunk * SPB, nslots);
     }
 }
 
-template <int BS, typename VT, bool SOR>
+template <int BS, typename VT, typename MT, bool SOR>
 int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
     constexpr int SPB = kWavesPerBlock * (kWave / BS);
     const int nchunks = (S.nslots + SPB - 1) / SPB;
@@ -217,15 +255,15 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
     if (ctx->trsv_mode == 1) {
         const int grid = std::max(1, std::min(nchunks, ctx->num_cus * ctx->sweep_wg_per_cu));
         a.base = S.tickets_used;
-        hipLaunchKernelGGL((k_sweep_syncfree<BS, VT, SOR>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, S.nslots);
+        hipLaunchKernelGGL((k_sweep_syncfree<BS, VT, MT, SOR>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, S.nslots);
         S.tickets_used += (unsigned long long)(nchunks + grid);
     } else {
         auto launch_levels = [&](hipStream_t st) {
             for (int l = 0; l < S.nlevels; ++l) {
                 const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
                 if (cnt <= 0) continue;
-                hipLaunchKernelGGL((k_sweep_level<BS, VT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, st, a,
-                                   s0, cnt);
+                hipLaunchKernelGGL((k_sweep_level<BS, VT, MT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, st,
+                                   a, s0, cnt);
             }
         };
         if (S.nlevels <= kDirectLevels || !ctx->sweep_graphs) {
@@ -267,19 +305,19 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
     return BSP_OK;
 }
 
-template <typename VT, bool SOR>
+template <typename VT, typename MT, bool SOR>
 int dispatch(bsp_ctx* ctx, Sweep& S, const SweepArgs& a) {
     switch (S.bs) {
-        case 1: return run_sweep<1, VT, SOR>(ctx, S, a);
-        case 2: return run_sweep<2, VT, SOR>(ctx, S, a);
-        case 3: return run_sweep<3, VT, SOR>(ctx, S, a);
-        case 4: return run_sweep<4, VT, SOR>(ctx, S, a);
-        case 5: return run_sweep<5, VT, SOR>(ctx, S, a);
-        case 6: return run_sweep<6, VT, SOR>(ctx, S, a);
-        case 7: return run_sweep<7, VT, SOR>(ctx, S, a);
-        case 8: return run_sweep<8, VT, SOR>(ctx, S, a);
-        case 16: return run_sweep<16, VT, SOR>(ctx, S, a);
-        case 32: return run_sweep<32, VT, SOR>(ctx, S, a);
+        case 1: return run_sweep<1, VT, MT, SOR>(ctx, S, a);
+        case 2: return run_sweep<2, VT, MT, SOR>(ctx, S, a);
+        case 3: return run_sweep<3, VT, MT, SOR>(ctx, S, a);
+        case 4: return run_sweep<4, VT, MT, SOR>(ctx, S, a);
+        case 5: return run_sweep<5, VT, MT, SOR>(ctx, S, a);
+        case 6: return run_sweep<6, VT, MT, SOR>(ctx, S, a);
+        case 7: return run_sweep<7, VT, MT, SOR>(ctx, S, a);
+        case 8: return run_sweep<8, VT, MT, SOR>(ctx, S, a);
+        case 16: return run_sweep<16, VT, MT, SOR>(ctx, S, a);
+        case 32: return run_sweep<32, VT, MT, SOR>(ctx, S, a);
         default: return fail(ctx, BSP_E_UNSUPPORTED, "sweep: unsupported block size " + std::to_string(S.bs));
     }
 }
@@ -305,7 +343,8 @@ int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoc
     SweepArgs a = make_args(ctx, S, epoch);
     a.dinv = dinv;
     a.x = x;
-    return S.f32 ? dispatch<float, false>(ctx, S, a) : dispatch<double, false>(ctx, S, a);
+    if (S.bf16) return dispatch<float, bf16_t, false>(ctx, S, a);
+    return S.f32 ? dispatch<float, float, false>(ctx, S, a) : dispatch<double, double, false>(ctx, S, a);
 }
 
 int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b, const void* xo,
@@ -318,7 +357,8 @@ int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* 
     a.xo = xo;
     a.w = w;
     a.forward = forward;
-    return S.f32 ? dispatch<float, true>(ctx, S, a) : dispatch<double, true>(ctx, S, a);
+    if (S.bf16) return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweeps have no BF16 form");
+    return S.f32 ? dispatch<float, float, true>(ctx, S, a) : dispatch<double, double, true>(ctx, S, a);
 }
 
 }  // namespace bsp

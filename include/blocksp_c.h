@@ -162,9 +162,12 @@ int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
  * natural order within a colour, i.e. ILU0 of P A P^T for that permutation; with a
  * colouring in which no two rows of one colour are coupled, each triangular sweep
  * has at most (number of colours) levels.  NULL = natural order (the reference's).
- * precision: BSP_PREC_F64 or BSP_PREC_F32 (factor, diagonal inverses and applies). */
+ * precision: BSP_PREC_F64 or BSP_PREC_F32 (factor, diagonal inverses and applies), or
+ * BSP_PREC_BF16 (factor and diagonal inverses stored in BF16, rounded to nearest even from the
+ * FP64 factor; applies in FP32 -- an FP32 preconditioner for bsp_fgmres_mixed). */
 #define BSP_PREC_F64 0
 #define BSP_PREC_F32 1
+#define BSP_PREC_BF16 2
 int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision,
                                bsp_precond** out);
 /* IChol0(A) (preconditioners.hpp:364-538), A symmetric positive definite. */

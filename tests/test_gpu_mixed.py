@@ -181,3 +181,65 @@ def test_fgmres_mixed_fp32_basis_reaches_fp64_residual(ctx, prec):
 def test_fgmres_basis_option_is_checked(ctx):
     with pytest.raises(Exception):
         ctx.set_option("fgmres_basis_f32", 2)
+
+
+def _bf16(a):
+    """FP64 -> FP32 (nearest) -> BF16 (nearest even), back to FP64: the library's rounding."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_bf16_ilu0_apply_matches_the_rounded_factor(ctx):
+    # natural order (the reference's ILU0, many levels): the device apply in FP32 arithmetic
+    # equals forward/backward block substitution with the oracle's factor rounded to BF16
+    # (strict L and U blocks, and the inverses of U's diagonal blocks)
+    p = pr.dg2d_veclap(9, 7, seed=4)
+    A = bsp.BSR.from_problem(ctx, p)
+    lu, _, _ = po.ilu0_factor(p)
+    bs, n = p.bs, p.n
+    r = np.random.default_rng(5).standard_normal(n * bs)
+    y = r.reshape(n, bs).copy()
+    for i in range(n):
+        for q in range(p.rowptr[i], p.rowptr[i + 1]):
+            if p.colidx[q] < i:
+                y[i] -= _bf16(lu[q]) @ y[p.colidx[q]]
+    x = y.copy()
+    for i in reversed(range(n)):
+        d = None
+        for q in range(p.rowptr[i], p.rowptr[i + 1]):
+            c = p.colidx[q]
+            if c > i:
+                x[i] -= _bf16(lu[q]) @ x[c]
+            elif c == i:
+                d = _bf16(np.linalg.inv(lu[q]))
+        x[i] = d @ x[i]
+    M = bsp.ILU0Ex(ctx, A, None, bsp.PREC_BF16)
+    rd = ctx.to_device(r)
+    M(rd)
+    got = rd.numpy()
+    assert np.linalg.norm(got - x.reshape(-1)) <= 1e-5 * np.linalg.norm(x)
+    # and it is a different operator from the FP32 one (the rounding is really BF16)
+    r32 = ctx.to_device(r)
+    bsp.ILU0Ex(ctx, A, None, bsp.PREC_F32)(r32)
+    assert np.linalg.norm(got - r32.numpy()) >= 1e-4 * np.linalg.norm(x)
+
+
+def test_fgmres_mixed_bf16_preconditioner_reaches_fp64_residual(ctx):
+    dims = (48, 48)
+    p = pr.dg2d_veclap(*dims, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    b = ctx.to_device(p.b)
+    col = pr.parity_colours(dims)
+    _, st32, rc32 = bsp.FGMRES_mixed(ctx, A, b, 1e-10, bsp.ILU0Ex(ctx, A, col, bsp.PREC_F32))
+    x, st, rc = bsp.FGMRES_mixed(ctx, A, b, 1e-10, bsp.ILU0Ex(ctx, A, col, bsp.PREC_BF16))
+    relres = np.linalg.norm(p.b - p.bsmv(x.numpy())) / np.linalg.norm(p.b)
+    assert rc == 0 and st.converged == 1 and relres <= 1e-10, (rc, relres, st.iters)
+    assert rc32 == 0 and st.iters <= 1.15 * st32.iters + 20, (st.iters, st32.iters)
+
+
+def test_ilu0_ex_rejects_unknown_precision(ctx):
+    p = pr.dg2d_veclap(6, 5, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    with pytest.raises(Exception):
+        bsp.ILU0Ex(ctx, A, None, 7)
