@@ -40,7 +40,7 @@ class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
                 ("precond_applies", C.c_int), ("res0", C.c_double), ("res_est", C.c_double),
                 ("true_relres", C.c_double), ("setup_ms", C.c_double), ("solve_ms", C.c_double),
-                ("fallback", C.c_int), ("reserved", C.c_int)]
+                ("fallback", C.c_int), ("reorth", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

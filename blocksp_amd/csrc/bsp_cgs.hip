@@ -9,8 +9,9 @@
 //   if ||w_new|| < 0.01 ||w_old|| (cancellation; the Daniel-Gragg-Kaufman-Stewart test with
 //   a threshold chosen for FP64) the pass is repeated once (A2, B2) and h = h1 + h2;
 //   q_m = w / ||w|| (and, on request, an FP32 copy of q_m for the next preconditioner apply).
-// The decision is taken on the device (a flag written after B); A2 and B2 are always
-// launched and return at once when the flag is clear, so the host never waits inside a step.
+// The decision is taken on the device (a flag written after B; A2 and B2 are launched and
+// return at once when it is clear), or by the caller, which reads the flag with the H column
+// and launches A2 and B2 only when it is set.
 // Each pass is followed by a small kernel that adds its per-workgroup partials in a fixed
 // order; a step is nine launches and reads 2m + 4 vectors (modified Gram-Schmidt: m + 2
 // launches, 4m + 4 vectors).  (Folding those sums into the last workgroup of each pass, with a
@@ -194,15 +195,18 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __re
     if (threadIdx.x == 0) part_norm[blockIdx.x] = acc[0];
 }
 
-// Fixed-order sums of the partials.
+// Fixed-order sums of the partials.  hcol has m + 2 entries.
 //   stage 0 (after A):  h1[s] = sum part[s];  wn2 = sum part[m]
-//   stage 1 (after B):  n2 = sum part_norm; flag = (n2 < 1e-4 wn2); if !flag: hcol = h1, norm
-//   stage 2 (after A2): h2[s] = sum part[s]; hcol[s] = h1[s] + h2[s]      (flag set only)
-//   stage 3 (after B2): hcol[m] = norm = sqrt(sum part_norm)            (flag set only)
+//   stage 1 (after B):  n2 = sum part_norm; reorth = (n2 < 1e-4 wn2); *flag_out = reorth;
+//                       hcol[0..m) = h1, hcol[m] = norm, hcol[m + 1] = reorth (as 0.0 / 1.0)
+//   stage 2 (after A2): h2[s] = sum part[s]; hcol[s] += h2[s]
+//   stage 3 (after B2): hcol[m] = norm = sqrt(sum part_norm)
+// Stages 2 and 3 return at once when gate is set and *gate == 0.
 __global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, Scratch sc, int nparts,
-                                                       int* __restrict__ flag, double* __restrict__ hcol) {
+                                                       const int* __restrict__ gate, int* __restrict__ flag_out,
+                                                       double* __restrict__ hcol) {
     // st: [0, m) h1, [m, 2m) h2, 2m wn2, 2m + 1 norm
-    if ((stage == 2 || stage == 3) && *flag == 0) return;
+    if ((stage == 2 || stage == 3) && gate && *gate == 0) return;
     __shared__ double sh[kBlock / kWave][1];
     double* st = sc.st;
     const int t = blockIdx.x;   // one block per output
@@ -217,16 +221,15 @@ __global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, Scratch
         else st[2 * m] = v;
     } else if (stage == 1) {
         const bool reorth = v < kReorth2 * st[2 * m];
-        *flag = reorth ? 1 : 0;
-        if (!reorth) {
-            const double nrm = sqrt(fabs(v));
-            for (int s = 0; s < m; ++s) hcol[s] = st[s];
-            hcol[m] = nrm;
-            st[2 * m + 1] = nrm;
-        }
+        *flag_out = reorth ? 1 : 0;
+        const double nrm = sqrt(fabs(v));
+        for (int s = 0; s < m; ++s) hcol[s] = st[s];
+        hcol[m] = nrm;
+        hcol[m + 1] = reorth ? 1.0 : 0.0;
+        st[2 * m + 1] = nrm;
     } else if (stage == 2) {
         st[m + t] = v;
-        hcol[t] = st[t] + v;
+        hcol[t] += v;
     } else {
         const double nrm = sqrt(fabs(v));
         hcol[m] = nrm;
@@ -256,29 +259,36 @@ Scratch carve(double* scratch, int m) {
     return sc;
 }
 
+// First pass (A, B) and, when `second` is 1, the second pass (A2, B2) gated on the device flag,
+// or, when `second` is 2, the second pass alone and ungated (the caller saw the flag of an
+// earlier first pass on this w); then q = w / ||w||.
 template <typename QT>
 int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, int m, double* hcol, QT* qnew,
-                float* znext, double* scratch, int* flag) {
+                float* znext, double* scratch, int* flag, int second) {
     if (m < 1 || m > kMaxBasis) return fail(ctx, BSP_E_INVALID, "cgs: basis size out of range");
     QPtrs<QT> qp{};
     for (int s = 0; s < m; ++s) qp.q[s] = Q[s];
     const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((len + 8 * kBlock - 1) / (8 * kBlock), kMaxParts));
     const Scratch sc = carve(scratch, m);
     const int ng = (m + kGroup - 1) / kGroup;
-    const int mm = std::max(m, 1);
-    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc, nb,
-                       (const int*)nullptr);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, sc, nb, flag, hcol);
-    hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st, sc.part_norm,
-                       (const int*)nullptr);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, sc, nb, flag, hcol);
-    // second pass, only when the device flagged cancellation
-    hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc, nb,
-                       (const int*)flag);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(mm), dim3(kBlock), 0, ctx->stream, 2, m, sc, nb, flag, hcol);
-    hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st + m,
-                       sc.part_norm, (const int*)flag);
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, sc, nb, flag, hcol);
+    if (second != 2) {
+        hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc, nb,
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, sc, nb, (const int*)nullptr,
+                           flag, hcol);
+        hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st,
+                           sc.part_norm, (const int*)nullptr);
+        hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, sc, nb, (const int*)nullptr,
+                           flag, hcol);
+    }
+    if (second != 0) {
+        const int* gate = second == 1 ? flag : nullptr;
+        hipLaunchKernelGGL(k_cgs_dots<QT>, dim3(nb, ng), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc, nb, gate);
+        hipLaunchKernelGGL(k_cgs_reduce, dim3(m), dim3(kBlock), 0, ctx->stream, 2, m, sc, nb, gate, flag, hcol);
+        hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st + m,
+                           sc.part_norm, gate);
+        hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 3, m, sc, nb, gate, flag, hcol);
+    }
     const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 4096);
     hipLaunchKernelGGL(k_cgs_scale<QT>, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, sc.st, m, qnew, znext);
     hipError_t e = hipGetLastError();
@@ -290,16 +300,18 @@ int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, 
 
 size_t cgs_scratch_doubles() { return (size_t)(kMaxBasis + 1) * kMaxParts + kMaxParts + 2 * kMaxBasis + 2; }
 
-// Q: host array of m device pointers; scratch: cgs_scratch_doubles() doubles; flag: one int.
+// Q: host array of m device pointers; hcol: m + 2 device doubles; scratch: cgs_scratch_doubles()
+// doubles; flag: one int.  second: 1 = conditional second pass decided on the device, 0 = first
+// pass only (hcol[m + 1] says whether a second pass is needed), 2 = second pass only.
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
-                       double* qnew, float* znext, double* scratch, int* flag) {
-    return arnoldi_cgs<double>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag);
+                       double* qnew, float* znext, double* scratch, int* flag, int second) {
+    return arnoldi_cgs<double>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag, second);
 }
 
 // The same step with the basis stored in FP32 (FP64 arithmetic; q_m is rounded to FP32 once).
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,
-                       float* qnew, float* znext, double* scratch, int* flag) {
-    return arnoldi_cgs<float>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag);
+                       float* qnew, float* znext, double* scratch, int* flag, int second) {
+    return arnoldi_cgs<float>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag, second);
 }
 
 }  // namespace bsp
@@ -316,9 +328,9 @@ int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs
     DevMem scratch, flag, hcol;
     BSP_HIP(scratch.alloc(sizeof(double) * cgs_scratch_doubles()));
     BSP_HIP(flag.alloc(sizeof(int)));
-    BSP_HIP(hcol.alloc(sizeof(double) * (m + 1)));
-    int rc = launch_arnoldi_cgs(ctx, len, w_dev, Q_dev_ptrs, m, hcol.as<double>(), qnew_dev, nullptr, scratch.as<double>(),
-                            flag.as<int>());
+    BSP_HIP(hcol.alloc(sizeof(double) * (m + 2)));
+    int rc = launch_arnoldi_cgs(ctx, len, w_dev, Q_dev_ptrs, m, hcol.as<double>(), qnew_dev, nullptr,
+                                scratch.as<double>(), flag.as<int>(), 1);
     if (rc) return rc;
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     if (h_host) BSP_HIP(hipMemcpy(h_host, hcol.p, sizeof(double) * (m + 1), hipMemcpyDeviceToHost));

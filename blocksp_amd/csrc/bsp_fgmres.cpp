@@ -62,6 +62,7 @@ struct InnerStats {
     double res = 0.0, beta = 0.0;
     bool stagnated = false;
     bool nonfinite = false;   // the pass met a non-finite value; its last update was dropped
+    int reorth = 0;           // steps that needed a second Gram-Schmidt pass
 };
 
 // One inner FGMRES-DR pass on A e = r for the current x (x += e on return).  VT is the
@@ -139,10 +140,16 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     bool restarted = false;
     double res = beta;
     int iter = 0, tot = 0;
-    double* w = r_work;   // reuse: r is not needed after V[0] is formed
+    // w of the step in slot 0 reuses r_work (r is not needed after V[0] is formed); slot 1 has
+    // its own, so a step's w survives while the next step runs
+    DevMem w1;
+    if (use_cgs) BSP_HIP(w1.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
+    double* wbuf[2] = {r_work, use_cgs ? w1.as<double>() : r_work};
     // One Arnoldi step for column `col`, enqueued without waiting: z = P(V[col]), w = A z,
     // Gram-Schmidt of w against V[0..col], q = w / h -> V[col + 1] (appended to Vb at once),
-    // and the H column copied to host slot `slot` (event ev.e[slot]).
+    // and the H column copied to host slot `slot` (event ev.e[slot]).  Classical Gram-Schmidt
+    // runs its first pass only; the flag copied back with the column says whether the rare
+    // second pass is needed, which the host then runs before using the column.
     // With an FP32 preconditioner the Gram-Schmidt step also writes the FP32 copy of V[col + 1]
     // (znext), which the next step preconditions in place instead of converting V[col + 1].
     struct Step {
@@ -173,6 +180,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             if (!r2) r2 = launch_d2f(ctx, len, t64, z);
         }
         if (r2) return r2;
+        double* w = wbuf[slot];
         // w = A z
         if (spmv_f32) {
             r2 = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
@@ -188,14 +196,14 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         if (use_cgs) {
             if (f32_pre && (r2 = Z.take(ctx, s.znext))) return r2;
             r2 = launch_arnoldi_cgs(ctx, len, w, Vb.data(), col + 1, hd, s.vn, s.znext, cgs_scratch.as<double>(),
-                                    cgs_flag.as<int>());
+                                    cgs_flag.as<int>(), 0);
         } else {
             if constexpr (!vf32)
                 r2 = launch_arnoldi_mgs(ctx, len, w, Vb.data(), col, hd, s.vn, ctx->red_partials.as<double>());
         }
         if (r2) return r2;
-        BSP_HIP(hipMemcpyAsync(hpin.p + slot * hslot, hd, sizeof(double) * (col + 2), hipMemcpyDeviceToHost,
-                               ctx->stream));
+        BSP_HIP(hipMemcpyAsync(hpin.p + slot * hslot, hd, sizeof(double) * (col + (use_cgs ? 3 : 2)),
+                               hipMemcpyDeviceToHost, ctx->stream));
         BSP_HIP(hipEventRecord(ev.e[slot], ctx->stream));
         Vb.push_back(s.vn);
         return BSP_OK;
@@ -216,8 +224,31 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             if (rc) return rc;
         }
         BSP_HIP(hipEventSynchronize(ev.e[cur.slot]));
-        H_resize(iter + 2, iter + 1);
         const double* hp = hpin.p + cur.slot * hslot;
+        if (use_cgs && hp[iter + 2] != 0.0) {
+            // cancellation in the first Gram-Schmidt pass: drop the step enqueued on the
+            // unfinished vector, orthogonalise once more, then enqueue that step again
+            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            if (spec) {
+                if (nxt.z != cur.znext) Z.give(nxt.z);
+                if (nxt.znext) Z.give(nxt.znext);
+                V.give(nxt.vn);
+                Vb.pop_back();
+            }
+            double* hd = hcol.as<double>() + cur.slot * hslot;
+            rc = launch_arnoldi_cgs(ctx, len, wbuf[cur.slot], Vb.data(), iter + 1, hd, cur.vn, cur.znext,
+                                    cgs_scratch.as<double>(), cgs_flag.as<int>(), 2);
+            if (rc) return rc;
+            BSP_HIP(hipMemcpyAsync(hpin.p + cur.slot * hslot, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            ++st.reorth;
+            if (spec) {
+                rc = enqueue_step(iter + 1, cur.slot ^ 1, nxt, cur.znext);
+                if (rc) return rc;
+            }
+        }
+        H_resize(iter + 2, iter + 1);
         for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hp[j];
         const double hlast = hp[iter + 1];
         const bool breakdown = std::fabs(hlast) < zero_thresh || !std::isfinite(hlast);
@@ -358,7 +389,7 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
     double bnorm = 0.0;
     rc = reduce(ctx, 1, len, b, b, SLOT_TMP2, &bnorm);
     if (rc) return rc;
-    int total = 0, restarts = 0, passes = 0;
+    int total = 0, restarts = 0, passes = 0, reorth = 0;
     double rr = 1.0, res0 = 0.0, res_est = 0.0;
     const bsp_precond* cur = M;
     bool switched = false;
@@ -377,6 +408,7 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
         if (passes == 0) res0 = st.beta;
         total += st.iters;
         restarts += st.restarts;
+        reorth += st.reorth;
         res_est = st.res;
         rc = true_relres_ext(ctx, A, b, x, rw.as<double>(), bnorm, rr);
         if (rc) return rc;
@@ -417,6 +449,7 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
         stats->true_relres = rr;
         stats->solve_ms = ms;
         stats->fallback = switched ? 1 : 0;
+        stats->reorth = reorth;
     }
     return conv ? BSP_OK : BSP_NOT_CONVERGED;
 }

@@ -224,13 +224,16 @@ int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double*
                        double* qnew, double* parts);
 // Classical Gram-Schmidt with one conditional reorthogonalisation (bsp_cgs.hip): w against
 // the m (<= 64) orthonormal device vectors Q[0..m-1] (a host array of device pointers);
-// hcol[0..m] (device), qnew = w / hcol[m] and, if znext is not null, znext = FP32 copy of
-// qnew.  scratch: cgs_scratch_doubles() device doubles; flag: one device int.
+// hcol[0..m] (device; m + 2 entries, hcol[m + 1] = 1.0 when the first pass asks for a second),
+// qnew = w / hcol[m] and, if znext is not null, znext = FP32 copy of qnew.  second: 1 = the
+// second pass is decided on the device, 0 = first pass only, 2 = second pass only (after a first
+// pass on the same w whose hcol[m + 1] was 1; hcol still holds that pass's coefficients).
+// scratch: cgs_scratch_doubles() device doubles; flag: one device int.
 size_t cgs_scratch_doubles();
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
-                       double* qnew, float* znext, double* scratch, int* flag);
+                       double* qnew, float* znext, double* scratch, int* flag, int second);
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,
-                       float* qnew, float* znext, double* scratch, int* flag);
+                       float* qnew, float* znext, double* scratch, int* flag, int second);
 // q = v / h[0]
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, float* q);

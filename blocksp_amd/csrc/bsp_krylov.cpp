@@ -363,6 +363,7 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
         stats->true_relres = rr;
         stats->solve_ms = ms;
         stats->fallback = 0;
+        stats->reorth = 0;
     }
     return conv ? BSP_OK : BSP_NOT_CONVERGED;
 }
@@ -465,6 +466,7 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
         stats->true_relres = rr;
         stats->solve_ms = ms;
         stats->fallback = 0;
+        stats->reorth = 0;
     }
     return conv ? BSP_OK : BSP_NOT_CONVERGED;
 }

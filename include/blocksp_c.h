@@ -203,7 +203,7 @@ typedef struct {
     double setup_ms;    /* not filled by the solvers (preconditioners are built separately) */
     double solve_ms;    /* wall time of the solve on the context stream */
     int fallback;       /* bsp_fgmres_fallback: 1 when the fallback preconditioner took over */
-    int reserved;
+    int reorth;         /* bsp_fgmres_*: steps whose Gram-Schmidt needed a second pass */
 } bsp_stats;
 
 /* Stopping rule: 0 = the reference's (pGMRES: least-squares residual estimate of the

@@ -243,3 +243,24 @@ def test_ilu0_ex_rejects_unknown_precision(ctx):
     A = bsp.BSR.from_problem(ctx, p)
     with pytest.raises(Exception):
         bsp.ILU0Ex(ctx, A, None, 7)
+
+
+@pytest.mark.parametrize("spmv", [bsp.PREC_F64, bsp.PREC_F32])
+def test_fgmres_mixed_second_gram_schmidt_pass_on_cancellation(ctx, spmv):
+    # diagonal A whose entries form three tight clusters: from the fourth step on, A z lies almost
+    # in the span of the basis, one classical pass cancels to ~1e-4 of ||w|| and the solver runs the
+    # second pass (counted in stats.reorth); the solution is b / diag(A)
+    n, bs = 3000, 8
+    rng = np.random.default_rng(11)
+    d = rng.choice([1.0, 2.0, 5.0], size=n * bs) * (1.0 + 1e-4 * rng.standard_normal(n * bs))
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.arange(n, dtype=np.int32)
+    vals = np.zeros((n, bs, bs))
+    for k in range(bs):
+        vals[:, k, k] = d.reshape(n, bs)[:, k]
+    A = bsp.BSR(ctx, n, n, bs, rp, ci, vals)
+    bh = rng.standard_normal(n * bs)
+    x, st, rc = bsp.FGMRES_mixed(ctx, A, ctx.to_device(bh), 1e-10, bsp.NoPre(ctx, A), spmv_precision=spmv)
+    assert rc == 0 and st.true_relres <= 1e-10, (rc, st.true_relres, st.iters)
+    assert st.reorth >= 1, st.reorth
+    np.testing.assert_allclose(x.numpy(), bh / d, rtol=0, atol=1e-9 * np.abs(bh / d).max())
