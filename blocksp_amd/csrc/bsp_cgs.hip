@@ -239,6 +239,48 @@ __global__ __launch_bounds__(kBlock) void k_cgs_reduce(int stage, int m, Scratch
 
 // q = w / ||w|| in the basis precision; z (optional) = FP32 copy of q, the input of the next
 // step's FP32 preconditioner apply.
+// After a first pass whose caller reads the H column on the host: every workgroup adds the
+// norm partials in the order of k_cgs_reduce stage 1 (so all get the same bits), workgroup 0
+// writes the stage-1 results to hcol and to the host copy hcol_host (pinned, mapped), and all
+// scale: q = w / ||w||, z (optional) = FP32 copy of q.
+template <typename QT>
+__global__ __launch_bounds__(kBlock) void k_cgs_finish_first(int64_t len, const double* __restrict__ w, Scratch sc,
+                                                             int nparts, int m, QT* __restrict__ q,
+                                                             float* __restrict__ z, double* __restrict__ hcol,
+                                                             double* hcol_host, int* __restrict__ flag_out) {
+    __shared__ double sh[kBlock / kWave][1];
+    __shared__ double s_norm;
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < nparts; b += kBlock) acc[0] += sc.part_norm[b];
+    block_sums<1>(acc, sh);
+    if (threadIdx.x == 0) {
+        const double v = acc[0];
+        const double nrm = sqrt(fabs(v));
+        s_norm = nrm;
+        if (blockIdx.x == 0) {
+            const bool reorth = v < kReorth2 * sc.st[2 * m];
+            *flag_out = reorth ? 1 : 0;
+            for (int s = 0; s < m; ++s) {
+                hcol[s] = sc.st[s];
+                hcol_host[s] = sc.st[s];
+            }
+            hcol[m] = nrm;
+            hcol[m + 1] = reorth ? 1.0 : 0.0;
+            hcol_host[m] = nrm;
+            hcol_host[m + 1] = reorth ? 1.0 : 0.0;
+            sc.st[2 * m + 1] = nrm;
+            __threadfence_system();
+        }
+    }
+    __syncthreads();
+    const double h = s_norm;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (int64_t)gridDim.x * kBlock) {
+        const QT v = (QT)(w[i] / h);
+        q[i] = v;
+        if (z) z[i] = (float)v;
+    }
+}
+
 template <typename QT>
 __global__ __launch_bounds__(kBlock) void k_cgs_scale(int64_t len, const double* __restrict__ w,
                                                       const double* __restrict__ st, int m,
@@ -264,7 +306,7 @@ Scratch carve(double* scratch, int m) {
 // earlier first pass on this w); then q = w / ||w||.
 template <typename QT>
 int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, int m, double* hcol, QT* qnew,
-                float* znext, double* scratch, int* flag, int second) {
+                float* znext, double* scratch, int* flag, int second, double* hcol_host) {
     if (m < 1 || m > kMaxBasis) return fail(ctx, BSP_E_INVALID, "cgs: basis size out of range");
     QPtrs<QT> qp{};
     for (int s = 0; s < m; ++s) qp.q[s] = Q[s];
@@ -278,6 +320,14 @@ int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, 
                            flag, hcol);
         hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st,
                            sc.part_norm, (const int*)nullptr);
+        if (second == 0 && hcol_host) {
+            const int gs = (int)std::min<int64_t>((len + kBlock - 1) / kBlock, 1024);
+            hipLaunchKernelGGL(k_cgs_finish_first<QT>, dim3(gs), dim3(kBlock), 0, ctx->stream, len, w, sc, nb, m,
+                               qnew, znext, hcol, hcol_host, flag);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("cgs launch: ") + hipGetErrorString(e));
+            return BSP_OK;
+        }
         hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, 1, m, sc, nb, (const int*)nullptr,
                            flag, hcol);
     }
@@ -303,15 +353,17 @@ size_t cgs_scratch_doubles() { return (size_t)(kMaxBasis + 1) * kMaxParts + kMax
 // Q: host array of m device pointers; hcol: m + 2 device doubles; scratch: cgs_scratch_doubles()
 // doubles; flag: one int.  second: 1 = conditional second pass decided on the device, 0 = first
 // pass only (hcol[m + 1] says whether a second pass is needed), 2 = second pass only.
+// hcol_host (second == 0, optional): pinned host memory that receives hcol[0..m+1] from the
+// kernel itself, complete when the stream reaches the next recorded event.
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
-                       double* qnew, float* znext, double* scratch, int* flag, int second) {
-    return arnoldi_cgs<double>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag, second);
+                       double* qnew, float* znext, double* scratch, int* flag, int second, double* hcol_host) {
+    return arnoldi_cgs<double>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag, second, hcol_host);
 }
 
 // The same step with the basis stored in FP32 (FP64 arithmetic; q_m is rounded to FP32 once).
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,
-                       float* qnew, float* znext, double* scratch, int* flag, int second) {
-    return arnoldi_cgs<float>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag, second);
+                       float* qnew, float* znext, double* scratch, int* flag, int second, double* hcol_host) {
+    return arnoldi_cgs<float>(ctx, len, w, Q, m, hcol, qnew, znext, scratch, flag, second, hcol_host);
 }
 
 }  // namespace bsp
@@ -330,7 +382,7 @@ int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs
     BSP_HIP(flag.alloc(sizeof(int)));
     BSP_HIP(hcol.alloc(sizeof(double) * (m + 2)));
     int rc = launch_arnoldi_cgs(ctx, len, w_dev, Q_dev_ptrs, m, hcol.as<double>(), qnew_dev, nullptr,
-                                scratch.as<double>(), flag.as<int>(), 1);
+                                scratch.as<double>(), flag.as<int>(), 1, nullptr);
     if (rc) return rc;
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     if (h_host) BSP_HIP(hipMemcpy(h_host, hcol.p, sizeof(double) * (m + 1), hipMemcpyDeviceToHost));

@@ -82,7 +82,12 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     BSP_HIP(hcol.alloc(sizeof(double) * 2 * (m_max + 2)));
     BSP_HIP(ydev.alloc(sizeof(double) * (m_max + 2)));
     Pinned hpin, ypin, ppin;
-    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * 2 * (m_max + 2), hipHostMallocDefault));
+    // H column slots; the classical Gram-Schmidt kernels write them directly (coherent, mapped,
+    // through the device address of the same memory)
+    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * 2 * (m_max + 2),
+                          hipHostMallocCoherent | hipHostMallocMapped));
+    double* hpin_dev = nullptr;
+    BSP_HIP(hipHostGetDevicePointer((void**)&hpin_dev, hpin.p, 0));
     BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
     // classical Gram-Schmidt (default): scratch and the reorthogonalisation flag
     const bool use_cgs = vf32 || ctx->fgmres_ortho == 0;
@@ -196,14 +201,16 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         if (use_cgs) {
             if (f32_pre && (r2 = Z.take(ctx, s.znext))) return r2;
             r2 = launch_arnoldi_cgs(ctx, len, w, Vb.data(), col + 1, hd, s.vn, s.znext, cgs_scratch.as<double>(),
-                                    cgs_flag.as<int>(), 0);
+                                    cgs_flag.as<int>(), 0, hpin_dev + slot * hslot);
         } else {
             if constexpr (!vf32)
                 r2 = launch_arnoldi_mgs(ctx, len, w, Vb.data(), col, hd, s.vn, ctx->red_partials.as<double>());
         }
         if (r2) return r2;
-        BSP_HIP(hipMemcpyAsync(hpin.p + slot * hslot, hd, sizeof(double) * (col + (use_cgs ? 3 : 2)),
-                               hipMemcpyDeviceToHost, ctx->stream));
+        // classical Gram-Schmidt wrote the column (and its flag) to the pinned slot itself
+        if (!use_cgs)
+            BSP_HIP(hipMemcpyAsync(hpin.p + slot * hslot, hd, sizeof(double) * (col + 2), hipMemcpyDeviceToHost,
+                                   ctx->stream));
         BSP_HIP(hipEventRecord(ev.e[slot], ctx->stream));
         Vb.push_back(s.vn);
         return BSP_OK;
@@ -237,7 +244,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             }
             double* hd = hcol.as<double>() + cur.slot * hslot;
             rc = launch_arnoldi_cgs(ctx, len, wbuf[cur.slot], Vb.data(), iter + 1, hd, cur.vn, cur.znext,
-                                    cgs_scratch.as<double>(), cgs_flag.as<int>(), 2);
+                                    cgs_scratch.as<double>(), cgs_flag.as<int>(), 2, nullptr);
             if (rc) return rc;
             BSP_HIP(hipMemcpyAsync(hpin.p + cur.slot * hslot, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost,
                                    ctx->stream));

@@ -228,12 +228,13 @@ int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double*
 // qnew = w / hcol[m] and, if znext is not null, znext = FP32 copy of qnew.  second: 1 = the
 // second pass is decided on the device, 0 = first pass only, 2 = second pass only (after a first
 // pass on the same w whose hcol[m + 1] was 1; hcol still holds that pass's coefficients).
-// scratch: cgs_scratch_doubles() device doubles; flag: one device int.
+// scratch: cgs_scratch_doubles() device doubles; flag: one device int.  hcol_host (second == 0,
+// optional): pinned host memory the kernel writes hcol[0..m+1] to directly.
 size_t cgs_scratch_doubles();
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
-                       double* qnew, float* znext, double* scratch, int* flag, int second);
+                       double* qnew, float* znext, double* scratch, int* flag, int second, double* hcol_host);
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,
-                       float* qnew, float* znext, double* scratch, int* flag, int second);
+                       float* qnew, float* znext, double* scratch, int* flag, int second, double* hcol_host);
 // q = v / h[0]
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, double* q);
 int launch_scale_div(const bsp_ctx* ctx, int64_t len, const double* v, const double* h, float* q);
