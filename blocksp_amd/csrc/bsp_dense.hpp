@@ -379,6 +379,255 @@ inline int invariant_subspace(int n, const double* A, int k, std::vector<double>
     return kout;
 }
 
+// H = Q^T A Q (upper Hessenberg), Q orthogonal; row-major n x n.
+inline void hessenberg_q(int n, std::vector<double>& H, std::vector<double>& Q) {
+    Q.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+    std::vector<double> v(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        double nrm2 = 0.0;
+        for (int i = k + 1; i < n; ++i) nrm2 += H[(size_t)i * n + k] * H[(size_t)i * n + k];
+        if (nrm2 == 0.0) continue;
+        const double x0 = H[(size_t)(k + 1) * n + k];
+        const double alpha = x0 > 0 ? -std::sqrt(nrm2) : std::sqrt(nrm2);
+        for (int i = 0; i < n; ++i) v[i] = 0.0;
+        v[k + 1] = x0 - alpha;
+        for (int i = k + 2; i < n; ++i) v[i] = H[(size_t)i * n + k];
+        double vv = 0.0;
+        for (int i = k + 1; i < n; ++i) vv += v[i] * v[i];
+        if (vv == 0.0) continue;
+        const double beta = 2.0 / vv;
+        for (int c = 0; c < n; ++c) {   // H <- (I - beta v v^T) H
+            double s = 0.0;
+            for (int i = k + 1; i < n; ++i) s += v[i] * H[(size_t)i * n + c];
+            s *= beta;
+            for (int i = k + 1; i < n; ++i) H[(size_t)i * n + c] -= s * v[i];
+        }
+        for (int r = 0; r < n; ++r) {   // H <- H (I - beta v v^T), Q <- Q (I - beta v v^T)
+            double s = 0.0, sq = 0.0;
+            for (int i = k + 1; i < n; ++i) {
+                s += H[(size_t)r * n + i] * v[i];
+                sq += Q[(size_t)r * n + i] * v[i];
+            }
+            s *= beta;
+            sq *= beta;
+            for (int i = k + 1; i < n; ++i) {
+                H[(size_t)r * n + i] -= s * v[i];
+                Q[(size_t)r * n + i] -= sq * v[i];
+            }
+        }
+        for (int i = k + 2; i < n; ++i) H[(size_t)i * n + k] = 0.0;
+    }
+}
+
+// Eigenvalues of an upper Hessenberg H (destroyed) by Francis double-shift QR steps in real
+// arithmetic on the active window (Golub & Van Loan, Matrix Computations, Alg. 7.5.1), with
+// negligible-subdiagonal splitting and an exceptional shift every 11th step without one.
+inline bool hqr_eigs(int n, std::vector<double>& H, std::vector<cplx>& ev) {
+    auto h = [&](int i, int j) -> double& { return H[(size_t)i * n + j]; };
+    ev.assign(n, cplx(0.0));
+    const double eps = std::numeric_limits<double>::epsilon();
+    int hi = n - 1, its = 0, total = 0;
+    while (hi >= 0) {
+        int lo = hi;
+        for (; lo > 0; --lo) {
+            double s = std::fabs(h(lo - 1, lo - 1)) + std::fabs(h(lo, lo));
+            if (s == 0.0) s = 1.0;
+            if (std::fabs(h(lo, lo - 1)) <= eps * s) {
+                h(lo, lo - 1) = 0.0;
+                break;
+            }
+        }
+        if (lo == hi) {   // 1 x 1
+            ev[hi] = h(hi, hi);
+            --hi;
+            its = 0;
+            continue;
+        }
+        if (lo == hi - 1) {   // 2 x 2
+            const double a = h(hi - 1, hi - 1), b = h(hi - 1, hi), c = h(hi, hi - 1), d = h(hi, hi);
+            const double tr = 0.5 * (a + d), det = a * d - b * c, disc = tr * tr - det;
+            if (disc >= 0.0) {
+                const double sq = std::sqrt(disc);
+                const double l1 = tr + (tr >= 0 ? sq : -sq);
+                const double l2 = l1 != 0.0 ? det / l1 : tr - (tr >= 0 ? sq : -sq);
+                ev[hi - 1] = l1;
+                ev[hi] = l2;
+            } else {
+                const double sq = std::sqrt(-disc);
+                ev[hi - 1] = cplx(tr, sq);
+                ev[hi] = cplx(tr, -sq);
+            }
+            hi -= 2;
+            its = 0;
+            continue;
+        }
+        if (++total > 60 * n) return false;
+        ++its;
+        double s, t;
+        if (its % 11 == 0) {   // exceptional shift
+            const double e = std::fabs(h(hi, hi - 1)) + std::fabs(h(hi - 1, hi - 2));
+            s = 1.5 * e;
+            t = e * e;
+        } else {
+            s = h(hi - 1, hi - 1) + h(hi, hi);
+            t = h(hi - 1, hi - 1) * h(hi, hi) - h(hi - 1, hi) * h(hi, hi - 1);
+        }
+        double x = h(lo, lo) * h(lo, lo) + h(lo, lo + 1) * h(lo + 1, lo) - s * h(lo, lo) + t;
+        double y = h(lo + 1, lo) * (h(lo, lo) + h(lo + 1, lo + 1) - s);
+        double z = h(lo + 1, lo) * h(lo + 2, lo + 1);
+        for (int k = lo - 1; k <= hi - 3; ++k) {
+            // reflector P = I - 2 u u^T / u^T u mapping (x, y, z) to a multiple of e1
+            const double nrm = std::sqrt(x * x + y * y + z * z);
+            if (nrm == 0.0) break;
+            const double alpha = x > 0 ? -nrm : nrm;
+            const double u0 = x - alpha, u1 = y, u2 = z;
+            const double uu = u0 * u0 + u1 * u1 + u2 * u2;
+            if (uu != 0.0) {
+                const double beta = 2.0 / uu;
+                const int c0 = std::max(k, lo);
+                for (int c = c0; c <= hi; ++c) {
+                    const double sv = beta * (u0 * h(k + 1, c) + u1 * h(k + 2, c) + u2 * h(k + 3, c));
+                    h(k + 1, c) -= sv * u0;
+                    h(k + 2, c) -= sv * u1;
+                    h(k + 3, c) -= sv * u2;
+                }
+                const int r1 = std::min(k + 4, hi);
+                for (int r = lo; r <= r1; ++r) {
+                    const double sv = beta * (h(r, k + 1) * u0 + h(r, k + 2) * u1 + h(r, k + 3) * u2);
+                    h(r, k + 1) -= sv * u0;
+                    h(r, k + 2) -= sv * u1;
+                    h(r, k + 3) -= sv * u2;
+                }
+            }
+            x = h(k + 2, k + 1);
+            y = h(k + 3, k + 1);
+            if (k < hi - 3) z = h(k + 4, k + 1);
+        }
+        // final 2-vector Givens on rows / columns hi-1, hi
+        {
+            const double r = std::hypot(x, y);
+            if (r != 0.0) {
+                const double c = x / r, sn = y / r;
+                const int k = hi - 2;
+                for (int col = std::max(k, lo); col <= hi; ++col) {
+                    const double a = h(hi - 1, col), b = h(hi, col);
+                    h(hi - 1, col) = c * a + sn * b;
+                    h(hi, col) = -sn * a + c * b;
+                }
+                for (int row = lo; row <= hi; ++row) {
+                    const double a = h(row, hi - 1), b = h(row, hi);
+                    h(row, hi - 1) = c * a + sn * b;
+                    h(row, hi) = -sn * a + c * b;
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// Solve (H - theta I) x = b for upper Hessenberg H (complex shift) by Gaussian elimination
+// with partial pivoting between adjacent rows; tiny pivots are replaced by eps * scale.
+inline void hess_shift_solve(int n, const std::vector<double>& H, cplx theta, std::vector<cplx>& x, double scale) {
+    std::vector<cplx> M((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) M[(size_t)i * n + j] = H[(size_t)i * n + j] - (i == j ? theta : cplx(0.0));
+    const double tiny = std::numeric_limits<double>::epsilon() * scale;
+    for (int k = 0; k + 1 < n; ++k) {
+        cplx& a = M[(size_t)k * n + k];
+        cplx& b = M[(size_t)(k + 1) * n + k];
+        if (std::abs(b) > std::abs(a)) {
+            for (int j = k; j < n; ++j) std::swap(M[(size_t)k * n + j], M[(size_t)(k + 1) * n + j]);
+            std::swap(x[k], x[k + 1]);
+        }
+        if (std::abs(M[(size_t)k * n + k]) < tiny) M[(size_t)k * n + k] = tiny;
+        const cplx f = M[(size_t)(k + 1) * n + k] / M[(size_t)k * n + k];
+        if (f != cplx(0.0)) {
+            for (int j = k; j < n; ++j) M[(size_t)(k + 1) * n + j] -= f * M[(size_t)k * n + j];
+            x[k + 1] -= f * x[k];
+        }
+    }
+    if (std::abs(M[(size_t)(n - 1) * n + n - 1]) < tiny) M[(size_t)(n - 1) * n + n - 1] = tiny;
+    for (int i = n - 1; i >= 0; --i) {
+        cplx s = x[i];
+        for (int j = i + 1; j < n; ++j) s -= M[(size_t)i * n + j] * x[j];
+        x[i] = s / M[(size_t)i * n + i];
+    }
+}
+
+// Orthonormal real basis (n x kout, row-major) of the invariant subspace of the real A for
+// its k eigenvalues of smallest modulus (conjugate pairs kept together), without a Schur
+// form: Hessenberg reduction H = Q^T A Q, eigenvalues by Francis steps, two steps of
+// inverse iteration on H - theta I per selected eigenvalue (one per conjugate pair), the
+// real and imaginary parts mapped back by Q and orthonormalised.  About 5x fewer flops
+// than the complex Schur route for n = 30 and the same subspace on non-defective spectra.
+// Returns kout, or -1 when the QR steps do not converge or the vectors are close to
+// dependent (nearly defective eigenvalues), where the caller uses invariant_subspace.
+inline int invariant_subspace_fast(int n, const double* A, int k, std::vector<double>& P) {
+    std::vector<double> H(A, A + (size_t)n * n), Q;
+    hessenberg_q(n, H, Q);
+    std::vector<double> Hw(H);
+    std::vector<cplx> ev;
+    if (!hqr_eigs(n, Hw, ev)) return -1;
+    std::vector<int> sel = select_smallest(ev, k);
+    const int kout = std::accumulate(sel.begin(), sel.end(), 0);
+    double scale = 0.0;
+    for (double v : H) scale = std::max(scale, std::fabs(v));
+    if (scale == 0.0) scale = 1.0;
+    std::vector<std::vector<double>> cand;
+    std::vector<char> done(n, 0);
+    const double zt = 10.0 * n * n * std::numeric_limits<double>::epsilon();
+    for (int i = 0; i < n; ++i) {
+        if (!sel[i] || done[i]) continue;
+        done[i] = 1;
+        const bool cpx = std::fabs(ev[i].imag()) > zt;
+        if (cpx)   // mark the partner
+            for (int j = 0; j < n; ++j)
+                if (j != i && sel[j] && !done[j] && std::fabs(ev[j].real() - ev[i].real()) < zt &&
+                    std::fabs(ev[j].imag() + ev[i].imag()) < zt) {
+                    done[j] = 1;
+                    break;
+                }
+        std::vector<cplx> x(n, cplx(1.0));
+        for (int it = 0; it < 2; ++it) {
+            hess_shift_solve(n, H, ev[i], x, scale);
+            double nr = 0.0;
+            for (auto& e : x) nr += std::norm(e);
+            nr = std::sqrt(nr);
+            if (!(nr > 0.0) || !std::isfinite(nr)) return -1;
+            for (auto& e : x) e /= nr;
+        }
+        std::vector<double> re(n, 0.0), im(n, 0.0);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) {
+                re[r] += Q[(size_t)r * n + c] * x[c].real();
+                im[r] += Q[(size_t)r * n + c] * x[c].imag();
+            }
+        cand.push_back(re);
+        if (cpx) cand.push_back(im);
+    }
+    if ((int)cand.size() != kout) return -1;
+    P.assign((size_t)n * kout, 0.0);
+    for (int col = 0; col < kout; ++col) {
+        std::vector<double>& q = cand[col];
+        double n0 = 0.0;
+        for (double e : q) n0 += e * e;
+        n0 = std::sqrt(n0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int j = 0; j < col; ++j) {
+                double d = 0.0;
+                for (int r = 0; r < n; ++r) d += P[(size_t)r * kout + j] * q[r];
+                for (int r = 0; r < n; ++r) q[r] -= d * P[(size_t)r * kout + j];
+            }
+        double n1 = 0.0;
+        for (double e : q) n1 += e * e;
+        n1 = std::sqrt(n1);
+        if (!(n1 > 1e-6 * n0)) return -1;
+        for (int r = 0; r < n; ++r) P[(size_t)r * kout + col] = q[r] / n1;
+    }
+    return kout;
+}
+
 // Ritz values of an Arnoldi Hessenberg H (m x n row-major, m = n + 1 or m = n): the
 // eigenvalues of its square part H_n, or with `harmonic` the harmonic Ritz values, the
 // eigenvalues of H_n + h_{n+1,n}^2 H_n^{-T} e_n e_n^T (dense_solvers.hpp:919-965).
@@ -403,8 +652,10 @@ inline bool ritz_values(int m, int n, const double* H, bool harmonic, std::vecto
 // Deflated restart of GMRES (Morgan, GMRES-DR; deflated_restart.hpp:38-269).
 // In: H (m x n, m = n+1, row-major), res_vec (m), k.  Out: P (m x kp, row-major,
 // kp = kout+1) such that Q_new = Q P, Hnew (kp x kout), c_r (kp).  Returns kout or -1.
+// fast: the harmonic Ritz subspace from invariant_subspace_fast (Schur route as fallback);
+// otherwise the Schur route, as the reference computes it.
 inline int deflate(int n, const double* H, const double* res_vec, int k, std::vector<double>& P,
-                   std::vector<double>& Hnew, std::vector<double>& c_r) {
+                   std::vector<double>& Hnew, std::vector<double>& c_r, bool fast = false) {
     const int m = n + 1;
     if (k > n) k = n;
     std::vector<double> Ht((size_t)n * n), w(n), e(n, 0.0);
@@ -415,7 +666,8 @@ inline int deflate(int n, const double* H, const double* res_vec, int k, std::ve
     if (!solve_transposed(n, Ht.data(), e.data(), w.data())) return -1;
     for (int i = 0; i < n; ++i) Ht[i * n + n - 1] += h_mn * h_mn * w[i];
     std::vector<double> Pk;
-    const int kout = invariant_subspace(n, Ht.data(), k, Pk, nullptr);
+    int kout = fast ? invariant_subspace_fast(n, Ht.data(), k, Pk) : -1;
+    if (kout < 0) kout = invariant_subspace(n, Ht.data(), k, Pk, nullptr);
     if (kout < 0) return -1;
     const int kp = kout + 1;
     P.assign((size_t)m * kp, 0.0);

@@ -302,7 +302,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             if (rc) return rc;
             ++st.restarts;
             std::vector<double> P, Hnew;
-            const int kout = dense::deflate(nn, H.data(), res_vec.data(), k, P, Hnew, c_r);
+            const int kout = dense::deflate(nn, H.data(), res_vec.data(), k, P, Hnew, c_r, true);
             BSP_CHECK(kout >= 0, BSP_E_NUMERIC, "flexible deflated restart: small eigenproblem failed");
             const int kp = kout + 1;
             BSP_CHECK((int)Vb.size() == m && (int)Zb.size() == nn, BSP_E_NUMERIC, "basis size mismatch");

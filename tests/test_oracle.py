@@ -404,7 +404,7 @@ def ldg():
 
 
 def test_ldg_fixture_structure(ldg):
-    """Blocking of the reference's fixture with its insertion order (SURVEY fixture table)."""
+    """Blocking of the reference's fixture with its insertion order (the fixture table of the reference tests)."""
     counts = {2: 116610, 4: 42210, 8: 12706, 32: 1666}
     for bs, nnzb in counts.items():
         rp, ci, v = mtx.entries_to_bsr(int(ldg["M"]), int(ldg["N"]), ldg["i"], ldg["j"], ldg["v"], bs, symmetric=True)
