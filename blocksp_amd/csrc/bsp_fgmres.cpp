@@ -324,7 +324,8 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             if (!rc && kout > 0)
                 rc = launch_basis_rotate_f32(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
             if (rc) return rc;
-            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            // the old vectors go back to the pools at once: the kernels that will write them
+            // again are enqueued on the same stream after the rotations that read them
             for (VT* p : Vb) V.give(p);
             for (float* p : Zb) Z.give(p);
             if (cur.znext) Z.give(cur.znext);   // an FP32 copy of a basis vector the rotation replaced
