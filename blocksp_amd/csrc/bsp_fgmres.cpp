@@ -380,7 +380,9 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
     BSP_CHECK(!M_fallback || (M_fallback->n == A->n_rows && M_fallback->bs == A->bs), BSP_E_INVALID,
               "fallback preconditioner does not match the matrix");
     BSP_CHECK(b != x, BSP_E_INVALID, "b and x must not alias");
-    BSP_CHECK(restart <= 62, BSP_E_UNSUPPORTED, "FGMRES restart length is limited to 62");
+    BSP_CHECK(restart >= 1 && restart <= 62, BSP_E_UNSUPPORTED,
+              "FGMRES needs a restart length in [1, 62] (the basis lives in the solver's pools)");
+    BSP_CHECK(k >= 0 && k < restart, BSP_E_INVALID, "FGMRES deflation size must be in [0, restart)");
     BSP_CHECK(spmv_precision == BSP_PREC_F64 || spmv_precision == BSP_PREC_F32, BSP_E_INVALID, "bad SpMV precision");
     BSP_HIP(hipSetDevice(ctx->device));
     const int64_t len = (int64_t)A->n_rows * A->bs;

@@ -247,7 +247,8 @@ int bsp_arnoldi_eigenvalues(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M
  * FP32 accumulation when spmv_precision == BSP_PREC_F32; Gram-Schmidt, least squares,
  * deflation, solution update and the true residual b - Ax are FP64.  Each refinement
  * pass reduces the current true residual by inner_reduction (<= 0: 1e-6); the solve
- * stops when ||b - Ax|| <= tol ||b||.  x_dev is output (initial guess 0). */
+ * stops when ||b - Ax|| <= tol ||b||.  1 <= restart <= 62 and 0 <= k < restart (there is
+ * no unrestarted form).  x_dev is output (initial guess 0). */
 int bsp_fgmres_mixed(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b_dev, double* x_dev,
                      double tol, int maxIters, int restart, int k, int spmv_precision, double inner_reduction,
                      bsp_stats* stats);

@@ -264,3 +264,12 @@ def test_fgmres_mixed_second_gram_schmidt_pass_on_cancellation(ctx, spmv):
     assert rc == 0 and st.true_relres <= 1e-10, (rc, st.true_relres, st.iters)
     assert st.reorth >= 1, st.reorth
     np.testing.assert_allclose(x.numpy(), bh / d, rtol=0, atol=1e-9 * np.abs(bh / d).max())
+
+
+@pytest.mark.parametrize("restart,k", [(0, 0), (-1, 10), (63, 10), (30, 30), (30, -1)])
+def test_fgmres_mixed_rejects_bad_restart(ctx, restart, k):
+    p = pr.dg2d_veclap(6, 5, seed=1)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.ILU0Ex(ctx, A, None, bsp.PREC_F32)
+    with pytest.raises(bsp.BspError):
+        bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, restart=restart, k=k)
