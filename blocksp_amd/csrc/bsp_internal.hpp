@@ -72,6 +72,8 @@ struct Sweep {
     std::vector<int> level_of;   // host: level of each row (unpadded)
     int nslots = 0;
     int nlevels = 0;
+    int fuse_boundary = -1;      // ILU0 lower sweep: its last level is the upper sweep's first
+                                 // (1 yes, 0 no, -1 not yet checked)
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
     // level-launch sweeps replayed from captured HIP graphs, keyed by their operands
@@ -208,6 +210,9 @@ int launch_scal(const bsp_ctx* ctx, int64_t len, double a, double* x);
 // Sweeps: trsv computes x_i <- Dinv_i (x_i - sum_j T_ij x_j) over the schedule.
 // Vectors and values are double, or float when S.f32 (bsp_sweeps.hip).
 int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoch);
+// ILU0 apply: forward sweep over `lower`, backward over `upper` with the diagonal inverses;
+// the shared boundary level is done once when both sweeps are launched level by level.
+int launch_ilu0_sweeps(bsp_ctx* ctx, Sweep& lower, Sweep& upper, const void* dinv, void* x, uint32_t& epoch);
 // SOR sweep: xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j!=i} A_ij x_j)), x_j from xn on the
 // dependency side (forward: j < i) and from xo otherwise.
 int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b,

@@ -578,9 +578,7 @@ int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* Mc, double* r) {
                 if (!rc) rc = launch_f2d(ctx, len, w, r);
                 return rc;
             }
-            int rc = launch_trsv(ctx, M->lower, nullptr, r, ++M->epoch);
-            if (rc) return rc;
-            return launch_trsv(ctx, M->upper, M->dinv_upper.as<double>(), r, ++M->epoch);
+            return launch_ilu0_sweeps(ctx, M->lower, M->upper, M->dinv_upper.as<double>(), r, M->epoch);
         }
         case BSP_PRE_ICHOL0: {
             int rc = launch_trsv(ctx, M->lower, M->dinv_lower.as<double>(), r, ++M->epoch);
@@ -649,11 +647,7 @@ int precond_apply_f32(bsp_ctx* ctx, const bsp_precond* Mc, float* r) {
     auto* M = const_cast<bsp_precond*>(Mc);
     if (!M->f32) return fail(ctx, BSP_E_INVALID, "FP32 apply needs an FP32 preconditioner");
     switch (M->kind) {
-        case BSP_PRE_ILU0: {
-            int rc = launch_trsv(ctx, M->lower, nullptr, r, ++M->epoch);
-            if (rc) return rc;
-            return launch_trsv(ctx, M->upper, M->dinv_upper.p, r, ++M->epoch);
-        }
+        case BSP_PRE_ILU0: return launch_ilu0_sweeps(ctx, M->lower, M->upper, M->dinv_upper.p, r, M->epoch);
         default: return fail(ctx, BSP_E_UNSUPPORTED, "FP32 apply is implemented for ILU0");
     }
 }

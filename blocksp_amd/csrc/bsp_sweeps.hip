@@ -247,8 +247,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_syncfree(SweepArgs a, int nslo
     }
 }
 
+// Levels [l0, l1) of the sweep (l1 < 0: to the last); a partial range is only asked for on
+// the direct level-launch path (few levels, level-launch mode).
 template <int BS, typename VT, typename MT, bool SOR>
-int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
+int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
     constexpr int SPB = kWavesPerBlock * (kWave / BS);
     const int nchunks = (S.nslots + SPB - 1) / SPB;
     a.nchunks = nchunks;
@@ -258,8 +260,12 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
         hipLaunchKernelGGL((k_sweep_syncfree<BS, VT, MT, SOR>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, S.nslots);
         S.tickets_used += (unsigned long long)(nchunks + grid);
     } else {
+        const bool partial = l0 != 0 || (l1 >= 0 && l1 != S.nlevels);
+        if (partial && (S.nlevels > kDirectLevels && ctx->sweep_graphs))
+            return fail(ctx, BSP_E_INVALID, "sweep: a level range needs the direct launch path");
+        const int lend = l1 < 0 ? S.nlevels : l1;
         auto launch_levels = [&](hipStream_t st) {
-            for (int l = 0; l < S.nlevels; ++l) {
+            for (int l = l0; l < lend; ++l) {
                 const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
                 if (cnt <= 0) continue;
                 hipLaunchKernelGGL((k_sweep_level<BS, VT, MT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, st,
@@ -306,18 +312,18 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a) {
 }
 
 template <typename VT, typename MT, bool SOR>
-int dispatch(bsp_ctx* ctx, Sweep& S, const SweepArgs& a) {
+int dispatch(bsp_ctx* ctx, Sweep& S, const SweepArgs& a, int l0 = 0, int l1 = -1) {
     switch (S.bs) {
-        case 1: return run_sweep<1, VT, MT, SOR>(ctx, S, a);
-        case 2: return run_sweep<2, VT, MT, SOR>(ctx, S, a);
-        case 3: return run_sweep<3, VT, MT, SOR>(ctx, S, a);
-        case 4: return run_sweep<4, VT, MT, SOR>(ctx, S, a);
-        case 5: return run_sweep<5, VT, MT, SOR>(ctx, S, a);
-        case 6: return run_sweep<6, VT, MT, SOR>(ctx, S, a);
-        case 7: return run_sweep<7, VT, MT, SOR>(ctx, S, a);
-        case 8: return run_sweep<8, VT, MT, SOR>(ctx, S, a);
-        case 16: return run_sweep<16, VT, MT, SOR>(ctx, S, a);
-        case 32: return run_sweep<32, VT, MT, SOR>(ctx, S, a);
+        case 1: return run_sweep<1, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 2: return run_sweep<2, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 3: return run_sweep<3, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 4: return run_sweep<4, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 5: return run_sweep<5, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 6: return run_sweep<6, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 7: return run_sweep<7, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 8: return run_sweep<8, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 16: return run_sweep<16, VT, MT, SOR>(ctx, S, a, l0, l1);
+        case 32: return run_sweep<32, VT, MT, SOR>(ctx, S, a, l0, l1);
         default: return fail(ctx, BSP_E_UNSUPPORTED, "sweep: unsupported block size " + std::to_string(S.bs));
     }
 }
@@ -338,13 +344,50 @@ SweepArgs make_args(bsp_ctx* ctx, Sweep& S, uint32_t epoch) {
 
 }  // namespace
 
-int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoch) {
+namespace {
+
+int trsv_levels(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoch, int l0, int l1) {
     if (S.n <= 0 || S.nslots <= 0) return BSP_OK;
     SweepArgs a = make_args(ctx, S, epoch);
     a.dinv = dinv;
     a.x = x;
-    if (S.bf16) return dispatch<float, bf16_t, false>(ctx, S, a);
-    return S.f32 ? dispatch<float, float, false>(ctx, S, a) : dispatch<double, double, false>(ctx, S, a);
+    if (S.bf16) return dispatch<float, bf16_t, false>(ctx, S, a, l0, l1);
+    return S.f32 ? dispatch<float, float, false>(ctx, S, a, l0, l1)
+                 : dispatch<double, double, false>(ctx, S, a, l0, l1);
+}
+
+// The last level of `lower` and the first level of `upper` hold the same rows (rows without
+// strict-upper blocks are exactly upper's first level).
+bool shared_boundary_level(const Sweep& lower, const Sweep& upper) {
+    if (lower.n != upper.n || lower.nlevels < 2 || upper.nlevels < 2) return false;
+    if ((int)lower.level_of.size() != lower.n || (int)upper.level_of.size() != upper.n) return false;
+    for (int i = 0; i < lower.n; ++i)
+        if ((lower.level_of[i] == lower.nlevels - 1) != (upper.level_of[i] == 0)) return false;
+    return true;
+}
+
+}  // namespace
+
+int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoch) {
+    return trsv_levels(ctx, S, dinv, x, epoch, 0, -1);
+}
+
+int launch_ilu0_sweeps(bsp_ctx* ctx, Sweep& lower, Sweep& upper, const void* dinv, void* x, uint32_t& epoch) {
+    if (lower.fuse_boundary < 0) lower.fuse_boundary = shared_boundary_level(lower, upper) ? 1 : 0;
+    const bool direct = ctx->trsv_mode == 0 && (!ctx->sweep_graphs || (lower.nlevels <= kDirectLevels &&
+                                                                        upper.nlevels <= kDirectLevels));
+    if (lower.fuse_boundary == 1 && direct) {
+        // rows of the shared level: x = D^-1 (b - L y) in the forward sweep; the backward sweep
+        // starts at its second level
+        const uint32_t e = ++epoch;
+        int rc = trsv_levels(ctx, lower, nullptr, x, e, 0, lower.nlevels - 1);
+        if (!rc) rc = trsv_levels(ctx, lower, dinv, x, e, lower.nlevels - 1, lower.nlevels);
+        if (!rc) rc = trsv_levels(ctx, upper, dinv, x, ++epoch, 1, upper.nlevels);
+        return rc;
+    }
+    int rc = launch_trsv(ctx, lower, nullptr, x, ++epoch);
+    if (rc) return rc;
+    return launch_trsv(ctx, upper, dinv, x, ++epoch);
 }
 
 int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b, const void* xo,
