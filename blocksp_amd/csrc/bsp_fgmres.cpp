@@ -22,10 +22,14 @@
 
 namespace bsp {
 
+// FP32 copy of A's values for the inner SpMV, in the chunk-major order of the chunked kernel
+// when the block size allows it.
 int ensure_values_f32(bsp_ctx* ctx, const bsp_bsr* A) {
     if (A->val32.p) return BSP_OK;
     const int64_t cnt = A->nnzb * (int64_t)A->bs * A->bs;
     BSP_HIP(A->val32.alloc(sizeof(float) * std::max<int64_t>(cnt, 1)));
+    A->val32_chunked = bsmv_f32_chunked_supported(A->bs);
+    if (A->val32_chunked) return launch_d2f_chunked(ctx, A->nnzb, A->bs, A->val.as<double>(), A->val32.as<float>());
     return launch_d2f(ctx, cnt, A->val.as<double>(), A->val32.as<float>());
 }
 
@@ -188,7 +192,10 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         double* w = wbuf[slot];
         // w = A z
         if (spmv_f32) {
-            r2 = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
+            if (A->val32_chunked)
+                r2 = launch_bsmv_f32_chunked(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
+            else
+                r2 = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
         } else {
             r2 = launch_f2d(ctx, len, z, t64);
             if (!r2) r2 = bsmv_matrix(ctx, A, t64, w, 0);

@@ -153,6 +153,7 @@ struct bsp_bsr {
     const bsp_bsr* mm_B = nullptr;
     bool mm_Atran = false;             // C = A^T B: the pairs index A's explicit transpose
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
+    mutable bool val32_chunked = false; // val32 is in the chunk-major order of k_bsmv_lp_perm
     mutable bsp::DevMem val16;         // BF16 copy of the values (MFMA block SpMM)
     mutable bsp::DevMem valp;          // chunk-major copy of the values (bsmv variant 3)
 };
@@ -258,6 +259,10 @@ int launch_fill_u32(const bsp_ctx* ctx, int64_t len, uint32_t v, uint32_t* p);
 // y (double) = A32 x32 with FP32 accumulation; y (float) variant
 int launch_bsmv_f32(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
                     double* y);
+bool bsmv_f32_chunked_supported(int bs);
+int launch_d2f_chunked(const bsp_ctx* ctx, int64_t nnzb, int bs, const double* val, float* valp);
+int launch_bsmv_f32_chunked(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* valp,
+                            const float* x, double* y);
 int launch_bsmv_f32f(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
                      float* y);
 int launch_d2f(const bsp_ctx* ctx, int64_t n, const double* a, float* b);

@@ -3,6 +3,7 @@
 //  * k_bsmv_lp<BS, AT, XT, YT>: y = A x with A in AT (float), x in XT, FP32
 //    accumulation, y stored as YT (float, or double for the FP64 Krylov basis);
 //    same lane mapping as the FP64 bsmv (BS lanes per block row, lane r owns row r).
+//  * k_bsmv_lp_perm<BS>: the FP32-value, FP64-result form over chunk-major FP32 values;
 //  * conversions double <-> float, and the FP64 combination x += sum_j c_j Z_j with
 //    FP32 basis vectors Z_j (the solution update of flexible GMRES).
 #include "bsp_internal.hpp"
@@ -86,6 +87,84 @@ __global__ __launch_bounds__(kBlock) void k_bsmv_lp(int n, const int* __restrict
         acc += s0;
     }
     y[(size_t)i * BS + r] = (YT)acc;
+}
+
+// Same product with the FP32 values in chunk-major order: in block p, lane r's row is stored as
+// BS/4 chunks of 4 floats, chunk k of row r at valp[p*BS*BS + (k*BS + r)*4], so for each k the BS
+// lanes of a block row read 16*BS contiguous bytes.  Each XCD walks a contiguous range of rows
+// (hardware workgroup w runs on XCD w % 8) and the values, read once, are streamed.  Per lane the
+// values and the order of the arithmetic are those of k_bsmv_lp: the results are the same bits.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bsmv_lp_perm(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                         const float* __restrict__ valp, const float* __restrict__ x,
+                                                         double* __restrict__ y, int nwg) {
+    static_assert(BS % 4 == 0, "chunk-major FP32 layout needs a block size divisible by 4");
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int per = (nwg + 7) / 8;
+    const int wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (wg >= nwg) return;
+    const int wave = (wg * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    auto load_row = [&](int p, float (&a)[BS]) {
+        const f4v* s4 = reinterpret_cast<const f4v*>(valp + (size_t)p * BS * BS) + r;
+#pragma unroll
+        for (int k = 0; k < BS / 4; ++k) {
+            const f4v t = __builtin_nontemporal_load(s4 + k * BS);
+            a[4 * k] = t.x;
+            a[4 * k + 1] = t.y;
+            a[4 * k + 2] = t.z;
+            a[4 * k + 3] = t.w;
+        }
+    };
+    float acc = 0.0f;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    int p = p0;
+    for (; p + 1 < p1; p += 2) {
+        const int c0 = ci[p], c1 = ci[p + 1];
+        float a0[BS], a1[BS], x0[BS], x1[BS];
+        load_row(p, a0);
+        load_row(p + 1, a1);
+        load_vec<BS, float>(x + (size_t)c0 * BS, x0);
+        load_vec<BS, float>(x + (size_t)c1 * BS, x1);
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            s0 = fmaf(a0[c], x0[c], s0);
+            s1 = fmaf(a1[c], x1[c], s1);
+        }
+        acc += s0;
+        acc += s1;
+    }
+    if (p < p1) {
+        const int c0 = ci[p];
+        float a0[BS], x0[BS];
+        load_row(p, a0);
+        load_vec<BS, float>(x + (size_t)c0 * BS, x0);
+        float s0 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) s0 = fmaf(a0[c], x0[c], s0);
+        acc += s0;
+    }
+    y[(size_t)i * BS + r] = (double)acc;
+}
+
+// FP64 row-major blocks -> FP32 chunk-major blocks (the layout of k_bsmv_lp_perm)
+__global__ __launch_bounds__(kBlock) void k_d2f_chunked(int64_t total, int bs, const double* __restrict__ val,
+                                                        float* __restrict__ valp) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const int b2 = bs * bs;
+    const int64_t p = t / b2;
+    const int e = (int)(t - p * b2);
+    const int r = e / bs, c = e - r * bs;
+    valp[p * b2 + ((c / 4) * bs + r) * 4 + (c & 3)] = (float)val[t];
 }
 
 __global__ __launch_bounds__(kBlock) void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -178,6 +257,37 @@ int launch_bsmv_f32(const bsp_ctx* ctx, int n, int bs, const int* rp, const int*
                     double* y) {
     if (n <= 0) return BSP_OK;
     return bsmv_lp_dispatch<float, float, double>(ctx, n, bs, rp, ci, val, x, y);
+}
+
+bool bsmv_f32_chunked_supported(int bs) { return bs == 4 || bs == 8 || bs == 16; }
+
+int launch_d2f_chunked(const bsp_ctx* ctx, int64_t nnzb, int bs, const double* val, float* valp) {
+    const int64_t total = nnzb * bs * bs;
+    if (total <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_d2f_chunked, dim3(grid_for(total)), dim3(kBlock), 0, ctx->stream, total, bs, val, valp);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+}
+
+int launch_bsmv_f32_chunked(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* valp,
+                            const float* x, double* y) {
+    if (n <= 0) return BSP_OK;
+    auto launch = [&](auto bsc) {
+        constexpr int BS = decltype(bsc)::value;
+        constexpr int RPW = kWave / BS;
+        const int64_t waves = (n + RPW - 1) / RPW;
+        const int nwg = (int)grid_for(waves * kWave);
+        const int grid = (nwg + 7) / 8 * 8;
+        hipLaunchKernelGGL((k_bsmv_lp_perm<BS>), dim3(grid), dim3(kBlock), 0, ctx->stream, n, rp, ci, valp, x, y, nwg);
+    };
+    switch (bs) {
+        case 4: launch(std::integral_constant<int, 4>{}); break;
+        case 8: launch(std::integral_constant<int, 8>{}); break;
+        case 16: launch(std::integral_constant<int, 16>{}); break;
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "chunked FP32 bsmv: block size must be 4, 8 or 16");
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, std::string("bsmv_lp_perm launch: ") + hipGetErrorString(e));
 }
 
 int launch_bsmv_f32f(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* val, const float* x,
