@@ -79,10 +79,12 @@ __device__ __forceinline__ void load_mrow(const MT* __restrict__ src, VT (&a)[BS
     } else {
         static_assert(std::is_same_v<MT, bf16_t> && std::is_same_v<VT, float>, "BF16 rows widen to FP32");
         if constexpr (BS % 8 == 0) {
-            const uint4* s8 = reinterpret_cast<const uint4*>(src);
+            // factor rows are read once per sweep: streamed, so the vector keeps the caches
+            typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+            const u4v* s8 = reinterpret_cast<const u4v*>(src);
 #pragma unroll
             for (int c = 0; c < BS / 8; ++c) {
-                const uint4 t = s8[c];
+                const u4v t = __builtin_nontemporal_load(s8 + c);
                 a[8 * c + 0] = bf16_lo(t.x);
                 a[8 * c + 1] = bf16_hi(t.x);
                 a[8 * c + 2] = bf16_lo(t.y);
