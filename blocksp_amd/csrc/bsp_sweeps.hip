@@ -379,10 +379,11 @@ int launch_ilu0_sweeps(bsp_ctx* ctx, Sweep& lower, Sweep& upper, const void* din
     const bool direct = ctx->trsv_mode == 0 && (!ctx->sweep_graphs || (lower.nlevels <= kDirectLevels &&
                                                                         upper.nlevels <= kDirectLevels));
     if (lower.fuse_boundary == 1 && direct) {
-        // rows of the shared level: x = D^-1 (b - L y) in the forward sweep; the backward sweep
-        // starts at its second level
+        // The forward sweep's first level holds the rows without strict-lower blocks, where
+        // y = b: nothing to do.  Rows of the shared level: x = D^-1 (b - L y) in the forward
+        // sweep; the backward sweep starts at its second level.
         const uint32_t e = ++epoch;
-        int rc = trsv_levels(ctx, lower, nullptr, x, e, 0, lower.nlevels - 1);
+        int rc = trsv_levels(ctx, lower, nullptr, x, e, 1, lower.nlevels - 1);
         if (!rc) rc = trsv_levels(ctx, lower, dinv, x, e, lower.nlevels - 1, lower.nlevels);
         if (!rc) rc = trsv_levels(ctx, upper, dinv, x, ++epoch, 1, upper.nlevels);
         return rc;
