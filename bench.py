@@ -23,11 +23,12 @@ Also measured in the same run:
     algorithm, 1 thread) on a bounded sample, scaled to the same unit;
   * `workloads`: the secondary configurations -- the 3D 64^3 DG Laplacian (BF16 MFMA
     block SpMM with 32 RHS, mixed vs plain FP64 solve with forward errors), 512 batched
-    uneven IChol0-pCG systems in one persistent launch, and the learned ILU0 correction
-    on the 48^3 random-coefficient problem (trained on the GPU in the same run).
+    uneven IChol0-pCG systems in one persistent launch, device IChol0-pCG on one large
+    system in natural vs parity-coloured order, and the learned ILU0 correction on the
+    48^3 random-coefficient problem (trained on the GPU in the same run).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-plain]
-                       [--skip-extras] [--only bf16|batch|learned]
+                       [--skip-extras] [--only bf16|batch|learned|pcg]
 """
 from __future__ import annotations
 
@@ -228,6 +229,47 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
            "subset_systems": len(sel), "subset_per_system_launches_s": round(per_sys, 3),
            "subset_batched_s": round(ms2 / 1e3, 4), "subset_speedup": round(per_sys / (ms2 / 1e3), 1)}
     log(f"batched 512 systems {ms:.0f} ms; subset per-system {per_sys:.2f} s vs batched {ms2:.0f} ms")
+    # the same batch with each system's IChol0 eliminated by coordinate-parity colours
+    del bat, bat2, A2
+    col = np.concatenate([pr.parity_colours(q.meta["dims"]) for q in probs]).astype(np.int32)
+    t0 = time.perf_counter()
+    batc = bsp.Batch(ctx, A, sysp, 4, colour=col)
+    setup_c = time.perf_counter() - t0
+    Xc = ctx.zeros(Bh.shape)
+    wgc, rcc = batc.autotune(B, Xc, probe_iters=8)
+    batc.pcg(B, Xc, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    itsc, relc, msc, rcode = batc.pcg(B, Xc, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    out["parity_coloured"] = {"batched_s": round(msc / 1e3, 4), "tiles": {"wg": wgc, "rhs_per_item": rcc},
+                              "setup_s": round(setup_c, 2), "converged": rcode == 0, "iters_max": int(itsc.max()),
+                              "iters_mean": round(float(itsc.mean()), 1), "relres_max": float(relc.max()),
+                              "speedup_vs_natural": round(ms / msc, 2)}
+    log(f"batched 512 systems, parity-coloured IChol0: {msc:.0f} ms (iterations mean {itsc.mean():.0f} "
+        f"vs {its.mean():.0f} natural)")
+    return out
+
+
+def bench_pcg(bsp, pr, ctx):
+    """Device pCG with IChol0 to true relres 1e-10 on one large system, the reference's natural
+    elimination order against the parity-coloured one: the largest 2D block Laplacian of the
+    batch (223 x 224 cells, B=4) and the 3D DG Laplacian 48^3 (bs=8)."""
+    out = {}
+    big = max(pr.batch_blocklap(nsys=512, lo=1e3, hi=5e4, seed=2, nrhs=4), key=lambda q: q.n)
+    cases = [("blocklap2d_223x224_B4", big, big.meta["rhs"][0], big.meta["dims"]),
+             ("dg3d_lap_48cubed_bs8", pr.dg3d_lap(48, 48, 48, seed=4), None, (48, 48, 48))]
+    for name, p, rhs, dims in cases:
+        A = bsp.BSR.from_problem(ctx, p)
+        b = ctx.to_device(p.b if rhs is None else rhs)
+        res = {}
+        for label, colour in (("natural", None), ("parity_coloured", pr.parity_colours(dims))):
+            M = bsp.IChol0(ctx, A, colour=colour)
+            bsp.pCG(ctx, A, b, 1e-10, M, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+            _, st, rc = bsp.pCG(ctx, A, b, 1e-10, M, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+            res[label] = {"ms": round(st.solve_ms, 2), "iters": st.iters, "true_relres": st.true_relres,
+                          "converged": rc == 0}
+        res["speedup"] = round(res["natural"]["ms"] / res["parity_coloured"]["ms"], 2)
+        out[name] = res
+        log(f"pCG IChol0 {name}: natural {res['natural']['ms']:.1f} ms ({res['natural']['iters']} it), "
+            f"parity-coloured {res['parity_coloured']['ms']:.1f} ms ({res['parity_coloured']['iters']} it)")
     return out
 
 
@@ -291,7 +333,7 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
     ap.add_argument("--skip-extras", action="store_true", help="skip the 3D BF16, batched and learned workloads")
-    ap.add_argument("--only", choices=["bf16", "batch", "learned"], default=None,
+    ap.add_argument("--only", choices=["bf16", "batch", "learned", "pcg"], default=None,
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -312,7 +354,8 @@ def main():
             ctx.set_option("verbose", args.verbose)
         fn = {"bf16": lambda: bench_bf16(bsp, pr, ctx, solve=False),
               "batch": lambda: bench_batch(bsp, pr, ctx, baseline_every=512),
-              "learned": lambda: bench_learned(bsp, pr, ctx, train_only=True)}[args.only]
+              "learned": lambda: bench_learned(bsp, pr, ctx, train_only=True),
+              "pcg": lambda: bench_pcg(bsp, pr, ctx)}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -444,7 +487,7 @@ def main():
     extras = None
     if not args.skip_extras:
         extras = {"dg3d_bf16_mixed": bench_bf16(bsp, pr, ctx), "batched_ichol0_pcg": bench_batch(bsp, pr, ctx),
-                  "learned_ilu0": bench_learned(bsp, pr, ctx)}
+                  "ichol0_pcg_single": bench_pcg(bsp, pr, ctx), "learned_ilu0": bench_learned(bsp, pr, ctx)}
 
     value = ms_per_step / 1e3
     out = {

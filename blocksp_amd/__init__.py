@@ -90,6 +90,7 @@ _SIGS = {
     "bsp_precond_create_sor": [_VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)],
     "bsp_precond_create_ilu0": [_VP, _VP, C.POINTER(_VP)],
     "bsp_precond_create_ichol0": [_VP, _VP, C.POINTER(_VP)],
+    "bsp_precond_create_ichol0_ex": [_VP, _VP, _VP, C.POINTER(_VP)],
     "bsp_precond_create_host": [_VP, _VP, _VP, _VP, C.POINTER(_VP)],
     "bsp_precond_destroy": [_VP],
     "bsp_precond_apply": [_VP, _VP, _VP],
@@ -119,6 +120,7 @@ _SIGS = {
     "bsp_ctx_record_basis": [_VP, _VP, C.c_int64, C.c_int, C.c_int],
     "bsp_ctx_recorded": [_VP, C.POINTER(C.c_int)],
     "bsp_batch_create": [_VP, _VP, C.c_int, _VP, C.c_int, C.POINTER(_VP)],
+    "bsp_batch_create_ex": [_VP, _VP, C.c_int, _VP, C.c_int, _VP, C.POINTER(_VP)],
     "bsp_batch_destroy": [_VP],
     "bsp_batch_set_tiles": [_VP, _VP, C.c_int, C.c_int],
     "bsp_batch_autotune": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
@@ -473,12 +475,18 @@ class HostPrecond(_Precond):
 
 
 class IChol0(_Precond):
-    """preConditioners::IChol0(A) (blocksp_preconditioners.hpp:364-538)."""
+    """preConditioners::IChol0(A) (blocksp_preconditioners.hpp:364-538); with `colour`, the
+    IChol0 of A eliminated colour by colour (natural order within a colour)."""
 
-    def __init__(self, ctx, A):
+    def __init__(self, ctx, A, colour=None):
         self.ctx, self.A = ctx, A
         h = C.c_void_p()
-        ctx.check(ctx.L.bsp_precond_create_ichol0(ctx.h, A.h, C.byref(h)), "IChol0")
+        if colour is None:
+            ctx.check(ctx.L.bsp_precond_create_ichol0(ctx.h, A.h, C.byref(h)), "IChol0")
+        else:
+            col = np.ascontiguousarray(colour, dtype=np.int32)
+            assert col.shape == (A.n_rows,)
+            ctx.check(ctx.L.bsp_precond_create_ichol0_ex(ctx.h, A.h, _ptr(col), C.byref(h)), "IChol0")
         self.h = h
 
 
@@ -656,13 +664,20 @@ class Batch:
     """Independent systems (diagonal blocks of the block-diagonal A) solved by IChol0-pCG in
     one persistent launch (bsp_batch_*).  sys_rowptr: first block row of each system, plus n."""
 
-    def __init__(self, ctx, A: BSR, sys_rowptr, nrhs):
+    def __init__(self, ctx, A: BSR, sys_rowptr, nrhs, colour=None):
+        """colour: optional colour per block row (each system's IChol0 eliminated colour by colour)."""
         self.ctx, self.A, self.nrhs = ctx, A, int(nrhs)
         self.sys_rowptr = np.ascontiguousarray(sys_rowptr, np.int32)
         self.nsys = len(self.sys_rowptr) - 1
         h = C.c_void_p()
-        ctx.check(ctx.L.bsp_batch_create(ctx.h, A.h, self.nsys, _ptr(self.sys_rowptr), self.nrhs, C.byref(h)),
-                  "batch_create")
+        if colour is None:
+            ctx.check(ctx.L.bsp_batch_create(ctx.h, A.h, self.nsys, _ptr(self.sys_rowptr), self.nrhs, C.byref(h)),
+                      "batch_create")
+        else:
+            col = np.ascontiguousarray(colour, np.int32)
+            assert col.shape == (A.n_rows,)
+            ctx.check(ctx.L.bsp_batch_create_ex(ctx.h, A.h, self.nsys, _ptr(self.sys_rowptr), self.nrhs, _ptr(col),
+                                                C.byref(h)), "batch_create")
         self.h = h
 
     def set_tiles(self, wg, rhs_per_item):

@@ -2,9 +2,10 @@
 // persistent launch (kernels in bsp_batch.hip).
 //
 // Per system the method is the reference's pCG (blocksp_solvers.hpp:51-105) with
-// IChol0 (blocksp_preconditioners.hpp:364-538).  The batch is one block-diagonal
-// matrix, so the IChol0 factor of the whole matrix is the set of per-system factors
-// and its level sets restricted to a system are that system's level sets.
+// IChol0 (blocksp_preconditioners.hpp:364-538), in natural order or eliminated colour by
+// colour.  The batch is one block-diagonal matrix, so the IChol0 factor of the whole
+// matrix is the set of per-system factors and its level sets restricted to a system are
+// that system's level sets.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -177,6 +178,11 @@ extern "C" {
 
 int bsp_batch_create(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
                      bsp_batch** out) {
+    return bsp_batch_create_ex(ctx, A, nsys, sys_rowptr, nrhs, nullptr, out);
+}
+
+int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
+                        const int32_t* colour, bsp_batch** out) {
     if (!ctx || !A || !sys_rowptr || !out || nsys <= 0 || nrhs <= 0) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "batched systems must be square");
@@ -200,7 +206,9 @@ int bsp_batch_create(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sy
                           "block row " + std::to_string(i) + " couples system " + std::to_string(s) +
                               " to another system");
     }
-    int rc = bsp_precond_create_ichol0(ctx, A, &b->M);
+    // a colour order over the whole block-diagonal matrix is, restricted to each system, that
+    // system's own colour order (no block couples two systems)
+    int rc = bsp_precond_create_ichol0_ex(ctx, A, colour, &b->M);
     if (rc) return rc;
     std::vector<int32_t> rows, lev, base;
     level_sets(b->M->lower.level_of, b->sys_row0, rows, lev, base);

@@ -12,6 +12,7 @@
 // over level schedules (bsp_kernels.hip).
 #include <algorithm>
 #include <cstring>
+#include <numeric>
 
 #include "bsp_dense.hpp"
 #include "bsp_internal.hpp"
@@ -412,36 +413,23 @@ int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int 
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
-    return create_ilu0_impl(ctx, A, nullptr, BSP_PREC_F64, out);
-}
-
-int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision,
-                               bsp_precond** out) {
-    if (!ctx) return BSP_E_INVALID;
-    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
-              "precision must be F64, F32 or BF16");
-    return create_ilu0_impl(ctx, A, colour, precision, out);
-}
-
-int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
-    if (!ctx || !A || !out) return BSP_E_INVALID;
-    *out = nullptr;
-    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "IChol0 needs a square matrix");
-    BSP_HIP(hipSetDevice(ctx->device));
-    if (int rc = sync_host_values(ctx, A)) return rc;
-    const int n = A->n_rows, bs = A->bs;
+// IChol0 (blocksp_preconditioners.hpp:374-429) of the block matrix (rp, ci, val) on the host, in
+// its own row order: the factor L on lowTri, each row's blocks in stored order.
+int ichol0_factor_rows(bsp_ctx* ctx, int n, int bs, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                       const std::vector<double>& val, std::vector<int32_t>& Lrp, std::vector<int32_t>& Lci,
+                       std::vector<double>& Lv) {
     const size_t b2 = (size_t)bs * bs;
     // lowTri (blocksp_containers.hpp:277-285): blocks with col <= row, in row order
-    std::vector<int32_t> Lrp(n + 1, 0), Lci;
-    std::vector<double> Lv;
+    Lrp.assign(n + 1, 0);
+    Lci.clear();
+    Lv.clear();
     for (int i = 0; i < n; ++i) {
-        for (int p = A->h_rp[i]; p < A->h_rp[i + 1]; ++p)
-            if (A->h_ci[p] <= i) {
-                Lci.push_back(A->h_ci[p]);
-                Lv.insert(Lv.end(), A->h_val.begin() + p * b2, A->h_val.begin() + (p + 1) * b2);
+        for (int p = rp[i]; p < rp[i + 1]; ++p)
+            if (ci[p] <= i) {
+                Lci.push_back(ci[p]);
+                Lv.insert(Lv.end(), val.begin() + p * b2, val.begin() + (p + 1) * b2);
             }
         Lrp[i + 1] = (int32_t)Lci.size();
     }
@@ -487,7 +475,7 @@ int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out)
             for (int p = Lrp[k]; p < Lrp[k + 1]; ++p) {
                 const int j = Lci[p];
                 if (j <= i || j > k) continue;
-                if (find_col(A->h_rp, A->h_ci, i, j) < 0) continue;
+                if (find_col(rp, ci, i, j) < 0) continue;
                 const int pji = find_col(Lrp, Lci, j, i);
                 if (pji < 0) continue;
                 dense::transpose_to(bs, Lv.data() + (size_t)pji * b2, ljt.data());
@@ -497,6 +485,16 @@ int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out)
             }
         }
     }
+    return BSP_OK;
+}
+
+// Device IChol0 from a factor whose row i holds the blocks (i, j) with pos[j] <= pos[i] (diagonal
+// included); elim lists the rows in elimination order and pos is its inverse.  The forward sweep
+// runs over the strict blocks of L in elimination order, the backward sweep over L^T (row i:
+// (L_ki)^T for the rows k eliminated after i, in elimination order) in reverse.
+int ichol0_build(bsp_ctx* ctx, int n, int bs, std::vector<int32_t>& Lrp, std::vector<int32_t>& Lci,
+                 std::vector<double>& Lv, const std::vector<int32_t>& elim, bsp_precond** out) {
+    const size_t b2 = (size_t)bs * bs;
     auto M = std::make_unique<bsp_precond>();
     M->ctx = ctx;
     M->kind = BSP_PRE_ICHOL0;
@@ -505,34 +503,125 @@ int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out)
     std::vector<double> dlo(n * b2), dup(n * b2);
     for (int i = 0; i < n; ++i) {
         const int pd = find_col(Lrp, Lci, i, i);
+        BSP_CHECK(pd >= 0, BSP_E_NUMERIC, "IChol0: missing diagonal block in row " + std::to_string(i));
         BSP_CHECK(dense::invert(bs, Lv.data() + (size_t)pd * b2, dlo.data() + i * b2), BSP_E_NUMERIC,
                   "IChol0: singular factor block");
         dense::transpose_to(bs, dlo.data() + i * b2, dup.data() + i * b2);
     }
-    std::vector<int32_t> lrp, lci;
+    std::vector<int32_t> lrp(n + 1, 0), lci;
     std::vector<double> lv;
-    split_strict(n, bs, Lrp, Lci, Lv, true, lrp, lci, lv);
-    // L^T strictly upper: row i holds (L_ki)^T for k > i, ascending k
-    std::vector<int32_t> urp(n + 1, 0), uci(cptr[n]);
-    std::vector<double> uv((size_t)cptr[n] * b2);
+    std::vector<int32_t> ucnt(n + 1, 0);
     for (int i = 0; i < n; ++i) {
-        urp[i + 1] = cptr[i + 1];
-        for (int q = cptr[i]; q < cptr[i + 1]; ++q) {
-            uci[q] = crow[q];
-            dense::transpose_to(bs, Lv.data() + (size_t)cpos[q] * b2, uv.data() + (size_t)q * b2);
+        for (int p = Lrp[i]; p < Lrp[i + 1]; ++p) {
+            const int c = Lci[p];
+            if (c == i) continue;
+            lci.push_back(c);
+            lv.insert(lv.end(), Lv.begin() + (size_t)p * b2, Lv.begin() + (size_t)(p + 1) * b2);
+            ucnt[c + 1]++;
+        }
+        lrp[i + 1] = (int32_t)lci.size();
+    }
+    // L^T strictly upper: row c holds (L_kc)^T for the rows k eliminated after c, in elimination order
+    std::vector<int32_t> urp(n + 1, 0);
+    for (int c = 0; c < n; ++c) urp[c + 1] = urp[c] + ucnt[c + 1];
+    std::vector<int32_t> uci(urp[n]), fill(urp.begin(), urp.end() - 1);
+    std::vector<double> uv((size_t)urp[n] * b2);
+    for (int t = 0; t < n; ++t) {
+        const int k = elim[t];
+        for (int p = Lrp[k]; p < Lrp[k + 1]; ++p) {
+            const int c = Lci[p];
+            if (c == k) continue;
+            const int q = fill[c]++;
+            uci[q] = k;
+            dense::transpose_to(bs, Lv.data() + (size_t)p * b2, uv.data() + (size_t)q * b2);
         }
     }
     BSP_HIP(upload_structure(M->lower, lrp, lci, lv, bs));
     BSP_HIP(upload_structure(M->upper, urp, uci, uv, bs));
-    build_schedule(M->lower, n, bs, lrp, lci, true);
-    build_schedule(M->upper, n, bs, urp, uci, false);
+    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
+    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
     BSP_HIP(upload(M->dinv_lower, dlo));
     BSP_HIP(upload(M->dinv_upper, dup));
+    M->elim_order = elim;
     M->f_rp.swap(Lrp);
     M->f_ci.swap(Lci);
     M->f_val.swap(Lv);
     *out = M.release();
     return BSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+    return create_ilu0_impl(ctx, A, nullptr, BSP_PREC_F64, out);
+}
+
+int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision,
+                               bsp_precond** out) {
+    if (!ctx) return BSP_E_INVALID;
+    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
+              "precision must be F64, F32 or BF16");
+    return create_ilu0_impl(ctx, A, colour, precision, out);
+}
+
+int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out) {
+    return bsp_precond_create_ichol0_ex(ctx, A, nullptr, out);
+}
+
+int bsp_precond_create_ichol0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bsp_precond** out) {
+    if (!ctx || !A || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "IChol0 needs a square matrix");
+    BSP_HIP(hipSetDevice(ctx->device));
+    if (int rc = sync_host_values(ctx, A)) return rc;
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    std::vector<int32_t> elim(n), pos(n);
+    std::iota(elim.begin(), elim.end(), 0);
+    std::vector<int32_t> Lrp, Lci;
+    std::vector<double> Lv;
+    if (!colour) {
+        int rc = ichol0_factor_rows(ctx, n, bs, A->h_rp, A->h_ci, A->h_val, Lrp, Lci, Lv);
+        if (rc) return rc;
+        return ichol0_build(ctx, n, bs, Lrp, Lci, Lv, elim, out);
+    }
+    // colour by colour, natural order within a colour: IChol0 of P A P^T, mapped back
+    for (int i = 0; i < n; ++i) BSP_CHECK(colour[i] >= 0, BSP_E_INVALID, "colours must be non-negative");
+    std::stable_sort(elim.begin(), elim.end(), [&](int a, int b) { return colour[a] < colour[b]; });
+    for (int t = 0; t < n; ++t) pos[elim[t]] = t;
+    std::vector<int32_t> prp(n + 1, 0), pci;
+    std::vector<double> pv;
+    pci.reserve(A->h_ci.size());
+    pv.reserve(A->h_val.size());
+    for (int t = 0; t < n; ++t) {
+        const int i = elim[t];
+        for (int p = A->h_rp[i]; p < A->h_rp[i + 1]; ++p) {
+            pci.push_back(pos[A->h_ci[p]]);
+            pv.insert(pv.end(), A->h_val.begin() + (size_t)p * b2, A->h_val.begin() + (size_t)(p + 1) * b2);
+        }
+        prp[t + 1] = (int32_t)pci.size();
+    }
+    std::vector<int32_t> Prp, Pci;
+    std::vector<double> Pv;
+    int rc = ichol0_factor_rows(ctx, n, bs, prp, pci, pv, Prp, Pci, Pv);
+    if (rc) return rc;
+    Lrp.assign(n + 1, 0);
+    for (int t = 0; t < n; ++t) Lrp[elim[t] + 1] = Prp[t + 1] - Prp[t];
+    for (int i = 0; i < n; ++i) Lrp[i + 1] += Lrp[i];
+    Lci.resize(Pci.size());
+    Lv.resize(Pv.size());
+    for (int t = 0; t < n; ++t) {
+        const int i = elim[t];
+        int q = Lrp[i];
+        for (int p = Prp[t]; p < Prp[t + 1]; ++p, ++q) {
+            Lci[q] = elim[Pci[p]];
+            std::copy(Pv.begin() + (size_t)p * b2, Pv.begin() + (size_t)(p + 1) * b2, Lv.begin() + (size_t)q * b2);
+        }
+    }
+    return ichol0_build(ctx, n, bs, Lrp, Lci, Lv, elim, out);
 }
 
 int bsp_precond_create_host(bsp_ctx* ctx, const bsp_bsr* A, bsp_host_precond_fn fn, void* user, bsp_precond** out) {

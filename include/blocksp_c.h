@@ -172,6 +172,11 @@ int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* co
                                bsp_precond** out);
 /* IChol0(A) (preconditioners.hpp:364-538), A symmetric positive definite. */
 int bsp_precond_create_ichol0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
+/* IChol0 of A in the elimination order given by `colour` (rows colour by colour, natural order
+ * within a colour; no reference counterpart): the IChol0 factor of P A P^T applied to vectors in
+ * A's numbering, with sweeps of at most (number of colours) levels.  NULL = natural order
+ * (bsp_precond_create_ichol0). */
+int bsp_precond_create_ichol0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bsp_precond** out);
 /* A user preconditioner (the reference's PreCond concept: any object with
  * `void operator()(solArray&) const`, preconditioners.hpp:542-556, lambdas in
  * blocksp_tests.hpp:113).  The library copies r to a host buffer, calls
@@ -329,6 +334,12 @@ typedef struct bsp_batch bsp_batch;
 /* Builds the IChol0 factor and the per-system level sets (A must outlive the batch). */
 int bsp_batch_create(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
                      bsp_batch** out);
+/* As bsp_batch_create with each system's IChol0 eliminated colour by colour (`colour`: one
+ * non-negative colour per block row of A, e.g. coordinate parity of the grid cells; NULL =
+ * natural order; no reference counterpart).  A few levels per sweep instead of ~2 sqrt(n):
+ * more pCG iterations, far fewer dependent steps per iteration. */
+int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
+                        const int32_t* colour, bsp_batch** out);
 int bsp_batch_destroy(bsp_batch* b);
 /* Tiles: workgroup size (multiple of 64, <= 1024) and right-hand sides per work item
  * (1, 2 or 4, dividing nrhs). */

@@ -133,3 +133,74 @@ def test_batched_equals_one_launch_per_system_bitwise(ctx, wg, r):
         assert rc1 == 0
         np.testing.assert_array_equal(its1[0], its[s])
         np.testing.assert_array_equal(Xs.numpy(), Xh[:, sys[s] * 4:sys[s + 1] * 4])
+
+
+def _permuted(p, perm):
+    """The problem with rows and columns renumbered: new row t = old row perm[t], each row's
+    blocks in its stored order with the columns renumbered (P A P^T)."""
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    rp, ci, vals = [0], [], []
+    for old in perm:
+        ci.append(inv[p.colidx[p.rowptr[old]:p.rowptr[old + 1]]])
+        vals.append(p.vals[p.rowptr[old]:p.rowptr[old + 1]])
+        rp.append(rp[-1] + len(ci[-1]))
+    return pr.Problem(p.n, p.bs, p.B, p.D, np.array(rp, np.int32), np.concatenate(ci).astype(np.int32),
+                      np.concatenate(vals), p.b[np.repeat(perm * p.bs, p.bs) + np.tile(np.arange(p.bs), p.n)])
+
+
+def test_coloured_ichol0_is_the_ichol0_of_the_permuted_matrix(ctx):
+    # the colour-ordered IChol0 applied in A's numbering equals the oracle's natural IChol0 of
+    # P A P^T applied to the permuted vector; its sweeps have 3 levels (2D parity colours)
+    p = pr.batch_blocklap(nsys=1, lo=600, hi=700, seed=3)[0]
+    col = pr.parity_colours(p.meta["dims"])
+    perm = np.argsort(col, kind="stable")
+    q = _permuted(p, perm)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.IChol0(ctx, A, colour=col)
+    r = np.random.default_rng(1).standard_normal(p.n * p.bs)
+    lrp, lci, lv = po.ichol0_factor(q)
+    rq = r.reshape(p.n, p.bs)[perm].reshape(-1)
+    zq = po.ichol0_apply(q.n, q.bs, lrp, lci, lv, rq)
+    rd = ctx.to_device(r)
+    M(rd)
+    z = rd.numpy().reshape(p.n, p.bs)
+    np.testing.assert_allclose(z[perm].reshape(-1), zq, rtol=0, atol=1e-12 * np.abs(zq).max())
+
+
+def test_coloured_batch_matches_oracle_pcg_on_permuted_systems(ctx):
+    probs, A, sys, Bh = _batch(ctx, nsys=8, lo=60, hi=900, seed=4)
+    col = np.concatenate([pr.parity_colours(p.meta["dims"]) for p in probs]).astype(np.int32)
+    bat = bsp.Batch(ctx, A, sys, 4, colour=col)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=2000, tol_mode=bsp.TOL_REFERENCE)
+    assert rc == 0
+    Xh = X.numpy()
+    for s, p in enumerate(probs):
+        perm = np.argsort(pr.parity_colours(p.meta["dims"]), kind="stable")
+        q = _permuted(p, perm)
+        for k in range(4):
+            b = p.meta["rhs"][k]
+            bq = b.reshape(p.n, p.bs)[perm].reshape(-1)
+            xo, st = po.pcg(q, bq, 1e-10, po.PRE_ICHOL0, maxIters=2000)
+            x = _solution_of(Xh, probs, sys, s, k)
+            assert abs(int(its[s, k]) - st.iters) <= 1, (s, k, its[s, k], st.iters)
+            xq = x.reshape(p.n, p.bs)[perm].reshape(-1)
+            assert np.linalg.norm(xq - xo) <= 1e-8 * np.linalg.norm(xo)
+            assert np.linalg.norm(b - p.bsmv(x)) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_device_pcg_with_coloured_ichol0_matches_oracle_on_permuted_system(ctx):
+    p = pr.batch_blocklap(nsys=1, lo=900, hi=1000, seed=6)[0]
+    perm = np.argsort(pr.parity_colours(p.meta["dims"]), kind="stable")
+    q = _permuted(p, perm)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.IChol0(ctx, A, colour=pr.parity_colours(p.meta["dims"]))
+    b = p.meta["rhs"][1]
+    x, st, rc = bsp.pCG(ctx, A, ctx.to_device(b), 1e-10, M, maxIters=2000)
+    bq = b.reshape(p.n, p.bs)[perm].reshape(-1)
+    xo, sto = po.pcg(q, bq, 1e-10, po.PRE_ICHOL0, maxIters=2000)
+    assert rc == 0 and abs(st.iters - sto.iters) <= 1, (st.iters, sto.iters)
+    xq = x.numpy().reshape(p.n, p.bs)[perm].reshape(-1)
+    assert np.linalg.norm(xq - xo) <= 1e-8 * np.linalg.norm(xo)
