@@ -88,6 +88,7 @@ _SIGS = {
     "bsp_precond_create_none": [_VP, _VP, C.POINTER(_VP)],
     "bsp_precond_create_diag": [_VP, _VP, C.c_int, C.POINTER(_VP)],
     "bsp_precond_create_sor": [_VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)],
+    "bsp_precond_create_sor_ex": [_VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, _VP, C.POINTER(_VP)],
     "bsp_precond_create_ilu0": [_VP, _VP, C.POINTER(_VP)],
     "bsp_precond_create_ichol0": [_VP, _VP, C.POINTER(_VP)],
     "bsp_precond_create_ichol0_ex": [_VP, _VP, _VP, C.POINTER(_VP)],
@@ -415,13 +416,21 @@ class Diag(_Precond):
 
 
 class SOR(_Precond):
-    """preConditioners::SOR<Inv>(A, w, nu, type) (blocksp_preconditioners.hpp:116-240)."""
+    """preConditioners::SOR<Inv>(A, w, nu, type) (blocksp_preconditioners.hpp:116-240); with
+    `colour`, multicolour SOR (rows swept colour by colour, natural order within a colour)."""
 
-    def __init__(self, ctx, A, w, nu, type=0, cholesky=False):
+    def __init__(self, ctx, A, w, nu, type=0, cholesky=False, colour=None):
         self.ctx, self.A = ctx, A
         h = C.c_void_p()
-        ctx.check(ctx.L.bsp_precond_create_sor(ctx.h, A.h, float(w), int(nu), int(type), int(cholesky), C.byref(h)),
-                  "SOR")
+        if colour is None:
+            ctx.check(ctx.L.bsp_precond_create_sor(ctx.h, A.h, float(w), int(nu), int(type), int(cholesky),
+                                                   C.byref(h)), "SOR")
+        else:
+            col = np.ascontiguousarray(colour, dtype=np.int32)
+            assert col.shape == (A.n_rows,)
+            self._colour = col
+            ctx.check(ctx.L.bsp_precond_create_sor_ex(ctx.h, A.h, float(w), int(nu), int(type), int(cholesky),
+                                                      _ptr(col), C.byref(h)), "SOR")
         self.h = h
 
 

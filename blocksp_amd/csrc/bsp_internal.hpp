@@ -61,6 +61,7 @@ struct Sweep {
     const int* drp = nullptr;    // the structure the kernels read: the owned arrays or
     const int* dci = nullptr;    // (SOR) the matrix's own arrays
     const void* dval = nullptr;
+    const int* dpos = nullptr;   // SOR in an elimination order: step of each row (device)
     bool f32 = false;            // values / vectors in FP32 instead of FP64
     bool bf16 = false;           // values (and the diagonal inverses applied with them) stored in
                                  // BF16, vectors and arithmetic in FP32 (f32 is set too)
@@ -173,7 +174,8 @@ struct bsp_precond {
     bool f32 = false;                 // factor, diagonal inverses and applies in FP32
     bool bf16 = false;                // factor and diagonal inverses stored in BF16 (applies in FP32)
     bsp::DevMem work32;               // FP32 staging vector for applies on FP64 vectors
-    std::vector<int32_t> elim_order;  // elimination order of the factorisation (ILU0)
+    std::vector<int32_t> elim_order;  // elimination order of the factorisation (ILU0, IChol0)
+    bsp::DevMem pos;                  // multicolour SOR: step of each row in the sweep order
     bsp_host_precond_fn host_fn = nullptr;   // BSP_PRE_HOST
     void* host_user = nullptr;
     // host copy of the factor in the reference layout (for parity download)

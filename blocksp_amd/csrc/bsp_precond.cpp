@@ -230,6 +230,11 @@ int bsp_precond_create_diag(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, bsp_pr
 
 int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int type, int inv_kind,
                            bsp_precond** out) {
+    return bsp_precond_create_sor_ex(ctx, A, w, nu, type, inv_kind, nullptr, out);
+}
+
+int bsp_precond_create_sor_ex(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int type, int inv_kind,
+                              const int32_t* colour, bsp_precond** out) {
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "SOR needs a square matrix");
@@ -249,13 +254,34 @@ int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int
     int rc = diag_inverses(ctx, A, inv_kind, dinv);
     if (rc) return rc;
     BSP_HIP(upload(M->dinv_upper, dinv));
-    build_schedule(M->lower, M->n, M->bs, A->h_rp, A->h_ci, true);
-    build_schedule(M->upper, M->n, M->bs, A->h_rp, A->h_ci, false);
+    if (!colour) {
+        build_schedule(M->lower, M->n, M->bs, A->h_rp, A->h_ci, true);
+        build_schedule(M->upper, M->n, M->bs, A->h_rp, A->h_ci, false);
+    } else {
+        // multicolour SOR: the sweeps visit the rows colour by colour (natural order within a
+        // colour), i.e. the reference's SOR on P A P^T; a row uses the new values of the rows
+        // before it in that order
+        const int n = M->n;
+        for (int i = 0; i < n; ++i) BSP_CHECK(colour[i] >= 0, BSP_E_INVALID, "colours must be non-negative");
+        std::vector<int32_t> elim(n), pos(n);
+        std::iota(elim.begin(), elim.end(), 0);
+        std::stable_sort(elim.begin(), elim.end(), [&](int a, int b) { return colour[a] < colour[b]; });
+        for (int t = 0; t < n; ++t) pos[elim[t]] = t;
+        std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+        build_schedule_topo(M->lower, n, M->bs, A->h_rp, A->h_ci, elim,
+                            [&pos](int i, int c) { return pos[c] < pos[i]; });
+        build_schedule_topo(M->upper, n, M->bs, A->h_rp, A->h_ci, rev,
+                            [&pos](int i, int c) { return pos[c] > pos[i]; });
+        BSP_HIP(M->pos.alloc(sizeof(int32_t) * std::max(n, 1)));
+        BSP_HIP(hipMemcpy(M->pos.p, pos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        M->elim_order = elim;
+    }
     for (Sweep* S : {&M->lower, &M->upper}) {
         S->drp = A->rp.as<int>();
         S->dci = A->ci.as<int>();
         S->dval = A->val.as<double>();
         S->nnzb = A->nnzb;
+        S->dpos = colour ? M->pos.as<int>() : nullptr;
     }
     const size_t len = (size_t)M->n * M->bs;
     BSP_HIP(M->work0.alloc(sizeof(double) * std::max<size_t>(len, 1)));

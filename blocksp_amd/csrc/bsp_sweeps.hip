@@ -140,6 +140,7 @@ struct SweepArgs {
     unsigned long long base;
     int64_t spin_limit;
     uint32_t* err;
+    const int* pos;     // SOR in an elimination order: step of each row (nullptr: natural order)
 };
 
 // One chunk of slots: [slot_begin, slot_end) with kWavesPerBlock * RPW slots per block.
@@ -163,7 +164,9 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
         if (active) {
             for (int p = p0; p < p1; ++p) {
                 const int c = a.ci[p];
-                const bool dep = SOR ? (a.forward ? c < i : c > i) : true;
+                bool dep = true;
+                if constexpr (SOR) dep = a.pos ? (a.forward ? a.pos[c] < a.pos[i] : a.pos[c] > a.pos[i])
+                                               : (a.forward ? c < i : c > i);
                 if (dep) wait_relaxed(&a.flags[c], a.epoch, a.spin_limit, a.err);
             }
         }
@@ -179,7 +182,8 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
             for (int p = p0; p < p1; ++p) {
                 const int c = a.ci[p];
                 if (c == i) continue;
-                const bool dep = a.forward ? c < i : c > i;
+                const bool dep = a.pos ? (a.forward ? a.pos[c] < a.pos[i] : a.pos[c] > a.pos[i])
+                                       : (a.forward ? c < i : c > i);
                 VT av[BS], xv[BS];
                 load_mrow<BS, MT, VT>(val + ((size_t)p * BS + r) * BS, av);
                 if (dep) {
@@ -341,6 +345,7 @@ SweepArgs make_args(bsp_ctx* ctx, Sweep& S, uint32_t epoch) {
     a.ticket = S.ticket.as<unsigned long long>();
     a.spin_limit = ctx->spin_limit;
     a.err = ctx->err_flag.as<uint32_t>();
+    a.pos = S.dpos;
     return a;
 }
 

@@ -154,6 +154,12 @@ int bsp_precond_create_diag(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, bsp_pr
  * (the reference keeps a non-owning pointer too, preconditioners.hpp:120). */
 int bsp_precond_create_sor(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int type, int inv_kind,
                            bsp_precond** out);
+/* Multicolour block SOR (no reference counterpart): the SOR above with the rows swept colour by
+ * colour (natural order within a colour; `colour`: one non-negative colour per block row), i.e.
+ * the reference's SOR of P A P^T applied in A's numbering; sweeps of (number of colours) levels.
+ * NULL = natural order (bsp_precond_create_sor). */
+int bsp_precond_create_sor_ex(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, int type, int inv_kind,
+                              const int32_t* colour, bsp_precond** out);
 /* ILU0(A) (preconditioners.hpp:246-357): factor + diagonal-block inverses. */
 int bsp_precond_create_ilu0(bsp_ctx* ctx, const bsp_bsr* A, bsp_precond** out);
 /* ILU0 with a chosen elimination order and storage precision (no reference

@@ -329,3 +329,32 @@ def test_bsmv_variants(ctx, bs, variant):
         check_bsmv(yd.numpy(), yref, absax)
     finally:
         ctx.set_option("bsmv_variant", 5)   # the default
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("typ", [-1, 0, 1])
+def test_multicolour_sor_is_the_sor_of_the_permuted_matrix(ctx, typ, mode):
+    # rows swept colour by colour (coordinate parity, natural order within a colour): equal to
+    # the reference's SOR of P A P^T (oracle) on the permuted vector; 3 levels per sweep
+    from test_gpu_mixed import permuted
+    print(f"multicolour sor type={typ} trsv={mode}", flush=True)
+    ctx.set_option("trsv", mode)
+    try:
+        dims = (11, 9)
+        p = pr.dg2d_veclap(*dims, seed=2)
+        col = pr.parity_colours(dims)
+        perm = np.argsort(col, kind="stable")
+        q = permuted(p, perm)
+        A = bsp.BSR.from_problem(ctx, p)
+        for w, nu in ((1.2, 3), (0.7, 1)):
+            M = bsp.SOR(ctx, A, w, nu, typ, colour=col)
+            r = np.random.default_rng(7).standard_normal(p.n * p.bs)
+            rd = ctx.to_device(r)
+            M(rd)
+            zq = po.sor_apply(q, r.reshape(p.n, p.bs)[perm].reshape(-1), w, nu, typ)
+            z = rd.numpy().reshape(p.n, p.bs)[perm].reshape(-1)
+            assert np.linalg.norm(z - zq) <= 1e-10 * np.linalg.norm(zq)
+        nl_lo, _ = M.levels(True)
+        assert nl_lo == 3
+    finally:
+        ctx.set_option("trsv", 0)
