@@ -190,6 +190,20 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
     bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     Xh = X.numpy()
+    # the same solve with the sweeps reading the CSR factor instead of its ELL copies
+    ctx.set_option("batch_ell", 0)
+    try:
+        batl = bsp.Batch(ctx, A, sysp, 4)
+    finally:
+        ctx.set_option("batch_ell", 1)
+    batl.set_tiles(wg, r)
+    Xl = ctx.zeros(Bh.shape)
+    batl.pcg(B, Xl, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    itsl, _, msl, _ = batl.pcg(B, Xl, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    csr_same = bool((itsl == its).all() and np.array_equal(Xl.numpy(), Xh))
+    log(f"batched 512 systems: ELL sweeps (width {bat.ell_width()}) {ms:.0f} ms, CSR sweeps {msl:.0f} ms, "
+        f"bitwise equal: {csr_same}")
+    del batl, Xl
     worst = 0.0
     for k in range(4):
         xk = ctx.to_device(Xh[k])
@@ -224,6 +238,8 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
     _, _, ms2, _ = bat2.pcg(B2, X2, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     out = {"workload": "512 SPD block Laplacians B=4, 1e3..5e4 block rows (log-uniform), 4 RHS, IChol0-pCG to 1e-10",
            "block_rows": n, "batched_s": round(ms / 1e3, 4), "tiles": {"wg": wg, "rhs_per_item": r},
+           "sweep_layout": f"ELL width {bat.ell_width()}" if bat.ell_width() else "CSR",
+           "csr_sweeps": {"batched_s": round(msl / 1e3, 4), "bitwise_equal": csr_same},
            "setup_s": round(setup, 2), "converged": rc == 0, "iters_max": int(its.max()),
            "iters_mean": round(float(its.mean()), 1), "true_relres_max": worst,
            "subset_systems": len(sel), "subset_per_system_launches_s": round(per_sys, 3),

@@ -127,6 +127,7 @@ _SIGS = {
     "bsp_batch_autotune": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsp_batch_pcg": [_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, _VP, _VP, C.POINTER(C.c_double)],
     "bsp_batch_info": [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "bsp_batch_ell_width": [_VP],
     "bsp_bsmm_rhs_bf16": [_VP, _VP, C.c_int, _VP, _VP],
     "bsp_rhs_pack_bf16": [_VP, C.c_int64, C.c_int, C.c_int, _VP, _VP],
     "bsp_rhs_unpack_f32": [_VP, C.c_int64, C.c_int, C.c_int, _VP, _VP],
@@ -702,6 +703,10 @@ class Batch:
         wg, r, ni = C.c_int(), C.c_int(), C.c_int()
         self.ctx.check(self.ctx.L.bsp_batch_info(self.h, C.byref(wg), C.byref(r), C.byref(ni)), "batch_info")
         return wg.value, r.value, ni.value
+
+    def ell_width(self):
+        """Entries per slot of the ELL sweep copies (0: the sweeps read the CSR factor)."""
+        return int(self.ctx.L.bsp_batch_ell_width(self.h))
 
     def pcg(self, B: DeviceArray, X: DeviceArray, tol, maxIters=5000, tol_mode=TOL_REFERENCE):
         its = np.zeros(self.nsys * self.nrhs, np.int32)

@@ -28,6 +28,8 @@ struct bsp_batch {
     DevMem d_sys_row0, frows, flev, flev_base, brows, blev, blev_base;
     DevMem Bi, Xi, Rv, P, Z, AP, iters, res, work;
     DevMem item_sys, item_rhs0;
+    int ell_k = 0;                    // entries per slot of the ELL sweep copies (0: CSR sweeps)
+    DevMem fcol, fval, fdinv, bcol, bval, bdinv;
     int items_R = 0, nitems = 0;
     int wg = 256, R = 1;
     bool tuned = false;
@@ -80,6 +82,46 @@ hipError_t upload_i(DevMem& m, const std::vector<int32_t>& v) {
     return hipMemcpy(m.p, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice);
 }
 
+// Slot-ordered ELL copies of both sweeps (entries in CSR order, -1 / zero padding) when every
+// row holds at most 4 strict blocks; the factor's host copy gives the row lengths.
+int build_ell(bsp_ctx* ctx, bsp_batch* b) {
+    const bsp_precond* M = b->M;
+    const int n = M->n;
+    if (n <= 0 || (int)M->f_rp.size() != n + 1) return BSP_OK;
+    int klo = 0;
+    std::vector<int> ucnt(n, 0);
+    for (int i = 0; i < n; ++i) {
+        int cnt = 0;
+        for (int p = M->f_rp[i]; p < M->f_rp[i + 1]; ++p)
+            if (M->f_ci[p] != i) {
+                ++cnt;
+                ucnt[M->f_ci[p]]++;
+            }
+        klo = std::max(klo, cnt);
+    }
+    const int kup = *std::max_element(ucnt.begin(), ucnt.end());
+    const int K = batch::ell_width(std::max(klo, kup));
+    if (K == 0) return BSP_OK;
+    const size_t b2 = (size_t)b->bs * b->bs;
+    BSP_HIP(b->fcol.alloc(sizeof(int) * (size_t)n * K));
+    BSP_HIP(b->bcol.alloc(sizeof(int) * (size_t)n * K));
+    BSP_HIP(b->fval.alloc(sizeof(double) * (size_t)n * K * b2));
+    BSP_HIP(b->bval.alloc(sizeof(double) * (size_t)n * K * b2));
+    BSP_HIP(b->fdinv.alloc(sizeof(double) * (size_t)n * b2));
+    BSP_HIP(b->bdinv.alloc(sizeof(double) * (size_t)n * b2));
+    int rc = batch::launch_ell_gather(ctx, n, K, b->bs, b->frows.as<int>(), M->lower.drp, M->lower.dci,
+                                      static_cast<const double*>(M->lower.dval), M->dinv_lower.as<double>(),
+                                      b->fcol.as<int>(), b->fval.as<double>(), b->fdinv.as<double>());
+    if (rc) return rc;
+    rc = batch::launch_ell_gather(ctx, n, K, b->bs, b->brows.as<int>(), M->upper.drp, M->upper.dci,
+                                  static_cast<const double*>(M->upper.dval), M->dinv_upper.as<double>(),
+                                  b->bcol.as<int>(), b->bval.as<double>(), b->bdinv.as<double>());
+    if (rc) return rc;
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    b->ell_k = K;
+    return BSP_OK;
+}
+
 // Work items (system, first RHS) for R right-hand sides per item, largest system first.
 int build_items(bsp_ctx* ctx, bsp_batch* b, int R) {
     if (b->items_R == R) return BSP_OK;
@@ -125,7 +167,15 @@ batch::Args make_args(bsp_batch* b, double tol, int maxIters, int mode) {
     a.brows = b->brows.as<int>();
     a.blev = b->blev.as<int>();
     a.blev_base = b->blev_base.as<int>();
+    a.ell_k = b->ell_k;
+    a.fcol = b->fcol.as<int>();
+    a.fval = b->fval.as<double>();
+    a.fdinv = b->fdinv.as<double>();
+    a.bcol = b->bcol.as<int>();
+    a.bval = b->bval.as<double>();
+    a.bdinv = b->bdinv.as<double>();
     a.nrhs = b->nrhs;
+    a.len = b->nrows * b->bs;
     a.B = b->Bi.as<double>();
     a.X = b->Xi.as<double>();
     a.Rv = b->Rv.as<double>();
@@ -143,7 +193,7 @@ batch::Args make_args(bsp_batch* b, double tol, int maxIters, int mode) {
 
 // Persistent grid: every resident workgroup slot, at most one per work item.
 int grid_size(const bsp_ctx* ctx, const bsp_batch* b) {
-    return std::max(1, std::min(b->nitems, batch::max_resident_blocks(ctx, b->bs, b->R, b->wg)));
+    return std::max(1, std::min(b->nitems, batch::max_resident_blocks(ctx, b->bs, b->R, b->ell_k, b->wg)));
 }
 
 // The whole solve as stream work: B -> interleaved, counter = 0, persistent pCG, X back.
@@ -151,11 +201,11 @@ int grid_size(const bsp_ctx* ctx, const bsp_batch* b) {
 int enqueue(bsp_ctx* ctx, bsp_batch* b, int grid, const double* B, double* X, double tol, int maxIters, int mode) {
     int rc;
     const int64_t len = b->nrows * b->bs;
-    if ((rc = batch::launch_interleave(ctx, len, b->nrhs, B, b->Bi.as<double>(), true))) return rc;
+    if ((rc = batch::launch_interleave(ctx, len, b->nrhs, b->R, B, b->Bi.as<double>(), true))) return rc;
     if ((rc = launch_fill_u32(ctx, 1, 0u, b->work.as<uint32_t>()))) return rc;
     batch::Args a = make_args(b, tol, maxIters, mode);
     if ((rc = batch::launch_pcg(ctx, b->bs, b->R, b->wg, grid, a))) return rc;
-    return batch::launch_interleave(ctx, len, b->nrhs, b->Xi.as<double>(), X, false);
+    return batch::launch_interleave(ctx, len, b->nrhs, b->R, b->Xi.as<double>(), X, false);
 }
 
 double time_solve(bsp_ctx* ctx, bsp_batch* b, const double* B, double* X, int iters, int& rc) {
@@ -189,6 +239,9 @@ int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t*
     BSP_CHECK(sys_rowptr[0] == 0 && sys_rowptr[nsys] == A->n_rows, BSP_E_INVALID,
               "system row pointers must cover the matrix");
     BSP_CHECK(batch::supported(A->bs, 1), BSP_E_UNSUPPORTED, "batched pCG supports block sizes 1, 2, 4 and 8");
+    // vectors are addressed with 32-bit offsets inside a group of up to 4 right-hand sides
+    BSP_CHECK((int64_t)A->n_rows * A->bs * 4 < (int64_t)INT32_MAX, BSP_E_UNSUPPORTED,
+              "batched pCG: rows * block size must stay below 2^29");
     BSP_HIP(hipSetDevice(ctx->device));
     auto b = std::make_unique<bsp_batch>();
     b->ctx = ctx;
@@ -220,6 +273,7 @@ int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t*
     BSP_HIP(upload_i(b->blev, lev));
     BSP_HIP(upload_i(b->blev_base, base));
     BSP_HIP(upload_i(b->d_sys_row0, b->sys_row0));
+    if (ctx->batch_ell && (rc = build_ell(ctx, b.get()))) return rc;
     const size_t vec = sizeof(double) * std::max<size_t>((size_t)b->nrows * b->bs * nrhs, 1);
     BSP_HIP(b->Bi.alloc(vec));
     BSP_HIP(b->Xi.alloc(vec));
@@ -375,5 +429,7 @@ int bsp_batch_info(const bsp_batch* b, int* wg_size, int* rhs_per_item, int* nit
     if (nitems) *nitems = b->nsys * (b->nrhs / b->R);
     return BSP_OK;
 }
+
+int bsp_batch_ell_width(const bsp_batch* b) { return b ? b->ell_k : BSP_E_INVALID; }
 
 }  // extern "C"

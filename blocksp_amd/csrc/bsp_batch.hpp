@@ -6,7 +6,8 @@
 namespace bsp {
 namespace batch {
 
-// Kernel arguments (all device pointers; vectors interleaved by right-hand side).
+// Kernel arguments (all device pointers; vectors stored per group of R right-hand sides,
+// interleaved inside the group).
 struct Args {
     int nitems = 0;
     const int* item_sys = nullptr;    // [nitems] system of each work item (largest first)
@@ -18,7 +19,13 @@ struct Args {
     const double *lval = nullptr, *uval = nullptr, *dinvL = nullptr, *dinvU = nullptr;   // IChol0 factor
     const int *frows = nullptr, *flev = nullptr, *flev_base = nullptr;   // forward level sets per system
     const int *brows = nullptr, *blev = nullptr, *blev_base = nullptr;   // backward level sets per system
+    // ELL copies of the two sweeps, slot-ordered like frows / brows (ell_k == 0: none, CSR sweeps)
+    int ell_k = 0;                    // entries per slot
+    const int *fcol = nullptr, *bcol = nullptr;           // [slots * ell_k] block column, -1 padding
+    const double *fval = nullptr, *bval = nullptr;        // [slots * ell_k][bs * bs] blocks
+    const double *fdinv = nullptr, *bdinv = nullptr;      // [slots][bs * bs] inverse diagonal blocks
     int nrhs = 1;
+    int64_t len = 0;                  // rows * bs (length of one right-hand side)
     const double* B = nullptr;
     double *X = nullptr, *Rv = nullptr, *P = nullptr, *Z = nullptr, *AP = nullptr;
     double tol = 0.0;
@@ -30,9 +37,16 @@ struct Args {
 };
 
 bool supported(int bs, int R);
-int max_resident_blocks(const bsp_ctx* ctx, int bs, int R, int wg);
+// ELL width for sweeps whose rows hold at most `max_entries` strict blocks (0: keep CSR)
+int ell_width(int max_entries);
+int max_resident_blocks(const bsp_ctx* ctx, int bs, int R, int K, int wg);
 int launch_pcg(const bsp_ctx* ctx, int bs, int R, int wg, int grid, const Args& a);
-int launch_interleave(const bsp_ctx* ctx, int64_t len, int nrhs, const double* src, double* dst, bool to_interleaved);
+// ELL copy of a sweep (slot q = position in `rows`): columns, blocks and inverse diagonal blocks
+int launch_ell_gather(const bsp_ctx* ctx, int64_t nslots, int K, int bs, const int* rows, const int* rp, const int* ci,
+                      const double* val, const double* dinv, int* ecol, double* eval, double* edinv);
+// RHS-major (nrhs vectors of len) <-> groups of R right-hand sides interleaved inside the group
+int launch_interleave(const bsp_ctx* ctx, int64_t len, int nrhs, int R, const double* src, double* dst,
+                      bool to_interleaved);
 
 }  // namespace batch
 }  // namespace bsp

@@ -106,6 +106,8 @@ struct bsp_ctx {
     int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
+    int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
+                                     // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +
                                      // XCD-aware row ranges + streaming value loads (117.6 us, 6.04 TB/s, the
                                      // default for even bs <= 8); 4 chunk-major + XCD (126.3 us); 3 chunk-major

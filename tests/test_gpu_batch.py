@@ -204,3 +204,104 @@ def test_device_pcg_with_coloured_ichol0_matches_oracle_on_permuted_system(ctx):
     assert rc == 0 and abs(st.iters - sto.iters) <= 1, (st.iters, sto.iters)
     xq = x.numpy().reshape(p.n, p.bs)[perm].reshape(-1)
     assert np.linalg.norm(xq - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def _grid_problem(dims, offsets, bs, rng, nrhs=4):
+    """SPD system on a grid: block (i, j) = -c I for every neighbour j = i + offset inside the
+    grid (offsets closed under negation), diagonal block c ((deg + 1) I + S S^T / bs), which
+    is diagonally dominant.  Rows keep their blocks in increasing column order."""
+    dims = tuple(dims)
+    n = int(np.prod(dims))
+    coords = np.stack(np.unravel_index(np.arange(n), dims), axis=1)
+    c = float(np.exp(0.5 * rng.standard_normal()))
+    S = rng.standard_normal((bs, bs))
+    rowptr, colidx, vals = [0], [], []
+    for i in range(n):
+        nb = []
+        for off in offsets:
+            q = coords[i] + np.asarray(off)
+            if (q >= 0).all() and (q < np.asarray(dims)).all():
+                nb.append(int(np.ravel_multi_index(tuple(q), dims)))
+        for j in sorted(nb + [i]):
+            colidx.append(j)
+            vals.append(c * ((len(nb) + 1) * np.eye(bs) + S @ S.T / bs) if j == i else -c * np.eye(bs))
+        rowptr.append(len(colidx))
+    rhs = rng.standard_normal((nrhs, n * bs))
+    return pr.Problem(n, bs, bs, 1, np.array(rowptr, np.int32), np.array(colidx, np.int32), np.array(vals),
+                      rhs[0].copy(), name=f"grid{dims}", meta={"rhs": rhs})
+
+
+_OFFS = {
+    "chain": [(-1,), (1,)],                                                     # 1 strict block per row
+    "7pt": [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)],   # 3
+    "9pt": [(a, b) for a in (-1, 0, 1) for b in (-1, 0, 1) if (a, b) != (0, 0)],     # 4
+    "27pt": [(a, b, d) for a in (-1, 0, 1) for b in (-1, 0, 1) for d in (-1, 0, 1) if (a, b, d) != (0, 0, 0)],  # 13
+}
+_DIMS = {
+    "chain": [(57,), (300,), (131,)],
+    "7pt": [(5, 6, 7), (8, 8, 8), (3, 9, 4)],
+    "9pt": [(10, 12), (20, 15), (7, 3)],
+    "27pt": [(5, 5, 5), (6, 7, 4), (3, 3, 9)],
+}
+
+
+@pytest.mark.parametrize("pattern,bs,width", [("chain", 1, 2), ("7pt", 2, 3), ("9pt", 8, 4), ("27pt", 4, 0),
+                                              ("9pt", 4, 4), ("7pt", 8, 3)])
+def test_ell_sweeps_equal_csr_sweeps_bitwise(ctx, pattern, bs, width):
+    """The ELL copy of the sweeps (rows with at most 4 strict blocks) runs the CSR sweeps'
+    operations in the same order: identical iterates for every tile; wider rows keep CSR."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    probs = [_grid_problem(d, _OFFS[pattern], bs, rng) for d in _DIMS[pattern]]
+    rp, ci, vals, sys = bsp.block_diagonal(probs)
+    n = int(sys[-1])
+    A = bsp.BSR(ctx, n, n, bs, rp, ci, vals)
+    Bh = np.stack([np.concatenate([p.meta["rhs"][k] for p in probs]) for k in range(4)])
+    B = ctx.to_device(Bh)
+    try:
+        ctx.set_option("batch_ell", 0)
+        csr = bsp.Batch(ctx, A, sys, 4)
+    finally:
+        ctx.set_option("batch_ell", 1)
+    ell = bsp.Batch(ctx, A, sys, 4)
+    assert csr.ell_width() == 0 and ell.ell_width() == width
+    for wg, r in ((256, 1), (1024, 2), (128, 4)):
+        out = []
+        for bat in (csr, ell):
+            bat.set_tiles(wg, r)
+            X = ctx.zeros(Bh.shape)
+            its, rel, _, rc = bat.pcg(B, X, 1e-10, maxIters=3000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+            assert rc == 0 and (rel <= 1e-10).all()
+            out.append((its, X.numpy()))
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+    Xh = out[1][1]
+    for s, p in enumerate(probs):
+        for k in range(4):
+            b = p.meta["rhs"][k]
+            x = Xh[k, sys[s] * bs:sys[s + 1] * bs]
+            assert np.linalg.norm(b - p.bsmv(x)) <= 2e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("pattern,bs", [("9pt", 2), ("7pt", 1)])
+def test_ell_batch_matches_oracle_pcg(ctx, pattern, bs):
+    """ELL sweeps with 4 and 3 strict blocks per row against the oracle's pCG with IChol0
+    (reference stopping rule)."""
+    rng = np.random.Generator(np.random.PCG64(37))
+    probs = [_grid_problem(d, _OFFS[pattern], bs, rng) for d in _DIMS[pattern]]
+    rp, ci, vals, sys = bsp.block_diagonal(probs)
+    n = int(sys[-1])
+    A = bsp.BSR(ctx, n, n, bs, rp, ci, vals)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    assert bat.ell_width() > 0
+    Bh = np.stack([np.concatenate([p.meta["rhs"][k] for p in probs]) for k in range(4)])
+    X = ctx.zeros(Bh.shape)
+    its, _, _, rc = bat.pcg(ctx.to_device(Bh), X, 1e-10, maxIters=3000, tol_mode=bsp.TOL_REFERENCE)
+    assert rc == 0
+    Xh = X.numpy()
+    for s, p in enumerate(probs):
+        for k in range(4):
+            b = p.meta["rhs"][k]
+            xo, st = po.pcg(p, b, 1e-10, po.PRE_ICHOL0, maxIters=3000)
+            x = Xh[k, sys[s] * bs:sys[s + 1] * bs]
+            assert abs(int(its[s, k]) - st.iters) <= 1, (s, k, its[s, k], st.iters)
+            assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
