@@ -186,7 +186,7 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
     setup = time.perf_counter() - t0
     B = ctx.to_device(Bh)
     X = ctx.zeros(Bh.shape)
-    wg, r = bat.autotune(B, X, probe_iters=8)
+    wg, r = bat.autotune_solve(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL, probe_iters=8)
     bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     Xh = X.numpy()
@@ -252,7 +252,7 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
     batc = bsp.Batch(ctx, A, sysp, 4, colour=col)
     setup_c = time.perf_counter() - t0
     Xc = ctx.zeros(Bh.shape)
-    wgc, rcc = batc.autotune(B, Xc, probe_iters=8)
+    wgc, rcc = batc.autotune_solve(B, Xc, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL, probe_iters=8)
     batc.pcg(B, Xc, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     itsc, relc, msc, rcode = batc.pcg(B, Xc, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     out["parity_coloured"] = {"batched_s": round(msc / 1e3, 4), "tiles": {"wg": wgc, "rhs_per_item": rcc},

@@ -125,6 +125,8 @@ _SIGS = {
     "bsp_batch_destroy": [_VP],
     "bsp_batch_set_tiles": [_VP, _VP, C.c_int, C.c_int],
     "bsp_batch_autotune": [_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "bsp_batch_autotune_solve": [_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)],
     "bsp_batch_pcg": [_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, _VP, _VP, C.POINTER(C.c_double)],
     "bsp_batch_info": [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsp_batch_ell_width": [_VP],
@@ -697,6 +699,15 @@ class Batch:
         wg, r = C.c_int(), C.c_int()
         self.ctx.check(self.ctx.L.bsp_batch_autotune(self.ctx.h, self.h, B.ptr, X.ptr, int(probe_iters), C.byref(wg),
                                                      C.byref(r)), "autotune")
+        return wg.value, r.value
+
+    def autotune_solve(self, B: DeviceArray, X: DeviceArray, tol, maxIters=5000, tol_mode=TOL_REFERENCE,
+                       probe_iters=8):
+        """Short probes, then complete solves of the finalists; returns the kept (wg, rhs_per_item)."""
+        wg, r = C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.L.bsp_batch_autotune_solve(self.ctx.h, self.h, B.ptr, X.ptr, float(tol), int(maxIters),
+                                                           int(tol_mode), int(probe_iters), C.byref(wg), C.byref(r)),
+                       "autotune_solve")
         return wg.value, r.value
 
     def info(self):

@@ -208,18 +208,63 @@ int enqueue(bsp_ctx* ctx, bsp_batch* b, int grid, const double* B, double* X, do
     return batch::launch_interleave(ctx, len, b->nrhs, b->R, b->Xi.as<double>(), X, false);
 }
 
-double time_solve(bsp_ctx* ctx, bsp_batch* b, const double* B, double* X, int iters, int& rc) {
+// Device time of one solve with the current tiles: `iters` iterations exactly (tol 0), or to
+// the stopping rule when tol > 0.
+double time_solve(bsp_ctx* ctx, bsp_batch* b, const double* B, double* X, int iters, int& rc, double tol = 0.0,
+                  int mode = 1) {
     rc = build_items(ctx, b, b->R);
     if (rc) return 0.0;
     const int grid = grid_size(ctx, b);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventRecord(ctx->events[10], ctx->stream);
-    rc = enqueue(ctx, b, grid, B, X, 0.0, iters, 1);
+    rc = enqueue(ctx, b, grid, B, X, tol, iters, mode);
     (void)hipEventRecord(ctx->events[11], ctx->stream);
     (void)hipEventSynchronize(ctx->events[11]);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->events[10], ctx->events[11]);
     return ms;
+}
+
+// Short probes: every workgroup size x RHS-per-item for probe_iters iterations.  Candidates
+// whose per-(system, RHS) residual norms and ||X|| do not reproduce the first candidate's
+// (up to rounding) are rejected.  best[R] = (fastest accepted workgroup size, its time).
+int probe_tiles(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, int probe_iters,
+                std::map<int, std::pair<int, double>>& best) {
+    const size_t cnt = (size_t)b->nsys * b->nrhs;
+    const int64_t xlen = b->nrows * b->bs * b->nrhs;
+    std::vector<double> ref_res, res(2 * cnt);
+    double ref_x = -1.0;
+    for (int R : {1, 2, 4}) {
+        if (b->nrhs % R || !batch::supported(b->bs, R)) continue;
+        for (int wg : {128, 256, 512, 1024}) {
+            b->wg = wg;
+            b->R = R;
+            int rc = BSP_OK;
+            time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);   // warm-up (code load, caches)
+            if (rc) return rc;
+            const double ms = time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);
+            if (rc) return rc;
+            BSP_HIP(hipMemcpy(res.data(), b->res.p, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost));
+            double xn = 0.0;
+            if ((rc = reduce(ctx, 1, xlen, X_dev, X_dev, SLOT_TMP0, &xn))) return rc;
+            bool same = true;
+            if (ref_x < 0.0) {
+                ref_res = res;
+                ref_x = xn;
+            } else {
+                same = std::fabs(xn - ref_x) <= 1e-9 * std::max(ref_x, 1e-300);
+                for (size_t t = 0; t < cnt && same; ++t)
+                    same = std::fabs(res[2 * t] - ref_res[2 * t]) <= 1e-6 * std::max(ref_res[2 * t], 1e-300);
+            }
+            if (ctx->verbose > 0) {
+                std::fprintf(stderr, "[batch autotune] wg %d rhs/item %d: %.3f ms for %d iterations%s\n", wg, R, ms,
+                             probe_iters, same ? "" : " (rejected: results differ)");
+                std::fflush(stderr);
+            }
+            if (same && (!best.count(R) || ms < best[R].second)) best[R] = {wg, ms};
+        }
+    }
+    return BSP_OK;
 }
 
 }  // namespace
@@ -314,53 +359,55 @@ int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* 
     auto& cache = tune_cache();
     auto it = cache.find(key);
     if (it == cache.end()) {
-        probe_iters = std::max(1, probe_iters);
-        double best = 1e300;
         std::pair<int, int> choice{b->wg, b->R};
-        // parity gate: after probe_iters iterations every candidate must reproduce the first
-        // candidate's per-(system, RHS) residual norms and ||X|| (reduction orders differ
-        // between tiles, so equal up to rounding)
-        const size_t cnt = (size_t)b->nsys * b->nrhs;
-        const int64_t xlen = b->nrows * b->bs * b->nrhs;
-        std::vector<double> ref_res, res(2 * cnt);
-        double ref_x = -1.0;
-        for (int R : {1, 2, 4}) {
-            if (b->nrhs % R || !batch::supported(b->bs, R)) continue;
-            for (int wg : {128, 256, 512, 1024}) {
-                b->wg = wg;
-                b->R = R;
-                int rc = BSP_OK;
-                time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);   // warm-up (code load, caches)
-                if (rc) return rc;
-                const double ms = time_solve(ctx, b, B_dev, X_dev, probe_iters, rc);
-                if (rc) return rc;
-                BSP_HIP(hipMemcpy(res.data(), b->res.p, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost));
-                double xn = 0.0;
-                if ((rc = reduce(ctx, 1, xlen, X_dev, X_dev, SLOT_TMP0, &xn))) return rc;
-                bool same = true;
-                if (ref_x < 0.0) {
-                    ref_res = res;
-                    ref_x = xn;
-                } else {
-                    same = std::fabs(xn - ref_x) <= 1e-9 * std::max(ref_x, 1e-300);
-                    for (size_t t = 0; t < cnt && same; ++t)
-                        same = std::fabs(res[2 * t] - ref_res[2 * t]) <= 1e-6 * std::max(ref_res[2 * t], 1e-300);
-                }
-                if (ctx->verbose > 0) {
-                    std::fprintf(stderr, "[batch autotune] wg %d rhs/item %d: %.3f ms for %d iterations%s\n", wg, R, ms,
-                                 probe_iters, same ? "" : " (rejected: results differ)");
-                    std::fflush(stderr);
-                }
-                if (same && ms < best) {
-                    best = ms;
-                    choice = {wg, R};
-                }
+        std::map<int, std::pair<int, double>> best;
+        int rc = probe_tiles(ctx, b, B_dev, X_dev, std::max(1, probe_iters), best);
+        if (rc) return rc;
+        double t = 1e300;
+        for (auto& [R, wt] : best)
+            if (wt.second < t) {
+                t = wt.second;
+                choice = {wt.first, R};
             }
-        }
         it = cache.emplace(key, choice).first;
     }
     b->wg = it->second.first;
     b->R = it->second.second;
+    b->tuned = true;
+    if (wg_size) *wg_size = b->wg;
+    if (rhs_per_item) *rhs_per_item = b->R;
+    return BSP_OK;
+}
+
+int bsp_batch_autotune_solve(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, double tol, int maxIters,
+                             int tol_mode, int probe_iters, int* wg_size, int* rhs_per_item) {
+    if (!ctx || !b || !B_dev || !X_dev || !(tol > 0.0) || maxIters <= 0) return BSP_E_INVALID;
+    BSP_CHECK(tol_mode == BSP_TOL_REFERENCE || tol_mode == BSP_TOL_TRUE_RESIDUAL, BSP_E_INVALID, "bad tol_mode");
+    BSP_HIP(hipSetDevice(ctx->device));
+    std::pair<int, int> choice{b->wg, b->R};
+    std::map<int, std::pair<int, double>> best;
+    int rc = probe_tiles(ctx, b, B_dev, X_dev, std::max(1, probe_iters), best);
+    if (rc) return rc;
+    // the finalists (best workgroup size per RHS-per-item) on complete solves: the largest
+    // systems' items set the solve time, which short probes do not show
+    const int mode = tol_mode == BSP_TOL_REFERENCE ? 0 : 1;
+    double t = 1e300;
+    for (auto& [R, wt] : best) {
+        b->wg = wt.first;
+        b->R = R;
+        const double ms = time_solve(ctx, b, B_dev, X_dev, maxIters, rc, tol, mode);
+        if (rc) return rc;
+        if (ctx->verbose > 0) {
+            std::fprintf(stderr, "[batch autotune] complete solve, wg %d rhs/item %d: %.3f ms\n", b->wg, R, ms);
+            std::fflush(stderr);
+        }
+        if (ms < t) {
+            t = ms;
+            choice = {wt.first, R};
+        }
+    }
+    b->wg = choice.first;
+    b->R = choice.second;
     b->tuned = true;
     if (wg_size) *wg_size = b->wg;
     if (rhs_per_item) *rhs_per_item = b->R;

@@ -356,6 +356,13 @@ int bsp_batch_set_tiles(bsp_ctx* ctx, bsp_batch* b, int wg_size, int rhs_per_ite
  * choice per (device, block size, nrhs, nsys, rows) for later batches. */
 int bsp_batch_autotune(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, int probe_iters,
                        int* wg_size, int* rhs_per_item);
+/* As bsp_batch_autotune's probes (not cached), then one complete solve (tol, maxIters,
+ * tol_mode as for bsp_batch_pcg; X_dev is overwritten) with each RHS-per-item's fastest
+ * workgroup size; keeps the fastest.  Short probes favour throughput, but a batch whose
+ * largest systems run far longer than the rest finishes with those items, which complete
+ * solves show. */
+int bsp_batch_autotune_solve(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev, double tol, int maxIters,
+                             int tol_mode, int probe_iters, int* wg_size, int* rhs_per_item);
 /* Solves A_s X_{s,k} = B_{s,k} for every system s and right-hand side k.  B_dev and X_dev
  * hold nrhs vectors of length n*bs one after another (vector k: every system's rows in
  * order).  Stopping rule per (s, k): tol_mode 0 ||r|| <= tol ||M^{-1} b|| (the reference's),

@@ -305,3 +305,19 @@ def test_ell_batch_matches_oracle_pcg(ctx, pattern, bs):
             x = Xh[k, sys[s] * bs:sys[s + 1] * bs]
             assert abs(int(its[s, k]) - st.iters) <= 1, (s, k, its[s, k], st.iters)
             assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_autotune_solve_keeps_a_finalist_and_solves(ctx):
+    """Complete-solve tuning keeps one of the probed tiles; the batch then solves with it."""
+    probs, A, sys, Bh = _batch(ctx, nsys=10, seed=17)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    wg, r = bat.autotune_solve(B, X, 1e-10, maxIters=2000, tol_mode=bsp.TOL_TRUE_RESIDUAL, probe_iters=4)
+    assert wg in (128, 256, 512, 1024) and r in (1, 2, 4)
+    assert bat.info()[:2] == (wg, r)
+    X.zero()
+    its, rel, _, rc = bat.pcg(B, X, 1e-10, maxIters=2000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    assert rc == 0 and (rel <= 1e-10).all()
+    with pytest.raises(bsp.BspError):
+        bat.autotune_solve(B, X, 0.0)
