@@ -83,7 +83,8 @@ hipError_t upload_i(DevMem& m, const std::vector<int32_t>& v) {
 }
 
 // Slot-ordered ELL copies of both sweeps (entries in CSR order, -1 / zero padding) when every
-// row holds at most 4 strict blocks; the factor's host copy gives the row lengths.
+// row holds at most 4 strict blocks of at most 4 x 4; the factor's host copy gives the row
+// lengths.
 int build_ell(bsp_ctx* ctx, bsp_batch* b) {
     const bsp_precond* M = b->M;
     const int n = M->n;
@@ -100,7 +101,7 @@ int build_ell(bsp_ctx* ctx, bsp_batch* b) {
         klo = std::max(klo, cnt);
     }
     const int kup = *std::max_element(ucnt.begin(), ucnt.end());
-    const int K = batch::ell_width(std::max(klo, kup));
+    const int K = batch::ell_width(b->bs, std::max(klo, kup));
     if (K == 0) return BSP_OK;
     const size_t b2 = (size_t)b->bs * b->bs;
     BSP_HIP(b->fcol.alloc(sizeof(int) * (size_t)n * K));

@@ -16,7 +16,7 @@
 // workgroup size).
 //
 // One workgroup streams a whole system's vectors, so a vector pass is bound by the loads a
-// thread has in flight: each thread takes U slots per round (8 / R, or 1 for R = 4), issues
+// thread has in flight: each thread takes U slots per round (4 / R, or 1 for R = 4), issues
 // all their loads, then computes them in slot order (the order of the one-slot loop, so
 // the sums are the same).
 //
@@ -24,9 +24,10 @@
 // the factor in CSR the chain is level pointer -> row -> row pointer -> column -> vector.
 // The ELL form (K > 0) stores each sweep slot-ordered (slot = position in the level-sorted
 // row list): K columns per slot (-1 padding), their K blocks and the slot's diagonal-block
-// inverse, so the slot's structure and values load at once and only the vector entries
-// wait on them.  Entries keep the CSR order and padding is skipped, so both forms run the
-// same floating-point operations in the same order.
+// inverse, so the slot's structure and values load at once -- a level ahead, while the
+// previous level computes -- and only the vector entries wait on earlier levels.  Entries
+// keep the CSR order and padding is skipped, so both forms run the same floating-point
+// operations in the same order.
 #include "bsp_internal.hpp"
 #include "bsp_batch.hpp"
 
@@ -36,6 +37,7 @@ namespace batch {
 namespace {
 
 constexpr int kMaxWaves = 16;
+constexpr int kEllMaxBs = 4;   // larger blocks: a slot a level ahead does not fit the registers
 
 template <int N>
 __device__ __forceinline__ void block_sum(double (&v)[N], double* sh) {
@@ -133,30 +135,35 @@ __device__ void sweep(const int* __restrict__ rows, const int* __restrict__ lev,
 // val[(q*K + k) * BS*BS] and the inverse diagonal block dinv[q * BS*BS]; rows[q] is its row.
 template <int BS, int R, int K>
 struct EllSlot {
-    int row;
+    int q, row;
     int cq[K];
-    double av[K][BS], dv[BS], t[R];
+    double av[K][BS];
 };
 
+// The slot's structure and its row of the K blocks (what a level ahead can load).
 template <int BS, int R, int K>
 __device__ __forceinline__ void ell_load(const int* __restrict__ rows, const int* __restrict__ col,
-                                         const double* __restrict__ val, const double* __restrict__ dinv,
-                                         const double* in, int q, int c, EllSlot<BS, R, K>& e) {
+                                         const double* __restrict__ val, int q, int c, EllSlot<BS, R, K>& e) {
     constexpr int B2 = BS * BS;
+    e.q = q;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         e.cq[k] = col[(size_t)q * K + k];
 #pragma unroll
         for (int j = 0; j < BS; ++j) e.av[k][j] = val[((size_t)q * K + k) * B2 + c * BS + j];
     }
-#pragma unroll
-    for (int j = 0; j < BS; ++j) e.dv[j] = dinv[(size_t)q * B2 + c * BS + j];
     e.row = rows[q];
-    ld<R>(in + (e.row * BS + c) * R, e.t);
 }
 
+// Right-hand side, inverse diagonal block and the gathered vector entries, then the row.
 template <int BS, int R, int K>
-__device__ __forceinline__ void ell_finish(double* out, int c, int gbase, EllSlot<BS, R, K>& e) {
+__device__ __forceinline__ void ell_finish(const double* __restrict__ dinv, const double* in, double* out, int c,
+                                           int gbase, const EllSlot<BS, R, K>& e) {
+    constexpr int B2 = BS * BS;
+    double t[R], dv[BS];
+    ld<R>(in + (e.row * BS + c) * R, t);
+#pragma unroll
+    for (int j = 0; j < BS; ++j) dv[j] = dinv[(size_t)e.q * B2 + c * BS + j];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (e.cq[k] < 0) continue;
@@ -166,7 +173,7 @@ __device__ __forceinline__ void ell_finish(double* out, int c, int gbase, EllSlo
             double xv[R];
             ld<R>(xc + j * R, xv);
 #pragma unroll
-            for (int kk = 0; kk < R; ++kk) e.t[kk] = fma(-e.av[k][j], xv[kk], e.t[kk]);
+            for (int kk = 0; kk < R; ++kk) t[kk] = fma(-e.av[k][j], xv[kk], t[kk]);
         }
     }
     double y[R];
@@ -175,7 +182,7 @@ __device__ __forceinline__ void ell_finish(double* out, int c, int gbase, EllSlo
 #pragma unroll
     for (int j = 0; j < BS; ++j)
 #pragma unroll
-        for (int k = 0; k < R; ++k) y[k] = fma(e.dv[j], __shfl(e.t[k], gbase + j), y[k]);
+        for (int k = 0; k < R; ++k) y[k] = fma(dv[j], __shfl(t[k], gbase + j), y[k]);
     st<R>(out + (e.row * BS + c) * R, y);
 }
 
@@ -185,18 +192,42 @@ __device__ void sweep_ell(const int* __restrict__ rows, const int* __restrict__ 
                           const double* __restrict__ dinv, const double* in, double* out) {
     const int T = blockDim.x, tid = threadIdx.x, lane = tid & 63;
     const int gbase = lane & ~(BS - 1);
+    const int c = tid % BS;   // T is a multiple of BS: every slot of this thread has component c
     if (lb >= le) return;
-    // level bounds are read one level ahead, off the chain of the level in hand
+    if constexpr (K * BS > 8) {
+        // wide slots: loading one a level ahead would spill registers
+        for (int l = lb; l < le; ++l) {
+            const int a0 = lev[l], cnt = (lev[l + 1] - a0) * BS;
+            for (int i0 = tid; i0 < cnt; i0 += T) {
+                EllSlot<BS, R, K> e;
+                ell_load<BS, R, K>(rows, col, val, a0 + i0 / BS, c, e);
+                ell_finish<BS, R, K>(dinv, in, out, c, gbase, e);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    // The structure and blocks of the thread's first slot of the next level are loaded while
+    // this level computes (the sweep writes none of them), so a level's chain is the loads
+    // of its vector entries, right-hand sides and inverse diagonal blocks.
     int a0 = lev[lb], a1 = lev[lb + 1];
+    EllSlot<BS, R, K> nxt;
+    bool have_n = tid < (a1 - a0) * BS;
+    if (have_n) ell_load<BS, R, K>(rows, col, val, a0 + tid / BS, c, nxt);
     for (int l = lb; l < le; ++l) {
         const int a2 = l + 2 <= le ? lev[l + 2] : a1;
         const int cnt = (a1 - a0) * BS;
-        int i0 = tid;
-        for (; i0 < cnt; i0 += T) {
+        EllSlot<BS, R, K> cur = nxt;
+        const bool have_c = have_n;
+        // slots beyond the first of a level wider than the workgroup: loaded in place
+        for (int i0 = tid + T; i0 < cnt; i0 += T) {
             EllSlot<BS, R, K> e;
-            ell_load<BS, R, K>(rows, col, val, dinv, in, a0 + i0 / BS, i0 % BS, e);
-            ell_finish<BS, R, K>(out, i0 % BS, gbase, e);
+            ell_load<BS, R, K>(rows, col, val, a0 + i0 / BS, c, e);
+            ell_finish<BS, R, K>(dinv, in, out, c, gbase, e);
         }
+        have_n = l + 1 < le && tid < (a2 - a1) * BS;
+        if (have_n) ell_load<BS, R, K>(rows, col, val, a1 + tid / BS, c, nxt);
+        if (have_c) ell_finish<BS, R, K>(dinv, in, out, c, gbase, cur);
         __syncthreads();
         a0 = a1;
         a1 = a2;
@@ -258,8 +289,8 @@ __global__ __launch_bounds__(1024) void k_batch_pcg(Args a) {
     __shared__ double red[kMaxWaves * 2 * R];
     __shared__ int item_sh;
     constexpr int B2 = BS * BS;
-    constexpr int U = R == 4 ? 1 : 8 / R;   // slots per thread per round of a vector pass (R = 4:
-                                            // one slot already has 16 values in flight)
+    constexpr int U = R == 4 ? 1 : 4 / R;   // slots per thread per round of a vector pass (more
+                                            // spills registers once the sweeps load a level ahead)
     const int T = blockDim.x, tid = threadIdx.x;
     for (;;) {
         if (tid == 0) item_sh = (int)atomicAdd(a.work, 1u);
@@ -486,13 +517,17 @@ __global__ void k_ell_gather(int64_t nslots, int K, int b2, const int* __restric
 
 template <int BS, int R>
 const void* pick_k(int K) {
-    switch (K) {
-        case 0: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 0>);
-        case 2: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 2>);
-        case 3: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 3>);
-        case 4: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 4>);
+    if constexpr (BS > kEllMaxBs) {
+        return K == 0 ? reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 0>) : nullptr;
+    } else {
+        switch (K) {
+            case 0: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 0>);
+            case 2: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 2>);
+            case 3: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 3>);
+            case 4: return reinterpret_cast<const void*>(&k_batch_pcg<BS, R, 4>);
+        }
+        return nullptr;
     }
-    return nullptr;
 }
 
 template <int BS>
@@ -519,7 +554,10 @@ const void* pick(int bs, int R, int K) {
 
 bool supported(int bs, int R) { return pick(bs, R, 0) != nullptr; }
 
-int ell_width(int max_entries) { return max_entries <= 2 ? 2 : max_entries <= 4 ? max_entries : 0; }
+int ell_width(int bs, int max_entries) {
+    if (bs > kEllMaxBs) return 0;
+    return max_entries <= 2 ? 2 : max_entries <= 4 ? max_entries : 0;
+}
 
 int max_resident_blocks(const bsp_ctx* ctx, int bs, int R, int K, int wg) {
     const void* k = pick(bs, R, K);

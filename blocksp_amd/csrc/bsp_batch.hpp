@@ -37,8 +37,9 @@ struct Args {
 };
 
 bool supported(int bs, int R);
-// ELL width for sweeps whose rows hold at most `max_entries` strict blocks (0: keep CSR)
-int ell_width(int max_entries);
+// ELL width for sweeps of bs x bs blocks whose rows hold at most `max_entries` strict blocks
+// (0: keep CSR)
+int ell_width(int bs, int max_entries);
 int max_resident_blocks(const bsp_ctx* ctx, int bs, int R, int K, int wg);
 int launch_pcg(const bsp_ctx* ctx, int bs, int R, int wg, int grid, const Args& a);
 // ELL copy of a sweep (slot q = position in `rows`): columns, blocks and inverse diagonal blocks

@@ -374,8 +374,8 @@ int bsp_batch_pcg(bsp_ctx* ctx, bsp_batch* b, const double* B_dev, double* X_dev
                   int tol_mode, int32_t* iters_host, double* relres_host, double* solve_ms);
 int bsp_batch_info(const bsp_batch* b, int* wg_size, int* rhs_per_item, int* nitems);
 /* Entries per slot of the slot-ordered ELL copies of the two factor sweeps (2, 3 or 4), or 0
- * when the sweeps read the CSR factor (rows with more than 4 strict blocks, or the context
- * option "batch_ell" = 0 when the batch was created).  Both forms give bitwise the same
+ * when the sweeps read the CSR factor (rows with more than 4 strict blocks, blocks larger
+ * than 4 x 4, or the context option "batch_ell" = 0 when the batch was created).  Both forms give bitwise the same
  * results; the ELL form shortens each level's chain of dependent loads. */
 int bsp_batch_ell_width(const bsp_batch* b);
 

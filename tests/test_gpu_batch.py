@@ -245,11 +245,12 @@ _DIMS = {
 }
 
 
-@pytest.mark.parametrize("pattern,bs,width", [("chain", 1, 2), ("7pt", 2, 3), ("9pt", 8, 4), ("27pt", 4, 0),
-                                              ("9pt", 4, 4), ("7pt", 8, 3)])
+@pytest.mark.parametrize("pattern,bs,width", [("chain", 1, 2), ("7pt", 2, 3), ("9pt", 2, 4), ("27pt", 4, 0),
+                                              ("9pt", 4, 4), ("7pt", 4, 3), ("7pt", 8, 0)])
 def test_ell_sweeps_equal_csr_sweeps_bitwise(ctx, pattern, bs, width):
-    """The ELL copy of the sweeps (rows with at most 4 strict blocks) runs the CSR sweeps'
-    operations in the same order: identical iterates for every tile; wider rows keep CSR."""
+    """The ELL copy of the sweeps (rows with at most 4 strict blocks of at most 4 x 4) runs the
+    CSR sweeps' operations in the same order: identical iterates for every tile; wider rows
+    and 8 x 8 blocks keep CSR."""
     rng = np.random.Generator(np.random.PCG64(31))
     probs = [_grid_problem(d, _OFFS[pattern], bs, rng) for d in _DIMS[pattern]]
     rp, ci, vals, sys = bsp.block_diagonal(probs)
