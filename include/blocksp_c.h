@@ -56,7 +56,9 @@ int bsp_ctx_destroy(bsp_ctx* ctx);
 const char* bsp_last_error(const bsp_ctx* ctx);
 int bsp_ctx_sync(bsp_ctx* ctx);
 /* Set a tuning option by name ("trsv" = 0 level launches / 1 single-launch
- * sync-free; "spin_limit" = polls before a sync-free wait gives up). */
+ * sync-free; "spin_limit" = polls before a sync-free wait gives up; "batch_ell" = 0 / 1
+ * CSR or ELL sweeps for batches created afterwards; others listed with the Python
+ * bindings). */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
 
 int bsp_malloc(bsp_ctx* ctx, size_t bytes, void** out_dev);
@@ -337,7 +339,8 @@ int bsp_ctx_recorded(const bsp_ctx* ctx, int* count);
  * workgroup per item, level-set sweeps separated by workgroup barriers), replayed from a
  * captured HIP graph.  Block sizes 1, 2, 4, 8. */
 typedef struct bsp_batch bsp_batch;
-/* Builds the IChol0 factor and the per-system level sets (A must outlive the batch). */
+/* Builds the IChol0 factor and the per-system level sets (A must outlive the batch and keep
+ * its values: the factor and its ELL copies are taken at creation). */
 int bsp_batch_create(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t* sys_rowptr, int nrhs,
                      bsp_batch** out);
 /* As bsp_batch_create with each system's IChol0 eliminated colour by colour (`colour`: one
