@@ -96,6 +96,7 @@ _SIGS = {
     "bsp_precond_destroy": [_VP],
     "bsp_precond_apply": [_VP, _VP, _VP],
     "bsp_precond_levels": [_VP, C.c_int, _VP, C.POINTER(C.c_int)],
+    "bsp_precond_ell_width": [_VP, C.c_int],
     "bsp_precond_factor_info": [_VP, C.POINTER(C.c_int64)],
     "bsp_precond_download_factor": [_VP, _VP, _VP, _VP, _VP],
     "bsp_pgmres": [_VP, _VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Stats)],
@@ -378,6 +379,10 @@ class _Precond:
         nl = C.c_int()
         self.ctx.check(self.ctx.L.bsp_precond_levels(self.h, int(lower), _ptr(lev), C.byref(nl)), "levels")
         return nl.value, lev
+
+    def ell_width(self, lower=True):
+        """Entries per slot of the sweep's ELL copy (0: the sweep reads the CSR factor)."""
+        return int(self.ctx.L.bsp_precond_ell_width(self.h, int(lower)))
 
     def factor(self):
         nz = C.c_int64()

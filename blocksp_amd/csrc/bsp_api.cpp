@@ -122,6 +122,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
         ctx->bf16_variant = (int)value;
     } else if (k == "sweep_graphs") {
         ctx->sweep_graphs = value != 0;
+    } else if (k == "sweep_ell") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "sweep_ell must be 0 or 1");
+        ctx->sweep_ell = (int)value;
     } else if (k == "batch_ell") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "batch_ell must be 0 or 1");
         ctx->batch_ell = (int)value;

@@ -75,6 +75,11 @@ struct Sweep {
     int nlevels = 0;
     int fuse_boundary = -1;      // ILU0 lower sweep: its last level is the upper sweep's first
                                  // (1 yes, 0 no, -1 not yet checked)
+    // slot-ordered ELL copy of the blocks (triangular solves; built where padding is small):
+    // slot s holds ell_k entries, columns ecol[s * ell_k + k] (-1 padding, and -1 on padded
+    // slots), blocks at eval + (s * ell_k + k) * bs * bs in the value type of dval
+    int ell_k = 0;
+    DevMem ecol, eval;
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
     // level-launch sweeps replayed from captured HIP graphs, keyed by their operands
@@ -106,6 +111,7 @@ struct bsp_ctx {
     int verbose = 0;                 // print solver progress every `verbose` iterations (0 = off)
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
+    int sweep_ell = 1;               // triangular sweeps read their slot-ordered ELL copy where built
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +
@@ -218,6 +224,10 @@ int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoc
 // ILU0 apply: forward sweep over `lower`, backward over `upper` with the diagonal inverses;
 // the shared boundary level is done once when both sweeps are launched level by level.
 int launch_ilu0_sweeps(bsp_ctx* ctx, Sweep& lower, Sweep& upper, const void* dinv, void* x, uint32_t& epoch);
+// Slot-ordered ELL copy of a triangular sweep's blocks (after its schedule is built; rp: the
+// sweep's host row pointers), when blocks are at most 8 x 8, every row has at most 4 blocks
+// and the padding stays within a quarter of the blocks; otherwise the sweep keeps reading CSR.
+int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp);
 // SOR sweep: xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j!=i} A_ij x_j)), x_j from xn on the
 // dependency side (forward: j < i) and from xo otherwise.
 int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b,

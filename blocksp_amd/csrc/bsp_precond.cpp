@@ -407,6 +407,9 @@ int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
     std::vector<int32_t> rev(elim.rbegin(), elim.rend());
     build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
     build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    int rc = build_sweep_ell(ctx, M->lower, lrp);
+    if (!rc) rc = build_sweep_ell(ctx, M->upper, urp);
+    if (rc) return rc;
     if (f32) {
         if (M->bf16) BSP_HIP(upload_bf16(M->dinv_upper, dinv));
         else BSP_HIP(upload_f32(M->dinv_upper, dinv));
@@ -567,6 +570,9 @@ int ichol0_build(bsp_ctx* ctx, int n, int bs, std::vector<int32_t>& Lrp, std::ve
     std::vector<int32_t> rev(elim.rbegin(), elim.rend());
     build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
     build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    int rc = build_sweep_ell(ctx, M->lower, lrp);
+    if (!rc) rc = build_sweep_ell(ctx, M->upper, urp);
+    if (rc) return rc;
     BSP_HIP(upload(M->dinv_lower, dlo));
     BSP_HIP(upload(M->dinv_upper, dup));
     M->elim_order = elim;
@@ -737,6 +743,11 @@ int bsp_precond_levels(const bsp_precond* M, int lower, int32_t* level, int* nle
     if (level) std::copy(S.level_of.begin(), S.level_of.end(), level);
     if (nlevels) *nlevels = S.nlevels;
     return BSP_OK;
+}
+
+int bsp_precond_ell_width(const bsp_precond* M, int lower) {
+    if (!M) return BSP_E_INVALID;
+    return lower ? M->lower.ell_k : M->upper.ell_k;
 }
 
 int bsp_precond_factor_info(const bsp_precond* M, int64_t* nnzb) {

@@ -141,7 +141,13 @@ struct SweepArgs {
     int64_t spin_limit;
     uint32_t* err;
     const int* pos;     // SOR in an elimination order: step of each row (nullptr: natural order)
+    const int* ecol;    // trsv: slot-ordered ELL copy (nullptr: CSR), ell_k entries per slot
+    const void* eval;
+    int ell_k;
 };
+
+constexpr int kEllMax = 4;     // entries per slot of the ELL copies
+constexpr int kEllMaxBs = 8;   // block sizes with an ELL form (a slot's blocks stay in registers)
 
 // One chunk of slots: [slot_begin, slot_end) with kWavesPerBlock * RPW slots per block.
 // Factor values and diagonal inverses are stored as MT, vectors and arithmetic are VT.
@@ -197,7 +203,7 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
                 ph += row_dot<BS, VT>(av, xv);
             }
             s = b[(size_t)i * BS + r] - ph;
-        } else {
+        } else if (!a.ecol) {
             s = x[(size_t)i * BS + r];
             for (int p = p0; p < p1; ++p) {
                 const int c = a.ci[p];
@@ -206,6 +212,32 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
                 load_row<BS, VT>(x + (size_t)c * BS, xv);
                 s -= row_dot<BS, VT>(av, xv);
             }
+        }
+    }
+    if constexpr (!SOR && BS <= kEllMaxBs) {
+        if (a.ecol) {
+            // ELL copy: columns and blocks are indexed by slot, so they load without waiting
+            // for the row number; padding (-1) ends a slot's entries, which keep the CSR order
+            const MT* __restrict__ ev = static_cast<const MT*>(a.eval);
+            const int K = a.ell_k;
+            int cq[kEllMax];
+            VT av[kEllMax][BS];
+#pragma unroll
+            for (int k = 0; k < kEllMax; ++k) {
+                cq[k] = -1;
+                if (k < K && have) {
+                    cq[k] = a.ecol[(size_t)slot * K + k];
+                    load_mrow<BS, MT, VT>(ev + (((size_t)slot * K + k) * BS + r) * BS, av[k]);
+                }
+            }
+            if (active) s = x[(size_t)i * BS + r];
+            VT xv[kEllMax][BS];
+#pragma unroll
+            for (int k = 0; k < kEllMax; ++k)
+                if (cq[k] >= 0) load_row<BS, VT>(x + (size_t)cq[k] * BS, xv[k]);
+#pragma unroll
+            for (int k = 0; k < kEllMax; ++k)
+                if (cq[k] >= 0) s -= row_dot<BS, VT>(av[k], xv[k]);
         }
     }
     VT out = s;
@@ -283,7 +315,8 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
         } else {
             // many levels: replay one captured graph instead of nlevels host launches
             uint64_t key[8] = {(uint64_t)(uintptr_t)a.x, (uint64_t)(uintptr_t)a.dinv, (uint64_t)(uintptr_t)a.b,
-                               (uint64_t)(uintptr_t)a.xo, 0, (uint64_t)a.forward, (uint64_t)(uintptr_t)a.val, 1};
+                               (uint64_t)(uintptr_t)a.xo, 0, (uint64_t)a.forward, (uint64_t)(uintptr_t)a.val,
+                               a.ecol ? 2u : 1u};
             std::memcpy(&key[4], &a.w, sizeof(double));
             Sweep::Graph* hit = nullptr;
             for (auto& g : S.graphs)
@@ -346,10 +379,62 @@ SweepArgs make_args(bsp_ctx* ctx, Sweep& S, uint32_t epoch) {
     a.spin_limit = ctx->spin_limit;
     a.err = ctx->err_flag.as<uint32_t>();
     a.pos = S.dpos;
+    a.ecol = S.ell_k > 0 && ctx->sweep_ell ? S.ecol.as<int>() : nullptr;
+    a.eval = S.eval.p;
+    a.ell_k = S.ell_k;
     return a;
 }
 
+// ELL copy of a sweep: thread t covers element e of entry k of slot q.
+template <typename MT>
+__global__ void k_sweep_ell_gather(int64_t nslots, int K, int b2, const int* __restrict__ order,
+                                   const int* __restrict__ rp, const int* __restrict__ ci,
+                                   const MT* __restrict__ val, int* __restrict__ ecol, MT* __restrict__ eval) {
+    const int64_t per = (int64_t)K * b2;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nslots * per) return;
+    const int64_t q = t / per;
+    const int r = (int)(t - q * per);
+    const int k = r / b2, e = r - k * b2;
+    const int row = order[q];
+    const int p = row >= 0 ? rp[row] + k : 0;
+    const bool have = row >= 0 && p < rp[row + 1];
+    eval[(q * K + k) * b2 + e] = have ? val[(size_t)p * b2 + e] : MT{};
+    if (e == 0) ecol[q * K + k] = have ? ci[p] : -1;
+}
+
 }  // namespace
+
+int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp) {
+    S.ell_k = 0;
+    if (S.n <= 0 || S.nslots <= 0 || S.bs > kEllMaxBs || (int)rp.size() != S.n + 1) return BSP_OK;
+    int K = 0;
+    for (int i = 0; i < S.n; ++i) K = std::max(K, rp[i + 1] - rp[i]);
+    const int64_t nnz = rp[S.n];
+    if (K == 0 || K > kEllMax || (int64_t)S.n * K * 4 > nnz * 5) return BSP_OK;   // padding <= 1/4
+    const size_t esz = S.bf16 ? sizeof(uint16_t) : S.f32 ? sizeof(float) : sizeof(double);
+    const int b2 = S.bs * S.bs;
+    BSP_HIP(S.ecol.alloc(sizeof(int) * (size_t)S.nslots * K));
+    BSP_HIP(S.eval.alloc(esz * (size_t)S.nslots * K * b2));
+    const int64_t total = (int64_t)S.nslots * K * b2;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    const int* order = S.order.as<int>();
+    if (S.bf16)
+        hipLaunchKernelGGL(k_sweep_ell_gather<uint16_t>, grid, dim3(256), 0, ctx->stream, (int64_t)S.nslots, K, b2,
+                           order, S.drp, S.dci, static_cast<const uint16_t*>(S.dval), S.ecol.as<int>(),
+                           S.eval.as<uint16_t>());
+    else if (S.f32)
+        hipLaunchKernelGGL(k_sweep_ell_gather<float>, grid, dim3(256), 0, ctx->stream, (int64_t)S.nslots, K, b2, order,
+                           S.drp, S.dci, static_cast<const float*>(S.dval), S.ecol.as<int>(), S.eval.as<float>());
+    else
+        hipLaunchKernelGGL(k_sweep_ell_gather<double>, grid, dim3(256), 0, ctx->stream, (int64_t)S.nslots, K, b2,
+                           order, S.drp, S.dci, static_cast<const double*>(S.dval), S.ecol.as<int>(),
+                           S.eval.as<double>());
+    BSP_HIP(hipGetLastError());
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    S.ell_k = K;
+    return BSP_OK;
+}
 
 namespace {
 
@@ -408,6 +493,7 @@ int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* 
     a.xo = xo;
     a.w = w;
     a.forward = forward;
+    a.ecol = nullptr;
     if (S.bf16) return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweeps have no BF16 form");
     return S.f32 ? dispatch<float, float, true>(ctx, S, a) : dispatch<double, double, true>(ctx, S, a);
 }
