@@ -124,7 +124,35 @@ __global__ __launch_bounds__(kBlock) void k_cgs_dots(int64_t len, const double* 
 #pragma unroll
     for (int k = 0; k < kGroup; ++k) q[k] = k < ns ? Q.q[s0 + k] : nullptr;
     const int64_t nvec = len / N;
-    for (int64_t i = (int64_t)slice * kBlock + threadIdx.x; i < nvec; i += (int64_t)nparts * kBlock) {
+    const int64_t stride = (int64_t)nparts * kBlock;
+    int64_t i = (int64_t)slice * kBlock + threadIdx.x;
+    // two element groups per step, all loads first (each sum keeps the one-group order)
+    for (; i + stride < nvec; i += 2 * stride) {
+        double wa[N], wb[N], qa[kGroup][N], qb[kGroup][N];
+        load_w<N>(w, i, wa);
+        load_w<N>(w, i + stride, wb);
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k)
+            if (k < ns) {
+                load_q(q[k], i, qa[k]);
+                load_q(q[k], i + stride, qb[k]);
+            }
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k)
+            if (k < ns) {
+#pragma unroll
+                for (int e = 0; e < N; ++e) acc[k] = fma(qa[k][e], wa[e], acc[k]);
+#pragma unroll
+                for (int e = 0; e < N; ++e) acc[k] = fma(qb[k][e], wb[e], acc[k]);
+            }
+        if (g == 0) {
+#pragma unroll
+            for (int e = 0; e < N; ++e) acc[kGroup] = fma(wa[e], wa[e], acc[kGroup]);
+#pragma unroll
+            for (int e = 0; e < N; ++e) acc[kGroup] = fma(wb[e], wb[e], acc[kGroup]);
+        }
+    }
+    if (i < nvec) {
         double wi[N];
         load_w<N>(w, i, wi);
 #pragma unroll
@@ -173,7 +201,31 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __re
     __syncthreads();
     double acc[1] = {0.0};
     const int64_t nvec = len / N;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nvec; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // two element groups per step (each keeps the one-group order of its updates)
+    for (; i + stride < nvec; i += 2 * stride) {
+        double wa[N], wb[N];
+        load_w<N>(w, i, wa);
+        load_w<N>(w, i + stride, wb);
+#pragma unroll 4
+        for (int s = 0; s < m; ++s) {
+            double qa[N], qb[N];
+            load_q(qs[s], i, qa);
+            load_q(qs[s], i + stride, qb);
+#pragma unroll
+            for (int e = 0; e < N; ++e) wa[e] = fma(-hs[s], qa[e], wa[e]);
+#pragma unroll
+            for (int e = 0; e < N; ++e) wb[e] = fma(-hs[s], qb[e], wb[e]);
+        }
+        store_w<N>(w, i, wa);
+        store_w<N>(w, i + stride, wb);
+#pragma unroll
+        for (int e = 0; e < N; ++e) acc[0] = fma(wa[e], wa[e], acc[0]);
+#pragma unroll
+        for (int e = 0; e < N; ++e) acc[0] = fma(wb[e], wb[e], acc[0]);
+    }
+    if (i < nvec) {
         double wi[N];
         load_w<N>(w, i, wi);
         for (int s = 0; s < m; ++s) {
