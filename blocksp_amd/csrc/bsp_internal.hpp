@@ -76,9 +76,14 @@ struct Sweep {
     int fuse_boundary = -1;      // ILU0 lower sweep: its last level is the upper sweep's first
                                  // (1 yes, 0 no, -1 not yet checked)
     // slot-ordered ELL copy of the blocks (triangular solves; built where padding is small):
-    // slot s holds ell_k entries, columns ecol[s * ell_k + k] (-1 padding, and -1 on padded
-    // slots), blocks at eval + (s * ell_k + k) * bs * bs in the value type of dval
+    // level l's slots hold ell_kl[l] entries each, slot s's entry k at ell_off[l] +
+    // (s - level_ptr[l]) * ell_kl[l] + k: column ecol[.] (-1 padding, and -1 on padded slots),
+    // block at eval + (.) * bs * bs in the value type of dval.  ell_k: the widest level (0: no
+    // copy); ell_uniform: every level has width ell_k (the single-launch sweeps need that).
     int ell_k = 0;
+    bool ell_uniform = false;
+    std::vector<int> ell_kl;
+    std::vector<int64_t> ell_off;
     DevMem ecol, eval;
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
@@ -225,8 +230,9 @@ int launch_trsv(bsp_ctx* ctx, Sweep& S, const void* dinv, void* x, uint32_t epoc
 // the shared boundary level is done once when both sweeps are launched level by level.
 int launch_ilu0_sweeps(bsp_ctx* ctx, Sweep& lower, Sweep& upper, const void* dinv, void* x, uint32_t& epoch);
 // Slot-ordered ELL copy of a triangular sweep's blocks (after its schedule is built; rp: the
-// sweep's host row pointers), when blocks are at most 8 x 8, every row has at most 4 blocks
-// and the padding stays within a quarter of the blocks; otherwise the sweep keeps reading CSR.
+// sweep's host row pointers), each level as wide as its widest row, when the sweep has more
+// than 8 levels, blocks are at most 8 x 8, every row has at most 4 blocks and the padding stays
+// within a quarter of the blocks; otherwise the sweep keeps reading CSR.
 int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp);
 // SOR sweep: xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j!=i} A_ij x_j)), x_j from xn on the
 // dependency side (forward: j < i) and from xo otherwise.

@@ -141,9 +141,10 @@ struct SweepArgs {
     int64_t spin_limit;
     uint32_t* err;
     const int* pos;     // SOR in an elimination order: step of each row (nullptr: natural order)
-    const int* ecol;    // trsv: slot-ordered ELL copy (nullptr: CSR), ell_k entries per slot
-    const void* eval;
+    const int* ecol;    // trsv: slot-ordered ELL copy (nullptr: CSR); slot s's entry k at
+    const void* eval;   // ell_base + s * ell_k + k (per level launch, or uniform)
     int ell_k;
+    int64_t ell_base;
 };
 
 constexpr int kEllMax = 4;     // entries per slot of the ELL copies
@@ -226,8 +227,9 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
             for (int k = 0; k < kEllMax; ++k) {
                 cq[k] = -1;
                 if (k < K && have) {
-                    cq[k] = a.ecol[(size_t)slot * K + k];
-                    load_mrow<BS, MT, VT>(ev + (((size_t)slot * K + k) * BS + r) * BS, av[k]);
+                    const int64_t e = a.ell_base + (int64_t)slot * K + k;
+                    cq[k] = a.ecol[e];
+                    load_mrow<BS, MT, VT>(ev + ((size_t)e * BS + r) * BS, av[k]);
                 }
             }
             if (active) s = x[(size_t)i * BS + r];
@@ -293,6 +295,7 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
     const int nchunks = (S.nslots + SPB - 1) / SPB;
     a.nchunks = nchunks;
     if (ctx->trsv_mode == 1) {
+        if (a.ecol && !S.ell_uniform) a.ecol = nullptr;   // levels of different widths: CSR
         const int grid = std::max(1, std::min(nchunks, ctx->num_cus * ctx->sweep_wg_per_cu));
         a.base = S.tickets_used;
         hipLaunchKernelGGL((k_sweep_syncfree<BS, VT, MT, SOR>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, S.nslots);
@@ -306,6 +309,10 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
             for (int l = l0; l < lend; ++l) {
                 const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - S.level_ptr[l];
                 if (cnt <= 0) continue;
+                if (a.ecol) {
+                    a.ell_k = S.ell_kl[l];
+                    a.ell_base = S.ell_off[l] - (int64_t)s0 * S.ell_kl[l];
+                }
                 hipLaunchKernelGGL((k_sweep_level<BS, VT, MT, SOR>), dim3((cnt + SPB - 1) / SPB), dim3(kBlock), 0, st,
                                    a, s0, cnt);
             }
@@ -381,13 +388,15 @@ SweepArgs make_args(bsp_ctx* ctx, Sweep& S, uint32_t epoch) {
     a.pos = S.dpos;
     a.ecol = S.ell_k > 0 && ctx->sweep_ell ? S.ecol.as<int>() : nullptr;
     a.eval = S.eval.p;
-    a.ell_k = S.ell_k;
+    a.ell_k = S.ell_k;   // uniform width (single launch); level launches set their level's
+    a.ell_base = 0;
     return a;
 }
 
-// ELL copy of a sweep: thread t covers element e of entry k of slot q.
+// ELL copy of the levels' slots [s0, s0 + nslots) of width K, entries from e0: thread t
+// covers element e of entry k of slot s0 + q.
 template <typename MT>
-__global__ void k_sweep_ell_gather(int64_t nslots, int K, int b2, const int* __restrict__ order,
+__global__ void k_sweep_ell_gather(int64_t nslots, int K, int b2, int s0, int64_t e0, const int* __restrict__ order,
                                    const int* __restrict__ rp, const int* __restrict__ ci,
                                    const MT* __restrict__ val, int* __restrict__ ecol, MT* __restrict__ eval) {
     const int64_t per = (int64_t)K * b2;
@@ -396,43 +405,62 @@ __global__ void k_sweep_ell_gather(int64_t nslots, int K, int b2, const int* __r
     const int64_t q = t / per;
     const int r = (int)(t - q * per);
     const int k = r / b2, e = r - k * b2;
-    const int row = order[q];
+    const int row = order[s0 + q];
     const int p = row >= 0 ? rp[row] + k : 0;
     const bool have = row >= 0 && p < rp[row + 1];
-    eval[(q * K + k) * b2 + e] = have ? val[(size_t)p * b2 + e] : MT{};
-    if (e == 0) ecol[q * K + k] = have ? ci[p] : -1;
+    const int64_t d = e0 + q * K + k;
+    eval[d * b2 + e] = have ? val[(size_t)p * b2 + e] : MT{};
+    if (e == 0) ecol[d] = have ? ci[p] : -1;
+}
+
+template <typename MT>
+hipError_t gather_levels(bsp_ctx* ctx, const Sweep& S, int* ecol, MT* eval) {
+    for (int l = 0; l < S.nlevels; ++l) {
+        const int s0 = S.level_ptr[l], cnt = S.level_ptr[l + 1] - s0, K = S.ell_kl[l];
+        const int64_t total = (int64_t)cnt * K * S.bs * S.bs;
+        if (total <= 0) continue;
+        hipLaunchKernelGGL(k_sweep_ell_gather<MT>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (int64_t)cnt, K, S.bs * S.bs, s0, S.ell_off[l], S.order.as<int>(), S.drp, S.dci,
+                           static_cast<const MT*>(S.dval), ecol, eval);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
 
 int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp) {
     S.ell_k = 0;
-    if (S.n <= 0 || S.nslots <= 0 || S.bs > kEllMaxBs || (int)rp.size() != S.n + 1) return BSP_OK;
+    S.ell_uniform = false;
+    S.ell_kl.clear();
+    S.ell_off.clear();
+    // a sweep of a few wide levels is bound by bandwidth, not by its load chains: the copy
+    // measured no gain there (parity-coloured BF16 ILU0 of the 512^2 DG matrix) and costs memory
+    if (S.n <= 0 || S.nslots <= 0 || S.bs > kEllMaxBs || S.nlevels <= kDirectLevels ||
+        (int)rp.size() != S.n + 1 || (int)S.level_of.size() != S.n)
+        return BSP_OK;
+    // each level as wide as its widest row; padding counted over the real rows
+    std::vector<int> kl(S.nlevels, 0);
+    for (int i = 0; i < S.n; ++i) kl[S.level_of[i]] = std::max(kl[S.level_of[i]], rp[i + 1] - rp[i]);
     int K = 0;
-    for (int i = 0; i < S.n; ++i) K = std::max(K, rp[i + 1] - rp[i]);
+    int64_t padded = 0;
+    for (int i = 0; i < S.n; ++i) padded += kl[S.level_of[i]];
+    for (int l = 0; l < S.nlevels; ++l) K = std::max(K, kl[l]);
     const int64_t nnz = rp[S.n];
-    if (K == 0 || K > kEllMax || (int64_t)S.n * K * 4 > nnz * 5) return BSP_OK;   // padding <= 1/4
+    if (K == 0 || K > kEllMax || padded * 4 > nnz * 5) return BSP_OK;   // padding <= 1/4
+    S.ell_kl = kl;
+    S.ell_off.assign(S.nlevels + 1, 0);
+    for (int l = 0; l < S.nlevels; ++l)
+        S.ell_off[l + 1] = S.ell_off[l] + (int64_t)(S.level_ptr[l + 1] - S.level_ptr[l]) * kl[l];
+    const int64_t entries = std::max<int64_t>(S.ell_off[S.nlevels], 1);
     const size_t esz = S.bf16 ? sizeof(uint16_t) : S.f32 ? sizeof(float) : sizeof(double);
-    const int b2 = S.bs * S.bs;
-    BSP_HIP(S.ecol.alloc(sizeof(int) * (size_t)S.nslots * K));
-    BSP_HIP(S.eval.alloc(esz * (size_t)S.nslots * K * b2));
-    const int64_t total = (int64_t)S.nslots * K * b2;
-    const dim3 grid((unsigned)((total + 255) / 256));
-    const int* order = S.order.as<int>();
-    if (S.bf16)
-        hipLaunchKernelGGL(k_sweep_ell_gather<uint16_t>, grid, dim3(256), 0, ctx->stream, (int64_t)S.nslots, K, b2,
-                           order, S.drp, S.dci, static_cast<const uint16_t*>(S.dval), S.ecol.as<int>(),
-                           S.eval.as<uint16_t>());
-    else if (S.f32)
-        hipLaunchKernelGGL(k_sweep_ell_gather<float>, grid, dim3(256), 0, ctx->stream, (int64_t)S.nslots, K, b2, order,
-                           S.drp, S.dci, static_cast<const float*>(S.dval), S.ecol.as<int>(), S.eval.as<float>());
-    else
-        hipLaunchKernelGGL(k_sweep_ell_gather<double>, grid, dim3(256), 0, ctx->stream, (int64_t)S.nslots, K, b2,
-                           order, S.drp, S.dci, static_cast<const double*>(S.dval), S.ecol.as<int>(),
-                           S.eval.as<double>());
-    BSP_HIP(hipGetLastError());
+    BSP_HIP(S.ecol.alloc(sizeof(int) * (size_t)entries));
+    BSP_HIP(S.eval.alloc(esz * (size_t)entries * S.bs * S.bs));
+    if (S.bf16) BSP_HIP(gather_levels<uint16_t>(ctx, S, S.ecol.as<int>(), S.eval.as<uint16_t>()));
+    else if (S.f32) BSP_HIP(gather_levels<float>(ctx, S, S.ecol.as<int>(), S.eval.as<float>()));
+    else BSP_HIP(gather_levels<double>(ctx, S, S.ecol.as<int>(), S.eval.as<double>()));
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     S.ell_k = K;
+    S.ell_uniform = std::all_of(kl.begin(), kl.end(), [&](int k) { return k == K; });
     return BSP_OK;
 }
 

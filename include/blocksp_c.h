@@ -198,9 +198,10 @@ int bsp_precond_destroy(bsp_precond* M);
 int bsp_precond_apply(bsp_ctx* ctx, const bsp_precond* M, double* r_dev);
 /* Level sets of the forward (lower=1) or backward (lower=0) sweep: level_host[n]. */
 int bsp_precond_levels(const bsp_precond* M, int lower, int32_t* level_host, int* nlevels);
-/* Entries per slot of the forward (lower=1) or backward (lower=0) sweep's slot-ordered ELL
- * copy (1-4), or 0 when the sweep reads the CSR factor (blocks above 8 x 8, rows with more
- * than 4 blocks, or padding above a quarter of the blocks).  No reference counterpart: the
+/* Widest level of the forward (lower=1) or backward (lower=0) sweep's slot-ordered ELL copy
+ * (1-4 entries per slot; each level as wide as its widest row), or 0 when the sweep reads the
+ * CSR factor (8 levels or fewer, blocks above 8 x 8, rows with more than 4 blocks, or padding
+ * above a quarter of the blocks).  No reference counterpart: the
  * copy shortens each level's chain of dependent loads and gives bitwise the same results;
  * the context option "sweep_ell" = 0 makes applies read CSR. */
 int bsp_precond_ell_width(const bsp_precond* M, int lower);

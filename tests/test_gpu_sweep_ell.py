@@ -21,6 +21,8 @@ def _apply(ctx, M, r, ell):
         ctx.set_option("sweep_ell", 1)
 
 
+_COL = pr.parity_colours((40, 36))
+
 CASES = {
     "dg2d_ilu0": lambda ctx, A: bsp.ILU0(ctx, A),
     "dg2d_ilu0_f32": lambda ctx, A: bsp.ILU0Ex(ctx, A, precision=bsp.PREC_F32),
@@ -78,9 +80,46 @@ def test_ell_pgmres_and_pcg_iterates_equal_csr(ctx):
     np.testing.assert_array_equal(out[1][3], out[0][3])
 
 
-def test_wide_or_padded_sweeps_keep_csr(ctx):
-    """A parity-coloured factor pads most rows (red rows have no earlier neighbours): no ELL."""
+@pytest.mark.parametrize("kind", ["ilu0", "ilu0_bf16", "ichol0"])
+def test_few_level_sweeps_keep_csr(ctx, kind):
+    """Parity-coloured factors sweep in 3 levels (bandwidth-bound): no ELL copy."""
     p = pr.dg2d_veclap(40, 36, seed=1)
     A = bsp.BSR.from_problem(ctx, p)
-    M = bsp.ILU0Ex(ctx, A, colour=pr.parity_colours((40, 36)))
+    M = {"ilu0": lambda: bsp.ILU0Ex(ctx, A, colour=_COL),
+         "ilu0_bf16": lambda: bsp.ILU0Ex(ctx, A, colour=_COL, precision=bsp.PREC_BF16),
+         "ichol0": lambda: bsp.IChol0(ctx, A, colour=_COL)}[kind]()
     assert M.ell_width(True) == 0 and M.ell_width(False) == 0
+
+
+def test_padded_or_wide_sweeps_keep_csr(ctx):
+    """Rows 0-9 form a chain (10 levels); rows 10.. all depend on row 9, and row 10 also on rows
+    6-8: a level whose widest row has 4 strict blocks while the others have 1 (padding far above
+    a quarter), and an upper sweep whose row 9 has n - 10 blocks.  Neither sweep gets an ELL
+    copy; the apply still runs."""
+    n, bs = 40, 2
+    nb = {i: set() for i in range(n)}
+
+    def couple(i, j):
+        nb[i].add(j)
+        nb[j].add(i)
+
+    for i in range(1, 10):
+        couple(i, i - 1)
+    for i in range(10, n):
+        couple(i, 9)
+    for j in (6, 7, 8):
+        couple(10, j)
+    rowptr, colidx, vals = [0], [], []
+    for i in range(n):
+        for j in sorted(nb[i] | {i}):
+            colidx.append(j)
+            vals.append((len(nb[i]) + 1.0) * np.eye(bs) if j == i else -0.5 * np.eye(bs))
+        rowptr.append(len(colidx))
+    p = pr.Problem(n, bs, bs, 1, np.array(rowptr, np.int32), np.array(colidx, np.int32), np.array(vals),
+                   np.ones(n * bs), name="padded")
+    A = bsp.BSR.from_problem(ctx, p)
+    M = bsp.ILU0(ctx, A)
+    assert M.levels(True)[0] > 8
+    assert M.ell_width(True) == 0 and M.ell_width(False) == 0
+    z = _apply(ctx, M, np.ones(n * bs), 1)
+    assert np.isfinite(z).all()
