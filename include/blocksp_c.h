@@ -55,10 +55,13 @@ int bsp_ctx_create(int device, void* stream, bsp_ctx** out);
 int bsp_ctx_destroy(bsp_ctx* ctx);
 const char* bsp_last_error(const bsp_ctx* ctx);
 int bsp_ctx_sync(bsp_ctx* ctx);
-/* Set a tuning option by name ("trsv" = 0 level launches / 1 single-launch
- * sync-free; "spin_limit" = polls before a sync-free wait gives up; "batch_ell" = 0 / 1
- * CSR or ELL sweeps for batches created afterwards; others listed with the Python
- * bindings). */
+/* Set a tuning option by name: "trsv" (0 level launches, default / 1 single-launch
+ * sync-free sweeps), "sweep_graphs" (1: replay many-level sweeps from HIP graphs),
+ * "sweep_wg_per_cu", "spin_limit" (polls before a sync-free wait gives up), "verbose",
+ * "bsmv_variant" (0-5), "bf16_variant", "fgmres_ortho" (0 classical / 1 modified
+ * Gram-Schmidt), "fgmres_basis_f32", "sweep_ell" (1: sweeps read their ELL copy where
+ * built / 0: CSR), "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards).
+ * Unknown names and out-of-range values return BSP_E_INVALID. */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
 
 int bsp_malloc(bsp_ctx* ctx, size_t bytes, void** out_dev);
