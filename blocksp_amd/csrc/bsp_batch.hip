@@ -133,7 +133,7 @@ __device__ void sweep(const int* __restrict__ rows, const int* __restrict__ lev,
 
 // The same sweep over the ELL copy: slot q holds columns col[q*K + k] (-1: padding), blocks
 // val[(q*K + k) * BS*BS] and the inverse diagonal block dinv[q * BS*BS]; rows[q] is its row.
-template <int BS, int R, int K>
+template <int BS, int K>
 struct EllSlot {
     int q, row;
     int cq[K];
@@ -141,9 +141,9 @@ struct EllSlot {
 };
 
 // The slot's structure and its row of the K blocks (what a level ahead can load).
-template <int BS, int R, int K>
+template <int BS, int K>
 __device__ __forceinline__ void ell_load(const int* __restrict__ rows, const int* __restrict__ col,
-                                         const double* __restrict__ val, int q, int c, EllSlot<BS, R, K>& e) {
+                                         const double* __restrict__ val, int q, int c, EllSlot<BS, K>& e) {
     constexpr int B2 = BS * BS;
     e.q = q;
 #pragma unroll
@@ -158,7 +158,7 @@ __device__ __forceinline__ void ell_load(const int* __restrict__ rows, const int
 // Right-hand side, inverse diagonal block and the gathered vector entries, then the row.
 template <int BS, int R, int K>
 __device__ __forceinline__ void ell_finish(const double* __restrict__ dinv, const double* in, double* out, int c,
-                                           int gbase, const EllSlot<BS, R, K>& e) {
+                                           int gbase, const EllSlot<BS, K>& e) {
     constexpr int B2 = BS * BS;
     double t[R], dv[BS];
     ld<R>(in + (e.row * BS + c) * R, t);
@@ -199,8 +199,8 @@ __device__ void sweep_ell(const int* __restrict__ rows, const int* __restrict__ 
         for (int l = lb; l < le; ++l) {
             const int a0 = lev[l], cnt = (lev[l + 1] - a0) * BS;
             for (int i0 = tid; i0 < cnt; i0 += T) {
-                EllSlot<BS, R, K> e;
-                ell_load<BS, R, K>(rows, col, val, a0 + i0 / BS, c, e);
+                EllSlot<BS, K> e;
+                ell_load<BS, K>(rows, col, val, a0 + i0 / BS, c, e);
                 ell_finish<BS, R, K>(dinv, in, out, c, gbase, e);
             }
             __syncthreads();
@@ -211,22 +211,22 @@ __device__ void sweep_ell(const int* __restrict__ rows, const int* __restrict__ 
     // this level computes (the sweep writes none of them), so a level's chain is the loads
     // of its vector entries, right-hand sides and inverse diagonal blocks.
     int a0 = lev[lb], a1 = lev[lb + 1];
-    EllSlot<BS, R, K> nxt;
+    EllSlot<BS, K> nxt;
     bool have_n = tid < (a1 - a0) * BS;
-    if (have_n) ell_load<BS, R, K>(rows, col, val, a0 + tid / BS, c, nxt);
+    if (have_n) ell_load<BS, K>(rows, col, val, a0 + tid / BS, c, nxt);
     for (int l = lb; l < le; ++l) {
         const int a2 = l + 2 <= le ? lev[l + 2] : a1;
         const int cnt = (a1 - a0) * BS;
-        EllSlot<BS, R, K> cur = nxt;
+        EllSlot<BS, K> cur = nxt;
         const bool have_c = have_n;
         // slots beyond the first of a level wider than the workgroup: loaded in place
         for (int i0 = tid + T; i0 < cnt; i0 += T) {
-            EllSlot<BS, R, K> e;
-            ell_load<BS, R, K>(rows, col, val, a0 + i0 / BS, c, e);
+            EllSlot<BS, K> e;
+            ell_load<BS, K>(rows, col, val, a0 + i0 / BS, c, e);
             ell_finish<BS, R, K>(dinv, in, out, c, gbase, e);
         }
         have_n = l + 1 < le && tid < (a2 - a1) * BS;
-        if (have_n) ell_load<BS, R, K>(rows, col, val, a1 + tid / BS, c, nxt);
+        if (have_n) ell_load<BS, K>(rows, col, val, a1 + tid / BS, c, nxt);
         if (have_c) ell_finish<BS, R, K>(dinv, in, out, c, gbase, cur);
         __syncthreads();
         a0 = a1;
