@@ -24,11 +24,12 @@ Also measured in the same run:
   * `workloads`: the secondary configurations -- the 3D 64^3 DG Laplacian (BF16 MFMA
     block SpMM with 32 RHS, mixed vs plain FP64 solve with forward errors), 512 batched
     uneven IChol0-pCG systems in one persistent launch, device IChol0-pCG on one large
-    system in natural vs parity-coloured order, and the learned ILU0 correction on the
-    48^3 random-coefficient problem (trained on the GPU in the same run).
+    system in natural vs parity-coloured order, the learned ILU0 correction on the
+    48^3 random-coefficient problem (trained on the GPU in the same run), and the
+    64x64 B=2 SOR-pGMRES case beside the oracle on one core.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-plain]
-                       [--skip-extras] [--only bf16|batch|learned|pcg]
+                       [--skip-extras] [--only bf16|batch|learned|pcg|c1]
 """
 from __future__ import annotations
 
@@ -264,6 +265,36 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
     return out
 
 
+def bench_c1(bsp, pr, ctx):
+    """The reference's CPU-sized case: 64 x 64 block stencil (B=2, 4096 block rows),
+    SOR(w=1.2, nu=3)-pGMRES(30, k=10) to tol 1e-10 with the reference's stopping rule, on the GPU
+    (exact-order SOR sweeps, and multicolour SOR as the option) beside the oracle on 1 core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    p = pr.stencil2d_b2(64, 64, seed=0)
+    A = bsp.BSR.from_problem(ctx, p)
+    b = ctx.to_device(p.b)
+    out = {"workload": "64x64 block stencil B=2 (4096 block rows), SOR(1.2, nu 3)-pGMRES(30, k 10), tol 1e-10"}
+    for label, colour in (("exact_order_sor", None), ("multicolour_sor", pr.parity_colours((64, 64)))):
+        M = bsp.SOR(ctx, A, 1.2, 3, 0, colour=colour)
+        bsp.pGMRES(ctx, A, b, 1e-10, M, maxIters=5000, restart=30, k=10)
+        ctx.sync()
+        t0 = time.perf_counter()
+        _, st, rc = bsp.pGMRES(ctx, A, b, 1e-10, M, maxIters=5000, restart=30, k=10)
+        ctx.sync()
+        out[label] = {"ms": round((time.perf_counter() - t0) * 1e3, 2), "iters": st.iters,
+                      "true_relres": st.true_relres, "converged": rc == 0}
+    t0 = time.perf_counter()
+    _, sto = po.pgmres(p, p.b, 1e-10, po.PRE_SOR, maxIters=5000, restart=30, k=10, w=1.2, nu=3)
+    out["cpu_oracle_1core"] = {"ms": round((time.perf_counter() - t0) * 1e3, 2), "iters": sto.iters,
+                               "true_relres": sto.true_relres}
+    log(f"C1 SOR-pGMRES: GPU exact-order {out['exact_order_sor']['ms']} ms ({out['exact_order_sor']['iters']} it), "
+        f"multicolour {out['multicolour_sor']['ms']} ms ({out['multicolour_sor']['iters']} it); "
+        f"oracle 1 core {out['cpu_oracle_1core']['ms']} ms ({sto.iters} it)")
+    return out
+
+
 def bench_pcg(bsp, pr, ctx):
     """Device pCG with IChol0 to true relres 1e-10 on one large system, the reference's natural
     elimination order against the parity-coloured one: the largest 2D block Laplacian of the
@@ -349,7 +380,7 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
     ap.add_argument("--skip-extras", action="store_true", help="skip the 3D BF16, batched and learned workloads")
-    ap.add_argument("--only", choices=["bf16", "batch", "learned", "pcg"], default=None,
+    ap.add_argument("--only", choices=["bf16", "batch", "learned", "pcg", "c1"], default=None,
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -371,7 +402,8 @@ def main():
         fn = {"bf16": lambda: bench_bf16(bsp, pr, ctx, solve=False),
               "batch": lambda: bench_batch(bsp, pr, ctx, baseline_every=512),
               "learned": lambda: bench_learned(bsp, pr, ctx, train_only=True),
-              "pcg": lambda: bench_pcg(bsp, pr, ctx)}[args.only]
+              "pcg": lambda: bench_pcg(bsp, pr, ctx),
+              "c1": lambda: bench_c1(bsp, pr, ctx)}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -503,7 +535,8 @@ def main():
     extras = None
     if not args.skip_extras:
         extras = {"dg3d_bf16_mixed": bench_bf16(bsp, pr, ctx), "batched_ichol0_pcg": bench_batch(bsp, pr, ctx),
-                  "ichol0_pcg_single": bench_pcg(bsp, pr, ctx), "learned_ilu0": bench_learned(bsp, pr, ctx)}
+                  "ichol0_pcg_single": bench_pcg(bsp, pr, ctx), "learned_ilu0": bench_learned(bsp, pr, ctx),
+                  "sor_pgmres_c1": bench_c1(bsp, pr, ctx)}
 
     value = ms_per_step / 1e3
     out = {
