@@ -70,6 +70,7 @@ struct Sweep {
     // the rows handled by one wavefront, so a wavefront never holds two levels.
     DevMem order;                // int32 [nslots]
     std::vector<int> level_ptr;  // host: slot offsets of each level (padded)
+    DevMem dlevel_ptr;           // the same on the device (one-workgroup sweeps)
     std::vector<int> level_of;   // host: level of each row (unpadded)
     int nslots = 0;
     int nlevels = 0;
@@ -117,6 +118,9 @@ struct bsp_ctx {
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
     int sweep_ell = 1;               // triangular sweeps read their slot-ordered ELL copy where built
+    int64_t sweep_onewg_max = 131072; // level-launch mode: sweeps of more than 8 levels of systems with
+                                     // n * bs up to this run in one workgroup (0: never; measured
+                                     // faster up to 256^2 x bs 2, slower on the 512^2 x bs 8 DG matrix)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +

@@ -59,6 +59,8 @@ void build_schedule_topo(Sweep& S, int n, int bs, const std::vector<int32_t>& rp
     for (int i = 0; i < n; ++i) order[fill[S.level_of[i]]++] = i;
     (void)S.order.alloc(sizeof(int) * std::max(S.nslots, 1));
     (void)hipMemcpy(S.order.p, order.data(), sizeof(int) * S.nslots, hipMemcpyHostToDevice);
+    (void)S.dlevel_ptr.alloc(sizeof(int) * S.level_ptr.size());
+    (void)hipMemcpy(S.dlevel_ptr.p, S.level_ptr.data(), sizeof(int) * S.level_ptr.size(), hipMemcpyHostToDevice);
     (void)S.flags.alloc(sizeof(uint32_t) * std::max(n, 1));
     (void)hipMemset(S.flags.p, 0, sizeof(uint32_t) * std::max(n, 1));
     (void)S.ticket.alloc(sizeof(unsigned long long));

@@ -287,13 +287,39 @@ __global__ __launch_bounds__(kBlock) void k_sweep_syncfree(SweepArgs a, int nslo
     }
 }
 
+// A whole sweep (levels [l0, l1)) in one workgroup of kOneBlock threads, a barrier between levels:
+// for small systems with many narrow levels a level costs a few dependent loads instead of a
+// kernel launch.
+constexpr int kOneBlock = 1024;
+
+template <int BS, typename VT, typename MT, bool SOR>
+__global__ __launch_bounds__(kOneBlock) void k_sweep_onewg(SweepArgs a, const int* __restrict__ level_ptr, int l0,
+                                                           int l1) {
+    constexpr int SPP = (kOneBlock / kWave) * (kWave / BS);   // slots per pass of the workgroup
+    for (int l = l0; l < l1; ++l) {
+        const int s0 = level_ptr[l], s1 = level_ptr[l + 1];
+        for (int sb = s0; sb < s1; sb += SPP) sweep_chunk<BS, VT, MT, false, SOR>(a, sb, s1);
+        __syncthreads();
+    }
+}
+
 // Levels [l0, l1) of the sweep (l1 < 0: to the last); a partial range is only asked for on
-// the direct level-launch path (few levels, level-launch mode).
+// the direct level-launch path (few levels, level-launch mode) or for a one-workgroup sweep.
 template <int BS, typename VT, typename MT, bool SOR>
 int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
     constexpr int SPB = kWavesPerBlock * (kWave / BS);
     const int nchunks = (S.nslots + SPB - 1) / SPB;
     a.nchunks = nchunks;
+    // (sweeps of a few wide levels keep their parallel level launches)
+    if (ctx->trsv_mode == 0 && S.nlevels > kDirectLevels && (int64_t)S.n * S.bs <= ctx->sweep_onewg_max &&
+        S.dlevel_ptr.p) {
+        a.ecol = nullptr;   // the CSR blocks (the ELL copy's level widths are per launch)
+        hipLaunchKernelGGL((k_sweep_onewg<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream, a,
+                           S.dlevel_ptr.as<int>(), l0, l1 < 0 ? S.nlevels : l1);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
+        return BSP_OK;
+    }
     if (ctx->trsv_mode == 1) {
         if (a.ecol && !S.ell_uniform) a.ecol = nullptr;   // levels of different widths: CSR
         const int grid = std::max(1, std::min(nchunks, ctx->num_cus * ctx->sweep_wg_per_cu));

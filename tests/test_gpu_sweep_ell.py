@@ -11,6 +11,15 @@ from blocksp_amd import problems as pr
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def _level_launches(ctx):
+    """These systems are small enough for one-workgroup sweeps, which read CSR: launch the
+    sweeps level by level so that the ELL copy is the one read."""
+    ctx.set_option("sweep_onewg_max", 0)
+    yield
+    ctx.set_option("sweep_onewg_max", 131072)
+
+
 def _apply(ctx, M, r, ell):
     ctx.set_option("sweep_ell", ell)
     try:

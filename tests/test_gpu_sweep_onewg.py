@@ -1,0 +1,73 @@
+"""Sweeps of small systems in one workgroup (context option sweep_onewg_max) against the same
+sweeps launched level by level: every row of a level is independent of the others, so the
+applies and whole solves must agree bitwise.  Level launches are checked against the oracle in
+test_gpu_parity.py."""
+import numpy as np
+import pytest
+
+import blocksp_amd as bsp
+from blocksp_amd import problems as pr
+
+pytestmark = pytest.mark.gpu
+
+
+def _with(ctx, onewg, fn):
+    ctx.set_option("sweep_onewg_max", onewg)
+    try:
+        return fn()
+    finally:
+        ctx.set_option("sweep_onewg_max", 131072)
+
+
+PRECONDS = {
+    "ilu0": lambda ctx, A, p: bsp.ILU0(ctx, A),
+    "ilu0_f32": lambda ctx, A, p: bsp.ILU0Ex(ctx, A, precision=bsp.PREC_F32),
+    "ilu0_bf16_coloured": lambda ctx, A, p: bsp.ILU0Ex(ctx, A, colour=pr.parity_colours(p.meta["dims"]),
+                                                      precision=bsp.PREC_BF16),
+    "ichol0": lambda ctx, A, p: bsp.IChol0(ctx, A),
+    "sor_sym": lambda ctx, A, p: bsp.SOR(ctx, A, 1.2, 3, 0),
+    "sor_fwd": lambda ctx, A, p: bsp.SOR(ctx, A, 0.9, 2, 1),
+    "sor_multicolour": lambda ctx, A, p: bsp.SOR(ctx, A, 1.2, 3, 0, colour=pr.parity_colours(p.meta["dims"])),
+}
+
+
+def _problem(kind):
+    if kind.startswith("ichol0") or kind.startswith("ilu0"):
+        return pr.dg2d_veclap(20, 18, seed=2)
+    return pr.stencil2d_b2(40, 36, seed=1)
+
+
+@pytest.mark.parametrize("kind", sorted(PRECONDS))
+def test_onewg_apply_equals_level_launches_bitwise(ctx, kind):
+    print(f"onewg {kind}", flush=True)
+    p = _problem(kind)
+    A = bsp.BSR.from_problem(ctx, p)
+    M = PRECONDS[kind](ctx, A, p)
+    r = np.random.default_rng(9).standard_normal(p.n * p.bs)
+    out = []
+    for onewg in (0, 1 << 20):
+        def run():
+            rd = ctx.to_device(r)
+            M(rd)
+            return rd.numpy()
+        out.append(_with(ctx, onewg, run))
+    np.testing.assert_array_equal(out[0], out[1])
+    assert np.isfinite(out[0]).all() and np.linalg.norm(out[0]) > 0
+
+
+def test_onewg_solves_equal_level_launches_bitwise(ctx):
+    p = pr.stencil2d_b2(48, 40, seed=3)
+    A = bsp.BSR.from_problem(ctx, p)
+    b = ctx.to_device(p.b)
+    res = []
+    for onewg in (0, 1 << 20):
+        def run():
+            M = bsp.SOR(ctx, A, 1.2, 3, 0)
+            x, st, rc = bsp.pGMRES(ctx, A, b, 1e-10, M, maxIters=3000, restart=30, k=10)
+            Mi = bsp.ILU0(ctx, A)
+            xi, sti, rci = bsp.pGMRES(ctx, A, b, 1e-10, Mi, maxIters=3000, restart=30, k=10)
+            return st.iters, x.numpy(), sti.iters, xi.numpy()
+        res.append(_with(ctx, onewg, run))
+    assert res[0][0] == res[1][0] and res[0][2] == res[1][2]
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    np.testing.assert_array_equal(res[0][3], res[1][3])
