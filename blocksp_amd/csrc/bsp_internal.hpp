@@ -74,6 +74,7 @@ struct Sweep {
     std::vector<int> level_of;   // host: level of each row (unpadded)
     int nslots = 0;
     int nlevels = 0;
+    int max_level_rows = 0;      // rows of the widest level (unpadded)
     int fuse_boundary = -1;      // ILU0 lower sweep: its last level is the upper sweep's first
                                  // (1 yes, 0 no, -1 not yet checked)
     // slot-ordered ELL copy of the blocks (triangular solves; built where padding is small):
@@ -118,9 +119,10 @@ struct bsp_ctx {
     int sweep_wg_per_cu = 8;         // persistent grid of the sync-free sweeps, per compute unit
     int sweep_graphs = 1;            // replay many-level sweeps from captured HIP graphs
     int sweep_ell = 1;               // triangular sweeps read their slot-ordered ELL copy where built
-    int64_t sweep_onewg_max = 131072; // level-launch mode: sweeps of more than 8 levels of systems with
-                                     // n * bs up to this run in one workgroup (0: never; measured
-                                     // faster up to 256^2 x bs 2, slower on the 512^2 x bs 8 DG matrix)
+    int64_t sweep_onewg_max = int64_t(1) << 22;   // level-launch mode: a sweep of more than 8
+                                     // levels whose widest level fits one pass of a 1024-thread
+                                     // workgroup (1024 / bs rows) runs in one workgroup when n * bs is
+                                     // at most this (0: never)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +

@@ -52,6 +52,7 @@ void build_schedule_topo(Sweep& S, int n, int bs, const std::vector<int32_t>& rp
     const int rpw = std::max(1, kWaveRows / bs);
     std::vector<int> cnt(nlev, 0);
     for (int i = 0; i < n; ++i) cnt[S.level_of[i]]++;
+    S.max_level_rows = nlev ? *std::max_element(cnt.begin(), cnt.end()) : 0;
     S.level_ptr.assign(nlev + 1, 0);
     for (int l = 0; l < nlev; ++l) S.level_ptr[l + 1] = S.level_ptr[l] + (cnt[l] + rpw - 1) / rpw * rpw;
     S.nslots = S.level_ptr[nlev];

@@ -310,9 +310,10 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
     constexpr int SPB = kWavesPerBlock * (kWave / BS);
     const int nchunks = (S.nslots + SPB - 1) / SPB;
     a.nchunks = nchunks;
-    // (sweeps of a few wide levels keep their parallel level launches)
-    if (ctx->trsv_mode == 0 && S.nlevels > kDirectLevels && (int64_t)S.n * S.bs <= ctx->sweep_onewg_max &&
-        S.dlevel_ptr.p) {
+    // many levels, each within one pass of the workgroup (wider levels keep their parallel
+    // level launches: measured 1.6x slower on the 256^2 x bs 8 DG matrix in one workgroup)
+    if (ctx->trsv_mode == 0 && S.nlevels > kDirectLevels && S.max_level_rows <= kOneBlock / BS &&
+        (int64_t)S.n * S.bs <= ctx->sweep_onewg_max && S.dlevel_ptr.p) {
         a.ecol = nullptr;   // the CSR blocks (the ELL copy's level widths are per launch)
         hipLaunchKernelGGL((k_sweep_onewg<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream, a,
                            S.dlevel_ptr.as<int>(), l0, l1 < 0 ? S.nlevels : l1);

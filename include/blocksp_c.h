@@ -60,9 +60,9 @@ int bsp_ctx_sync(bsp_ctx* ctx);
  * "sweep_wg_per_cu", "spin_limit" (polls before a sync-free wait gives up), "verbose",
  * "bsmv_variant" (0-5), "bf16_variant", "fgmres_ortho" (0 classical / 1 modified
  * Gram-Schmidt), "fgmres_basis_f32", "sweep_ell" (1: sweeps read their ELL copy where
- * built / 0: CSR), "sweep_onewg_max" (sweeps of more than 8 levels of systems with n * bs up
- * to this run in one workgroup; default 131072, 0: never), "batch_ell" (1 / 0: ELL or CSR
- * sweeps for batches created afterwards).
+ * built / 0: CSR), "sweep_onewg_max" (a sweep of more than 8 levels, each at most 1024 / bs
+ * rows, runs in one workgroup when n * bs is at most this; default 4194304, 0: never),
+ * "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards).
  * Unknown names and out-of-range values return BSP_E_INVALID. */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
 

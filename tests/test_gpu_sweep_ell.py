@@ -17,7 +17,7 @@ def _level_launches(ctx):
     sweeps level by level so that the ELL copy is the one read."""
     ctx.set_option("sweep_onewg_max", 0)
     yield
-    ctx.set_option("sweep_onewg_max", 131072)
+    ctx.set_option("sweep_onewg_max", 1 << 22)
 
 
 def _apply(ctx, M, r, ell):

@@ -16,7 +16,7 @@ def _with(ctx, onewg, fn):
     try:
         return fn()
     finally:
-        ctx.set_option("sweep_onewg_max", 131072)
+        ctx.set_option("sweep_onewg_max", 1 << 22)
 
 
 PRECONDS = {
