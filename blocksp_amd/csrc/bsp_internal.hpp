@@ -87,6 +87,7 @@ struct Sweep {
     std::vector<int> ell_kl;
     std::vector<int64_t> ell_off;
     DevMem ecol, eval;
+    DevMem dell_kl, dell_off;    // ell_kl / ell_off on the device (one-workgroup sweeps)
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
     // level-launch sweeps replayed from captured HIP graphs, keyed by their operands

@@ -293,12 +293,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep_syncfree(SweepArgs a, int nslo
 constexpr int kOneBlock = 1024;
 
 template <int BS, typename VT, typename MT, bool SOR>
-__global__ __launch_bounds__(kOneBlock) void k_sweep_onewg(SweepArgs a, const int* __restrict__ level_ptr, int l0,
-                                                           int l1) {
+__global__ __launch_bounds__(kOneBlock) void k_sweep_onewg(SweepArgs a, const int* __restrict__ level_ptr,
+                                                           const int* __restrict__ ell_kl,
+                                                           const int64_t* __restrict__ ell_off, int l0, int l1) {
     constexpr int SPP = (kOneBlock / kWave) * (kWave / BS);   // slots per pass of the workgroup
     for (int l = l0; l < l1; ++l) {
         const int s0 = level_ptr[l], s1 = level_ptr[l + 1];
-        for (int sb = s0; sb < s1; sb += SPP) sweep_chunk<BS, VT, MT, false, SOR>(a, sb, s1);
+        SweepArgs b = a;
+        if (a.ecol) {   // the level's width and the offset of its entries in the ELL copy
+            b.ell_k = ell_kl[l];
+            b.ell_base = ell_off[l] - (int64_t)s0 * b.ell_k;
+        }
+        for (int sb = s0; sb < s1; sb += SPP) sweep_chunk<BS, VT, MT, false, SOR>(b, sb, s1);
         __syncthreads();
     }
 }
@@ -314,9 +320,10 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
     // level launches: measured 1.6x slower on the 256^2 x bs 8 DG matrix in one workgroup)
     if (ctx->trsv_mode == 0 && S.nlevels > kDirectLevels && S.max_level_rows <= kOneBlock / BS &&
         (int64_t)S.n * S.bs <= ctx->sweep_onewg_max && S.dlevel_ptr.p) {
-        a.ecol = nullptr;   // the CSR blocks (the ELL copy's level widths are per launch)
+        if (!S.dell_kl.p) a.ecol = nullptr;
         hipLaunchKernelGGL((k_sweep_onewg<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream, a,
-                           S.dlevel_ptr.as<int>(), l0, l1 < 0 ? S.nlevels : l1);
+                           S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0,
+                           l1 < 0 ? S.nlevels : l1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
         return BSP_OK;
@@ -485,6 +492,10 @@ int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp) {
     if (S.bf16) BSP_HIP(gather_levels<uint16_t>(ctx, S, S.ecol.as<int>(), S.eval.as<uint16_t>()));
     else if (S.f32) BSP_HIP(gather_levels<float>(ctx, S, S.ecol.as<int>(), S.eval.as<float>()));
     else BSP_HIP(gather_levels<double>(ctx, S, S.ecol.as<int>(), S.eval.as<double>()));
+    BSP_HIP(S.dell_kl.alloc(sizeof(int) * kl.size()));
+    BSP_HIP(S.dell_off.alloc(sizeof(int64_t) * S.ell_off.size()));
+    BSP_HIP(hipMemcpy(S.dell_kl.p, kl.data(), sizeof(int) * kl.size(), hipMemcpyHostToDevice));
+    BSP_HIP(hipMemcpy(S.dell_off.p, S.ell_off.data(), sizeof(int64_t) * S.ell_off.size(), hipMemcpyHostToDevice));
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     S.ell_k = K;
     S.ell_uniform = std::all_of(kl.begin(), kl.end(), [&](int k) { return k == K; });
