@@ -11,8 +11,11 @@
 //    from xn on the dependency side (forward: j < i) and from xo elsewhere
 //    (blocksp_preconditioners.hpp:135-202).
 //
-// Two launch modes:
-//  * level launches: one kernel per level (kernel boundaries order the levels);
+// Launch modes:
+//  * level launches: one kernel per level (kernel boundaries order the levels), replayed
+//    from a captured HIP graph when there are many levels;
+//  * one workgroup: a sweep of many narrow levels (each within one pass of a 1024-thread
+//    workgroup) in a single workgroup, with a barrier between levels;
 //  * single launch ("sync-free"): a persistent grid takes chunks of the
 //    schedule in order from a device ticket counter, so every row a chunk waits
 //    on belongs to a chunk that is already held by a running workgroup.  A row
@@ -21,6 +24,9 @@
 //    fence followed by relaxed flag stores (flag = epoch of this launch).  Every
 //    wait is bounded (spin_limit polls, or immediately once any wait has failed
 //    and set *err), so every wavefront always exits.
+// Triangular solves of many levels read a slot-ordered ELL copy of their blocks when it is
+// built (columns and blocks indexed by schedule slot, each level as wide as its widest row), so
+// a lane's loads do not wait for the row number; the copy keeps each row's CSR order.
 #include "bsp_internal.hpp"
 
 #include <algorithm>
