@@ -128,6 +128,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "sweep_onewg_max") {
         BSP_CHECK(value >= 0, BSP_E_INVALID, "sweep_onewg_max must be >= 0");
         ctx->sweep_onewg_max = value;
+    } else if (k == "sweep_ahead") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "sweep_ahead must be 0 or 1");
+        ctx->sweep_ahead = (int)value;
     } else if (k == "batch_ell") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "batch_ell must be 0 or 1");
         ctx->batch_ell = (int)value;

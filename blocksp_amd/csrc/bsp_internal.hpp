@@ -75,6 +75,7 @@ struct Sweep {
     int nslots = 0;
     int nlevels = 0;
     int max_level_rows = 0;      // rows of the widest level (unpadded)
+    int max_level_slots = 0;     // slots of the widest level (padded)
     int fuse_boundary = -1;      // ILU0 lower sweep: its last level is the upper sweep's first
                                  // (1 yes, 0 no, -1 not yet checked)
     // slot-ordered ELL copy of the blocks (triangular solves; built where padding is small):
@@ -87,6 +88,10 @@ struct Sweep {
     std::vector<int> ell_kl;
     std::vector<int64_t> ell_off;
     DevMem ecol, eval;
+    // SOR sweeps: ecol = column on the dependency side, -2 - column on the other (-1 padding),
+    // epos = the block's position in the matrix; sor_forward: the direction encoded (-1 none)
+    DevMem epos;
+    int sor_forward = -1;
     DevMem dell_kl, dell_off;    // ell_kl / ell_off on the device (one-workgroup sweeps)
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
@@ -124,6 +129,9 @@ struct bsp_ctx {
                                      // levels whose widest level fits one pass of a 1024-thread
                                      // workgroup (1024 / bs rows) runs in one workgroup when n * bs is
                                      // at most this (0: never)
+    int sweep_ahead = 1;             // one-workgroup sweeps over an ELL copy (blocks up to 4 x 4,
+                                     // levels within one pass) load each level's static operands
+                                     // before the previous level's barrier
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +
@@ -241,6 +249,12 @@ int launch_ilu0_sweeps(bsp_ctx* ctx, Sweep& lower, Sweep& upper, const void* din
 // than 8 levels, blocks are at most 8 x 8, every row has at most 4 blocks and the padding stays
 // within a quarter of the blocks; otherwise the sweep keeps reading CSR.
 int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp);
+// The SOR form of the copy (after the sweep's schedule and its drp / dci / dpos are set; rp, ci:
+// the matrix's host structure): each slot's off-diagonal blocks as columns tagged with the side
+// they are read from and positions in the matrix, for sweeps in direction `forward`, under the
+// same rules counted over off-diagonal blocks, for blocks up to 4 x 4.
+int build_sor_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                  int forward);
 // SOR sweep: xn_i = xo_i + (-w xo_i + w Dinv_i (b_i - sum_{j!=i} A_ij x_j)), x_j from xn on the
 // dependency side (forward: j < i) and from xo otherwise.
 int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* dinv, const void* b,

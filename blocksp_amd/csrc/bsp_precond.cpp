@@ -55,6 +55,8 @@ void build_schedule_topo(Sweep& S, int n, int bs, const std::vector<int32_t>& rp
     S.max_level_rows = nlev ? *std::max_element(cnt.begin(), cnt.end()) : 0;
     S.level_ptr.assign(nlev + 1, 0);
     for (int l = 0; l < nlev; ++l) S.level_ptr[l + 1] = S.level_ptr[l] + (cnt[l] + rpw - 1) / rpw * rpw;
+    S.max_level_slots = 0;
+    for (int l = 0; l < nlev; ++l) S.max_level_slots = std::max(S.max_level_slots, S.level_ptr[l + 1] - S.level_ptr[l]);
     S.nslots = S.level_ptr[nlev];
     std::vector<int> order(S.nslots, -1), fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
     for (int i = 0; i < n; ++i) order[fill[S.level_of[i]]++] = i;
@@ -286,6 +288,9 @@ int bsp_precond_create_sor_ex(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, 
         S->nnzb = A->nnzb;
         S->dpos = colour ? M->pos.as<int>() : nullptr;
     }
+    rc = build_sor_ell(ctx, M->lower, A->h_rp, A->h_ci, 1);
+    if (!rc) rc = build_sor_ell(ctx, M->upper, A->h_rp, A->h_ci, 0);
+    if (rc) return rc;
     const size_t len = (size_t)M->n * M->bs;
     BSP_HIP(M->work0.alloc(sizeof(double) * std::max<size_t>(len, 1)));
     BSP_HIP(M->work1.alloc(sizeof(double) * std::max<size_t>(len, 1)));

@@ -26,7 +26,9 @@
 //    and set *err), so every wavefront always exits.
 // Triangular solves of many levels read a slot-ordered ELL copy of their blocks when it is
 // built (columns and blocks indexed by schedule slot, each level as wide as its widest row), so
-// a lane's loads do not wait for the row number; the copy keeps each row's CSR order.
+// a lane's loads do not wait for the row number; the copy keeps each row's CSR order.  SOR
+// sweeps of many levels read one of columns and block positions of the off-diagonal blocks
+// (the matrix values stay where they are, so a change of values takes effect at once).
 #include "bsp_internal.hpp"
 
 #include <algorithm>
@@ -147,14 +149,17 @@ struct SweepArgs {
     int64_t spin_limit;
     uint32_t* err;
     const int* pos;     // SOR in an elimination order: step of each row (nullptr: natural order)
-    const int* ecol;    // trsv: slot-ordered ELL copy (nullptr: CSR); slot s's entry k at
+    const int* ecol;    // slot-ordered ELL copy (nullptr: CSR); slot s's entry k at
     const void* eval;   // ell_base + s * ell_k + k (per level launch, or uniform)
+    const int* epos;    // SOR: the entry's block position in the matrix (its values stay live)
     int ell_k;
     int64_t ell_base;
 };
 
 constexpr int kEllMax = 4;     // entries per slot of the ELL copies
 constexpr int kEllMaxBs = 8;   // block sizes with an ELL form (a slot's blocks stay in registers)
+constexpr int kSorEllMaxBs = 4;   // the SOR form also holds the old iterate's rows: larger
+                                  // blocks spill in the 1024-thread one-workgroup sweeps
 
 // One chunk of slots: [slot_begin, slot_end) with kWavesPerBlock * RPW slots per block.
 // Factor values and diagonal inverses are stored as MT, vectors and arithmetic are VT.
@@ -186,30 +191,76 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     VT s = VT(0), xold = VT(0);
-    if (active) {
-        if constexpr (SOR) {
+    if constexpr (SOR && BS <= kSorEllMaxBs) {
+        if (a.ecol) {
+            // ELL copy of the off-diagonal blocks, each slot's in its row's CSR order, indexed
+            // by slot so they load without waiting for the row number: ecol = column on the
+            // dependency side, -2 - column on the other (read from the previous iterate), -1
+            // padding; epos = the block's position in the matrix, whose values are read live
             const VT* xo = static_cast<const VT*>(a.xo);
             const VT* b = static_cast<const VT*>(a.b);
-            xold = xo ? xo[(size_t)i * BS + r] : VT(0);
-            VT ph = VT(0);
-            for (int p = p0; p < p1; ++p) {
-                const int c = a.ci[p];
-                if (c == i) continue;
-                const bool dep = a.pos ? (a.forward ? a.pos[c] < a.pos[i] : a.pos[c] > a.pos[i])
-                                       : (a.forward ? c < i : c > i);
-                VT av[BS], xv[BS];
-                load_mrow<BS, MT, VT>(val + ((size_t)p * BS + r) * BS, av);
-                if (dep) {
-                    load_row<BS, VT>(x + (size_t)c * BS, xv);
+            const int K = a.ell_k;
+            int cq[kEllMax], pq[kEllMax];
+#pragma unroll
+            for (int k = 0; k < kEllMax; ++k) {
+                cq[k] = -1;
+                pq[k] = 0;
+                if (k < K && have) {
+                    const int64_t e = a.ell_base + (int64_t)slot * K + k;
+                    cq[k] = a.ecol[e];
+                    pq[k] = a.epos[e];
+                }
+            }
+            VT av[kEllMax][BS], xv[kEllMax][BS];
+#pragma unroll
+            for (int k = 0; k < kEllMax; ++k) {
+                if (cq[k] == -1) continue;
+                load_mrow<BS, MT, VT>(val + ((size_t)pq[k] * BS + r) * BS, av[k]);
+                if (cq[k] >= 0) {
+                    load_row<BS, VT>(x + (size_t)cq[k] * BS, xv[k]);
                 } else if (xo) {
-                    load_row<BS, VT>(xo + (size_t)c * BS, xv);
+                    load_row<BS, VT>(xo + (size_t)(-2 - cq[k]) * BS, xv[k]);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < BS; ++k) xv[k] = VT(0);
+                    for (int c = 0; c < BS; ++c) xv[k][c] = VT(0);
                 }
-                ph += row_dot<BS, VT>(av, xv);
             }
-            s = b[(size_t)i * BS + r] - ph;
+            VT ph = VT(0);
+#pragma unroll
+            for (int k = 0; k < kEllMax; ++k)
+                if (cq[k] != -1) ph += row_dot<BS, VT>(av[k], xv[k]);
+            if (active) {
+                xold = xo ? xo[(size_t)i * BS + r] : VT(0);
+                s = b[(size_t)i * BS + r] - ph;
+            }
+        }
+    }
+    if (active) {
+        if constexpr (SOR) {
+            if (!a.ecol) {
+                const VT* xo = static_cast<const VT*>(a.xo);
+                const VT* b = static_cast<const VT*>(a.b);
+                xold = xo ? xo[(size_t)i * BS + r] : VT(0);
+                VT ph = VT(0);
+                for (int p = p0; p < p1; ++p) {
+                    const int c = a.ci[p];
+                    if (c == i) continue;
+                    const bool dep = a.pos ? (a.forward ? a.pos[c] < a.pos[i] : a.pos[c] > a.pos[i])
+                                           : (a.forward ? c < i : c > i);
+                    VT av[BS], xv[BS];
+                    load_mrow<BS, MT, VT>(val + ((size_t)p * BS + r) * BS, av);
+                    if (dep) {
+                        load_row<BS, VT>(x + (size_t)c * BS, xv);
+                    } else if (xo) {
+                        load_row<BS, VT>(xo + (size_t)c * BS, xv);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < BS; ++k) xv[k] = VT(0);
+                    }
+                    ph += row_dot<BS, VT>(av, xv);
+                }
+                s = b[(size_t)i * BS + r] - ph;
+            }
         } else if (!a.ecol) {
             s = x[(size_t)i * BS + r];
             for (int p = p0; p < p1; ++p) {
@@ -315,6 +366,139 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg(SweepArgs a, const in
     }
 }
 
+// The one-workgroup sweep over an ELL copy when every level fits in one pass of the workgroup
+// (one slot per lane group): after a level's stores each lane loads what its slot of the next
+// level needs that no row of the sweep writes -- the row number, the columns, the blocks and
+// the diagonal inverse; the row's own entry (triangular solves), or the right-hand side and
+// the previous iterate's rows (SOR) -- so those loads overlap the stores and the barrier, and
+// after the barrier only the dependency columns' new values are read.  The arithmetic is
+// sweep_chunk's, in the same order, so results are bitwise the same.
+template <int BS, typename VT>
+struct EllRow {
+    int i;
+    int cq[kEllMax];
+    VT av[kEllMax][BS];
+    VT xv[kEllMax][BS];   // SOR: rows of the previous iterate (columns on the other side)
+    VT d[BS];
+    VT s;                 // triangular solve: x_i; SOR: b_i
+    VT xold;
+};
+
+template <int BS, typename VT, typename MT, bool SOR>
+__device__ __forceinline__ void ell_row_load(const SweepArgs& a, int slot, bool have, int K, int64_t base,
+                                             EllRow<BS, VT>& q) {
+    const int r = (threadIdx.x & (kWave - 1)) % BS;
+    const MT* __restrict__ val = static_cast<const MT*>(SOR ? a.val : a.eval);
+    const MT* __restrict__ dinv = static_cast<const MT*>(a.dinv);
+    const VT* xo = static_cast<const VT*>(a.xo);
+    q.i = have ? a.order[slot] : -1;
+    int pq[kEllMax];
+#pragma unroll
+    for (int k = 0; k < kEllMax; ++k) {
+        q.cq[k] = -1;
+        pq[k] = 0;
+        if (k < K && have) {
+            const int64_t e = base + (int64_t)slot * K + k;
+            q.cq[k] = a.ecol[e];
+            if constexpr (SOR) pq[k] = a.epos[e];
+            else load_mrow<BS, MT, VT>(val + ((size_t)e * BS + r) * BS, q.av[k]);
+        }
+    }
+    if constexpr (SOR) {
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k) {
+            if (q.cq[k] == -1) continue;
+            load_mrow<BS, MT, VT>(val + ((size_t)pq[k] * BS + r) * BS, q.av[k]);
+            if (q.cq[k] < 0) {
+                if (xo) {
+                    load_row<BS, VT>(xo + (size_t)(-2 - q.cq[k]) * BS, q.xv[k]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) q.xv[k][c] = VT(0);
+                }
+            }
+        }
+    }
+    q.s = VT(0);
+    q.xold = VT(0);
+    if (q.i >= 0) {
+        if (dinv) load_mrow<BS, MT, VT>(dinv + ((size_t)q.i * BS + r) * BS, q.d);
+        if constexpr (SOR) {
+            q.s = static_cast<const VT*>(a.b)[(size_t)q.i * BS + r];
+            q.xold = xo ? xo[(size_t)q.i * BS + r] : VT(0);
+        } else {
+            q.s = static_cast<const VT*>(a.x)[(size_t)q.i * BS + r];
+        }
+    }
+}
+
+template <int BS, typename VT, bool SOR>
+__device__ __forceinline__ void ell_row_finish(const SweepArgs& a, EllRow<BS, VT>& q, bool have_dinv) {
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    VT* x = static_cast<VT*>(a.x);
+    const bool active = q.i >= 0;
+#pragma unroll
+    for (int k = 0; k < kEllMax; ++k)
+        if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
+    VT s;
+    if constexpr (SOR) {
+        VT ph = VT(0);
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] != -1) ph += row_dot<BS, VT>(q.av[k], q.xv[k]);
+        s = active ? q.s - ph : VT(0);
+    } else {
+        s = q.s;
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] >= 0) s -= row_dot<BS, VT>(q.av[k], q.xv[k]);
+    }
+    VT out = s;
+    if (have_dinv) {
+        const int base = (g < RPW ? g : 0) * BS;
+        out = VT(0);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const VT sc = __shfl(s, base + c, kWave);
+            if (active) out = fma(q.d[c], sc, out);
+        }
+    }
+    if constexpr (SOR) {
+        const VT w = (VT)a.w;
+        out = q.xold + (-w * q.xold + w * out);
+    }
+    if (active) x[(size_t)q.i * BS + r] = out;
+}
+
+template <int BS, typename VT, typename MT, bool SOR>
+__global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, const int* __restrict__ level_ptr,
+                                                               const int* __restrict__ ell_kl,
+                                                               const int64_t* __restrict__ ell_off, int l0, int l1) {
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = lane / BS;
+    const int off = (int)(threadIdx.x / kWave) * RPW + g;   // the lane group's slot within a level
+    const bool grp = g < RPW;
+    const bool have_dinv = a.dinv != nullptr;
+    if (l0 >= l1) return;
+    EllRow<BS, VT> q;
+    int s0 = level_ptr[l0], s1 = level_ptr[l0 + 1], K = ell_kl[l0];
+    ell_row_load<BS, VT, MT, SOR>(a, s0 + off, grp && s0 + off < s1, K, ell_off[l0] - (int64_t)s0 * K, q);
+    for (int l = l0; l < l1; ++l) {
+        ell_row_finish<BS, VT, SOR>(a, q, have_dinv);
+        if (l + 1 < l1) {
+            s0 = s1;
+            s1 = level_ptr[l + 2];
+            K = ell_kl[l + 1];
+            ell_row_load<BS, VT, MT, SOR>(a, s0 + off, grp && s0 + off < s1, K, ell_off[l + 1] - (int64_t)s0 * K, q);
+        }
+        __syncthreads();
+    }
+}
+
 // Levels [l0, l1) of the sweep (l1 < 0: to the last); a partial range is only asked for on
 // the direct level-launch path (few levels, level-launch mode) or for a one-workgroup sweep.
 template <int BS, typename VT, typename MT, bool SOR>
@@ -327,9 +511,20 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
     if (ctx->trsv_mode == 0 && S.nlevels > kDirectLevels && S.max_level_rows <= kOneBlock / BS &&
         (int64_t)S.n * S.bs <= ctx->sweep_onewg_max && S.dlevel_ptr.p) {
         if (!S.dell_kl.p) a.ecol = nullptr;
-        hipLaunchKernelGGL((k_sweep_onewg<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream, a,
-                           S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0,
-                           l1 < 0 ? S.nlevels : l1);
+        constexpr int SPP = (kOneBlock / kWave) * (kWave / BS);
+        bool ahead = false;
+        if constexpr (BS <= kSorEllMaxBs) {
+            if (a.ecol && S.max_level_slots <= SPP && ctx->sweep_ahead) {
+                ahead = true;
+                hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream,
+                                   a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0,
+                                   l1 < 0 ? S.nlevels : l1);
+            }
+        }
+        if (!ahead)
+            hipLaunchKernelGGL((k_sweep_onewg<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream, a,
+                               S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0,
+                               l1 < 0 ? S.nlevels : l1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
         return BSP_OK;
@@ -428,6 +623,7 @@ SweepArgs make_args(bsp_ctx* ctx, Sweep& S, uint32_t epoch) {
     a.pos = S.dpos;
     a.ecol = S.ell_k > 0 && ctx->sweep_ell ? S.ecol.as<int>() : nullptr;
     a.eval = S.eval.p;
+    a.epos = S.epos.as<int>();
     a.ell_k = S.ell_k;   // uniform width (single launch); level launches set their level's
     a.ell_base = 0;
     return a;
@@ -466,7 +662,92 @@ hipError_t gather_levels(bsp_ctx* ctx, const Sweep& S, int* ecol, MT* eval) {
     return hipGetLastError();
 }
 
+// SOR copy of the levels' slots [s0, s0 + nslots) of width K, entries from e0: thread t covers
+// entry k of slot s0 + q, the k-th off-diagonal block of the slot's row in CSR order.
+__global__ void k_sor_ell_gather(int64_t nslots, int K, int s0, int64_t e0, const int* __restrict__ order,
+                                 const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ pos,
+                                 int forward, int* __restrict__ ecol, int* __restrict__ epos) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nslots * K) return;
+    const int64_t q = t / K;
+    const int k = (int)(t - q * K);
+    const int row = order[s0 + q];
+    int col = -1, at = 0;
+    if (row >= 0) {
+        int seen = 0;
+        for (int p = rp[row]; p < rp[row + 1]; ++p) {
+            const int c = ci[p];
+            if (c == row) continue;
+            if (seen++ == k) {
+                const bool dep = pos ? (forward ? pos[c] < pos[row] : pos[c] > pos[row])
+                                     : (forward ? c < row : c > row);
+                col = dep ? c : -2 - c;
+                at = p;
+                break;
+            }
+        }
+    }
+    ecol[e0 + q * K + k] = col;
+    epos[e0 + q * K + k] = at;
+}
+
+// Level widths kl -> entry offsets, the device copies of both, and the entry count.
+hipError_t set_ell_levels(Sweep& S, const std::vector<int>& kl, int64_t& entries) {
+    S.ell_kl = kl;
+    S.ell_off.assign(S.nlevels + 1, 0);
+    for (int l = 0; l < S.nlevels; ++l)
+        S.ell_off[l + 1] = S.ell_off[l] + (int64_t)(S.level_ptr[l + 1] - S.level_ptr[l]) * kl[l];
+    entries = std::max<int64_t>(S.ell_off[S.nlevels], 1);
+    hipError_t e = S.dell_kl.alloc(sizeof(int) * kl.size());
+    if (e == hipSuccess) e = S.dell_off.alloc(sizeof(int64_t) * S.ell_off.size());
+    if (e == hipSuccess) e = hipMemcpy(S.dell_kl.p, kl.data(), sizeof(int) * kl.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(S.dell_off.p, S.ell_off.data(), sizeof(int64_t) * S.ell_off.size(), hipMemcpyHostToDevice);
+    return e;
+}
+
 }  // namespace
+
+int build_sor_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci,
+                  int forward) {
+    S.ell_k = 0;
+    S.ell_uniform = false;
+    S.sor_forward = -1;
+    S.ell_kl.clear();
+    S.ell_off.clear();
+    // the triangular solves' rules: many levels, little padding
+    if (S.n <= 0 || S.nslots <= 0 || S.bs > kSorEllMaxBs || S.nlevels <= kDirectLevels ||
+        (int)rp.size() != S.n + 1 || (int)S.level_of.size() != S.n)
+        return BSP_OK;
+    std::vector<int> kl(S.nlevels, 0), cnt(S.n, 0);
+    int64_t real = 0, padded = 0;
+    for (int i = 0; i < S.n; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) cnt[i] += ci[p] != i;
+        kl[S.level_of[i]] = std::max(kl[S.level_of[i]], cnt[i]);
+        real += cnt[i];
+    }
+    for (int i = 0; i < S.n; ++i) padded += kl[S.level_of[i]];
+    const int K = *std::max_element(kl.begin(), kl.end());
+    if (K == 0 || K > kEllMax || padded * 4 > real * 5) return BSP_OK;
+    int64_t entries = 0;
+    BSP_HIP(set_ell_levels(S, kl, entries));
+    BSP_HIP(S.ecol.alloc(sizeof(int) * (size_t)entries));
+    BSP_HIP(S.epos.alloc(sizeof(int) * (size_t)entries));
+    for (int l = 0; l < S.nlevels; ++l) {
+        const int s0 = S.level_ptr[l], n = S.level_ptr[l + 1] - s0;
+        const int64_t total = (int64_t)n * kl[l];
+        if (total <= 0) continue;
+        hipLaunchKernelGGL(k_sor_ell_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (int64_t)n, kl[l], s0, S.ell_off[l], S.order.as<int>(), S.drp, S.dci, S.dpos, forward,
+                           S.ecol.as<int>(), S.epos.as<int>());
+    }
+    BSP_HIP(hipGetLastError());
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    S.ell_k = K;
+    S.ell_uniform = std::all_of(kl.begin(), kl.end(), [&](int k) { return k == K; });
+    S.sor_forward = forward;
+    return BSP_OK;
+}
 
 int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp) {
     S.ell_k = 0;
@@ -487,21 +768,14 @@ int build_sweep_ell(bsp_ctx* ctx, Sweep& S, const std::vector<int32_t>& rp) {
     for (int l = 0; l < S.nlevels; ++l) K = std::max(K, kl[l]);
     const int64_t nnz = rp[S.n];
     if (K == 0 || K > kEllMax || padded * 4 > nnz * 5) return BSP_OK;   // padding <= 1/4
-    S.ell_kl = kl;
-    S.ell_off.assign(S.nlevels + 1, 0);
-    for (int l = 0; l < S.nlevels; ++l)
-        S.ell_off[l + 1] = S.ell_off[l] + (int64_t)(S.level_ptr[l + 1] - S.level_ptr[l]) * kl[l];
-    const int64_t entries = std::max<int64_t>(S.ell_off[S.nlevels], 1);
+    int64_t entries = 0;
+    BSP_HIP(set_ell_levels(S, kl, entries));
     const size_t esz = S.bf16 ? sizeof(uint16_t) : S.f32 ? sizeof(float) : sizeof(double);
     BSP_HIP(S.ecol.alloc(sizeof(int) * (size_t)entries));
     BSP_HIP(S.eval.alloc(esz * (size_t)entries * S.bs * S.bs));
     if (S.bf16) BSP_HIP(gather_levels<uint16_t>(ctx, S, S.ecol.as<int>(), S.eval.as<uint16_t>()));
     else if (S.f32) BSP_HIP(gather_levels<float>(ctx, S, S.ecol.as<int>(), S.eval.as<float>()));
     else BSP_HIP(gather_levels<double>(ctx, S, S.ecol.as<int>(), S.eval.as<double>()));
-    BSP_HIP(S.dell_kl.alloc(sizeof(int) * kl.size()));
-    BSP_HIP(S.dell_off.alloc(sizeof(int64_t) * S.ell_off.size()));
-    BSP_HIP(hipMemcpy(S.dell_kl.p, kl.data(), sizeof(int) * kl.size(), hipMemcpyHostToDevice));
-    BSP_HIP(hipMemcpy(S.dell_off.p, S.ell_off.data(), sizeof(int64_t) * S.ell_off.size(), hipMemcpyHostToDevice));
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     S.ell_k = K;
     S.ell_uniform = std::all_of(kl.begin(), kl.end(), [&](int k) { return k == K; });
@@ -565,7 +839,7 @@ int launch_sor_sweep(bsp_ctx* ctx, Sweep& S, int forward, double w, const void* 
     a.xo = xo;
     a.w = w;
     a.forward = forward;
-    a.ecol = nullptr;
+    if (!a.epos || forward != S.sor_forward) a.ecol = nullptr;   // the copy encodes one direction
     if (S.bf16) return fail(ctx, BSP_E_UNSUPPORTED, "SOR sweeps have no BF16 form");
     return S.f32 ? dispatch<float, float, true>(ctx, S, a) : dispatch<double, double, true>(ctx, S, a);
 }

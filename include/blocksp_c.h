@@ -62,6 +62,8 @@ int bsp_ctx_sync(bsp_ctx* ctx);
  * Gram-Schmidt), "fgmres_basis_f32", "sweep_ell" (1: sweeps read their ELL copy where
  * built / 0: CSR), "sweep_onewg_max" (a sweep of more than 8 levels, each at most 1024 / bs
  * rows, runs in one workgroup when n * bs is at most this; default 4194304, 0: never),
+ * "sweep_ahead" (1: such a sweep over an ELL copy, blocks up to 4 x 4, loads each level's
+ * operands that the sweep does not write before the previous level's barrier / 0: after it),
  * "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards).
  * Unknown names and out-of-range values return BSP_E_INVALID. */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
@@ -206,7 +208,9 @@ int bsp_precond_levels(const bsp_precond* M, int lower, int32_t* level_host, int
 /* Widest level of the forward (lower=1) or backward (lower=0) sweep's slot-ordered ELL copy
  * (1-4 entries per slot; each level as wide as its widest row), or 0 when the sweep reads the
  * CSR factor (8 levels or fewer, blocks above 8 x 8, rows with more than 4 blocks, or padding
- * above a quarter of the blocks).  No reference counterpart: the
+ * above a quarter of the blocks).  SOR sweeps count their off-diagonal blocks and have the copy
+ * for blocks up to 4 x 4; it holds columns and block positions only, so values set on the matrix
+ * after the preconditioner was made still take effect.  No reference counterpart: the
  * copy shortens each level's chain of dependent loads and gives bitwise the same results;
  * the context option "sweep_ell" = 0 makes applies read CSR. */
 int bsp_precond_ell_width(const bsp_precond* M, int lower);

@@ -71,3 +71,68 @@ def test_onewg_solves_equal_level_launches_bitwise(ctx):
     assert res[0][0] == res[1][0] and res[0][2] == res[1][2]
     np.testing.assert_array_equal(res[0][1], res[1][1])
     np.testing.assert_array_equal(res[0][3], res[1][3])
+
+
+def _blocklap(B, n, seed):
+    return pr.batch_blocklap(nsys=1, lo=n, hi=n, seed=seed, B=B, nrhs=1)[0]
+
+
+AHEAD_PRECONDS = {
+    "ilu0": lambda ctx, A: bsp.ILU0(ctx, A),
+    "ilu0_f32": lambda ctx, A: bsp.ILU0Ex(ctx, A, precision=bsp.PREC_F32),
+    "ilu0_bf16": lambda ctx, A: bsp.ILU0Ex(ctx, A, precision=bsp.PREC_BF16),
+    "ichol0": lambda ctx, A: bsp.IChol0(ctx, A),
+    "sor_sym": lambda ctx, A: bsp.SOR(ctx, A, 1.2, 3, 0),
+    "sor_fwd": lambda ctx, A: bsp.SOR(ctx, A, 0.9, 1, 1),
+    "sor_bwd": lambda ctx, A: bsp.SOR(ctx, A, 1.1, 2, -1),
+}
+AHEAD_PROBLEMS = {
+    "b1": lambda: _blocklap(1, 3000, 11),
+    "b2": lambda: pr.stencil2d_b2(44, 40, seed=4),
+    "b3": lambda: _blocklap(3, 1800, 12),
+    "b4": lambda: _blocklap(4, 2200, 13),
+}
+
+
+@pytest.mark.parametrize("size", sorted(AHEAD_PROBLEMS))
+@pytest.mark.parametrize("kind", sorted(AHEAD_PRECONDS))
+def test_onewg_loads_ahead_bitwise(ctx, kind, size):
+    """One-workgroup sweeps over an ELL copy that load each level's operands before the previous
+    level's barrier (sweep_ahead = 1) against the same sweeps loading them after it."""
+    print(f"onewg ahead {kind} {size}", flush=True)
+    p = AHEAD_PROBLEMS[size]()
+    A = bsp.BSR.from_problem(ctx, p)
+    M = AHEAD_PRECONDS[kind](ctx, A)
+    assert M.ell_width(True) > 0 and M.ell_width(False) > 0
+    r = np.random.default_rng(10).standard_normal(p.n * p.bs)
+    out = []
+    for ahead in (1, 0):
+        ctx.set_option("sweep_ahead", ahead)
+        try:
+            rd = ctx.to_device(r)
+            M(rd)
+            out.append(rd.numpy())
+        finally:
+            ctx.set_option("sweep_ahead", 1)
+    np.testing.assert_array_equal(out[0], out[1])
+    assert np.isfinite(out[0]).all() and np.linalg.norm(out[0]) > 0
+
+
+def test_onewg_level_wider_than_one_pass(ctx):
+    """bs 3 on a 340 x 340 grid: levels of up to 340 rows (within 1024 / 3) pad to 357 slots,
+    more than one pass of the workgroup holds (16 wavefronts x 21 rows), so the sweep takes the
+    multi-pass one-workgroup kernel; it must still agree with level launches."""
+    p = _blocklap(3, 340 * 340, 14)
+    assert p.meta["dims"] == (340, 340)
+    A = bsp.BSR.from_problem(ctx, p)
+    r = np.random.default_rng(11).standard_normal(p.n * p.bs)
+    for make in (lambda: bsp.ILU0(ctx, A), lambda: bsp.SOR(ctx, A, 1.2, 1, 0)):
+        M = make()
+        out = []
+        for onewg in (0, 1 << 22):
+            def run():
+                rd = ctx.to_device(r)
+                M(rd)
+                return rd.numpy()
+            out.append(_with(ctx, onewg, run))
+        np.testing.assert_array_equal(out[0], out[1])
