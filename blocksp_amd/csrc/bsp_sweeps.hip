@@ -367,12 +367,21 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg(SweepArgs a, const in
 }
 
 // The one-workgroup sweep over an ELL copy when every level fits in one pass of the workgroup
-// (one slot per lane group): after a level's stores each lane loads what its slot of the next
-// level needs that no row of the sweep writes -- the row number, the columns, the blocks and
-// the diagonal inverse; the row's own entry (triangular solves), or the right-hand side and
-// the previous iterate's rows (SOR) -- so those loads overlap the stores and the barrier, and
-// after the barrier only the dependency columns' new values are read.  The arithmetic is
-// sweep_chunk's, in the same order, so results are bitwise the same.
+// (one slot per lane group; launched with as many wavefronts as the widest level needs, at most
+// 1024 threads), with the loads of what no row of the sweep writes pipelined ahead
+// of the levels: each slot's row number and entries (columns, and block positions or ELL entry
+// indices) two levels ahead, and from them -- right after a level's stores, so they overlap the
+// stores and the barrier -- the next level's blocks, diagonal inverse, and the row's own entry
+// (triangular solves) or the right-hand side and the previous iterate's rows (SOR).  After the
+// barrier only the dependency columns' new values are read.  The arithmetic is sweep_chunk's,
+// in the same order, so results are bitwise the same.
+struct EllIdx {
+    bool have;         // the lane group holds a slot of the level
+    int i;
+    int cq[kEllMax];
+    int pq[kEllMax];   // SOR: block positions in the matrix; triangular solves: ELL entry indices
+};
+
 template <int BS, typename VT>
 struct EllRow {
     int i;
@@ -384,31 +393,40 @@ struct EllRow {
     VT xold;
 };
 
+// Lanes without a slot read the level's first slot (every level holds a row there) and are
+// masked by `have` when the operands are loaded, so the loads need no per-lane branch.
+template <bool SOR>
+__device__ __forceinline__ void ell_idx_load(const SweepArgs& a, int slot, int first, bool have, int K,
+                                             int64_t base, EllIdx& t) {
+    const int sl = have ? slot : first;
+    t.have = have;
+    t.i = a.order[sl];
+#pragma unroll
+    for (int k = 0; k < kEllMax; ++k) {
+        if (k < K) {
+            const int64_t e = base + (int64_t)sl * K + k;
+            t.cq[k] = a.ecol[e];
+            t.pq[k] = SOR ? a.epos[e] : (int)e;
+        } else {
+            t.cq[k] = -1;
+            t.pq[k] = 0;
+        }
+    }
+}
+
 template <int BS, typename VT, typename MT, bool SOR>
-__device__ __forceinline__ void ell_row_load(const SweepArgs& a, int slot, bool have, int K, int64_t base,
-                                             EllRow<BS, VT>& q) {
+__device__ __forceinline__ void ell_row_load(const SweepArgs& a, const EllIdx& t, EllRow<BS, VT>& q) {
     const int r = (threadIdx.x & (kWave - 1)) % BS;
     const MT* __restrict__ val = static_cast<const MT*>(SOR ? a.val : a.eval);
     const MT* __restrict__ dinv = static_cast<const MT*>(a.dinv);
     const VT* xo = static_cast<const VT*>(a.xo);
-    q.i = have ? a.order[slot] : -1;
-    int pq[kEllMax];
+    q.i = t.have ? t.i : -1;
 #pragma unroll
     for (int k = 0; k < kEllMax; ++k) {
-        q.cq[k] = -1;
-        pq[k] = 0;
-        if (k < K && have) {
-            const int64_t e = base + (int64_t)slot * K + k;
-            q.cq[k] = a.ecol[e];
-            if constexpr (SOR) pq[k] = a.epos[e];
-            else load_mrow<BS, MT, VT>(val + ((size_t)e * BS + r) * BS, q.av[k]);
-        }
-    }
-    if constexpr (SOR) {
-#pragma unroll
-        for (int k = 0; k < kEllMax; ++k) {
-            if (q.cq[k] == -1) continue;
-            load_mrow<BS, MT, VT>(val + ((size_t)pq[k] * BS + r) * BS, q.av[k]);
+        q.cq[k] = t.have ? t.cq[k] : -1;
+        if (q.cq[k] == -1) continue;
+        load_mrow<BS, MT, VT>(val + ((size_t)t.pq[k] * BS + r) * BS, q.av[k]);
+        if constexpr (SOR) {
             if (q.cq[k] < 0) {
                 if (xo) {
                     load_row<BS, VT>(xo + (size_t)(-2 - q.cq[k]) * BS, q.xv[k]);
@@ -440,17 +458,39 @@ __device__ __forceinline__ void ell_row_finish(const SweepArgs& a, EllRow<BS, VT
     const int r = lane - g * BS;
     VT* x = static_cast<VT*>(a.x);
     const bool active = q.i >= 0;
-#pragma unroll
-    for (int k = 0; k < kEllMax; ++k)
-        if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
     VT s;
-    if constexpr (SOR) {
+    if constexpr (SOR && BS * sizeof(VT) > 24) {
+        // (4 x 4 FP64 blocks: the registers of the form below spill at 1024 threads)
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
         VT ph = VT(0);
 #pragma unroll
         for (int k = 0; k < kEllMax; ++k)
             if (q.cq[k] != -1) ph += row_dot<BS, VT>(q.av[k], q.xv[k]);
         s = active ? q.s - ph : VT(0);
+    } else if constexpr (SOR) {
+        // the previous iterate's products first (their rows were loaded a level ago), which
+        // frees their registers, so the new values' loads all issue together; the products are
+        // then added in the row's order
+        VT rd[kEllMax];
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k) rd[k] = q.cq[k] < -1 ? row_dot<BS, VT>(q.av[k], q.xv[k]) : VT(0);
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] >= 0) rd[k] = row_dot<BS, VT>(q.av[k], q.xv[k]);
+        VT ph = VT(0);
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] != -1) ph += rd[k];
+        s = active ? q.s - ph : VT(0);
     } else {
+#pragma unroll
+        for (int k = 0; k < kEllMax; ++k)
+            if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
         s = q.s;
 #pragma unroll
         for (int k = 0; k < kEllMax; ++k)
@@ -484,16 +524,27 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, cons
     const bool grp = g < RPW;
     const bool have_dinv = a.dinv != nullptr;
     if (l0 >= l1) return;
+    EllIdx t;
     EllRow<BS, VT> q;
     int s0 = level_ptr[l0], s1 = level_ptr[l0 + 1], K = ell_kl[l0];
-    ell_row_load<BS, VT, MT, SOR>(a, s0 + off, grp && s0 + off < s1, K, ell_off[l0] - (int64_t)s0 * K, q);
+    ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l0] - (int64_t)s0 * K, t);
+    ell_row_load<BS, VT, MT, SOR>(a, t, q);
+    if (l0 + 1 < l1) {   // (s0, s1): the slots of the level whose indices t holds
+        s0 = s1;
+        s1 = level_ptr[l0 + 2];
+        K = ell_kl[l0 + 1];
+        ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l0 + 1] - (int64_t)s0 * K, t);
+    }
     for (int l = l0; l < l1; ++l) {
         ell_row_finish<BS, VT, SOR>(a, q, have_dinv);
         if (l + 1 < l1) {
-            s0 = s1;
-            s1 = level_ptr[l + 2];
-            K = ell_kl[l + 1];
-            ell_row_load<BS, VT, MT, SOR>(a, s0 + off, grp && s0 + off < s1, K, ell_off[l + 1] - (int64_t)s0 * K, q);
+            ell_row_load<BS, VT, MT, SOR>(a, t, q);
+            if (l + 2 < l1) {
+                s0 = s1;
+                s1 = level_ptr[l + 3];
+                K = ell_kl[l + 2];
+                ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l + 2] - (int64_t)s0 * K, t);
+            }
         }
         __syncthreads();
     }
@@ -514,9 +565,12 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
         constexpr int SPP = (kOneBlock / kWave) * (kWave / BS);
         bool ahead = false;
         if constexpr (BS <= kSorEllMaxBs) {
-            if (a.ecol && S.max_level_slots <= SPP && ctx->sweep_ahead) {
+            // (ELL entry indices are held as int)
+            if (a.ecol && S.max_level_slots <= SPP && ctx->sweep_ahead && S.ell_off.back() < (int64_t(1) << 31)) {
                 ahead = true;
-                hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR>), dim3(1), dim3(kOneBlock), 0, ctx->stream,
+                // as many wavefronts as the widest level needs (idle ones would only add loads)
+                const int waves = std::max(1, (S.max_level_slots + kWave / BS - 1) / (kWave / BS));
+                hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR>), dim3(1), dim3(waves * kWave), 0, ctx->stream,
                                    a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0,
                                    l1 < 0 ? S.nlevels : l1);
             }
