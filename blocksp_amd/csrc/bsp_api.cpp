@@ -45,7 +45,10 @@ int bsp_ctx_create(int device, void* stream, bsp_ctx** out) {
     auto* ctx = new bsp_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->num_cus = prop.multiProcessorCount;
+        ctx->lds_per_block = (int64_t)prop.sharedMemPerBlock;
+    }
     if (stream) {
         ctx->stream = static_cast<hipStream_t>(stream);
     } else {
@@ -128,6 +131,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "sweep_onewg_max") {
         BSP_CHECK(value >= 0, BSP_E_INVALID, "sweep_onewg_max must be >= 0");
         ctx->sweep_onewg_max = value;
+    } else if (k == "sweep_lds") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "sweep_lds must be 0 or 1");
+        ctx->sweep_lds = (int)value;
     } else if (k == "sweep_ahead") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "sweep_ahead must be 0 or 1");
         ctx->sweep_ahead = (int)value;

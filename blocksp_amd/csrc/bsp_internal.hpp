@@ -118,6 +118,7 @@ struct bsp_ctx {
     bool own_stream = false;
     mutable std::string last_error;
     int num_cus = 256;
+    int64_t lds_per_block = 0;       // device limit of LDS bytes per workgroup
     // options
     int trsv_mode = 0;               // 0 level launches (default: measured faster), 1 sync-free single launch
     int64_t spin_limit = int64_t(1) << 22;
@@ -132,6 +133,8 @@ struct bsp_ctx {
     int sweep_ahead = 1;             // one-workgroup sweeps over an ELL copy (blocks up to 4 x 4,
                                      // levels within one pass) load each level's static operands
                                      // before the previous level's barrier
+    int sweep_lds = 1;               // such sweeps keep the solution vector in LDS when it fits
+                                     // (at most 64 KiB and the device's limit per workgroup)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +

@@ -450,13 +450,14 @@ __device__ __forceinline__ void ell_row_load(const SweepArgs& a, const EllIdx& t
     }
 }
 
-template <int BS, typename VT, bool SOR>
-__device__ __forceinline__ void ell_row_finish(const SweepArgs& a, EllRow<BS, VT>& q, bool have_dinv) {
+template <int BS, typename VT, bool SOR, bool LX>
+__device__ __forceinline__ void ell_row_finish(const SweepArgs& a, EllRow<BS, VT>& q, bool have_dinv, VT* xs) {
     constexpr int RPW = kWave / BS;
     const int lane = threadIdx.x & (kWave - 1);
     const int g = lane / BS;
     const int r = lane - g * BS;
-    VT* x = static_cast<VT*>(a.x);
+    VT* xg = static_cast<VT*>(a.x);
+    VT* x = LX ? xs : xg;   // where the dependency columns' new values are read
     const bool active = q.i >= 0;
     VT s;
     if constexpr (SOR && BS * sizeof(VT) > 24) {
@@ -510,13 +511,28 @@ __device__ __forceinline__ void ell_row_finish(const SweepArgs& a, EllRow<BS, VT
         const VT w = (VT)a.w;
         out = q.xold + (-w * q.xold + w * out);
     }
-    if (active) x[(size_t)q.i * BS + r] = out;
+    if (active) {
+        xg[(size_t)q.i * BS + r] = out;
+        if constexpr (LX) xs[(size_t)q.i * BS + r] = out;
+    }
 }
 
-template <int BS, typename VT, typename MT, bool SOR>
+template <int BS, typename VT, typename MT, bool SOR, bool LX>
 __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, const int* __restrict__ level_ptr,
                                                                const int* __restrict__ ell_kl,
-                                                               const int64_t* __restrict__ ell_off, int l0, int l1) {
+                                                               const int64_t* __restrict__ ell_off, int l0, int l1,
+                                                               int nx) {
+    // LX: the solution vector's rows written by this sweep are also kept in LDS (nx entries), and
+    // dependency columns are read from there; rows of levels before l0 are copied in first
+    extern __shared__ __align__(16) unsigned char sweep_lds[];
+    VT* xs = reinterpret_cast<VT*>(sweep_lds);
+    if constexpr (LX) {
+        if (l0 > 0) {
+            const VT* xg = static_cast<const VT*>(a.x);
+            for (int t = threadIdx.x; t < nx; t += blockDim.x) xs[t] = xg[t];
+            __syncthreads();
+        }
+    }
     constexpr int RPW = kWave / BS;
     const int lane = threadIdx.x & (kWave - 1);
     const int g = lane / BS;
@@ -536,7 +552,7 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, cons
         ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l0 + 1] - (int64_t)s0 * K, t);
     }
     for (int l = l0; l < l1; ++l) {
-        ell_row_finish<BS, VT, SOR>(a, q, have_dinv);
+        ell_row_finish<BS, VT, SOR, LX>(a, q, have_dinv, xs);
         if (l + 1 < l1) {
             ell_row_load<BS, VT, MT, SOR>(a, t, q);
             if (l + 2 < l1) {
@@ -570,9 +586,17 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
                 ahead = true;
                 // as many wavefronts as the widest level needs (idle ones would only add loads)
                 const int waves = std::max(1, (S.max_level_slots + kWave / BS - 1) / (kWave / BS));
-                hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR>), dim3(1), dim3(waves * kWave), 0, ctx->stream,
-                                   a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0,
-                                   l1 < 0 ? S.nlevels : l1);
+                const int nx = S.n * BS;
+                const int64_t lds = (int64_t)nx * sizeof(VT);
+                const int lend = l1 < 0 ? S.nlevels : l1;
+                if (ctx->sweep_lds && lds <= std::min<int64_t>(ctx->lds_per_block, 65536))
+                    hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR, true>), dim3(1), dim3(waves * kWave),
+                                       (size_t)lds, ctx->stream, a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(),
+                                       S.dell_off.as<int64_t>(), l0, lend, nx);
+                else
+                    hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR, false>), dim3(1), dim3(waves * kWave), 0,
+                                       ctx->stream, a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(),
+                                       S.dell_off.as<int64_t>(), l0, lend, nx);
             }
         }
         if (!ahead)

@@ -64,6 +64,8 @@ int bsp_ctx_sync(bsp_ctx* ctx);
  * rows, runs in one workgroup when n * bs is at most this; default 4194304, 0: never),
  * "sweep_ahead" (1: such a sweep over an ELL copy, blocks up to 4 x 4, loads each level's
  * operands that the sweep does not write before the previous level's barrier / 0: after it),
+ * "sweep_lds" (1: such a sweep keeps the solution vector in LDS when it fits, at most 64 KiB /
+ * 0: reads it from global memory),
  * "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards).
  * Unknown names and out-of-range values return BSP_E_INVALID. */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);

@@ -98,7 +98,8 @@ AHEAD_PROBLEMS = {
 @pytest.mark.parametrize("kind", sorted(AHEAD_PRECONDS))
 def test_onewg_loads_ahead_bitwise(ctx, kind, size):
     """One-workgroup sweeps over an ELL copy that load each level's operands before the previous
-    level's barrier (sweep_ahead = 1) against the same sweeps loading them after it."""
+    level's barrier (sweep_ahead = 1), with the solution vector in LDS where it fits
+    (sweep_lds = 1) or not, against the same sweeps loading the operands after the barrier."""
     print(f"onewg ahead {kind} {size}", flush=True)
     p = AHEAD_PROBLEMS[size]()
     A = bsp.BSR.from_problem(ctx, p)
@@ -106,16 +107,44 @@ def test_onewg_loads_ahead_bitwise(ctx, kind, size):
     assert M.ell_width(True) > 0 and M.ell_width(False) > 0
     r = np.random.default_rng(10).standard_normal(p.n * p.bs)
     out = []
-    for ahead in (1, 0):
+    for ahead, lds in ((1, 1), (1, 0), (0, 0)):
         ctx.set_option("sweep_ahead", ahead)
+        ctx.set_option("sweep_lds", lds)
         try:
             rd = ctx.to_device(r)
             M(rd)
             out.append(rd.numpy())
         finally:
             ctx.set_option("sweep_ahead", 1)
-    np.testing.assert_array_equal(out[0], out[1])
+            ctx.set_option("sweep_lds", 1)
+    np.testing.assert_array_equal(out[0], out[2])
+    np.testing.assert_array_equal(out[1], out[2])
     assert np.isfinite(out[0]).all() and np.linalg.norm(out[0]) > 0
+
+
+@pytest.mark.parametrize("lds", [1, 0])
+def test_onewg_partial_level_ranges(ctx, lds):
+    """Without sweep graphs, ILU0 applies launch its sweeps over level ranges (the forward
+    sweep's first level and the shared boundary level are split off), so one-workgroup sweeps
+    start past level 0 and read rows of earlier launches: against level launches, bitwise."""
+    p = pr.stencil2d_b2(40, 36, seed=6)
+    A = bsp.BSR.from_problem(ctx, p)
+    r = np.random.default_rng(12).standard_normal(p.n * p.bs)
+    ctx.set_option("sweep_graphs", 0)
+    ctx.set_option("sweep_lds", lds)
+    try:
+        M = bsp.ILU0(ctx, A)
+        out = []
+        for onewg in (0, 1 << 22):
+            def run():
+                rd = ctx.to_device(r)
+                M(rd)
+                return rd.numpy()
+            out.append(_with(ctx, onewg, run))
+        np.testing.assert_array_equal(out[0], out[1])
+    finally:
+        ctx.set_option("sweep_graphs", 1)
+        ctx.set_option("sweep_lds", 1)
 
 
 def test_onewg_level_wider_than_one_pass(ctx):
