@@ -15,7 +15,8 @@
 //  * level launches: one kernel per level (kernel boundaries order the levels), replayed
 //    from a captured HIP graph when there are many levels;
 //  * one workgroup: a sweep of many narrow levels (each within one pass of a 1024-thread
-//    workgroup) in a single workgroup, with a barrier between levels;
+//    workgroup) in a single workgroup, with a barrier between levels; over an ELL copy of blocks
+//    up to 4 x 4 the operands of later levels are loaded ahead and the vector may sit in LDS;
 //  * single launch ("sync-free"): a persistent grid takes chunks of the
 //    schedule in order from a device ticket counter, so every row a chunk waits
 //    on belongs to a chunk that is already held by a running workgroup.  A row
