@@ -131,6 +131,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "sweep_onewg_max") {
         BSP_CHECK(value >= 0, BSP_E_INVALID, "sweep_onewg_max must be >= 0");
         ctx->sweep_onewg_max = value;
+    } else if (k == "mgs_onewg") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "mgs_onewg must be 0 or 1");
+        ctx->mgs_onewg = (int)value;
     } else if (k == "sweep_lds") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "sweep_lds must be 0 or 1");
         ctx->sweep_lds = (int)value;

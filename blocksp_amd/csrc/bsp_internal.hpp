@@ -133,6 +133,8 @@ struct bsp_ctx {
     int sweep_ahead = 1;             // one-workgroup sweeps over an ELL copy (blocks up to 4 x 4,
                                      // levels within one pass) load each level's static operands
                                      // before the previous level's barrier
+    int mgs_onewg = 1;               // Arnoldi steps on vectors of at most 8192 entries: all the
+                                     // Gram-Schmidt steps in one single-workgroup kernel
     int sweep_lds = 1;               // such sweeps keep the solution vector in LDS when it fits
                                      // (at most 64 KiB and the device's limit per workgroup)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps

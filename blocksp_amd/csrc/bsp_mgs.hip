@@ -5,7 +5,8 @@
 //     if s > 0: h_{s-1} = sum of the previous step's block partials;  w -= h_{s-1} q_{s-1}
 //     partial_s[block] = sum over the block's elements of q_s w   (s <= j)   or   w w   (s = j+1)
 //   scale kernel: h_{j+1} = sqrt(|sum partial_{j+1}|);  q_{j+1} = w / h_{j+1}
-// Every block of a kernel sums the previous partials itself, in one fixed order
+// Short vectors (at most 4 partials) run all steps in one kernel of one workgroup, with the
+// same arithmetic.  Every block of a kernel sums the previous partials itself, in one fixed order
 // (thread t takes partials t, t+256, ..., then a fixed tree), so all blocks use the
 // bitwise-identical h, and block 0 records it in hcol for the host.  The kernel
 // boundary orders the partials; no device-scope fence or atomic is needed.
@@ -111,6 +112,113 @@ __global__ __launch_bounds__(kBlock) void k_mgs_scale(int64_t len, const double*
         q[i] = v[i] / h;
 }
 
+// One-workgroup form for short vectors (at most 4 partials): thread group g (256 threads) does
+// what block g of k_mgs2 / k_mgs_scale does, over all steps of the Arnoldi step in one kernel,
+// the partials passing through LDS; every group re-sums them in k_mgs2's order, so h and the
+// vectors are bitwise those of the launch per step.  Each thread rewrites only elements it owns.
+constexpr int kOneGroups = 4;
+constexpr int kOneMaxQ = 64;
+
+struct QList {
+    const double* q[kOneMaxQ];
+};
+
+__device__ __forceinline__ double group_sum(double v, double* sh, int tid) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((tid & (kWave - 1)) == 0) sh[tid / kWave] = v;
+    __syncthreads();
+    const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg(int64_t len, double* __restrict__ v, QList Q,
+                                                                   int j, int nparts, double* __restrict__ hcol,
+                                                                   double* __restrict__ qnew) {
+    __shared__ double sh[kOneGroups][kBlock / kWave];
+    __shared__ double part[2][kOneGroups];
+    const int grp = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
+    auto sum_parts = [&](const double* pin) {
+        double s = 0.0;
+        for (int b = tid; b < nparts; b += kBlock) s += pin[b];
+        return group_sum(s, sh[grp], tid);
+    };
+    for (int st = 0; st <= j + 1; ++st) {
+        const double* qprev = st ? Q.q[st - 1] : nullptr;
+        const double* qnext = st <= j ? Q.q[st] : nullptr;
+        double mh = 0.0;
+        if (qprev) {
+            const double h = sum_parts(part[(st + 1) & 1]);
+            if (grp == 0 && tid == 0) hcol[st - 1] = h;
+            mh = -h;
+        }
+        double s0 = 0.0, s1 = 0.0;
+        for (int64_t base = (int64_t)grp * kTile; base < len; base += (int64_t)nparts * kTile) {
+            if (VEC && base + kTile <= len) {
+                double2* v2 = reinterpret_cast<double2*>(v + base);
+                double2 a0 = v2[tid], a1 = v2[tid + kBlock];
+                double2 p0{0.0, 0.0}, p1{0.0, 0.0}, q0{0.0, 0.0}, q1{0.0, 0.0};
+                if (qprev) {
+                    const double2* p2 = reinterpret_cast<const double2*>(qprev + base);
+                    p0 = p2[tid];
+                    p1 = p2[tid + kBlock];
+                }
+                if (qnext) {
+                    const double2* q2 = reinterpret_cast<const double2*>(qnext + base);
+                    q0 = q2[tid];
+                    q1 = q2[tid + kBlock];
+                }
+                if (qprev) {
+                    a0.x = a0.x + mh * p0.x;
+                    a0.y = a0.y + mh * p0.y;
+                    a1.x = a1.x + mh * p1.x;
+                    a1.y = a1.y + mh * p1.y;
+                    v2[tid] = a0;
+                    v2[tid + kBlock] = a1;
+                }
+                if (qnext) {
+                    s0 = fma(q0.x, a0.x, s0);
+                    s1 = fma(q0.y, a0.y, s1);
+                    s0 = fma(q1.x, a1.x, s0);
+                    s1 = fma(q1.y, a1.y, s1);
+                } else {
+                    s0 = fma(a0.x, a0.x, s0);
+                    s1 = fma(a0.y, a0.y, s1);
+                    s0 = fma(a1.x, a1.x, s0);
+                    s1 = fma(a1.y, a1.y, s1);
+                }
+            } else {
+                for (int64_t i = base + tid; i < len && i < base + kTile; i += kBlock) {
+                    double vi = v[i];
+                    if (qprev) {
+                        vi = vi + mh * qprev[i];
+                        v[i] = vi;
+                    }
+                    s0 = qnext ? fma(qnext[i], vi, s0) : fma(vi, vi, s0);
+                }
+            }
+        }
+        const double ps = group_sum(s0 + s1, sh[grp], tid);
+        if (tid == 0) part[st & 1][grp] = ps;
+        __syncthreads();
+    }
+    const double h = sqrt(fabs(sum_parts(part[(j + 1) & 1])));
+    if (grp == 0 && tid == 0) hcol[j + 1] = h;
+    for (int64_t base = (int64_t)grp * kTile; base < len; base += (int64_t)nparts * kTile) {
+        if (VEC && base + kTile <= len) {
+            for (int e = 0; e < 2; ++e) {
+                const int64_t i = base + 2 * (tid + e * kBlock);
+                qnew[i] = v[i] / h;
+                qnew[i + 1] = v[i + 1] / h;
+            }
+        } else {
+            for (int64_t i = base + tid; i < len && i < base + kTile; i += kBlock) qnew[i] = v[i] / h;
+        }
+    }
+}
+
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 }  // namespace
@@ -126,6 +234,19 @@ int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double*
     double* P[2] = {parts, parts + kMaxParts};
     bool vec = aligned16(w);
     for (int s = 0; s <= j; ++s) vec = vec && aligned16(Q[s]);
+    if (ctx->mgs_onewg && nb <= kOneGroups && j + 1 <= kOneMaxQ) {
+        QList ql{};
+        for (int s = 0; s <= j; ++s) ql.q[s] = Q[s];
+        if (vec)
+            hipLaunchKernelGGL(k_mgs_onewg<true>, dim3(1), dim3(nb * kBlock), 0, ctx->stream, len, w, ql, j, nb, hcol,
+                               qnew);
+        else
+            hipLaunchKernelGGL(k_mgs_onewg<false>, dim3(1), dim3(nb * kBlock), 0, ctx->stream, len, w, ql, j, nb, hcol,
+                               qnew);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("mgs launch: ") + hipGetErrorString(e));
+        return BSP_OK;
+    }
     for (int s = 0; s <= j + 1; ++s) {
         const double* qprev = s ? Q[s - 1] : nullptr;
         const double* qnext = s <= j ? Q[s] : nullptr;

@@ -65,7 +65,9 @@ int bsp_ctx_sync(bsp_ctx* ctx);
  * "sweep_ahead" (1: such a sweep over an ELL copy, blocks up to 4 x 4, loads each level's
  * operands that the sweep does not write before the previous level's barrier / 0: after it),
  * "sweep_lds" (1: such a sweep keeps the solution vector in LDS when it fits, at most 64 KiB /
- * 0: reads it from global memory),
+ * 0: reads it from global memory), "mgs_onewg" (1: the modified Gram-Schmidt of an Arnoldi step
+ * on at most 8192 entries and 64 basis vectors runs in one single-workgroup kernel / 0: one
+ * launch per step; bitwise the same),
  * "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards).
  * Unknown names and out-of-range values return BSP_E_INVALID. */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
