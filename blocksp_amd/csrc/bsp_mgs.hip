@@ -219,6 +219,102 @@ __global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg(int64_t len, 
     }
 }
 
+// The same for vectors of whole tiles (at most 2 per group): each thread keeps its elements of w
+// and of the previous basis vector in registers across the steps and loads the next basis vector
+// one step ahead, so a step's loads overlap the previous step's reductions; w is written back
+// once at the end.  Same operations in the same order as k_mgs_onewg.
+struct Tile2 {
+    double2 h0, h1;   // elements 2 tid, 2 tid + 1 and 512 + 2 tid, 513 + 2 tid of the tile
+};
+
+__global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg_reg(int64_t len, double* __restrict__ v, QList Q,
+                                                                       int j, int nparts, double* __restrict__ hcol,
+                                                                       double* __restrict__ qnew) {
+    __shared__ double sh[kOneGroups][kBlock / kWave];
+    __shared__ double part[2][kOneGroups];
+    const int grp = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
+    int64_t base[2];
+    bool own[2];
+    Tile2 a[2], prev[2], cur[2], nxt[2];
+    auto ld = [&](const double* q, int tt) {
+        const double2* q2 = reinterpret_cast<const double2*>(q + base[tt]);
+        return Tile2{q2[tid], q2[tid + kBlock]};
+    };
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+        base[tt] = ((int64_t)grp + (int64_t)tt * nparts) * kTile;
+        own[tt] = base[tt] < len;
+        if (own[tt]) {
+            a[tt] = ld(v, tt);
+            cur[tt] = ld(Q.q[0], tt);
+        }
+    }
+    auto sum_parts = [&](const double* pin) {
+        double s = 0.0;
+        for (int b = tid; b < nparts; b += kBlock) s += pin[b];
+        return group_sum(s, sh[grp], tid);
+    };
+    for (int st = 0; st <= j + 1; ++st) {
+        if (st + 1 <= j) {
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+                if (own[tt]) nxt[tt] = ld(Q.q[st + 1], tt);
+        }
+        double mh = 0.0;
+        if (st > 0) {
+            const double h = sum_parts(part[(st + 1) & 1]);
+            if (grp == 0 && tid == 0) hcol[st - 1] = h;
+            mh = -h;
+        }
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+            if (!own[tt]) continue;
+            double2 a0 = a[tt].h0, a1 = a[tt].h1;
+            if (st > 0) {
+                a0.x = a0.x + mh * prev[tt].h0.x;
+                a0.y = a0.y + mh * prev[tt].h0.y;
+                a1.x = a1.x + mh * prev[tt].h1.x;
+                a1.y = a1.y + mh * prev[tt].h1.y;
+                a[tt].h0 = a0;
+                a[tt].h1 = a1;
+            }
+            if (st <= j) {
+                const double2 q0 = cur[tt].h0, q1 = cur[tt].h1;
+                s0 = fma(q0.x, a0.x, s0);
+                s1 = fma(q0.y, a0.y, s1);
+                s0 = fma(q1.x, a1.x, s0);
+                s1 = fma(q1.y, a1.y, s1);
+            } else {
+                s0 = fma(a0.x, a0.x, s0);
+                s1 = fma(a0.y, a0.y, s1);
+                s0 = fma(a1.x, a1.x, s0);
+                s1 = fma(a1.y, a1.y, s1);
+            }
+        }
+        const double ps = group_sum(s0 + s1, sh[grp], tid);
+        if (tid == 0) part[st & 1][grp] = ps;
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+            prev[tt] = cur[tt];
+            cur[tt] = nxt[tt];
+        }
+    }
+    const double h = sqrt(fabs(sum_parts(part[(j + 1) & 1])));
+    if (grp == 0 && tid == 0) hcol[j + 1] = h;
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+        if (!own[tt]) continue;
+        double2* v2 = reinterpret_cast<double2*>(v + base[tt]);
+        double2* q2 = reinterpret_cast<double2*>(qnew + base[tt]);
+        v2[tid] = a[tt].h0;
+        v2[tid + kBlock] = a[tt].h1;
+        q2[tid] = double2{a[tt].h0.x / h, a[tt].h0.y / h};
+        q2[tid + kBlock] = double2{a[tt].h1.x / h, a[tt].h1.y / h};
+    }
+}
+
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 }  // namespace
@@ -237,7 +333,11 @@ int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double*
     if (ctx->mgs_onewg && nb <= kOneGroups && j + 1 <= kOneMaxQ) {
         QList ql{};
         for (int s = 0; s <= j; ++s) ql.q[s] = Q[s];
-        if (vec)
+        // whole tiles, at most 2 per group (nb >= tiles / 2 by construction), 16-byte aligned qnew
+        if (vec && len % kTile == 0 && (len / kTile) <= 2 * nb && aligned16(qnew))
+            hipLaunchKernelGGL(k_mgs_onewg_reg, dim3(1), dim3(nb * kBlock), 0, ctx->stream, len, w, ql, j, nb, hcol,
+                               qnew);
+        else if (vec)
             hipLaunchKernelGGL(k_mgs_onewg<true>, dim3(1), dim3(nb * kBlock), 0, ctx->stream, len, w, ql, j, nb, hcol,
                                qnew);
         else

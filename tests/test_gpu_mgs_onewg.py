@@ -16,8 +16,10 @@ def _blocklap(B, n, seed):
     return pr.batch_blocklap(nsys=1, lo=n, hi=n, seed=seed, B=B, nrhs=1)[0]
 
 
-CASES = {   # len = n * bs: 8192 (4 partials, vector loads), 2880 (2 partials), 2997 (odd: scalar path)
+CASES = {   # len = n * bs: 8192 and 2048 (whole tiles: w kept in registers), 2880 (2 partials, last
+            # tile partial), 2997 (odd: misaligned basis vectors, scalar path)
     "stencil_64x64_b2": lambda: pr.stencil2d_b2(64, 64, seed=0),
+    "stencil_32x32_b2": lambda: pr.stencil2d_b2(32, 32, seed=2),
     "stencil_40x36_b2": lambda: pr.stencil2d_b2(40, 36, seed=1),
     "blocklap_b3_odd": lambda: _blocklap(3, 1000, 5),
 }
