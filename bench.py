@@ -75,32 +75,63 @@ def measure_bsmv(bsp, ctx, A, p, reps=50):
     return ms, bsmv_algorithmic_bytes(p.n, p.bs, p.nnzb)
 
 
-def cpu_baseline(p, gpu_iters, sample_iters=12):
-    """Oracle ILU0-pGMRES(30,k=10) on 1 thread: setup + a bounded number of iterations on
-    the full matrix, scaled to the plain GPU run's iteration count."""
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(p, gpu_iters, sample_iters=60):
+    """Oracle ILU0-pGMRES(30,k=10) built with the reference's release flags (-O3 -funroll-loops
+    -fstrict-aliasing -fomit-frame-pointer -ffast-math), pinned to one core: setup + a bounded
+    number of iterations on the full matrix, scaled to the plain GPU run's iteration count;
+    bsmv is the median of 20 runs after 3 warm-ups."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    t0 = time.perf_counter()
-    po.ilu0_factor(p)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, st = po.pgmres(p, p.b, 1e-30, po.PRE_ILU0, maxIters=sample_iters - 1, restart=30, k=10)
-    t_run = time.perf_counter() - t0
-    per_iter = max(t_run - t_setup, 1e-9) / max(st.iters, 1)   # pgmres rebuilds the factor
-    x = np.random.default_rng(0).standard_normal(p.n * p.bs)
-    t0 = time.perf_counter()
-    reps = 3
-    for _ in range(reps):
-        po.bsmv(p, x)
-    t_bsmv = (time.perf_counter() - t0) / reps
+    model, ncpu, nallowed = cpu_info()
+    before = os.sched_getaffinity(0)
+    core = min(before)
+    os.sched_setaffinity(0, {core})
+    po.use_fast_build(True)
+    try:
+        t0 = time.perf_counter()
+        po.ilu0_factor(p)
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, st = po.pgmres(p, p.b, 1e-30, po.PRE_ILU0, maxIters=sample_iters - 1, restart=30, k=10)
+        t_run = time.perf_counter() - t0
+        per_iter = max(t_run - t_setup, 1e-9) / max(st.iters, 1)   # pgmres rebuilds the factor
+        x = np.random.default_rng(0).standard_normal(p.n * p.bs)
+        for _ in range(3):
+            po.bsmv(p, x)
+        ts = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            po.bsmv(p, x)
+            ts.append(time.perf_counter() - t0)
+        t_bsmv = float(np.median(ts))
+    finally:
+        po.use_fast_build(False)
+        os.sched_setaffinity(0, before)
     return {
         "value": round(t_setup + per_iter * gpu_iters, 3),
         "unit": "s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle ILU0 factor + {st.iters} ILU0-pGMRES(30,k=10) iterations on the full matrix; "
+        "sample": f"oracle (reference release flags, pinned to core {core}) ILU0 factor + {st.iters} "
+                  f"ILU0-pGMRES(30,k=10) iterations on the full matrix ({t_setup + t_run:.1f} s of CPU work); "
                   f"time/iteration x {gpu_iters} (the plain GPU solve's iterations) + setup",
+        "cpu_model": model,
+        "nproc": ncpu,
+        "cores_allowed": nallowed,
         "setup_s": round(t_setup, 3),
         "s_per_iter": round(per_iter, 4),
         "bsmv_ms": round(t_bsmv * 1e3, 3),
@@ -117,6 +148,22 @@ def timed_ms(ctx, fn, reps):
     ctx.record(1)
     ctx.sync()
     return ctx.elapsed_ms(0, 1) / reps
+
+
+def tight_reference(bsp, ctx, A, b, p, colour):
+    """FP64 reference solution for forward errors: all-FP64 flexible GMRES (FP64 parity-coloured
+    ILU0, FP64 SpMV and bases) to true relres 1e-13, its residual recomputed by the CPU oracle's
+    bsmv (parity build)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    M = bsp.ILU0Ex(ctx, A, colour=colour, precision=bsp.PREC_F64)
+    xr, st, rc = bsp.FGMRES_mixed(ctx, A, b, 1e-13, M, maxIters=40000, spmv_precision=bsp.PREC_F64)
+    xref = xr.numpy()
+    bh = b.numpy()
+    rel = float(np.linalg.norm(bh - po.bsmv(p, xref)) / np.linalg.norm(bh))
+    del M
+    return xref, f"all-FP64 coloured FGMRES to true relres {st.true_relres:.1e} (oracle bsmv: {rel:.1e})"
 
 
 def bench_bf16(bsp, pr, ctx, solve=True):
@@ -149,26 +196,32 @@ def bench_bf16(bsp, pr, ctx, solve=True):
     if not solve:
         return out
     b = ctx.to_device(p.b)
-    M0 = bsp.ILU0(ctx, A)
-    xr, st_r, _ = bsp.pGMRES(ctx, A, b, 1e-13, M0, maxIters=20000, restart=30, k=10, tol_mode=bsp.TOL_TRUE_RESIDUAL)
-    xref = xr.numpy()
-    fwd = lambda x: float(np.linalg.norm(x.numpy() - xref) / np.linalg.norm(xref))
-    x0, st0, _ = bsp.pGMRES(ctx, A, b, 1e-10, M0, maxIters=20000, restart=30, k=10, tol_mode=bsp.TOL_TRUE_RESIDUAL)
     col3 = pr.parity_colours((64, 64, 64))
+    xref, ref_desc = tight_reference(bsp, ctx, A, b, p, col3)
+    fwd = lambda x: float(np.linalg.norm(x.numpy() - xref) / np.linalg.norm(xref))
+    M0 = bsp.ILU0(ctx, A)
+    x0, st0, _ = bsp.pGMRES(ctx, A, b, 1e-10, M0, maxIters=20000, restart=30, k=10, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    del M0
+    M64 = bsp.ILU0Ex(ctx, A, colour=col3, precision=bsp.PREC_F64)
+    x3, st3, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M64, spmv_precision=bsp.PREC_F64)
+    del M64
     Mc = bsp.ILU0Ex(ctx, A, colour=col3, precision=bsp.PREC_F32)
     x1, st1, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mc)
     Mh = bsp.ILU0Ex(ctx, A, colour=col3, precision=bsp.PREC_BF16)
     x2, st2, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mh)
     out.update({"plain_fp64_ms": round(st0.solve_ms, 2), "plain_fp64_iters": st0.iters,
                 "plain_fp64_relres": st0.true_relres, "plain_fp64_fwd_err": fwd(x0),
+                "fp64_fgmres_coloured_ms": round(st3.solve_ms, 2), "fp64_fgmres_coloured_iters": st3.iters,
+                "fp64_fgmres_coloured_relres": st3.true_relres, "fp64_fgmres_coloured_fwd_err": fwd(x3),
                 "mixed_f32_ms": round(st1.solve_ms, 2), "mixed_f32_iters": st1.iters,
                 "mixed_f32_relres": st1.true_relres, "mixed_f32_fwd_err": fwd(x1),
                 "mixed_bf16_factor_ms": round(st2.solve_ms, 2), "mixed_bf16_factor_iters": st2.iters,
                 "mixed_bf16_factor_relres": st2.true_relres, "mixed_bf16_factor_fwd_err": fwd(x2),
                 "mixed_speedup": round(st0.solve_ms / min(st1.solve_ms, st2.solve_ms), 2),
-                "fwd_err_reference": f"FP64 ILU0-pGMRES to true relres {st_r.true_relres:.1e}"})
-    log(f"3D mixed {st1.solve_ms:.0f} ms (FP32 factor), {st2.solve_ms:.0f} ms (BF16 factor) vs plain FP64 "
-        f"{st0.solve_ms:.0f} ms")
+                "precision_gain_same_ordering": round(st3.solve_ms / min(st1.solve_ms, st2.solve_ms), 2),
+                "fwd_err_reference": ref_desc})
+    log(f"3D mixed {st1.solve_ms:.0f} ms (FP32 factor), {st2.solve_ms:.0f} ms (BF16 factor) vs all-FP64 coloured "
+        f"FGMRES {st3.solve_ms:.0f} ms vs plain FP64 {st0.solve_ms:.0f} ms")
     return out
 
 
@@ -471,9 +524,13 @@ def main():
     setup_mixed = time.perf_counter() - t0
     log(f"BF16 parity-coloured ILU0 setup {setup_mixed:.2f}s")
 
+    _x_last = [None]
+
     def mixed_step(M=Mh):
-        return bsp.FGMRES_mixed(ctx, A, b, 1e-10, M, maxIters=20000, restart=30, k=10,
-                                spmv_precision=bsp.PREC_F32, inner_reduction=1e-6)
+        x, st, rc = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M, maxIters=20000, restart=30, k=10,
+                                     spmv_precision=bsp.PREC_F32, inner_reduction=1e-6)
+        _x_last[0] = x
+        return x, st, rc
 
     for w in range(args.warmup):
         _, st, rc = mixed_step()
@@ -489,6 +546,19 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
     ok = all(s["true_relres"] <= 1e-10 for s in stats)
+    x_head = _x_last[0]
+    # forward error of the headline solution against a tight FP64 reference solve
+    xref, ref_desc = tight_reference(bsp, ctx, A, b, p, colour)
+    fwd_err = float(np.linalg.norm(x_head.numpy() - xref) / np.linalg.norm(xref))
+    log(f"headline forward error {fwd_err:.2e} ({ref_desc})")
+    # the same ordering in all-FP64 (FP64 factor, SpMV and bases): the precision gain alone
+    M64 = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F64)
+    x64, st64, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, M64, maxIters=20000, restart=30, k=10,
+                                    spmv_precision=bsp.PREC_F64)
+    fp64_coloured = {"time_s": round(st64.solve_ms / 1e3, 4), "iters": st64.iters, "true_relres": st64.true_relres,
+                     "fwd_err": float(np.linalg.norm(x64.numpy() - xref) / np.linalg.norm(xref))}
+    log(f"all-FP64 coloured FGMRES: iters={st64.iters} {st64.solve_ms:.1f} ms")
+    del M64, x64
     # the same solve with the factor stored in FP32 (one run, for comparison)
     M32 = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_F32)
     _, st32, _ = mixed_step(M32)
@@ -524,7 +594,7 @@ def main():
         log(f"plain FP64 ILU0-pGMRES: iters={stp.iters} relres={stp.true_relres:.2e} {t_plain:.2f} s")
         plain = {"time_s": round(t_plain, 4), "iters": stp.iters, "restarts": stp.restarts,
                  "true_relres": stp.true_relres, "converged": bool(stp.true_relres <= 1e-10),
-                 "setup_s": round(setup_plain, 3),
+                 "setup_s": round(setup_plain, 3), "setup_plus_solve_s": round(setup_plain + t_plain, 4),
                  "ilu0_apply_ms": {"syncfree": round(apply_ms[1], 4), "level_launch": round(apply_ms[0], 4)}}
 
     cpu = None
@@ -561,10 +631,17 @@ def main():
         "iters": stats[-1]["iters"],
         "restarts": stats[-1]["restarts"],
         "true_relres": stats[-1]["true_relres"],
+        "fwd_err": fwd_err,
+        "fwd_err_reference": ref_desc,
         "setup_s": round(setup_mixed, 3),
+        "setup_plus_solve_s": round(setup_mixed + value, 4),
         "mixed_with_f32_factor": f32_factor,
+        "fp64_fgmres_same_coloured_ilu0": fp64_coloured,
+        "precision_gain_same_ordering": round(fp64_coloured["time_s"] / value, 3),
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": round(plain["time_s"] / value, 3) if plain else None,
+        "speedup_vs_plain_fp64_with_setup": (round((plain["time_s"] + plain["setup_s"]) / (value + setup_mixed), 3)
+                                             if plain else None),
         "bsmv_GBps": round(bsmv_gbps, 1),
         "bsmm_AA": bsmm,
         "roofline": roofline,

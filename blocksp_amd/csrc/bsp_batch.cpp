@@ -27,10 +27,13 @@ struct bsp_batch {
     bsp_precond* M = nullptr;
     DevMem d_sys_row0, frows, flev, flev_base, brows, blev, blev_base;
     DevMem Bi, Xi, Rv, P, Z, AP, iters, res, work;
-    DevMem item_sys, item_rhs0;
+    // work items for 1, 2 and 4 right-hand sides per item ([log2 R]), built once per R and kept:
+    // captured graphs hold these device pointers
+    DevMem item_sys[3], item_rhs0[3];
+    int nitems_R[3] = {0, 0, 0};
     int ell_k = 0;                    // entries per slot of the ELL sweep copies (0: CSR sweeps)
     DevMem fcol, fval, fdinv, bcol, bval, bdinv;
-    int items_R = 0, nitems = 0;
+    int nitems = 0;
     int wg = 256, R = 1;
     bool tuned = false;
     struct Cached {
@@ -123,9 +126,16 @@ int build_ell(bsp_ctx* ctx, bsp_batch* b) {
     return BSP_OK;
 }
 
-// Work items (system, first RHS) for R right-hand sides per item, largest system first.
+int log2_rhs(int R) { return R == 4 ? 2 : R == 2 ? 1 : 0; }
+
+// Work items (system, first RHS) for R right-hand sides per item, largest system first.  Each
+// R's arrays are uploaded once and never reallocated, so graphs captured with them stay valid.
 int build_items(bsp_ctx* ctx, bsp_batch* b, int R) {
-    if (b->items_R == R) return BSP_OK;
+    const int l = log2_rhs(R);
+    if (b->nitems_R[l] > 0) {
+        b->nitems = b->nitems_R[l];
+        return BSP_OK;
+    }
     std::vector<int32_t> order(b->nsys);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -137,19 +147,20 @@ int build_items(bsp_ctx* ctx, bsp_batch* b, int R) {
             isys.push_back(s);
             ir0.push_back(k0);
         }
-    BSP_HIP(upload_i(b->item_sys, isys));
-    BSP_HIP(upload_i(b->item_rhs0, ir0));
-    b->nitems = (int)isys.size();
-    b->items_R = R;
+    BSP_HIP(upload_i(b->item_sys[l], isys));
+    BSP_HIP(upload_i(b->item_rhs0[l], ir0));
+    b->nitems_R[l] = (int)isys.size();
+    b->nitems = b->nitems_R[l];
     return BSP_OK;
 }
 
 batch::Args make_args(bsp_batch* b, double tol, int maxIters, int mode) {
     const bsp_precond* M = b->M;
     batch::Args a;
-    a.nitems = b->nitems;
-    a.item_sys = b->item_sys.as<int>();
-    a.item_rhs0 = b->item_rhs0.as<int>();
+    const int l = log2_rhs(b->R);
+    a.nitems = b->nitems_R[l];
+    a.item_sys = b->item_sys[l].as<int>();
+    a.item_rhs0 = b->item_rhs0[l].as<int>();
     a.sys_row0 = b->d_sys_row0.as<int>();
     a.rp = b->A->rp.as<int>();
     a.ci = b->A->ci.as<int>();

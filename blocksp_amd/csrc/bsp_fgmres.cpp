@@ -70,8 +70,10 @@ struct InnerStats {
 };
 
 // One inner FGMRES-DR pass on A e = r for the current x (x += e on return).  VT is the
-// storage type of the Arnoldi basis V (FP64, or FP32 with FP64 Gram-Schmidt arithmetic).
-template <typename VT>
+// storage type of the Arnoldi basis V (FP64, or FP32 with FP64 Gram-Schmidt arithmetic); ZT
+// that of the preconditioned basis Z (FP32, or FP64 when the preconditioner and the SpMV are
+// both FP64: the all-FP64 flexible GMRES).
+template <typename VT, typename ZT>
 int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double reduction,
                 int maxIters, int restart, int k, bool spmv_f32, InnerStats& st, double* r_work, double* t64) {
     const int n = A->n_rows, bs = A->bs;
@@ -79,8 +81,12 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     const double eps = std::numeric_limits<double>::epsilon();
     const int m_max = restart > 0 ? restart : maxIters + 1;
     constexpr bool vf32 = std::is_same_v<VT, float>;
+    constexpr bool z64 = std::is_same_v<ZT, double>;
+    if constexpr (z64) {
+        BSP_CHECK(!M->f32 && !spmv_f32, BSP_E_INVALID, "an FP64 Z basis needs an FP64 preconditioner and SpMV");
+    }
     Pool<VT> V;
-    Pool<float> Z;
+    Pool<ZT> Z;
     V.len = Z.len = (size_t)len;
     DevMem hcol, ydev, Pdev, Pkdev;
     BSP_HIP(hcol.alloc(sizeof(double) * 2 * (m_max + 2)));
@@ -117,7 +123,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     st.res = beta;
     if (!(beta > 0.0) || !std::isfinite(beta)) return std::isfinite(beta) ? BSP_OK : fail(ctx, BSP_E_NUMERIC, "residual not finite");
     std::vector<VT*> Vb;
-    std::vector<float*> Zb;
+    std::vector<ZT*> Zb;
     VT* v0 = nullptr;
     rc = V.take(ctx, v0);
     if (rc) return rc;
@@ -162,18 +168,28 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     // With an FP32 preconditioner the Gram-Schmidt step also writes the FP32 copy of V[col + 1]
     // (znext), which the next step preconditions in place instead of converting V[col + 1].
     struct Step {
-        float* z = nullptr;
-        float* znext = nullptr;
+        ZT* z = nullptr;
+        ZT* znext = nullptr;
         VT* vn = nullptr;
         int slot = 0;
     };
-    auto enqueue_step = [&](int col, int slot, Step& s, float* zin) -> int {
+    // the Gram-Schmidt kernels write an FP32 copy of the new basis vector only for FP32 Z
+    auto z32 = [](ZT* p) -> float* {
+        if constexpr (z64) return nullptr;
+        else return p;
+    };
+    auto enqueue_step = [&](int col, int slot, Step& s, ZT* zin) -> int {
         s.slot = slot;
         int r2 = BSP_OK;
         if (zin) s.z = zin;
         else if ((r2 = Z.take(ctx, s.z))) return r2;
-        float* z = s.z;
-        if (f32_pre && zin) {
+        ZT* z = s.z;
+        if constexpr (z64) {
+            // all-FP64: z = M^{-1} V[col] in place on the FP64 Z vector, w = A z (FP64)
+            if constexpr (vf32) r2 = launch_f2d(ctx, len, Vb[col], z);
+            else r2 = launch_copy(ctx, len, Vb[col], z);
+            if (!r2) r2 = bsp_precond_apply(ctx, M, z);
+        } else if (f32_pre && zin) {
             r2 = precond_apply_f32(ctx, M, z);
         } else if (f32_pre) {
             if constexpr (vf32) {
@@ -191,7 +207,9 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         if (r2) return r2;
         double* w = wbuf[slot];
         // w = A z
-        if (spmv_f32) {
+        if constexpr (z64) {
+            r2 = bsmv_matrix(ctx, A, z, w, 0);
+        } else if (spmv_f32) {
             if (A->val32_chunked)
                 r2 = launch_bsmv_f32_chunked(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
             else
@@ -206,8 +224,9 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         s.znext = nullptr;
         double* hd = hcol.as<double>() + slot * hslot;
         if (use_cgs) {
-            if (f32_pre && (r2 = Z.take(ctx, s.znext))) return r2;
-            r2 = launch_arnoldi_cgs(ctx, len, w, Vb.data(), col + 1, hd, s.vn, s.znext, cgs_scratch.as<double>(),
+            if constexpr (!z64)
+                if (f32_pre && (r2 = Z.take(ctx, s.znext))) return r2;
+            r2 = launch_arnoldi_cgs(ctx, len, w, Vb.data(), col + 1, hd, s.vn, z32(s.znext), cgs_scratch.as<double>(),
                                     cgs_flag.as<int>(), 0, hpin_dev + slot * hslot);
         } else {
             if constexpr (!vf32)
@@ -250,7 +269,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
                 Vb.pop_back();
             }
             double* hd = hcol.as<double>() + cur.slot * hslot;
-            rc = launch_arnoldi_cgs(ctx, len, wbuf[cur.slot], Vb.data(), iter + 1, hd, cur.vn, cur.znext,
+            rc = launch_arnoldi_cgs(ctx, len, wbuf[cur.slot], Vb.data(), iter + 1, hd, cur.vn, z32(cur.znext),
                                     cgs_scratch.as<double>(), cgs_flag.as<int>(), 2, nullptr);
             if (rc) return rc;
             BSP_HIP(hipMemcpyAsync(hpin.p + cur.slot * hslot, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost,
@@ -305,7 +324,8 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             // x += Z y
             for (int i = 0; i < nn; ++i) ypin.p[i] = y[i];
             BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
-            rc = launch_combine_add_f32(ctx, len, nn, Zb.data(), ydev.as<double>(), x);
+            if constexpr (z64) rc = launch_combine_add(ctx, len, nn, Zb.data(), ydev.as<double>(), x);
+            else rc = launch_combine_add_f32(ctx, len, nn, Zb.data(), ydev.as<double>(), x);
             if (rc) return rc;
             ++st.restarts;
             std::vector<double> P, Hnew;
@@ -321,20 +341,22 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             BSP_HIP(hipMemcpyAsync(Pdev.p, ppin.p, sizeof(double) * P.size(), hipMemcpyHostToDevice, ctx->stream));
             BSP_HIP(hipMemcpyAsync(Pkdev.p, Pk_host, sizeof(double) * nn * kout, hipMemcpyHostToDevice, ctx->stream));
             std::vector<VT*> Vn(kp);
-            std::vector<float*> Zn(kout);
+            std::vector<ZT*> Zn(kout);
             for (int j = 0; j < kp; ++j)
                 if ((rc = V.take(ctx, Vn[j]))) return rc;
             for (int j = 0; j < kout; ++j)
                 if ((rc = Z.take(ctx, Zn[j]))) return rc;
             if constexpr (vf32) rc = launch_basis_rotate_f32(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
             else rc = launch_basis_rotate(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
-            if (!rc && kout > 0)
-                rc = launch_basis_rotate_f32(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
+            if (!rc && kout > 0) {
+                if constexpr (z64) rc = launch_basis_rotate(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
+                else rc = launch_basis_rotate_f32(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
+            }
             if (rc) return rc;
             // the old vectors go back to the pools at once: the kernels that will write them
             // again are enqueued on the same stream after the rotations that read them
             for (VT* p : Vb) V.give(p);
-            for (float* p : Zb) Z.give(p);
+            for (ZT* p : Zb) Z.give(p);
             if (cur.znext) Z.give(cur.znext);   // an FP32 copy of a basis vector the rotation replaced
             Vb = Vn;
             Zb = Zn;
@@ -355,7 +377,8 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     if (!y.empty()) {
         for (size_t i = 0; i < y.size(); ++i) ypin.p[i] = y[i];
         BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
-        rc = launch_combine_add_f32(ctx, len, (int)y.size(), Zb.data(), ydev.as<double>(), x);
+        if constexpr (z64) rc = launch_combine_add(ctx, len, (int)y.size(), Zb.data(), ydev.as<double>(), x);
+        else rc = launch_combine_add_f32(ctx, len, (int)y.size(), Zb.data(), ydev.as<double>(), x);
         if (rc) return rc;
     }
     BSP_HIP(hipStreamSynchronize(ctx->stream));
@@ -415,12 +438,21 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
         // the pass reduces the current residual by `inner_reduction` (or to the target)
         const double pass_red = std::max(inner_reduction, 0.5 * tol / std::max(rr, 1e-300));
         InnerStats st;
-        if (ctx->fgmres_basis_f32)
-            rc = fgmres_pass<float>(ctx, A, cur, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k,
-                                    spmv_f32, st, rw.as<double>(), t64.as<double>());
+        const double red = std::min(pass_red, 0.5);
+        // all-FP64 flexible GMRES (FP64 preconditioner and SpMV): Z is stored in FP64 too
+        const bool z64 = !cur->f32 && !spmv_f32;
+        if (ctx->fgmres_basis_f32 && z64)
+            rc = fgmres_pass<float, double>(ctx, A, cur, b, x, red, maxIters - total, restart, k, spmv_f32, st,
+                                            rw.as<double>(), t64.as<double>());
+        else if (ctx->fgmres_basis_f32)
+            rc = fgmres_pass<float, float>(ctx, A, cur, b, x, red, maxIters - total, restart, k, spmv_f32, st,
+                                           rw.as<double>(), t64.as<double>());
+        else if (z64)
+            rc = fgmres_pass<double, double>(ctx, A, cur, b, x, red, maxIters - total, restart, k, spmv_f32, st,
+                                             rw.as<double>(), t64.as<double>());
         else
-            rc = fgmres_pass<double>(ctx, A, cur, b, x, std::min(pass_red, 0.5), maxIters - total, restart, k,
-                                     spmv_f32, st, rw.as<double>(), t64.as<double>());
+            rc = fgmres_pass<double, float>(ctx, A, cur, b, x, red, maxIters - total, restart, k, spmv_f32, st,
+                                            rw.as<double>(), t64.as<double>());
         if (rc) return rc;
         if (passes == 0) res0 = st.beta;
         total += st.iters;

@@ -13,6 +13,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+# the same oracle built with the reference's release flags (-O3 -funroll-loops -fstrict-aliasing
+# -fomit-frame-pointer -ffast-math); timing only, never a parity checker
+_LIB_FAST = os.path.join(_HERE, "liboracle_fast.so")
+_use_fast = False
 
 PRE_NONE, PRE_DIAG, PRE_SOR, PRE_ILU0, PRE_ICHOL0 = 0, 1, 2, 3, 4
 
@@ -28,7 +32,7 @@ class Stats(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(_LIB):
+    if force or not (os.path.exists(_LIB) and os.path.exists(_LIB_FAST)):
         subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE])
     return _LIB
 
@@ -36,12 +40,21 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+def use_fast_build(on: bool = True) -> None:
+    """Route the following calls to liboracle_fast.so (True) or the parity build (False)."""
+    global _use_fast, _lib
+    if on != _use_fast:
+        _use_fast = on
+        _lib = None
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
+        path = _LIB_FAST if _use_fast else _LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         for name in ("orc_pcg", "orc_pgmres", "orc_harmonic_subspace", "orc_ichol0_factor",
                      "orc_levels", "orc_bsmm", "orc_bsmm_transA"):
             getattr(L, name).restype = C.c_int

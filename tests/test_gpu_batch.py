@@ -322,3 +322,37 @@ def test_autotune_solve_keeps_a_finalist_and_solves(ctx):
     assert rc == 0 and (rel <= 1e-10).all()
     with pytest.raises(bsp.BspError):
         bat.autotune_solve(B, X, 0.0)
+
+
+def test_graph_cache_survives_tile_changes(ctx):
+    """A graph captured for one RHS-per-item setting stays valid after solves with others.
+
+    pcg(R=4) captures a graph over the R=4 work-item arrays; solves at R=1 and R=2 build their
+    own arrays; the R=4 graph is then replayed and must give the first solve's bits.
+    """
+    probs, A, sys, Bh = _batch(ctx, nsys=6, seed=11)
+    bat = bsp.Batch(ctx, A, sys, 4)
+    B = ctx.to_device(Bh)
+    bat.set_tiles(256, 4)
+    X0 = ctx.zeros(Bh.shape)
+    its0, _, _, rc = bat.pcg(B, X0, 1e-10, maxIters=2000)
+    assert rc == 0
+    ref = X0.numpy().copy()
+    for wg, r in [(256, 1), (512, 2), (256, 4), (1024, 1), (256, 4)]:
+        bat.set_tiles(wg, r)
+        X = ctx.zeros(Bh.shape)
+        its, _, _, rc = bat.pcg(B, X, 1e-10, maxIters=2000)
+        assert rc == 0
+        if (wg, r) == (256, 4):
+            assert np.array_equal(its, its0)
+            assert np.array_equal(X.numpy(), ref)
+        # the same operands again replay the cached graph of this setting
+        its2, _, _, rc = bat.pcg(B, X, 1e-10, maxIters=2000)
+        assert rc == 0 and np.array_equal(its2, its)
+    # autotune probes rebuild items for every R; the cached R=4 graph must still be valid
+    Xt = ctx.zeros(Bh.shape)
+    bat.autotune(B, Xt, probe_iters=4)
+    bat.set_tiles(256, 4)
+    X = ctx.zeros(Bh.shape)
+    its, _, _, rc = bat.pcg(B, X, 1e-10, maxIters=2000)
+    assert rc == 0 and np.array_equal(X.numpy(), ref)
