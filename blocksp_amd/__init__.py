@@ -26,6 +26,9 @@ LIB_PATH = os.path.join(_HERE, "libblocksp_amd.so")
 TOL_REFERENCE = 0
 TOL_TRUE_RESIDUAL = 1
 
+E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_NUMERIC, E_TIMEOUT = -1, -2, -3, -4, -5, -6
+NOT_CONVERGED = 1
+
 _ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
            -5: "numerical failure", -6: "sync-free wait timed out"}
 

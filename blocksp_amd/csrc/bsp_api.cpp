@@ -143,6 +143,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "batch_ell") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "batch_ell must be 0 or 1");
         ctx->batch_ell = (int)value;
+    } else if (k == "factor_device") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "factor_device must be 0 or 1");
+        ctx->factor_device = (int)value;
     } else if (k == "verbose") {
         ctx->verbose = (int)std::max<int64_t>(0, value);
     } else if (k == "spin_limit") {

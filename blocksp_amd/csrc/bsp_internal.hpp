@@ -137,6 +137,8 @@ struct bsp_ctx {
                                      // Gram-Schmidt steps in one single-workgroup kernel
     int sweep_lds = 1;               // such sweeps keep the solution vector in LDS when it fits
                                      // (at most 64 KiB and the device's limit per workgroup)
+    int factor_device = 1;           // ILU0 / IChol0 factorisation: 1 on the device (blocks up to 8 x 8), 0 the
+                                     // host loop (the method of the reference, one row at a time)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
     int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +
@@ -211,9 +213,12 @@ struct bsp_precond {
     bsp::DevMem pos;                  // multicolour SOR: step of each row in the sweep order
     bsp_host_precond_fn host_fn = nullptr;   // BSP_PRE_HOST
     void* host_user = nullptr;
-    // host copy of the factor in the reference layout (for parity download)
+    // the factor in the reference layout (ILU0: A's blocks; IChol0: the blocks (k, j) with j
+    // eliminated no later than k, in A's row order): structure on the host, values on the
+    // device (fdev, device factorisation) or on the host (f_val, host factorisation)
     std::vector<int32_t> f_rp, f_ci;
     std::vector<double> f_val;
+    bsp::DevMem fdev;
 };
 
 namespace bsp {

@@ -15,6 +15,7 @@
 #include <numeric>
 
 #include "bsp_dense.hpp"
+#include "bsp_factor.hpp"
 #include "bsp_internal.hpp"
 
 namespace bsp {
@@ -133,6 +134,33 @@ hipError_t upload_structure(Sweep& S, const std::vector<int32_t>& rp, const std:
     S.dci = S.ci.as<int>();
     S.dval = S.val.p;
     return hipSuccess;
+}
+
+// Structure of a sweep on the device with room for its values in `precision` (filled on the
+// device afterwards).
+hipError_t upload_pattern(Sweep& S, const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, int bs,
+                          int precision) {
+    const size_t nnz = ci.size();
+    const size_t esz = precision == BSP_PREC_BF16 ? 2 : precision == BSP_PREC_F32 ? 4 : 8;
+    S.nnzb = (int64_t)nnz;
+    hipError_t e;
+    if ((e = S.rp.alloc(sizeof(int) * rp.size())) != hipSuccess) return e;
+    if ((e = S.ci.alloc(sizeof(int) * std::max<size_t>(nnz, 1))) != hipSuccess) return e;
+    if ((e = hipMemcpy(S.rp.p, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if (nnz && (e = hipMemcpy(S.ci.p, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = S.val.alloc(esz * std::max<size_t>(nnz * bs * bs, 1))) != hipSuccess) return e;
+    S.f32 = precision != BSP_PREC_F64;
+    S.bf16 = precision == BSP_PREC_BF16;
+    S.drp = S.rp.as<int>();
+    S.dci = S.ci.as<int>();
+    S.dval = S.val.p;
+    return hipSuccess;
+}
+
+hipError_t upload_ints(DevMem& m, const std::vector<int32_t>& v) {
+    hipError_t e = m.alloc(sizeof(int32_t) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(m.p, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice);
 }
 
 // Split a square block matrix (host) into its strictly lower / strictly upper parts,
@@ -302,18 +330,10 @@ int bsp_precond_create_sor_ex(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, 
 
 namespace bsp {
 
-// Block ILU(0) in a given elimination order (pos[i] = step at which row i is
-// eliminated; natural order pos[i] = i is the reference's, preconditioners.hpp:274-291).
-// The factor is returned in A's block layout (LU[p] for A's block p).
-int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std::vector<int32_t>& elim,
-                     std::vector<int32_t>& pos, std::vector<double>& LU) {
-    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
-    if (int rc = sync_host_values(ctx, A)) return rc;
-    const int n = A->n_rows, bs = A->bs;
-    const size_t b2 = (size_t)bs * bs;
-    const auto& rp = A->h_rp;
-    const auto& ci = A->h_ci;
-    // elimination order: colour by colour, natural order within a colour
+// Elimination order: colour by colour, natural order within a colour (natural order when
+// colour == nullptr); pos is its inverse.
+int elimination_order(bsp_ctx* ctx, int n, const int32_t* colour, std::vector<int32_t>& elim,
+                      std::vector<int32_t>& pos) {
     elim.assign(n, 0);
     pos.assign(n, 0);
     for (int i = 0; i < n; ++i) elim[i] = i;
@@ -322,6 +342,22 @@ int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std:
         std::stable_sort(elim.begin(), elim.end(), [&](int a, int b) { return colour[a] < colour[b]; });
     }
     for (int t = 0; t < n; ++t) pos[elim[t]] = t;
+    return BSP_OK;
+}
+
+namespace {
+
+// Block ILU(0) on the host in a given elimination order (pos[i] = step at which row i is
+// eliminated; natural order pos[i] = i is the reference's, preconditioners.hpp:274-291), for
+// block sizes the device factorisation does not take.  The factor is returned in A's block
+// layout (LU[p] for A's block p).
+int ilu0_factor_rows_host(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
+                          const std::vector<int32_t>& pos, std::vector<double>& LU) {
+    if (int rc = sync_host_values(ctx, A)) return rc;
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    const auto& rp = A->h_rp;
+    const auto& ci = A->h_ci;
     LU = A->h_val;
     // rows holding a block in column c, from A's pattern
     std::vector<int32_t> cptr(n + 1, 0), crow, cpos;
@@ -365,6 +401,25 @@ int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std:
             }
         }
     }
+    return BSP_OK;
+}
+
+}  // namespace
+
+// Block ILU(0) in the elimination order of `colour`, returned on the host in A's block layout
+// (device factorisation for blocks up to 8 x 8, downloaded; the host loop otherwise).
+int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std::vector<int32_t>& elim,
+                     std::vector<int32_t>& pos, std::vector<double>& LU) {
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
+    if (int rc = elimination_order(ctx, A->n_rows, colour, elim, pos)) return rc;
+    if (!factor_device_supported(A->bs) || !ctx->factor_device) return ilu0_factor_rows_host(ctx, A, elim, pos, LU);
+    FactorPlan P;
+    DevMem F, dinv;
+    int rc = plan_factor(ctx, A, pos, false, P);
+    if (!rc) rc = factor_device(ctx, A, P, false, F, dinv);
+    if (rc) return rc;
+    LU.resize((size_t)A->nnzb * A->bs * A->bs);
+    BSP_HIP(hipMemcpy(LU.data(), F.p, sizeof(double) * LU.size(), hipMemcpyDeviceToHost));
     return BSP_OK;
 }
 
@@ -433,6 +488,74 @@ int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
     return BSP_OK;
 }
 
+// Device preconditioner from an ILU(0) factor F on the device in A's block layout (F is taken
+// over as the preconditioner's copy of the factor): the strict parts are gathered on the device
+// in the storage precision, the inverses of U's diagonal blocks computed on the device.
+int ilu0_build_device(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
+                      const std::vector<int32_t>& pos, const int* diag_dev, DevMem& F, int precision,
+                      bsp_precond** out) {
+    BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
+              "ILU0 precision must be F64, F32 or BF16");
+    BSP_HIP(hipSetDevice(ctx->device));
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    const auto& rp = A->h_rp;
+    const auto& ci = A->h_ci;
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_ILU0;
+    M->n = n;
+    M->bs = bs;
+    M->f32 = precision != BSP_PREC_F64;
+    M->bf16 = precision == BSP_PREC_BF16;
+    std::vector<int32_t> lrp(n + 1, 0), lci, lmap, urp(n + 1, 0), uci, umap;
+    lci.reserve(A->nnzb);
+    uci.reserve(A->nnzb);
+    for (int i = 0; i < n; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            const int c = ci[p];
+            if (pos[c] < pos[i]) {
+                lci.push_back(c);
+                lmap.push_back(p);
+            } else if (pos[c] > pos[i]) {
+                uci.push_back(c);
+                umap.push_back(p);
+            }
+        }
+        lrp[i + 1] = (int32_t)lci.size();
+        urp[i + 1] = (int32_t)uci.size();
+    }
+    BSP_HIP(upload_pattern(M->lower, lrp, lci, bs, precision));
+    BSP_HIP(upload_pattern(M->upper, urp, uci, bs, precision));
+    DevMem dl, du;
+    BSP_HIP(upload_ints(dl, lmap));
+    BSP_HIP(upload_ints(du, umap));
+    int rc = gather_blocks(ctx, (int64_t)lmap.size(), bs, dl.as<int>(), F.as<double>(), M->lower.val.p, precision, false);
+    if (!rc) rc = gather_blocks(ctx, (int64_t)umap.size(), bs, du.as<int>(), F.as<double>(), M->upper.val.p, precision, false);
+    if (rc) return rc;
+    const size_t esz = precision == BSP_PREC_BF16 ? 2 : precision == BSP_PREC_F32 ? 4 : 8;
+    BSP_HIP(M->dinv_upper.alloc(esz * std::max<size_t>(n * b2, 1)));
+    if ((rc = invert_diag_device(ctx, n, bs, diag_dev, F.as<double>(), M->dinv_upper.p, nullptr, precision)))
+        return fail(ctx, BSP_E_NUMERIC, "ILU0: " + ctx->last_error);
+    if (M->f32) BSP_HIP(M->work32.alloc(sizeof(float) * std::max<size_t>((size_t)n * bs, 1)));
+    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
+    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    rc = build_sweep_ell(ctx, M->lower, lrp);
+    if (!rc) rc = build_sweep_ell(ctx, M->upper, urp);
+    if (rc) return rc;
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    M->elim_order = elim;
+    M->f_rp = rp;
+    M->f_ci = ci;
+    M->fdev.p = F.p;
+    M->fdev.bytes = F.bytes;
+    F.p = nullptr;
+    F.bytes = 0;
+    *out = M.release();
+    return BSP_OK;
+}
+
 }  // namespace bsp
 
 namespace {
@@ -440,8 +563,18 @@ namespace {
 int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int precision, bsp_precond** out) {
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
+    BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
     BSP_HIP(hipSetDevice(ctx->device));
     std::vector<int32_t> elim, pos;
+    if (factor_device_supported(A->bs) && ctx->factor_device) {
+        if (int rc = elimination_order(ctx, A->n_rows, colour, elim, pos)) return rc;
+        FactorPlan P;
+        DevMem F, dinv;
+        int rc = plan_factor(ctx, A, pos, false, P);
+        if (!rc) rc = factor_device(ctx, A, P, false, F, dinv);
+        if (rc) return rc;
+        return ilu0_build_device(ctx, A, elim, pos, P.diag.as<int>(), F, precision, out);
+    }
     std::vector<double> LU;
     int rc = ilu0_factor_host(ctx, A, colour, elim, pos, LU);
     if (rc) return rc;
@@ -591,6 +724,86 @@ int ichol0_build(bsp_ctx* ctx, int n, int bs, std::vector<int32_t>& Lrp, std::ve
     return BSP_OK;
 }
 
+// IChol0 on the device in the elimination order of `colour` (natural order when null): the
+// factor L on the blocks (k, j) with pos[j] <= pos[k] in A's row order (the reference's
+// lowTri(A) for the natural order, IChol0 of P A P^T mapped back otherwise), the forward sweep
+// over its strict blocks in elimination order and the backward sweep over L^T (row c: (L_kc)^T
+// for the rows k eliminated after c, in elimination order), as ichol0_build.
+int ichol0_create_device(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, bsp_precond** out) {
+    const int n = A->n_rows, bs = A->bs;
+    const size_t b2 = (size_t)bs * bs;
+    const auto& rp = A->h_rp;
+    const auto& ci = A->h_ci;
+    std::vector<int32_t> elim, pos;
+    if (int rc = elimination_order(ctx, n, colour, elim, pos)) return rc;
+    FactorPlan P;
+    DevMem F, dinv;
+    int rc = plan_factor(ctx, A, pos, true, P);
+    if (!rc) rc = factor_device(ctx, A, P, true, F, dinv);
+    if (rc) return rc;
+    auto M = std::make_unique<bsp_precond>();
+    M->ctx = ctx;
+    M->kind = BSP_PRE_ICHOL0;
+    M->n = n;
+    M->bs = bs;
+    std::vector<int32_t> Lrp(n + 1, 0), Lci, fmap, lrp(n + 1, 0), lci, lmap, ucnt(n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        for (int p = rp[k]; p < rp[k + 1]; ++p) {
+            const int c = ci[p];
+            if (pos[c] > pos[k]) continue;
+            Lci.push_back(c);
+            fmap.push_back(p);
+            if (c == k) continue;
+            lci.push_back(c);
+            lmap.push_back(p);
+            ucnt[c + 1]++;
+        }
+        Lrp[k + 1] = (int32_t)Lci.size();
+        lrp[k + 1] = (int32_t)lci.size();
+    }
+    std::vector<int32_t> urp(n + 1, 0);
+    for (int c = 0; c < n; ++c) urp[c + 1] = urp[c] + ucnt[c + 1];
+    std::vector<int32_t> uci(urp[n]), umap(urp[n]), fill(urp.begin(), urp.end() - 1);
+    for (int t = 0; t < n; ++t) {
+        const int k = elim[t];
+        for (int p = rp[k]; p < rp[k + 1]; ++p) {
+            const int c = ci[p];
+            if (c == k || pos[c] > pos[k]) continue;
+            const int q = fill[c]++;
+            uci[q] = k;
+            umap[q] = p;
+        }
+    }
+    BSP_HIP(upload_pattern(M->lower, lrp, lci, bs, BSP_PREC_F64));
+    BSP_HIP(upload_pattern(M->upper, urp, uci, bs, BSP_PREC_F64));
+    DevMem dl, du, df;
+    BSP_HIP(upload_ints(dl, lmap));
+    BSP_HIP(upload_ints(du, umap));
+    BSP_HIP(upload_ints(df, fmap));
+    BSP_HIP(M->fdev.alloc(sizeof(double) * std::max<size_t>(fmap.size() * b2, 1)));
+    rc = gather_blocks(ctx, (int64_t)lmap.size(), bs, dl.as<int>(), F.as<double>(), M->lower.val.p, BSP_PREC_F64, false);
+    if (!rc) rc = gather_blocks(ctx, (int64_t)umap.size(), bs, du.as<int>(), F.as<double>(), M->upper.val.p, BSP_PREC_F64, true);
+    if (!rc) rc = gather_blocks(ctx, (int64_t)fmap.size(), bs, df.as<int>(), F.as<double>(), M->fdev.p, BSP_PREC_F64, false);
+    if (rc) return rc;
+    BSP_HIP(M->dinv_lower.alloc(sizeof(double) * std::max<size_t>(n * b2, 1)));
+    BSP_HIP(M->dinv_upper.alloc(sizeof(double) * std::max<size_t>(n * b2, 1)));
+    if ((rc = invert_diag_device(ctx, n, bs, P.diag.as<int>(), F.as<double>(), M->dinv_lower.p, M->dinv_upper.p,
+                                 BSP_PREC_F64)))
+        return fail(ctx, BSP_E_NUMERIC, "IChol0: singular factor block (" + ctx->last_error + ")");
+    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
+    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    rc = build_sweep_ell(ctx, M->lower, lrp);
+    if (!rc) rc = build_sweep_ell(ctx, M->upper, urp);
+    if (rc) return rc;
+    BSP_HIP(hipStreamSynchronize(ctx->stream));
+    M->elim_order = elim;
+    M->f_rp.swap(Lrp);
+    M->f_ci.swap(Lci);
+    *out = M.release();
+    return BSP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -616,6 +829,7 @@ int bsp_precond_create_ichol0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* 
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "IChol0 needs a square matrix");
     BSP_HIP(hipSetDevice(ctx->device));
+    if (factor_device_supported(A->bs) && ctx->factor_device) return ichol0_create_device(ctx, A, colour, out);
     if (int rc = sync_host_values(ctx, A)) return rc;
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
@@ -769,7 +983,15 @@ int bsp_precond_download_factor(bsp_ctx* ctx, const bsp_precond* M, int32_t* row
     BSP_CHECK(!M->f_rp.empty(), BSP_E_UNSUPPORTED, "this preconditioner has no factor");
     if (rowptr) std::copy(M->f_rp.begin(), M->f_rp.end(), rowptr);
     if (colidx) std::copy(M->f_ci.begin(), M->f_ci.end(), colidx);
-    if (vals) std::copy(M->f_val.begin(), M->f_val.end(), vals);
+    if (vals) {
+        if (M->fdev.p) {
+            BSP_HIP(hipStreamSynchronize(ctx->stream));
+            BSP_HIP(hipMemcpy(vals, M->fdev.p, sizeof(double) * M->f_ci.size() * M->bs * M->bs,
+                              hipMemcpyDeviceToHost));
+        } else {
+            std::copy(M->f_val.begin(), M->f_val.end(), vals);
+        }
+    }
     return BSP_OK;
 }
 
