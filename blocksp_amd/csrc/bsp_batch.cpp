@@ -318,8 +318,11 @@ int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t*
     }
     // a colour order over the whole block-diagonal matrix is, restricted to each system, that
     // system's own colour order (no block couples two systems)
+    PhaseTimer tm(ctx, "batch");
+    tm.mark("checks");
     int rc = bsp_precond_create_ichol0_ex(ctx, A, colour, &b->M);
     if (rc) return rc;
+    tm.mark("IChol0");
     std::vector<int32_t> rows, lev, base;
     level_sets(b->M->lower.level_of, b->sys_row0, rows, lev, base);
     BSP_HIP(upload_i(b->frows, rows));
@@ -330,7 +333,9 @@ int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t*
     BSP_HIP(upload_i(b->blev, lev));
     BSP_HIP(upload_i(b->blev_base, base));
     BSP_HIP(upload_i(b->d_sys_row0, b->sys_row0));
+    tm.mark("per-system level sets");
     if (ctx->batch_ell && (rc = build_ell(ctx, b.get()))) return rc;
+    tm.mark("ELL copies");
     const size_t vec = sizeof(double) * std::max<size_t>((size_t)b->nrows * b->bs * nrhs, 1);
     BSP_HIP(b->Bi.alloc(vec));
     BSP_HIP(b->Xi.alloc(vec));

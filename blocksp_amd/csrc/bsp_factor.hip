@@ -11,7 +11,8 @@
 // blocks, i.e. the forward-sweep level sets: one launch per level, and within a level one
 // group of bs*bs lanes per row (lane = entry (r, c) of a block), several groups per 64-wide
 // wavefront for small blocks.  The symbolic part (which block updates which, in what order)
-// is integer work done once on the host from the structure.
+// is done by each row's group from A's structure as it goes, so nothing is planned on the host
+// beyond the level schedule, which the forward sweep needs anyway.
 #include <algorithm>
 #include <climits>
 #include <cstring>
@@ -25,16 +26,15 @@ namespace {
 
 constexpr int kWave = 64;
 
-// Plan arrays on the device (see bsp_factor.hpp for their meaning).
-struct PlanArgs {
-    const int* rows;        // rows sorted by level
-    const int* row_task;    // [n + 1] strict-lower tasks of each row (by row id)
-    const int* task_p;      // block (k, i) of the task
-    const int* task_i;      // its column i
-    const int* task_u;      // [ntasks + 1] update range of each task
-    const int* upd_src;     // ILU0: block (i, j); IChol0: block (j, i)
-    const int* upd_dst;     // block (k, j)
-    const int* diag;        // [n] position of the diagonal block
+// What a row's group reads: A's structure, the elimination position of every row, and the
+// schedule (rows sorted by level, -1 padding slots skipped).
+struct FactorArgs {
+    const int* rp;
+    const int* ci;
+    const int* pos;
+    const int* order;
+    int* diag;          // out: position of each row's (first) diagonal block
+    int* err;           // [0]: smallest failing pivot row + 1; [1]: smallest row without diagonal + 1
 };
 
 template <int BS>
@@ -138,12 +138,20 @@ __device__ bool group_cholesky(double* W, int e) {
     return true;
 }
 
-// One level of the factorisation: `cnt` rows starting at rows[s0].  F: the factor in A's
-// block layout (A's values on entry), dinv: [n][G] inverse of each finished row's pivot block
-// (ILU0: U_kk^{-1}; IChol0: L_kk^{-1}).  err: smallest failing row + 1 (atomicMin).
+__device__ inline int find_in_row(const int* __restrict__ rp, const int* __restrict__ ci, int i, int j) {
+    for (int p = rp[i]; p < rp[i + 1]; ++p)
+        if (ci[p] == j) return p;
+    return -1;
+}
+
+// One level of the factorisation: the rows in schedule slots [s0, s0 + cnt).  F: the factor in
+// A's block layout (A's values on entry), dinv: [n][G] inverse of each finished row's pivot
+// block (ILU0: U_kk^{-1}; IChol0: L_kk^{-1}).  Row k finds its own tasks: its strict-lower
+// blocks (pos[i] < pos[k]) in increasing (pos[i], position) order, and for each task the
+// blocks it updates, from A's structure -- the symbolic part of the reference's loops.
 template <int BS, bool CHOL>
-__global__ __launch_bounds__(kWave) void k_factor_level(PlanArgs pa, int s0, int cnt, double* __restrict__ F,
-                                                        double* __restrict__ dinv, int* __restrict__ err) {
+__global__ __launch_bounds__(kWave) void k_factor_level(FactorArgs fa, int s0, int cnt, double* __restrict__ F,
+                                                        double* __restrict__ dinv) {
     constexpr int G = Groups<BS>::G, R = Groups<BS>::R;
     __shared__ double X[R][G];
     __shared__ double Y[R][G];
@@ -152,10 +160,35 @@ __global__ __launch_bounds__(kWave) void k_factor_level(PlanArgs pa, int s0, int
     const int g = lane / G, e = lane % G;
     const int idx = blockIdx.x * R + g;
     if (g >= R || idx >= cnt) return;
+    const int k = fa.order[s0 + idx];
+    if (k < 0) return;
     const int r = e / BS, c = e % BS;
-    const int k = pa.rows[s0 + idx];
-    for (int t = pa.row_task[k]; t < pa.row_task[k + 1]; ++t) {
-        const int pki = pa.task_p[t], i = pa.task_i[t];
+    const int rs = fa.rp[k], re = fa.rp[k + 1];
+    const int pk = fa.pos[k];
+    int pd = -1;
+    for (int p = rs; p < re && pd < 0; ++p)
+        if (fa.ci[p] == k) pd = p;
+    if (pd < 0) {
+        if (e == 0) atomicMin(fa.err + 1, k + 1);
+        return;
+    }
+    int last_q = -1, last_p = -1;
+    for (;;) {
+        // next strict-lower block in (elimination position, position) order
+        int pki = -1, bq = INT_MAX;
+        for (int p = rs; p < re; ++p) {
+            const int q = fa.pos[fa.ci[p]];
+            if (q >= pk) continue;
+            if (q < last_q || (q == last_q && p <= last_p)) continue;
+            if (q < bq) {
+                bq = q;
+                pki = p;
+            }
+        }
+        if (pki < 0) break;
+        last_q = bq;
+        last_p = pki;
+        const int i = fa.ci[pki];
         // updates of earlier tasks of this row (other lanes' stores) must be visible
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         X[g][e] = F[(size_t)pki * G + e];
@@ -170,8 +203,21 @@ __global__ __launch_bounds__(kWave) void k_factor_level(PlanArgs pa, int s0, int
         __builtin_amdgcn_wave_barrier();
         X[g][e] = s;
         F[(size_t)pki * G + e] = s;
-        for (int u = pa.task_u[t]; u < pa.task_u[t + 1]; ++u) {
-            const int src = pa.upd_src[u], dst = pa.upd_dst[u];
+        for (int p = rs; p < re; ++p) {
+            const int j = fa.ci[p];
+            const int qj = fa.pos[j];
+            int src;
+            if constexpr (CHOL) {
+                // L_kj -= L_ki L_ji^T for pos[i] < pos[j] <= pos[k], A(i, j) and L(j, i) present
+                if (qj <= bq || qj > pk) continue;
+                if (find_in_row(fa.rp, fa.ci, i, j) < 0) continue;
+                src = find_in_row(fa.rp, fa.ci, j, i);
+            } else {
+                // A_kj -= L_ki U_ij for pos[j] > pos[i] and (i, j) in the pattern
+                if (qj <= bq) continue;
+                src = find_in_row(fa.rp, fa.ci, i, j);
+            }
+            if (src < 0) continue;
             __builtin_amdgcn_wave_barrier();
             // IChol0's diagonal update reads the block this task has just written
             Y[g][e] = src == pki ? s : F[(size_t)src * G + e];
@@ -182,13 +228,13 @@ __global__ __launch_bounds__(kWave) void k_factor_level(PlanArgs pa, int s0, int
             } else {
                 for (int m = 0; m < BS; ++m) prod += X[g][r * BS + m] * Y[g][m * BS + c];   // L_ki U_ij
             }
-            double* d = F + (size_t)dst * G + e;
+            double* d = F + (size_t)p * G + e;
             *d = *d - prod;
         }
         __builtin_amdgcn_wave_barrier();
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    const int pd = pa.diag[k];
+    if (e == 0) fa.diag[k] = pd;
     W[g][e] = F[(size_t)pd * G + e];
     __builtin_amdgcn_wave_barrier();
     bool ok = true;
@@ -199,7 +245,7 @@ __global__ __launch_bounds__(kWave) void k_factor_level(PlanArgs pa, int s0, int
     }
     if (ok) ok = group_invert<BS>(W[g], e);
     if (!ok) {
-        if (e == 0) atomicMin(err, k + 1);
+        if (e == 0) atomicMin(fa.err, k + 1);
         return;
     }
     dinv[(size_t)k * G + e] = W[g][G + e];
@@ -258,41 +304,27 @@ __global__ void k_gather_blocks(int64_t cnt, int bs, const int* __restrict__ map
     }
 }
 
-__global__ void k_set_int(int* p, int v) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
-}
-
-template <typename T>
-hipError_t upload_vec(DevMem& m, const std::vector<T>& v) {
-    hipError_t e = m.alloc(sizeof(T) * std::max<size_t>(v.size(), 1));
-    if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpy(m.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
-}
-
-int find_col(const std::vector<int32_t>& rp, const std::vector<int32_t>& ci, int i, int j) {
-    for (int p = rp[i]; p < rp[i + 1]; ++p)
-        if (ci[p] == j) return p;
-    return -1;
+__global__ void k_set_int(int* p, int v, int cnt) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < cnt) p[threadIdx.x] = v;
 }
 
 template <int BS, bool CHOL>
-void launch_level(hipStream_t st, const PlanArgs& pa, int s0, int cnt, double* F, double* dinv, int* err) {
+void launch_level(hipStream_t st, const FactorArgs& fa, int s0, int cnt, double* F, double* dinv) {
     constexpr int R = Groups<BS>::R;
-    hipLaunchKernelGGL((k_factor_level<BS, CHOL>), dim3((cnt + R - 1) / R), dim3(kWave), 0, st, pa, s0, cnt, F, dinv,
-                       err);
+    hipLaunchKernelGGL((k_factor_level<BS, CHOL>), dim3((cnt + R - 1) / R), dim3(kWave), 0, st, fa, s0, cnt, F, dinv);
 }
 
 template <bool CHOL>
-bool launch_level_bs(int bs, hipStream_t st, const PlanArgs& pa, int s0, int cnt, double* F, double* dinv, int* err) {
+bool launch_level_bs(int bs, hipStream_t st, const FactorArgs& fa, int s0, int cnt, double* F, double* dinv) {
     switch (bs) {
-        case 1: launch_level<1, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 2: launch_level<2, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 3: launch_level<3, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 4: launch_level<4, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 5: launch_level<5, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 6: launch_level<6, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 7: launch_level<7, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
-        case 8: launch_level<8, CHOL>(st, pa, s0, cnt, F, dinv, err); return true;
+        case 1: launch_level<1, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 2: launch_level<2, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 3: launch_level<3, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 4: launch_level<4, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 5: launch_level<5, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 6: launch_level<6, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 7: launch_level<7, CHOL>(st, fa, s0, cnt, F, dinv); return true;
+        case 8: launch_level<8, CHOL>(st, fa, s0, cnt, F, dinv); return true;
         default: return false;
     }
 }
@@ -319,11 +351,12 @@ bool launch_inv_bs(int bs, hipStream_t st, int n, const int* diag, const double*
     }
 }
 
-int check_err(bsp_ctx* ctx, const DevMem& err, const char* what) {
-    int h = INT_MAX;
-    BSP_HIP(hipMemcpyAsync(&h, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+int check_err(bsp_ctx* ctx, const DevMem& err, const char* what, const char* what2 = nullptr) {
+    int h[2] = {INT_MAX, INT_MAX};
+    BSP_HIP(hipMemcpyAsync(h, err.p, sizeof(int) * (what2 ? 2 : 1), hipMemcpyDeviceToHost, ctx->stream));
     BSP_HIP(hipStreamSynchronize(ctx->stream));
-    if (h != INT_MAX) return fail(ctx, BSP_E_NUMERIC, std::string(what) + " in row " + std::to_string(h - 1));
+    if (what2 && h[1] != INT_MAX) return fail(ctx, BSP_E_NUMERIC, std::string(what2) + " in row " + std::to_string(h[1] - 1));
+    if (h[0] != INT_MAX) return fail(ctx, BSP_E_NUMERIC, std::string(what) + " in row " + std::to_string(h[0] - 1));
     return BSP_OK;
 }
 
@@ -331,124 +364,40 @@ int check_err(bsp_ctx* ctx, const DevMem& err, const char* what) {
 
 bool factor_device_supported(int bs) { return bs >= 1 && bs <= 8; }
 
-int plan_factor(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& pos, bool chol, FactorPlan& P) {
-    const int n = A->n_rows;
-    const auto& rp = A->h_rp;
-    const auto& ci = A->h_ci;
-    std::vector<int32_t> elim(n);
-    for (int i = 0; i < n; ++i) elim[pos[i]] = i;
-    std::vector<int32_t> row_task(n + 1, 0), task_p, task_i, task_u, src, dst, diag(n, -1), level(n, 0);
-    task_p.reserve(A->nnzb);
-    task_i.reserve(A->nnzb);
-    task_u.reserve(A->nnzb + 1);
-    src.reserve(3 * A->nnzb);
-    dst.reserve(3 * A->nnzb);
-    // tasks are stored by row id; a row's strict-lower blocks in elimination order
-    std::vector<std::pair<int32_t, int32_t>> lo;
-    std::vector<int32_t> cnt(n, 0);
-    int nlev = 0;
-    // levels in elimination order (a row's strict-lower rows are eliminated before it)
-    for (int t = 0; t < n; ++t) {
-        const int k = elim[t];
-        int l = 0;
-        for (int p = rp[k]; p < rp[k + 1]; ++p)
-            if (pos[ci[p]] < t) l = std::max(l, level[ci[p]] + 1);
-        level[k] = l;
-        nlev = std::max(nlev, l + 1);
-    }
-    for (int k = 0; k < n; ++k) {
-        lo.clear();
-        for (int p = rp[k]; p < rp[k + 1]; ++p) {
-            const int j = ci[p];
-            if (j == k && diag[k] < 0) diag[k] = p;
-            if (pos[j] < pos[k]) lo.emplace_back(pos[j], p);
-        }
-        BSP_CHECK(diag[k] >= 0, BSP_E_NUMERIC,
-                  std::string(chol ? "IChol0" : "ILU0") + ": missing diagonal block in row " + std::to_string(k));
-        std::sort(lo.begin(), lo.end());
-        for (auto [pi, pki] : lo) {
-            const int i = ci[pki];
-            task_p.push_back(pki);
-            task_i.push_back(i);
-            task_u.push_back((int32_t)src.size());
-            for (int p = rp[k]; p < rp[k + 1]; ++p) {
-                const int j = ci[p];
-                if (!chol) {
-                    // A_kj -= L_ki U_ij for pos[j] > pos[i] and (i, j) in the pattern
-                    if (pos[j] <= pi) continue;
-                    const int pij = find_col(rp, ci, i, j);
-                    if (pij < 0) continue;
-                    src.push_back(pij);
-                    dst.push_back(p);
-                } else {
-                    // L_kj -= L_ki L_ji^T for pos[i] < pos[j] <= pos[k], A(i, j) and L(j, i) present
-                    if (pos[j] <= pi || pos[j] > pos[k]) continue;
-                    if (find_col(rp, ci, i, j) < 0) continue;
-                    const int pji = find_col(rp, ci, j, i);
-                    if (pji < 0) continue;
-                    src.push_back(pji);
-                    dst.push_back(p);
-                }
-            }
-        }
-        row_task[k + 1] = (int32_t)task_p.size();
-    }
-    task_u.push_back((int32_t)src.size());
-    // rows sorted by level, ascending row within a level
-    std::vector<int32_t> lev_cnt(nlev + 1, 0);
-    for (int k = 0; k < n; ++k) lev_cnt[level[k] + 1]++;
-    for (int l = 0; l < nlev; ++l) lev_cnt[l + 1] += lev_cnt[l];
-    P.level_ptr.assign(lev_cnt.begin(), lev_cnt.end());
-    std::vector<int32_t> rows(n), fill(lev_cnt.begin(), lev_cnt.end() - 1);
-    for (int k = 0; k < n; ++k) rows[fill[level[k]]++] = k;
-    P.n = n;
-    P.ntasks = (int64_t)task_p.size();
-    P.nupd = (int64_t)src.size();
-    BSP_HIP(upload_vec(P.rows, rows));
-    BSP_HIP(upload_vec(P.row_task, row_task));
-    BSP_HIP(upload_vec(P.task_p, task_p));
-    BSP_HIP(upload_vec(P.task_i, task_i));
-    BSP_HIP(upload_vec(P.task_u, task_u));
-    BSP_HIP(upload_vec(P.upd_src, src));
-    BSP_HIP(upload_vec(P.upd_dst, dst));
-    BSP_HIP(upload_vec(P.diag, diag));
-    return BSP_OK;
-}
-
-int factor_device(bsp_ctx* ctx, const bsp_bsr* A, const FactorPlan& P, bool chol, DevMem& F, DevMem& dinv) {
+int factor_device(bsp_ctx* ctx, const bsp_bsr* A, const int* pos_dev, const Sweep& lower, bool chol, DevMem& F,
+                  DevMem& dinv, DevMem& diag) {
     const int n = A->n_rows, bs = A->bs;
     BSP_CHECK(factor_device_supported(bs), BSP_E_UNSUPPORTED, "device factorisation supports block sizes 1..8");
+    BSP_CHECK(lower.n == n && (int)lower.level_ptr.size() == lower.nlevels + 1, BSP_E_INVALID,
+              "device factorisation needs the forward sweep's schedule");
     const size_t b2 = (size_t)bs * bs;
     BSP_HIP(hipSetDevice(ctx->device));
     BSP_HIP(F.alloc(sizeof(double) * std::max<size_t>((size_t)A->nnzb * b2, 1)));
     BSP_HIP(dinv.alloc(sizeof(double) * std::max<size_t>((size_t)n * b2, 1)));
+    BSP_HIP(diag.alloc(sizeof(int) * std::max(n, 1)));
     if (A->nnzb)
         BSP_HIP(hipMemcpyAsync(F.p, A->val.p, sizeof(double) * A->nnzb * b2, hipMemcpyDeviceToDevice, ctx->stream));
     DevMem err;
-    BSP_HIP(err.alloc(sizeof(int)));
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, err.as<int>(), INT_MAX);
-    PlanArgs pa{P.rows.as<int>(),   P.row_task.as<int>(), P.task_p.as<int>(),  P.task_i.as<int>(),
-                P.task_u.as<int>(), P.upd_src.as<int>(),  P.upd_dst.as<int>(), P.diag.as<int>()};
-    const int nlev = (int)P.level_ptr.size() - 1;
-    for (int l = 0; l < nlev; ++l) {
-        const int s0 = P.level_ptr[l], cnt = P.level_ptr[l + 1] - s0;
+    BSP_HIP(err.alloc(2 * sizeof(int)));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, err.as<int>(), INT_MAX, 2);
+    FactorArgs fa{A->rp.as<int>(), A->ci.as<int>(), pos_dev, lower.order.as<int>(), diag.as<int>(), err.as<int>()};
+    for (int l = 0; l < lower.nlevels; ++l) {
+        const int s0 = lower.level_ptr[l], cnt = lower.level_ptr[l + 1] - s0;
         if (cnt <= 0) continue;
-        const bool ok = chol ? launch_level_bs<true>(bs, ctx->stream, pa, s0, cnt, F.as<double>(), dinv.as<double>(),
-                                                     err.as<int>())
-                             : launch_level_bs<false>(bs, ctx->stream, pa, s0, cnt, F.as<double>(),
-                                                      dinv.as<double>(), err.as<int>());
+        const bool ok = chol ? launch_level_bs<true>(bs, ctx->stream, fa, s0, cnt, F.as<double>(), dinv.as<double>())
+                             : launch_level_bs<false>(bs, ctx->stream, fa, s0, cnt, F.as<double>(), dinv.as<double>());
         if (!ok) return fail(ctx, BSP_E_UNSUPPORTED, "device factorisation: block size");
     }
     BSP_HIP(hipGetLastError());
-    return check_err(ctx, err, chol ? "IChol0: pivot block is not positive definite"
-                                    : "ILU0: singular pivot block");
+    return check_err(ctx, err, chol ? "IChol0: pivot block is not positive definite" : "ILU0: singular pivot block",
+                     chol ? "IChol0: missing diagonal block" : "ILU0: missing diagonal block");
 }
 
 int invert_diag_device(bsp_ctx* ctx, int n, int bs, const int* diag, const double* F, void* out, void* out_t,
                        int precision) {
     DevMem err;
     BSP_HIP(err.alloc(sizeof(int)));
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, err.as<int>(), INT_MAX);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, ctx->stream, err.as<int>(), INT_MAX, 1);
     bool ok;
     if (n > 0) {
         if (precision == BSP_PREC_BF16)

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <ctime>
 #include <memory>
 #include <string>
 #include <vector>
@@ -109,6 +110,9 @@ struct Sweep {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
     }
 };
+
+// Host timing of set-up phases, printed to stderr when the context's verbose option is on.
+struct PhaseTimer;
 
 }  // namespace bsp
 
@@ -360,6 +364,29 @@ enum : int {
     SLOT_H = 64,      // Hessenberg column scratch (<= 1024 entries)
     SLOT_Y = 1200,    // combination coefficients
     SLOT_P = 1400,    // restart rotation matrix (<= 2048 entries)
+};
+
+}  // namespace bsp
+
+namespace bsp {
+
+struct PhaseTimer {
+    const bsp_ctx* ctx;
+    const char* what;
+    double t0;
+    static double now() {
+        timespec ts;
+        clock_gettime(CLOCK_MONOTONIC, &ts);
+        return ts.tv_sec + 1e-9 * ts.tv_nsec;
+    }
+    PhaseTimer(const bsp_ctx* c, const char* w) : ctx(c), what(w), t0(now()) {}
+    void mark(const char* phase) {
+        if (ctx->verbose <= 0) return;
+        const double t = now();
+        std::fprintf(stderr, "[setup %s] %s %.2f ms\n", what, phase, (t - t0) * 1e3);
+        std::fflush(stderr);
+        t0 = t;
+    }
 };
 
 }  // namespace bsp

@@ -297,8 +297,8 @@ int bsp_mpnn_graph_create(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, 
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "the learned correction needs a square matrix");
     BSP_CHECK(A->bs >= 1 && A->bs <= 8, BSP_E_UNSUPPORTED, "the learned correction supports block sizes 1..8");
-    BSP_CHECK(M->kind == BSP_PRE_ILU0 && M->n == A->n_rows && M->bs == A->bs && !M->f_val.empty(), BSP_E_INVALID,
-              "the factor to correct must be an ILU0 preconditioner of A");
+    BSP_CHECK(M->kind == BSP_PRE_ILU0 && M->n == A->n_rows && M->bs == A->bs && (!M->f_val.empty() || M->fdev.p),
+              BSP_E_INVALID, "the factor to correct must be an ILU0 preconditioner of A");
     BSP_HIP(hipSetDevice(ctx->device));
     const int n = A->n_rows, bs = A->bs, b2 = bs * bs;
     const auto& rp = A->h_rp;
@@ -375,7 +375,13 @@ int bsp_mpnn_graph_create(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, 
     BSP_HIP(up_i(g->tslot, tslot));
     BSP_HIP(g->lower.alloc(std::max<size_t>(nnzb, 1)));
     BSP_HIP(hipMemcpy(g->lower.p, lower.data(), nnzb, hipMemcpyHostToDevice));
-    {
+    if (M->fdev.p) {
+        // the factor is on the device (A's block layout): FP32 copy there
+        const int64_t cnt = nnzb * b2;
+        BSP_HIP(g->lu32.alloc(sizeof(float) * std::max<int64_t>(cnt, 1)));
+        if (int rc = launch_d2f(ctx, cnt, M->fdev.as<double>(), g->lu32.as<float>())) return rc;
+        BSP_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
         std::vector<float> lu(M->f_val.begin(), M->f_val.end());
         BSP_HIP(g->lu32.alloc(sizeof(float) * std::max<size_t>(lu.size(), 1)));
         BSP_HIP(hipMemcpy(g->lu32.p, lu.data(), sizeof(float) * lu.size(), hipMemcpyHostToDevice));

@@ -413,10 +413,18 @@ int ilu0_factor_host(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, std:
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "ILU0 needs a square matrix");
     if (int rc = elimination_order(ctx, A->n_rows, colour, elim, pos)) return rc;
     if (!factor_device_supported(A->bs) || !ctx->factor_device) return ilu0_factor_rows_host(ctx, A, elim, pos, LU);
-    FactorPlan P;
-    DevMem F, dinv;
-    int rc = plan_factor(ctx, A, pos, false, P);
-    if (!rc) rc = factor_device(ctx, A, P, false, F, dinv);
+    const int n = A->n_rows;
+    std::vector<int32_t> lrp(n + 1, 0), lci;
+    for (int i = 0; i < n; ++i) {
+        for (int p = A->h_rp[i]; p < A->h_rp[i + 1]; ++p)
+            if (pos[A->h_ci[p]] < pos[i]) lci.push_back(A->h_ci[p]);
+        lrp[i + 1] = (int32_t)lci.size();
+    }
+    Sweep lower;
+    build_schedule_topo(lower, n, A->bs, lrp, lci, elim, [](int, int) { return true; });
+    DevMem dpos, F, dinv, diag;
+    BSP_HIP(upload_ints(dpos, pos));
+    int rc = factor_device(ctx, A, dpos.as<int>(), lower, false, F, dinv, diag);
     if (rc) return rc;
     LU.resize((size_t)A->nnzb * A->bs * A->bs);
     BSP_HIP(hipMemcpy(LU.data(), F.p, sizeof(double) * LU.size(), hipMemcpyDeviceToHost));
@@ -488,15 +496,16 @@ int ilu0_build(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
     return BSP_OK;
 }
 
-// Device preconditioner from an ILU(0) factor F on the device in A's block layout (F is taken
-// over as the preconditioner's copy of the factor): the strict parts are gathered on the device
-// in the storage precision, the inverses of U's diagonal blocks computed on the device.
-int ilu0_build_device(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
-                      const std::vector<int32_t>& pos, const int* diag_dev, DevMem& F, int precision,
-                      bsp_precond** out) {
+// Device ILU(0) preconditioner for blocks up to 8 x 8, factored on the device in the
+// elimination order elim / pos: sweep structures and schedules from A's structure (host), the
+// factor level by level over the forward sweep's schedule, then the strict parts gathered on
+// the device in the storage precision and the inverses of U's diagonal blocks computed there.
+int ilu0_create_device(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>& elim,
+                       const std::vector<int32_t>& pos, int precision, bsp_precond** out) {
     BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
               "ILU0 precision must be F64, F32 or BF16");
     BSP_HIP(hipSetDevice(ctx->device));
+    PhaseTimer tm(ctx, "ilu0");
     const int n = A->n_rows, bs = A->bs;
     const size_t b2 = (size_t)bs * bs;
     const auto& rp = A->h_rp;
@@ -511,6 +520,8 @@ int ilu0_build_device(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>
     std::vector<int32_t> lrp(n + 1, 0), lci, lmap, urp(n + 1, 0), uci, umap;
     lci.reserve(A->nnzb);
     uci.reserve(A->nnzb);
+    lmap.reserve(A->nnzb);
+    umap.reserve(A->nnzb);
     for (int i = 0; i < n; ++i) {
         for (int p = rp[i]; p < rp[i + 1]; ++p) {
             const int c = ci[p];
@@ -527,24 +538,33 @@ int ilu0_build_device(bsp_ctx* ctx, const bsp_bsr* A, const std::vector<int32_t>
     }
     BSP_HIP(upload_pattern(M->lower, lrp, lci, bs, precision));
     BSP_HIP(upload_pattern(M->upper, urp, uci, bs, precision));
-    DevMem dl, du;
+    DevMem dl, du, dpos;
     BSP_HIP(upload_ints(dl, lmap));
     BSP_HIP(upload_ints(du, umap));
-    int rc = gather_blocks(ctx, (int64_t)lmap.size(), bs, dl.as<int>(), F.as<double>(), M->lower.val.p, precision, false);
+    BSP_HIP(upload_ints(dpos, pos));
+    tm.mark("sweep structures");
+    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
+    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    tm.mark("schedules");
+    DevMem F, dinv, diag;
+    int rc = factor_device(ctx, A, dpos.as<int>(), M->lower, false, F, dinv, diag);
+    if (rc) return rc;
+    tm.mark("device factorisation");
+    rc = gather_blocks(ctx, (int64_t)lmap.size(), bs, dl.as<int>(), F.as<double>(), M->lower.val.p, precision, false);
     if (!rc) rc = gather_blocks(ctx, (int64_t)umap.size(), bs, du.as<int>(), F.as<double>(), M->upper.val.p, precision, false);
     if (rc) return rc;
     const size_t esz = precision == BSP_PREC_BF16 ? 2 : precision == BSP_PREC_F32 ? 4 : 8;
     BSP_HIP(M->dinv_upper.alloc(esz * std::max<size_t>(n * b2, 1)));
-    if ((rc = invert_diag_device(ctx, n, bs, diag_dev, F.as<double>(), M->dinv_upper.p, nullptr, precision)))
+    if ((rc = invert_diag_device(ctx, n, bs, diag.as<int>(), F.as<double>(), M->dinv_upper.p, nullptr, precision)))
         return fail(ctx, BSP_E_NUMERIC, "ILU0: " + ctx->last_error);
     if (M->f32) BSP_HIP(M->work32.alloc(sizeof(float) * std::max<size_t>((size_t)n * bs, 1)));
-    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
-    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
-    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    tm.mark("gathers + inverses");
     rc = build_sweep_ell(ctx, M->lower, lrp);
     if (!rc) rc = build_sweep_ell(ctx, M->upper, urp);
     if (rc) return rc;
     BSP_HIP(hipStreamSynchronize(ctx->stream));
+    tm.mark("ELL copies");
     M->elim_order = elim;
     M->f_rp = rp;
     M->f_ci = ci;
@@ -568,12 +588,7 @@ int create_ilu0_impl(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, int 
     std::vector<int32_t> elim, pos;
     if (factor_device_supported(A->bs) && ctx->factor_device) {
         if (int rc = elimination_order(ctx, A->n_rows, colour, elim, pos)) return rc;
-        FactorPlan P;
-        DevMem F, dinv;
-        int rc = plan_factor(ctx, A, pos, false, P);
-        if (!rc) rc = factor_device(ctx, A, P, false, F, dinv);
-        if (rc) return rc;
-        return ilu0_build_device(ctx, A, elim, pos, P.diag.as<int>(), F, precision, out);
+        return ilu0_create_device(ctx, A, elim, pos, precision, out);
     }
     std::vector<double> LU;
     int rc = ilu0_factor_host(ctx, A, colour, elim, pos, LU);
@@ -734,19 +749,19 @@ int ichol0_create_device(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, 
     const size_t b2 = (size_t)bs * bs;
     const auto& rp = A->h_rp;
     const auto& ci = A->h_ci;
+    PhaseTimer tm(ctx, "ichol0");
     std::vector<int32_t> elim, pos;
     if (int rc = elimination_order(ctx, n, colour, elim, pos)) return rc;
-    FactorPlan P;
-    DevMem F, dinv;
-    int rc = plan_factor(ctx, A, pos, true, P);
-    if (!rc) rc = factor_device(ctx, A, P, true, F, dinv);
-    if (rc) return rc;
     auto M = std::make_unique<bsp_precond>();
     M->ctx = ctx;
     M->kind = BSP_PRE_ICHOL0;
     M->n = n;
     M->bs = bs;
     std::vector<int32_t> Lrp(n + 1, 0), Lci, fmap, lrp(n + 1, 0), lci, lmap, ucnt(n + 1, 0);
+    Lci.reserve(A->nnzb);
+    fmap.reserve(A->nnzb);
+    lci.reserve(A->nnzb);
+    lmap.reserve(A->nnzb);
     for (int k = 0; k < n; ++k) {
         for (int p = rp[k]; p < rp[k + 1]; ++p) {
             const int c = ci[p];
@@ -776,10 +791,20 @@ int ichol0_create_device(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, 
     }
     BSP_HIP(upload_pattern(M->lower, lrp, lci, bs, BSP_PREC_F64));
     BSP_HIP(upload_pattern(M->upper, urp, uci, bs, BSP_PREC_F64));
-    DevMem dl, du, df;
+    DevMem dl, du, df, dpos;
     BSP_HIP(upload_ints(dl, lmap));
     BSP_HIP(upload_ints(du, umap));
     BSP_HIP(upload_ints(df, fmap));
+    BSP_HIP(upload_ints(dpos, pos));
+    tm.mark("sweep structures");
+    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
+    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
+    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    tm.mark("schedules");
+    DevMem F, dinv, diag;
+    int rc = factor_device(ctx, A, dpos.as<int>(), M->lower, true, F, dinv, diag);
+    if (rc) return rc;
+    tm.mark("device factorisation");
     BSP_HIP(M->fdev.alloc(sizeof(double) * std::max<size_t>(fmap.size() * b2, 1)));
     rc = gather_blocks(ctx, (int64_t)lmap.size(), bs, dl.as<int>(), F.as<double>(), M->lower.val.p, BSP_PREC_F64, false);
     if (!rc) rc = gather_blocks(ctx, (int64_t)umap.size(), bs, du.as<int>(), F.as<double>(), M->upper.val.p, BSP_PREC_F64, true);
@@ -787,16 +812,15 @@ int ichol0_create_device(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* colour, 
     if (rc) return rc;
     BSP_HIP(M->dinv_lower.alloc(sizeof(double) * std::max<size_t>(n * b2, 1)));
     BSP_HIP(M->dinv_upper.alloc(sizeof(double) * std::max<size_t>(n * b2, 1)));
-    if ((rc = invert_diag_device(ctx, n, bs, P.diag.as<int>(), F.as<double>(), M->dinv_lower.p, M->dinv_upper.p,
+    if ((rc = invert_diag_device(ctx, n, bs, diag.as<int>(), F.as<double>(), M->dinv_lower.p, M->dinv_upper.p,
                                  BSP_PREC_F64)))
         return fail(ctx, BSP_E_NUMERIC, "IChol0: singular factor block (" + ctx->last_error + ")");
-    std::vector<int32_t> rev(elim.rbegin(), elim.rend());
-    build_schedule_topo(M->lower, n, bs, lrp, lci, elim, [](int, int) { return true; });
-    build_schedule_topo(M->upper, n, bs, urp, uci, rev, [](int, int) { return true; });
+    tm.mark("gathers + inverses");
     rc = build_sweep_ell(ctx, M->lower, lrp);
     if (!rc) rc = build_sweep_ell(ctx, M->upper, urp);
     if (rc) return rc;
     BSP_HIP(hipStreamSynchronize(ctx->stream));
+    tm.mark("ELL copies");
     M->elim_order = elim;
     M->f_rp.swap(Lrp);
     M->f_ci.swap(Lci);
