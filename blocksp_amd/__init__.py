@@ -80,6 +80,7 @@ _SIGS = {
     "bsp_pgmres_ritz": [_VP, _VP, _VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsp_bsr_save": [_VP, _VP, C.c_char_p],
     "bsp_orthogonalize": [_VP, C.c_int64, _VP, C.c_int, _VP, _VP, _VP],
+    "bsp_orthogonalize_noalloc": [_VP, C.c_int64, _VP, C.c_int, _VP, _VP, _VP],
     "bsp_bsr_load": [_VP, C.c_char_p, C.POINTER(_VP)],
     "bsp_arnoldi_eigenvalues": [_VP, _VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int, C.POINTER(C.c_int)],
     "bsp_bsmv_dim_transpose": [_VP, _VP, C.c_int, C.c_int, _VP, _VP, C.c_int],
@@ -651,6 +652,16 @@ def orthogonalize(ctx, Q, w: DeviceArray, qnew: DeviceArray):
     h = np.zeros(m + 1)
     ctx.check(ctx.L.bsp_orthogonalize(ctx.h, int(w.size), ptrs, m, w.ptr, _ptr(h), qnew.ptr), "orthogonalize")
     return h
+
+
+def orthogonalize_noalloc(ctx, Q, w: DeviceArray, h: DeviceArray, qnew: DeviceArray):
+    """bsp_orthogonalize_noalloc: the same step on the library's static scratch, no allocation and
+    no synchronisation; h (device, m + 2 doubles) receives the coefficients, ||w|| and the
+    second-pass indicator."""
+    m = len(Q)
+    ptrs = (C.c_void_p * m)(*[q.ptr.value for q in Q])
+    ctx.check(ctx.L.bsp_orthogonalize_noalloc(ctx.h, int(w.size), ptrs, m, w.ptr, h.ptr, qnew.ptr),
+              "orthogonalize_noalloc")
 
 
 def pgmres_ritz(ctx, cap=256):

@@ -74,6 +74,10 @@ int bsp_ctx_create(int device, void* stream, bsp_ctx** out) {
         bsp_ctx_destroy(ctx);
         return BSP_E_HIP;
     }
+    if (int rc = bsp::noalloc_scratch_init(ctx)) {
+        bsp_ctx_destroy(ctx);
+        return rc;
+    }
     *out = ctx;
     return BSP_OK;
 }

@@ -19,6 +19,7 @@
 #include "bsp_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace bsp {
 
@@ -404,6 +405,35 @@ int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, 
 
 size_t cgs_scratch_doubles() { return (size_t)(kMaxBasis + 1) * kMaxParts + kMaxParts + 2 * kMaxBasis + 2; }
 
+namespace {
+
+// Scratch of the allocation-free entry point (bsp_orthogonalize_noalloc): the per-workgroup
+// partials and reductions of one Gram-Schmidt step with up to kMaxBasis vectors, then the
+// reorthogonalisation flag in the last slot -- 33,922 doubles (265 KiB) on every device the
+// library runs on, held until the process ends.  In host code `scratch` names a host-side
+// stand-in: kernels get the device address taken from hipGetSymbolAddress per device.
+constexpr size_t kNoAllocDoubles = (size_t)(kMaxBasis + 1) * kMaxParts + kMaxParts + 2 * kMaxBasis + 2 + 1;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};   // device address of `scratch` on each device
+
+}  // namespace
+
+// Set-up of the allocation-free entry point, called by bsp_ctx_create with the context's
+// device current: records that device's address of `scratch` once.
+int noalloc_scratch_init(const bsp_ctx* ctx) {
+    BSP_CHECK(ctx->device >= 0 && ctx->device < kMaxDevices, BSP_E_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[ctx->device]) return BSP_OK;
+    double* ptr = nullptr;
+    BSP_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    BSP_CHECK(ptr != nullptr, BSP_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_dev[ctx->device] = ptr;
+    return BSP_OK;
+}
+
 // Q: host array of m device pointers; hcol: m + 2 device doubles; scratch: cgs_scratch_doubles()
 // doubles; flag: one int.  second: 1 = conditional second pass decided on the device, 0 = first
 // pass only (hcol[m + 1] says whether a second pass is needed), 2 = second pass only.
@@ -441,6 +471,24 @@ int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     if (h_host) BSP_HIP(hipMemcpy(h_host, hcol.p, sizeof(double) * (m + 1), hipMemcpyDeviceToHost));
     return BSP_OK;
+}
+
+int bsp_orthogonalize_noalloc(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs, int m, double* w_dev,
+                              double* h_dev, double* qnew_dev) {
+    if (!ctx || !Q_dev_ptrs || !w_dev || !h_dev || !qnew_dev || len <= 0) return BSP_E_INVALID;
+    BSP_CHECK(m >= 1 && m <= kMaxBasis, BSP_E_INVALID,
+              "orthogonalize_noalloc: 1 <= m <= 64 vectors (the static scratch holds the partials of 65 sums)");
+    double* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        ptr = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch_dev[ctx->device] : nullptr;
+    }
+    BSP_CHECK(ptr != nullptr, BSP_E_INVALID, "orthogonalize_noalloc: scratch not set up for this device");
+    BSP_HIP(hipSetDevice(ctx->device));
+    // partials and sums are written before they are read in every call; the flag is written by
+    // the first pass's reduction before the gated second pass reads it: nothing to clear
+    int* flag = reinterpret_cast<int*>(ptr + cgs_scratch_doubles());
+    return launch_arnoldi_cgs(ctx, len, w_dev, Q_dev_ptrs, m, h_dev, qnew_dev, nullptr, ptr, flag, 1, nullptr);
 }
 
 }  // extern "C"

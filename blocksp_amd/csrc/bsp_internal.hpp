@@ -292,6 +292,9 @@ int launch_arnoldi_mgs(const bsp_ctx* ctx, int64_t len, double* w, const double*
 // scratch: cgs_scratch_doubles() device doubles; flag: one device int.  hcol_host (second == 0,
 // optional): pinned host memory the kernel writes hcol[0..m+1] to directly.
 size_t cgs_scratch_doubles();
+// Records this context's device address of the static scratch of bsp_orthogonalize_noalloc
+// (bsp_cgs.hip); called once per context at creation.
+int noalloc_scratch_init(const bsp_ctx* ctx);
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const double* const* Q, int m, double* hcol,
                        double* qnew, float* znext, double* scratch, int* flag, int second, double* hcol_host);
 int launch_arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const float* const* Q, int m, double* hcol,

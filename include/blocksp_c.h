@@ -262,6 +262,16 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
  * w_dev is overwritten with the orthogonalised w. */
 int bsp_orthogonalize(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs, int m, double* w_dev,
                       double* h_host, double* qnew_dev);
+/* Allocation-free variant of bsp_orthogonalize.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  The same kernels run on a file-scope static device scratch array of 265 KiB per GPU
+ * (set up by bsp_ctx_create, kept until the process ends) instead of per-call allocations; the
+ * call makes no hipMalloc / hipFree and does not synchronise.  h_dev: m + 2 device doubles
+ * (h_dev[0..m-1] coefficients, h_dev[m] = ||w||, h_dev[m + 1] = 1 when the second pass ran).
+ * m above 64 is rejected with BSP_E_INVALID.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams or host threads) race. */
+int bsp_orthogonalize_noalloc(bsp_ctx* ctx, int64_t len, const double* const* Q_dev_ptrs, int m, double* w_dev,
+                              double* h_dev, double* qnew_dev);
 
 /* Ritz-value diagnostics (printEigInfo, blocksp_solvers.hpp:272-288): eigenvalues of the
  * square part of the last bsp_pgmres run's Hessenberg on this context -- harmonic Ritz values
