@@ -85,6 +85,11 @@ int bsp_ctx_create(int device, void* stream, bsp_ctx** out) {
 int bsp_ctx_destroy(bsp_ctx* ctx) {
     if (!ctx) return BSP_OK;
     (void)hipSetDevice(ctx->device);
+    if (ctx->aux_stream) {
+        (void)hipStreamSynchronize(ctx->aux_stream);
+        (void)hipStreamDestroy(ctx->aux_stream);
+        ctx->aux_stream = nullptr;
+    }
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& e : ctx->events)
         if (e) (void)hipEventDestroy(e);
@@ -147,6 +152,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "batch_ell") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "batch_ell must be 0 or 1");
         ctx->batch_ell = (int)value;
+    } else if (k == "deflate_device") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "deflate_device must be 0 or 1");
+        ctx->deflate_device = (int)value;
     } else if (k == "factor_device") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "factor_device must be 0 or 1");
         ctx->factor_device = (int)value;

@@ -88,28 +88,28 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     Pool<VT> V;
     Pool<ZT> Z;
     V.len = Z.len = (size_t)len;
-    DevMem hcol, ydev, Pdev, Pkdev;
+    DevMem hcol;
     BSP_HIP(hcol.alloc(sizeof(double) * 2 * (m_max + 2)));
-    BSP_HIP(ydev.alloc(sizeof(double) * (m_max + 2)));
-    Pinned hpin, ypin, ppin;
-    // H column slots; the classical Gram-Schmidt kernels write them directly (coherent, mapped,
-    // through the device address of the same memory)
+    // least squares and deflated restart on the device
+    DeviceLS ls;
+    if (int rc = ls_alloc(ctx, ls, m_max, restart > 0 ? restart : 0)) return rc;
+    Pinned hpin, outp;
+    // H column slots (for the reorthogonalisation flag); the classical Gram-Schmidt kernels write
+    // them directly (coherent, mapped, through the device address of the same memory)
     BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * 2 * (m_max + 2),
                           hipHostMallocCoherent | hipHostMallocMapped));
     double* hpin_dev = nullptr;
     BSP_HIP(hipHostGetDevicePointer((void**)&hpin_dev, hpin.p, 0));
-    BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (m_max + 2), hipHostMallocDefault));
+    // per-column {residual estimate, |h_{j+1,j}|} from the device least squares
+    BSP_HIP(hipHostMalloc((void**)&outp.p, sizeof(double) * 2 * (m_max + 1), hipHostMallocCoherent | hipHostMallocMapped));
+    double* out_dev = nullptr;
+    BSP_HIP(hipHostGetDevicePointer((void**)&out_dev, outp.p, 0));
     // classical Gram-Schmidt (default): scratch and the reorthogonalisation flag
     const bool use_cgs = vf32 || ctx->fgmres_ortho == 0;
     DevMem cgs_scratch, cgs_flag;
     if (use_cgs) {
         BSP_HIP(cgs_scratch.alloc(sizeof(double) * cgs_scratch_doubles()));
         BSP_HIP(cgs_flag.alloc(sizeof(int)));
-    }
-    if (restart > 0) {
-        BSP_HIP(Pdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
-        BSP_HIP(Pkdev.alloc(sizeof(double) * (m_max + 1) * (k + 2)));
-        BSP_HIP(hipHostMalloc((void**)&ppin.p, sizeof(double) * 2 * (m_max + 1) * (k + 2), hipHostMallocDefault));
     }
     double* sc = ctx->scalars.as<double>();
     // r = b - A x (FP64), beta = ||r||, V[0] = r / beta
@@ -130,29 +130,31 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     rc = launch_scale_div(ctx, len, r_work, sc + SLOT_TMP0, v0);
     if (rc) return rc;
     Vb.push_back(v0);
+    BSP_HIP(hipMemcpyAsync(sc + SLOT_BETA, sc + SLOT_TMP0, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = ls_fresh(ctx, ls, sc + SLOT_BETA))) return rc;
     const double zero_thresh = std::min(beta * eps, eps);
     const double target = beta * reduction;
-    std::vector<double> H, c_r, y, res_vec, res_hist;
-    int Hm = 0, Hn = 0;
-    auto H_resize = [&](int m, int nn) {
-        std::vector<double> Hnew((size_t)m * nn, 0.0);
-        for (int i = 0; i < std::min(m, Hm); ++i)
-            for (int j = 0; j < std::min(nn, Hn); ++j) Hnew[(size_t)i * nn + j] = H[(size_t)i * Hn + j];
-        H.swap(Hnew);
-        Hm = m;
-        Hn = nn;
-    };
+    std::vector<double> res_hist;
     const bool f32_pre = M->f32;
     const size_t hslot = (size_t)m_max + 2;   // H column slots: step j uses slot j % 2
+    // e[slot]: the step's Gram-Schmidt done (main stream); e[2 + slot]: its least-squares update
+    // done (second stream)
     struct Events {
-        hipEvent_t e[2] = {nullptr, nullptr};
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
         ~Events() {
             for (hipEvent_t x : e)
                 if (x) (void)hipEventDestroy(x);
         }
     } ev;
     for (hipEvent_t& x : ev.e) BSP_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
-    bool restarted = false;
+    // the least-squares updates run on a second stream: the next Arnoldi step does not need them,
+    // only the host (the residual estimate) and the next update do
+    if (!ctx->aux_stream) BSP_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    hipStream_t ls_stream = ctx->aux_stream;
+    struct SyncAux {
+        hipStream_t s;
+        ~SyncAux() { (void)hipStreamSynchronize(s); }
+    } sync_aux{ls_stream};
     double res = beta;
     int iter = 0, tot = 0;
     // w of the step in slot 0 reuses r_work (r is not needed after V[0] is formed); slot 1 has
@@ -161,10 +163,12 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
     if (use_cgs) BSP_HIP(w1.alloc(sizeof(double) * std::max<int64_t>(len, 1)));
     double* wbuf[2] = {r_work, use_cgs ? w1.as<double>() : r_work};
     // One Arnoldi step for column `col`, enqueued without waiting: z = P(V[col]), w = A z,
-    // Gram-Schmidt of w against V[0..col], q = w / h -> V[col + 1] (appended to Vb at once),
-    // and the H column copied to host slot `slot` (event ev.e[slot]).  Classical Gram-Schmidt
-    // runs its first pass only; the flag copied back with the column says whether the rare
-    // second pass is needed, which the host then runs before using the column.
+    // Gram-Schmidt of w against V[0..col], q = w / h -> V[col + 1] (appended to Vb at once), event
+    // ev.e[slot]; then, on the second stream after that event, the column rotated into the device
+    // least squares (which writes the residual estimate for the host), event ev.e[2 + slot].
+    // Classical Gram-Schmidt runs its first pass only; the flag it writes with the H column to
+    // host slot `slot` says whether the rare second pass is needed -- then the least-squares update
+    // of this column skips itself and the host applies the corrected column.
     // With an FP32 preconditioner the Gram-Schmidt step also writes the FP32 copy of V[col + 1]
     // (znext), which the next step preconditions in place instead of converting V[col + 1].
     struct Step {
@@ -233,35 +237,38 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
                 r2 = launch_arnoldi_mgs(ctx, len, w, Vb.data(), col, hd, s.vn, ctx->red_partials.as<double>());
         }
         if (r2) return r2;
-        // classical Gram-Schmidt wrote the column (and its flag) to the pinned slot itself
-        if (!use_cgs)
-            BSP_HIP(hipMemcpyAsync(hpin.p + slot * hslot, hd, sizeof(double) * (col + 2), hipMemcpyDeviceToHost,
-                                   ctx->stream));
         BSP_HIP(hipEventRecord(ev.e[slot], ctx->stream));
+        BSP_HIP(hipStreamWaitEvent(ls_stream, ev.e[slot], 0));
+        r2 = ls_column(ctx, ls, col, hd, out_dev + 2 * col, use_cgs ? hd + col + 2 : nullptr, ls_stream);
+        if (r2) return r2;
+        BSP_HIP(hipEventRecord(ev.e[2 + slot], ls_stream));
         Vb.push_back(s.vn);
         return BSP_OK;
     };
-    // The step after the current one is enqueued before the host waits for the current H
-    // column, so the device works on step j + 1 while the host solves the least-squares
-    // problem of step j.  A step enqueued past the end of the pass (convergence, breakdown,
-    // the iteration limit) is dropped unused; none is enqueued past a restart.
+    // The step after the current one is enqueued before the host waits for the current step, so
+    // the device works on step j + 1 while the host reads step j's residual estimate.  A step
+    // enqueued past the end of the pass (convergence, breakdown, the iteration limit) is dropped
+    // unused; none is enqueued past a restart.
     Step cur, nxt;
     if (res > target) {
         rc = enqueue_step(iter, 0, cur, nullptr);
         if (rc) return rc;
     }
+    int nc = 0;   // columns of the current cycle
     while (res > target) {
         const bool spec = (restart <= 0 || iter + 1 < restart) && tot + 1 < maxIters;
         if (spec) {
             rc = enqueue_step(iter + 1, cur.slot ^ 1, nxt, cur.znext);
             if (rc) return rc;
         }
-        BSP_HIP(hipEventSynchronize(ev.e[cur.slot]));
+        BSP_HIP(hipEventSynchronize(ev.e[2 + cur.slot]));
         const double* hp = hpin.p + cur.slot * hslot;
         if (use_cgs && hp[iter + 2] != 0.0) {
             // cancellation in the first Gram-Schmidt pass: drop the step enqueued on the
-            // unfinished vector, orthogonalise once more, then enqueue that step again
+            // unfinished vector, orthogonalise once more, apply the corrected column to the
+            // least squares, then enqueue that step again
             BSP_HIP(hipStreamSynchronize(ctx->stream));
+            BSP_HIP(hipStreamSynchronize(ls_stream));
             if (spec) {
                 if (nxt.z != cur.znext) Z.give(nxt.z);
                 if (nxt.znext) Z.give(nxt.znext);
@@ -272,8 +279,7 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             rc = launch_arnoldi_cgs(ctx, len, wbuf[cur.slot], Vb.data(), iter + 1, hd, cur.vn, z32(cur.znext),
                                     cgs_scratch.as<double>(), cgs_flag.as<int>(), 2, nullptr);
             if (rc) return rc;
-            BSP_HIP(hipMemcpyAsync(hpin.p + cur.slot * hslot, hd, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost,
-                                   ctx->stream));
+            if ((rc = ls_column(ctx, ls, iter, hd, out_dev + 2 * iter, nullptr))) return rc;
             BSP_HIP(hipStreamSynchronize(ctx->stream));
             ++st.reorth;
             if (spec) {
@@ -281,36 +287,19 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
                 if (rc) return rc;
             }
         }
-        H_resize(iter + 2, iter + 1);
-        for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hp[j];
-        const double hlast = hp[iter + 1];
-        const bool breakdown = std::fabs(hlast) < zero_thresh || !std::isfinite(hlast);
+        res = outp.p[2 * iter];
+        const double hlast = outp.p[2 * iter + 1];
+        const bool breakdown = hlast < zero_thresh || !std::isfinite(hlast);
         Zb.push_back(cur.z);
-        const int m = iter + 2, nn = iter + 1;
-        std::vector<double> c(m, 0.0);
-        if (!restarted) c[0] = beta;
-        else
-            for (int i = 0; i < (int)c_r.size() && i < m; ++i) c[i] = c_r[i];
-        y.assign(nn, 0.0);
-        dense::lstsq(m, nn, H.data(), c.data(), y.data());
-        res_vec.assign(m, 0.0);
-        double r2 = 0.0;
-        for (int i = 0; i < m; ++i) {
-            double s = 0.0;
-            for (int j = 0; j < nn; ++j) s += -1.0 * H[(size_t)i * nn + j] * y[j];
-            res_vec[i] = s + c[i];
-            r2 += res_vec[i] * res_vec[i];
-        }
-        res = std::sqrt(r2);
         ++iter;
         ++tot;
+        nc = iter;
         if (ctx->verbose > 0 && tot % ctx->verbose == 0) {
             std::fprintf(stderr, "[fgmres] iter %d res %.3e (target %.3e) restarts %d\n", tot, res, target, st.restarts);
             std::fflush(stderr);
         }
         if (!std::isfinite(res)) {
             st.nonfinite = true;
-            y.clear();
             break;
         }
         if (tot >= maxIters || breakdown) break;
@@ -321,38 +310,34 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
                 st.stagnated = true;
                 break;
             }
-            // x += Z y
-            for (int i = 0; i < nn; ++i) ypin.p[i] = y[i];
-            BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
-            if constexpr (z64) rc = launch_combine_add(ctx, len, nn, Zb.data(), ydev.as<double>(), x);
-            else rc = launch_combine_add_f32(ctx, len, nn, Zb.data(), ydev.as<double>(), x);
+            // x += Z y, then the deflated restart on the device (after the cycle's last update)
+            BSP_HIP(hipStreamWaitEvent(ctx->stream, ev.e[2 + cur.slot], 0));
+            if ((rc = ls_solve(ctx, ls, nc, sc + SLOT_BETA, true))) return rc;
+            if ((rc = ls_deflate_begin(ctx, ls, nc, k))) return rc;
+            // the solution update runs on the device while the host solves the eigenproblem
+            if constexpr (z64) rc = launch_combine_add(ctx, len, nc, Zb.data(), ls.y.as<double>(), x);
+            else rc = launch_combine_add_f32(ctx, len, nc, Zb.data(), ls.y.as<double>(), x);
             if (rc) return rc;
             ++st.restarts;
-            std::vector<double> P, Hnew;
-            const int kout = dense::deflate(nn, H.data(), res_vec.data(), k, P, Hnew, c_r, true);
-            BSP_CHECK(kout >= 0, BSP_E_NUMERIC, "flexible deflated restart: small eigenproblem failed");
-            const int kp = kout + 1;
-            BSP_CHECK((int)Vb.size() == m && (int)Zb.size() == nn, BSP_E_NUMERIC, "basis size mismatch");
-            // P (m x kp) and P_k (nn x kout: first kout columns, first nn rows)
-            double* Pk_host = ppin.p + (size_t)m * kp;
-            for (size_t t = 0; t < P.size(); ++t) ppin.p[t] = P[t];
-            for (int i = 0; i < nn; ++i)
-                for (int j = 0; j < kout; ++j) Pk_host[(size_t)i * kout + j] = P[(size_t)i * kp + j];
-            BSP_HIP(hipMemcpyAsync(Pdev.p, ppin.p, sizeof(double) * P.size(), hipMemcpyHostToDevice, ctx->stream));
-            BSP_HIP(hipMemcpyAsync(Pkdev.p, Pk_host, sizeof(double) * nn * kout, hipMemcpyHostToDevice, ctx->stream));
+            int kout = 0;
+            if ((rc = ls_deflate_end(ctx, ls, nc, k, true, kout))) return rc;
+            const int kp = kout + 1, m = nc + 1;
+            BSP_CHECK((int)Vb.size() == m && (int)Zb.size() == nc, BSP_E_NUMERIC, "basis size mismatch");
             std::vector<VT*> Vn(kp);
             std::vector<ZT*> Zn(kout);
             for (int j = 0; j < kp; ++j)
                 if ((rc = V.take(ctx, Vn[j]))) return rc;
             for (int j = 0; j < kout; ++j)
                 if ((rc = Z.take(ctx, Zn[j]))) return rc;
-            if constexpr (vf32) rc = launch_basis_rotate_f32(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
-            else rc = launch_basis_rotate(ctx, len, m, kp, Vb.data(), Pdev.as<double>(), Vn.data());
+            // V' = V P_{k+1} (m x kp), Z' = Z P_k (nc x kout), both on the device
+            if constexpr (vf32) rc = launch_basis_rotate_f32(ctx, len, m, kp, Vb.data(), ls.P, Vn.data());
+            else rc = launch_basis_rotate(ctx, len, m, kp, Vb.data(), ls.P, Vn.data());
             if (!rc && kout > 0) {
-                if constexpr (z64) rc = launch_basis_rotate(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
-                else rc = launch_basis_rotate_f32(ctx, len, nn, kout, Zb.data(), Pkdev.as<double>(), Zn.data());
+                if constexpr (z64) rc = launch_basis_rotate(ctx, len, nc, kout, Zb.data(), ls.Pk, Zn.data());
+                else rc = launch_basis_rotate_f32(ctx, len, nc, kout, Zb.data(), ls.Pk, Zn.data());
             }
             if (rc) return rc;
+            if ((rc = ls_restart(ctx, ls, kout))) return rc;
             // the old vectors go back to the pools at once: the kernels that will write them
             // again are enqueued on the same stream after the rotations that read them
             for (VT* p : Vb) V.give(p);
@@ -360,12 +345,8 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
             if (cur.znext) Z.give(cur.znext);   // an FP32 copy of a basis vector the rotation replaced
             Vb = Vn;
             Zb = Zn;
-            H = Hnew;
-            Hm = kp;
-            Hn = kout;
             iter = kout;
-            restarted = true;
-            y.clear();
+            nc = 0;
             rc = enqueue_step(iter, 0, cur, nullptr);
             if (rc) return rc;
             continue;
@@ -374,14 +355,15 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         BSP_CHECK(spec, BSP_E_NUMERIC, "fgmres: no step enqueued");
         cur = nxt;
     }
-    if (!y.empty()) {
-        for (size_t i = 0; i < y.size(); ++i) ypin.p[i] = y[i];
-        BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
-        if constexpr (z64) rc = launch_combine_add(ctx, len, (int)y.size(), Zb.data(), ydev.as<double>(), x);
-        else rc = launch_combine_add_f32(ctx, len, (int)y.size(), Zb.data(), ydev.as<double>(), x);
+    if (!st.nonfinite && nc > 0) {
+        BSP_HIP(hipStreamWaitEvent(ctx->stream, ev.e[2 + cur.slot], 0));
+        if ((rc = ls_solve(ctx, ls, nc, sc + SLOT_BETA, false))) return rc;
+        if constexpr (z64) rc = launch_combine_add(ctx, len, nc, Zb.data(), ls.y.as<double>(), x);
+        else rc = launch_combine_add_f32(ctx, len, nc, Zb.data(), ls.y.as<double>(), x);
         if (rc) return rc;
     }
     BSP_HIP(hipStreamSynchronize(ctx->stream));
+    BSP_HIP(hipStreamSynchronize(ls_stream));
     st.iters = tot;
     st.res = res;
     return BSP_OK;

@@ -120,6 +120,8 @@ struct bsp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t aux_stream = nullptr;   // second stream (default priority) for work off the critical
+                                        // path: the flexible GMRES's least-squares updates
     mutable std::string last_error;
     int num_cus = 256;
     int64_t lds_per_block = 0;       // device limit of LDS bytes per workgroup
@@ -141,6 +143,9 @@ struct bsp_ctx {
                                      // Gram-Schmidt steps in one single-workgroup kernel
     int sweep_lds = 1;               // such sweeps keep the solution vector in LDS when it fits
                                      // (at most 64 KiB and the device's limit per workgroup)
+    int deflate_device = 0;          // GMRES restarts: 1 solve the harmonic-Ritz eigenproblem in one workgroup on the
+                                     // device (~1.6 ms per restart at 30 columns, latency-bound), 0 on the host
+                                     // (~0.1 ms); least squares, residual vector and rotations stay on the device
     int factor_device = 1;           // ILU0 / IChol0 factorisation: 1 on the device (blocks up to 8 x 8), 0 the
                                      // host loop (the method of the reference, one row at a time)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
@@ -358,12 +363,66 @@ int launch_cg_update_xr(const bsp_ctx* ctx, int64_t len, const double* num, cons
 int launch_cg_update_p(const bsp_ctx* ctx, int64_t len, const double* num, const double* den,
                        const double* z, double* p);
 
+// Least squares and deflated restart of a GMRES solve on the device (bsp_restart.hip): the
+// Hessenberg and its Givens-rotated triangle, the rotated right-hand side, the solution y and
+// the outputs of a deflated restart (P_{k+1}, P_k, H_new, c_r, kout).
+struct DeviceLS {
+    int maxm = 0, ld = 0;
+    bool restarted = false;     // the current cycle started from a deflated restart
+    bool ws_in_lds = false;     // the deflation's workspace fits in one workgroup's LDS
+    int host_deflations = 0;    // restarts whose eigenproblem ran on the host
+    DevMem H, R, cs, sn, g, Q0t, y, res, meta, resv, tbuf, ws, wsg;
+    hipEvent_t ev = nullptr;    // the restart's copies down are complete
+    bool defl_device = false;   // the current restart runs the device route
+    // restart outputs, one device block: P ((n+1) x (n+1)) | P_k (n x n) | H_new ((n+1) x n) |
+    // c_r (n+1) | kout; packed by their actual sizes when the host computes them
+    DevMem rbuf;
+    double *P = nullptr, *Pk = nullptr, *Hnew = nullptr, *cr = nullptr;
+    int* kout = nullptr;
+    size_t rbuf_doubles = 0;
+    double* hpin = nullptr;     // pinned staging: the cycle's H and residual vector down, the
+                                // host route's restart outputs up (rbuf_doubles + (n+1)(n+2))
+    DeviceLS() = default;
+    DeviceLS(const DeviceLS&) = delete;
+    DeviceLS& operator=(const DeviceLS&) = delete;
+    ~DeviceLS() {
+        if (ev) (void)hipEventDestroy(ev);
+        if (hpin) (void)hipHostFree(hpin);
+    }
+};
+// Storage for a Hessenberg of maxm columns (restart > 0: also the restart outputs).
+int ls_alloc(bsp_ctx* ctx, DeviceLS& ls, int maxm, int restart);
+// A fresh cycle with c = beta e_1 (beta: device scalar).
+int ls_fresh(bsp_ctx* ctx, DeviceLS& ls, const double* beta_dev);
+// Column j from the Arnoldi column hcol_dev[0..j+1]; out_dev (optional, device address of
+// host-visible memory) receives {residual, |h_{j+1,j}|}; skip_dev (optional): no update when
+// *skip_dev != 0.  stream: the context's stream when null.
+int ls_column(bsp_ctx* ctx, DeviceLS& ls, int j, const double* hcol_dev, double* out_dev, const double* skip_dev,
+              hipStream_t stream = nullptr);
+// y = R^{-1} g over the first nc columns (ls.y), and with want_resv the residual vector c - H y
+// (ls.resv); beta_dev: the fresh cycle's beta (device scalar, read also when restarted).
+int ls_solve(bsp_ctx* ctx, DeviceLS& ls, int nc, const double* beta_dev, bool want_resv);
+// Deflated restart of the n-column cycle (ls.resv must hold its residual vector): ls.P, ls.Pk,
+// ls.Hnew, ls.cr on the device, kout to the host.  The harmonic-Ritz eigenproblem runs on the
+// host (default; fast: the inverse-iteration route, else the complex Schur route) or, with the
+// context option deflate_device, in one workgroup on the device (the inverse-iteration route;
+// the host Schur route where it declines).  _begin enqueues the device part and the copies
+// down; the caller may enqueue more work before _end, which waits for the copies only, runs
+// the host part and enqueues the copy up.
+int ls_deflate_begin(bsp_ctx* ctx, DeviceLS& ls, int n, int k);
+int ls_deflate_end(bsp_ctx* ctx, DeviceLS& ls, int n, int k, bool fast, int& kout);
+// Start the next cycle from the deflation's outputs.
+int ls_restart(bsp_ctx* ctx, DeviceLS& ls, int kout);
+// The first nc + 1 rows and nc columns of the Hessenberg, row-major, to the host.
+int ls_download_h(bsp_ctx* ctx, const DeviceLS& ls, int nc, std::vector<double>& H);
+
 // Scalar slots used by the solvers.
 enum : int {
     SLOT_TMP0 = 0,
     SLOT_TMP1 = 1,
     SLOT_TMP2 = 2,
     SLOT_TMP3 = 3,
+    SLOT_BETA = 4,    // beta of the current GMRES cycle (read by the device least squares)
     SLOT_H = 64,      // Hessenberg column scratch (<= 1024 entries)
     SLOT_Y = 1200,    // combination coefficients
     SLOT_P = 1400,    // restart rotation matrix (<= 2048 entries)

@@ -597,29 +597,63 @@ __global__ __launch_bounds__(kBlock) void k_combine_add(int64_t len, int m, PtrT
     x[e] = acc;
 }
 
-// Qout_j = sum_i Q_i P_ij (deflated_restart.hpp:250-258), m <= 64, kp <= 64.
+// Qout_j = sum_i Q_i P_ij (deflated_restart.hpp:250-258), m <= 64, kp <= 64.  Two consecutive
+// entries per thread (16-byte loads and stores), the basis read once per 16 output vectors,
+// four input vectors' loads issued before their FMAs.
 __global__ __launch_bounds__(kBlock) void k_basis_rotate(int64_t len, int m, int kp, PtrTable Q,
-                                                         const double* __restrict__ P, OutTable Out) {
+                                                         const double* __restrict__ P, OutTable Out, int vec) {
     __shared__ double sP[kMaxBasis * 16];
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 2;
     for (int j0 = 0; j0 < kp; j0 += 16) {
         const int jn = min(16, kp - j0);
         __syncthreads();
         for (int t = threadIdx.x; t < m * jn; t += kBlock) sP[t] = P[(t / jn) * kp + j0 + (t % jn)];
         __syncthreads();
-        if (e < len) {
-            double acc[16];
+        if (vec && e + 1 < len) {
+            double a0[16], a1[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double qv = Q.q[i][e];
+            for (int j = 0; j < 16; ++j) a0[j] = a1[j] = 0.0;
+            int i = 0;
+            for (; i + 4 <= m; i += 4) {
+                double2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const double2*>(Q.q[i + u] + e);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        if (j < jn) {
+                            a0[j] = fma(v[u].x, sP[(i + u) * jn + j], a0[j]);
+                            a1[j] = fma(v[u].y, sP[(i + u) * jn + j], a1[j]);
+                        }
+            }
+            for (; i < m; ++i) {
+                const double2 v = *reinterpret_cast<const double2*>(Q.q[i] + e);
 #pragma unroll
                 for (int j = 0; j < 16; ++j)
-                    if (j < jn) acc[j] = fma(qv, sP[i * jn + j], acc[j]);
+                    if (j < jn) {
+                        a0[j] = fma(v.x, sP[i * jn + j], a0[j]);
+                        a1[j] = fma(v.y, sP[i * jn + j], a1[j]);
+                    }
             }
 #pragma unroll
             for (int j = 0; j < 16; ++j)
-                if (j < jn) Out.q[j0 + j][e] = acc[j];
+                if (j < jn) *reinterpret_cast<double2*>(Out.q[j0 + j] + e) = make_double2(a0[j], a1[j]);
+        } else {
+            for (int64_t f = e; f < e + 2 && f < len; ++f) {
+                double acc[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    const double qv = Q.q[i][f];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        if (j < jn) acc[j] = fma(qv, sP[i * jn + j], acc[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j < jn) Out.q[j0 + j][f] = acc[j];
+            }
         }
     }
 }
@@ -906,9 +940,17 @@ int launch_basis_rotate(const bsp_ctx* ctx, int64_t len, int m, int kp, const do
     BSP_CHECK(m <= kMaxBasis && kp <= kMaxBasis, BSP_E_UNSUPPORTED, "Krylov basis larger than 64 vectors");
     PtrTable t{};
     OutTable o{};
-    for (int i = 0; i < m; ++i) t.q[i] = Q[i];
-    for (int j = 0; j < kp; ++j) o.q[j] = Qout[j];
-    hipLaunchKernelGGL(k_basis_rotate, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, m, kp, t, P, o);
+    bool vec = true;
+    for (int i = 0; i < m; ++i) {
+        t.q[i] = Q[i];
+        vec = vec && (uintptr_t)Q[i] % 16 == 0;
+    }
+    for (int j = 0; j < kp; ++j) {
+        o.q[j] = Qout[j];
+        vec = vec && (uintptr_t)Qout[j] % 16 == 0;
+    }
+    hipLaunchKernelGGL(k_basis_rotate, dim3(grid_for((len + 1) / 2)), dim3(kBlock), 0, ctx->stream, len, m, kp, t, P, o,
+                       vec ? 1 : 0);
     BSP_LAUNCH_CHECK();
     return BSP_OK;
 }

@@ -61,10 +61,18 @@ int check_sweep_errors(bsp_ctx* ctx) {
 
 // One pGMRES run in the reference's semantics, starting from x (x0 = x on entry).
 // tolerance is absolute on the preconditioned least-squares residual estimate.
+// The least squares stays on the device (bsp_restart.hip): each Arnoldi step's column is
+// rotated into the device triangle by the step's last kernel, which also writes the residual
+// estimate to host-visible memory.  The host keeps up to kAhead steps in flight and stops at
+// the first step whose estimate meets the tolerance (or breaks down, or reaches maxIters):
+// steps enqueued after it only add columns beyond the ones the solution uses.  At a restart
+// the solution update, the residual vector and the deflation's eigenproblem run on the device;
+// the host reads back kout once per restart.
 int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
                bool tol_is_relative, int maxIters, int restart, int k, int& iters_out, int& restarts_out,
                double& res0_out, double& res_out, bool& converged, int& applies, int stag_window,
                bool& stagnated) {
+    constexpr int kAhead = 3;
     stagnated = false;
     std::vector<double> res_hist;
     const int n = A->n_rows, bs = A->bs;
@@ -72,20 +80,29 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     const double eps = std::numeric_limits<double>::epsilon();
     double* sc = ctx->scalars.as<double>();
     const int maxm = restart > 0 ? restart : maxIters + 1;
-    // Hessenberg column (device) and its pinned host mirror, sized for the largest basis
     DevMem hcol_mem;
     BSP_HIP(hcol_mem.alloc(sizeof(double) * (maxm + 2)));
+    DeviceLS ls;
+    if (int rc = ls_alloc(ctx, ls, maxm, restart > 0 && k > 0 ? restart : 0)) return rc;
+    // per-column {residual estimate, |h_{j+1,j}|}, written by the device (coherent, mapped)
     struct Pinned {
         double* p = nullptr;
         ~Pinned() { if (p) (void)hipHostFree(p); }
-    } hpin, ypin;
-    BSP_HIP(hipHostMalloc((void**)&hpin.p, sizeof(double) * (maxm + 2), hipHostMallocDefault));
-    BSP_HIP(hipHostMalloc((void**)&ypin.p, sizeof(double) * (maxm + 2), hipHostMallocDefault));
-    DevMem ydev;
-    BSP_HIP(ydev.alloc(sizeof(double) * (maxm + 2)));
+    } outp;
+    BSP_HIP(hipHostMalloc((void**)&outp.p, sizeof(double) * 2 * (maxm + 1), hipHostMallocCoherent | hipHostMallocMapped));
+    double* out_dev = nullptr;
+    BSP_HIP(hipHostGetDevicePointer((void**)&out_dev, outp.p, 0));
+    hipEvent_t ev[kAhead + 1] = {};
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() {
+            for (int i = 0; i <= kAhead; ++i)
+                if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+    } evg{ev};
+    for (hipEvent_t& e : ev) BSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     VecPool pool;
     pool.len = (size_t)len;
-    // basis slots: Q[0..m] plus k+2 spare vectors for the restart rotation, plus v
     BSP_HIP(pool.grow(2));
     std::vector<double*> Q;             // current basis (device pointers into pool)
     std::vector<double*> spare;
@@ -122,161 +139,154 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     rc = launch_scale_div(ctx, len, q0, sc + SLOT_TMP0, q0);
     if (rc) return rc;
     Q.push_back(q0);
+    // the cycle's beta stays in its own slot (SLOT_TMP0 is reused by the reductions)
+    BSP_HIP(hipMemcpyAsync(sc + SLOT_BETA, sc + SLOT_TMP0, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = ls_fresh(ctx, ls, sc + SLOT_BETA))) return rc;
     const double zero_thresh = std::min(beta * eps, eps);
     const double tolerance = tol_is_relative ? beta * tol : tol;
-    // host Hessenberg (row-major, capacity (maxm+1) x maxm) and LS right-hand side
-    std::vector<double> H, c_r, y, res_vec;
-    int Hm = 0, Hn = 0;
-    auto H_resize = [&](int m, int nn) {
-        std::vector<double> Hnew((size_t)m * nn, 0.0);
-        for (int i = 0; i < std::min(m, Hm); ++i)
-            for (int j = 0; j < std::min(nn, Hn); ++j) Hnew[(size_t)i * nn + j] = H[(size_t)i * Hn + j];
-        H.swap(Hnew);
-        Hm = m;
-        Hn = nn;
-    };
-    bool restarted = false;
     double res = beta;
     int iter = 0, tot = 0;
     double* hcol_dev = hcol_mem.as<double>();
-    while (res > tolerance) {
-        // ---- one_arnoldi (solvers.hpp:109-136) ----
-        H_resize(iter + 2, iter + 1);
+    // One Arnoldi step for column j (one_arnoldi, solvers.hpp:109-136), enqueued without waiting.
+    auto enqueue = [&](int j) -> int {
         if (ctx->rec_buf && ctx->rec_len == len && ctx->rec_count < ctx->rec_cap &&
             ctx->rec_seen++ % ctx->rec_every == 0) {
-            BSP_HIP(hipMemcpyAsync(ctx->rec_buf + (size_t)ctx->rec_count * len, Q[iter], sizeof(double) * len,
+            BSP_HIP(hipMemcpyAsync(ctx->rec_buf + (size_t)ctx->rec_count * len, Q[j], sizeof(double) * len,
                                    hipMemcpyDeviceToDevice, ctx->stream));
             ++ctx->rec_count;
         }
-        rc = bsmv_matrix(ctx, A, Q[iter], v, 0);
-        if (rc) return rc;
-        rc = bsp_precond_apply(ctx, M, v);
+        int r2 = bsmv_matrix(ctx, A, Q[j], v, 0);
+        if (r2) return r2;
+        r2 = bsp_precond_apply(ctx, M, v);
         ++applies;
-        if (rc) return rc;
+        if (r2) return r2;
         double* qn = nullptr;
-        rc = take(qn);
-        if (rc) return rc;
-        rc = launch_arnoldi_mgs(ctx, len, v, Q.data(), iter, hcol_dev, qn, ctx->red_partials.as<double>());
-        if (rc) return rc;
-        BSP_HIP(hipMemcpyAsync(hpin.p, hcol_dev, sizeof(double) * (iter + 2), hipMemcpyDeviceToHost, ctx->stream));
-        BSP_HIP(hipStreamSynchronize(ctx->stream));
-        for (int j = 0; j < iter + 2; ++j) H[(size_t)j * Hn + iter] = hpin.p[j];
-        const double hlast = hpin.p[iter + 1];
-        const bool breakdown = std::fabs(hlast) < zero_thresh || !std::isfinite(hlast);
-        if (!breakdown) Q.push_back(qn);
-        else spare.push_back(qn);
-        // ---- least squares (solvers.hpp:192-232) ----
-        const int m = iter + 2, nn = iter + 1;
-        std::vector<double> c(m, 0.0);
-        if (!restarted) c[0] = beta;
-        else
-            for (int i = 0; i < (int)c_r.size() && i < m; ++i) c[i] = c_r[i];
-        y.assign(nn, 0.0);
-        dense::lstsq(m, nn, H.data(), c.data(), y.data());
-        res_vec.assign(m, 0.0);
-        double r2 = 0.0;
-        for (int i = 0; i < m; ++i) {
-            double s = 0.0;
-            for (int j = 0; j < nn; ++j) s += -1.0 * H[(size_t)i * nn + j] * y[j];
-            res_vec[i] = s + c[i];
-            r2 += res_vec[i] * res_vec[i];
+        if ((r2 = take(qn))) return r2;
+        r2 = launch_arnoldi_mgs(ctx, len, v, Q.data(), j, hcol_dev, qn, ctx->red_partials.as<double>());
+        if (r2) return r2;
+        Q.push_back(qn);   // speculative: dropped again if the run stops before this column
+        if ((r2 = ls_column(ctx, ls, j, hcol_dev, out_dev + 2 * j, nullptr))) return r2;
+        BSP_HIP(hipEventRecord(ev[j % (kAhead + 1)], ctx->stream));
+        return BSP_OK;
+    };
+    bool done = false;
+    int nc = 0;   // columns of the cycle the solution uses
+    while (!done) {
+        const int cycle_end = restart > 0 ? restart : maxm;
+        int enq = iter;
+        bool stop = false;
+        while (iter < cycle_end) {
+            // keep up to kAhead steps in flight, never past maxIters + 1 steps in total
+            while (enq < cycle_end && enq - iter < kAhead && tot + (enq - iter) <= maxIters) {
+                if ((rc = enqueue(enq))) return rc;
+                ++enq;
+            }
+            if (enq == iter) break;   // nothing left to run (maxIters)
+            BSP_HIP(hipEventSynchronize(ev[iter % (kAhead + 1)]));
+            res = outp.p[2 * iter];
+            const double hlast = outp.p[2 * iter + 1];
+            const bool breakdown = hlast < zero_thresh || !std::isfinite(hlast);
+            ++iter;
+            ++tot;
+            if (ctx->verbose > 0 && tot % ctx->verbose == 0) {
+                std::fprintf(stderr, "[pgmres] iter %d res %.3e (target %.3e) restarts %d\n", tot, res, tolerance,
+                             restarts_out);
+                std::fflush(stderr);
+            }
+            if (res <= tolerance || tot > maxIters || breakdown || !std::isfinite(res)) {
+                stop = true;
+                if (breakdown) {
+                    // one_arnoldi leaves the basis one vector short (solvers.hpp:133)
+                }
+                break;
+            }
         }
-        res = std::sqrt(r2);
-        ++iter;
-        ++tot;
-        if (ctx->verbose > 0 && tot % ctx->verbose == 0) {
-            std::fprintf(stderr, "[pgmres] iter %d res %.3e (target %.3e) restarts %d\n", tot, res, tolerance,
-                         restarts_out);
-            std::fflush(stderr);
+        // drop the speculative basis vectors past the columns in use (steps still in flight
+        // write them; the stream orders those writes before any later use of the buffers)
+        while ((int)Q.size() > iter + 1) {
+            spare.push_back(Q.back());
+            Q.pop_back();
         }
-        if (tot > maxIters || breakdown || !std::isfinite(res)) break;
-        if (iter == restart && res > tolerance && stag_window > 0) {
+        nc = iter;
+        if (stop || iter < cycle_end) {
+            done = true;
+            break;
+        }
+        // ---- restart (iter == restart, not converged) ----
+        if (stag_window > 0) {
             // stagnation of the restarted recurrence (true-residual mode only): less than a
             // factor 2 gained over the last stag_window restarts
             res_hist.push_back(res);
             const size_t h = res_hist.size();
             if (h > (size_t)stag_window && res > 0.5 * res_hist[h - 1 - stag_window]) {
                 stagnated = true;
+                done = true;
                 break;
             }
         }
-        if (iter == restart && res > tolerance) {
-            // x0 += Q y (solvers.hpp:242-245); the reference's r0 recomputation (248) is only
-            // consumed by a pure restart and is skipped for k > 0.
-            for (int i = 0; i < nn; ++i) ypin.p[i] = y[i];
-            BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->stream));
-            rc = launch_combine_add(ctx, len, nn, Q.data(), ydev.as<double>(), x);
-            if (rc) return rc;
-            ++restarts_out;
-            if (k == 0) {
-                // pure restart: r0 = M^{-1}(b - A x0), Q = [r0/||r0||]
-                for (double* p : Q) spare.push_back(p);
-                Q.clear();
-                double* r0 = nullptr;
-                rc = take(r0);
-                if (rc) return rc;
-                rc = launch_residual(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x, r0);
-                if (rc) return rc;
-                rc = bsp_precond_apply(ctx, M, r0);
-                ++applies;
-                if (rc) return rc;
-                rc = reduce(ctx, 1, len, r0, r0, SLOT_TMP0, &beta);
-                if (rc) return rc;
-                rc = launch_scale_div(ctx, len, r0, sc + SLOT_TMP0, r0);
-                if (rc) return rc;
-                Q.push_back(r0);
-                H.clear();
-                Hm = Hn = 0;
-                iter = 0;
-                restarted = false;
-                res = beta;
-                y.clear();
-                continue;
-            }
-            std::vector<double> P, Hnew;
-            const int kout = dense::deflate(nn, H.data(), res_vec.data(), k, P, Hnew, c_r);
-            BSP_CHECK(kout >= 0, BSP_E_NUMERIC, "deflated restart: small eigenproblem failed");
-            const int kp = kout + 1;
-            // Q_new = Q P  (Q has m = nn+1 vectors)
-            BSP_CHECK((size_t)m * kp <= (size_t)(bsp_ctx::kScalarSlots - SLOT_P), BSP_E_UNSUPPORTED,
-                      "restart rotation too large");
-            for (size_t t = 0; t < P.size(); ++t) ctx->pinned[SLOT_P + t] = P[t];
-            BSP_HIP(hipMemcpyAsync(sc + SLOT_P, ctx->pinned + SLOT_P, sizeof(double) * P.size(), hipMemcpyHostToDevice,
-                                   ctx->stream));
-            std::vector<double*> Qnew(kp);
-            for (int j = 0; j < kp; ++j) {
-                rc = take(Qnew[j]);
-                if (rc) return rc;
-            }
-            BSP_CHECK((int)Q.size() == m, BSP_E_NUMERIC, "basis size mismatch at restart");
-            rc = launch_basis_rotate(ctx, len, m, kp, Q.data(), sc + SLOT_P, Qnew.data());
-            if (rc) return rc;
-            BSP_HIP(hipStreamSynchronize(ctx->stream));   // pinned buffer reuse
+        // x0 += Q y (solvers.hpp:242-245); the reference's r0 recomputation (248) is only
+        // consumed by a pure restart and is skipped for k > 0.
+        if ((rc = ls_solve(ctx, ls, nc, sc + SLOT_BETA, k > 0))) return rc;
+        if (k > 0 && (rc = ls_deflate_begin(ctx, ls, nc, k))) return rc;
+        // the solution update runs on the device while the host solves the restart's eigenproblem
+        rc = launch_combine_add(ctx, len, nc, Q.data(), ls.y.as<double>(), x);
+        if (rc) return rc;
+        ++restarts_out;
+        if (k == 0) {
+            // pure restart: r0 = M^{-1}(b - A x0), Q = [r0/||r0||]
             for (double* p : Q) spare.push_back(p);
-            Q = Qnew;
-            H = Hnew;
-            Hm = kp;
-            Hn = kout;
-            iter = kout;
-            restarted = true;
-            y.clear();
+            Q.clear();
+            double* r0 = nullptr;
+            if ((rc = take(r0))) return rc;
+            rc = launch_residual(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), b, x, r0);
+            if (rc) return rc;
+            rc = bsp_precond_apply(ctx, M, r0);
+            ++applies;
+            if (rc) return rc;
+            rc = reduce(ctx, 1, len, r0, r0, SLOT_TMP0, &beta);
+            if (rc) return rc;
+            rc = launch_scale_div(ctx, len, r0, sc + SLOT_TMP0, r0);
+            if (rc) return rc;
+            Q.push_back(r0);
+            BSP_HIP(hipMemcpyAsync(sc + SLOT_BETA, sc + SLOT_TMP0, sizeof(double), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+            if ((rc = ls_fresh(ctx, ls, sc + SLOT_BETA))) return rc;
+            iter = 0;
+            res = beta;
+            continue;
         }
+        int kout = 0;
+        if ((rc = ls_deflate_end(ctx, ls, nc, k, false, kout))) return rc;
+        const int kp = kout + 1;
+        const int m = nc + 1;
+        BSP_CHECK((int)Q.size() == m, BSP_E_NUMERIC, "basis size mismatch at restart");
+        std::vector<double*> Qnew(kp);
+        for (int j = 0; j < kp; ++j)
+            if ((rc = take(Qnew[j]))) return rc;
+        // Q_new = Q P (P: m x kp on the device)
+        rc = launch_basis_rotate(ctx, len, m, kp, Q.data(), ls.P, Qnew.data());
+        if (rc) return rc;
+        if ((rc = ls_restart(ctx, ls, kout))) return rc;
+        for (double* p : Q) spare.push_back(p);
+        Q = Qnew;
+        iter = kout;
     }
     // x = x0 + Q y (solvers.hpp:290-294)
-    if (!y.empty()) {
-        for (size_t i = 0; i < y.size(); ++i) ypin.p[i] = y[i];
-        BSP_HIP(hipMemcpyAsync(ydev.p, ypin.p, sizeof(double) * y.size(), hipMemcpyHostToDevice, ctx->stream));
-        rc = launch_combine_add(ctx, len, (int)y.size(), Q.data(), ydev.as<double>(), x);
+    if (nc > 0 && std::isfinite(res)) {
+        if ((rc = ls_solve(ctx, ls, nc, sc + SLOT_BETA, false))) return rc;
+        rc = launch_combine_add(ctx, len, nc, Q.data(), ls.y.as<double>(), x);
         if (rc) return rc;
     }
     BSP_HIP(hipStreamSynchronize(ctx->stream));
     iters_out = tot;
     res_out = res;
     converged = res <= tolerance;
-    ctx->last_H = H;
-    ctx->last_Hm = Hm;
-    ctx->last_Hn = Hn;
-    ctx->last_restarted = restarted;
+    std::vector<double> Hh;
+    if ((rc = ls_download_h(ctx, ls, nc, Hh))) return rc;
+    ctx->last_H = Hh;
+    ctx->last_Hm = nc + 1;
+    ctx->last_Hn = nc;
+    ctx->last_restarted = ls.restarted;
     return check_sweep_errors(ctx);
 }
 

@@ -185,39 +185,95 @@ struct FOut {
     float* q[kMaxBasis];
 };
 
-// x += sum_j c_j Z_j (FP64 accumulation of FP32 vectors)
+// x += sum_j c_j Z_j (FP64 accumulation of FP32 vectors), two entries per thread, four
+// vectors' loads issued before their FMAs.
 __global__ __launch_bounds__(kBlock) void k_combine_add_f32(int64_t len, int m, FPtrs Z, const double* __restrict__ c,
-                                                            double* __restrict__ x) {
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= len) return;
-    double acc = x[e];
-    for (int j = 0; j < m; ++j) acc += c[j] * (double)Z.q[j][e];
-    x[e] = acc;
+                                                            double* __restrict__ x, int vec) {
+    const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 2;
+    if (vec && e + 1 < len) {
+        double2 acc = *reinterpret_cast<const double2*>(x + e);
+        int j = 0;
+        for (; j + 4 <= m; j += 4) {
+            float2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const float2*>(Z.q[j + u] + e);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc.x += c[j + u] * (double)v[u].x;
+                acc.y += c[j + u] * (double)v[u].y;
+            }
+        }
+        for (; j < m; ++j) {
+            const float2 v = *reinterpret_cast<const float2*>(Z.q[j] + e);
+            acc.x += c[j] * (double)v.x;
+            acc.y += c[j] * (double)v.y;
+        }
+        *reinterpret_cast<double2*>(x + e) = acc;
+    } else {
+        for (int64_t f = e; f < e + 2 && f < len; ++f) {
+            double acc = x[f];
+            for (int j = 0; j < m; ++j) acc += c[j] * (double)Z.q[j][f];
+            x[f] = acc;
+        }
+    }
 }
 
-// Zout_j = sum_i Z_i P_ij (FP64 accumulation, FP32 storage), m, kp <= 64
+// Zout_j = sum_i Z_i P_ij (FP64 accumulation, FP32 storage), m, kp <= 64; two entries per
+// thread, four input vectors' loads issued before their FMAs.
 __global__ __launch_bounds__(kBlock) void k_basis_rotate_f32(int64_t len, int m, int kp, FPtrs Z,
-                                                             const double* __restrict__ P, FOut Out) {
+                                                             const double* __restrict__ P, FOut Out, int vec) {
     __shared__ double sP[kMaxBasis * 16];
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 2;
     for (int j0 = 0; j0 < kp; j0 += 16) {
         const int jn = min(16, kp - j0);
         __syncthreads();
         for (int t = threadIdx.x; t < m * jn; t += kBlock) sP[t] = P[(t / jn) * kp + j0 + (t % jn)];
         __syncthreads();
-        if (e < len) {
-            double acc[16];
+        if (vec && e + 1 < len) {
+            double a0[16], a1[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double zv = (double)Z.q[i][e];
+            for (int j = 0; j < 16; ++j) a0[j] = a1[j] = 0.0;
+            int i = 0;
+            for (; i + 4 <= m; i += 4) {
+                float2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const float2*>(Z.q[i + u] + e);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        if (j < jn) {
+                            a0[j] = fma((double)v[u].x, sP[(i + u) * jn + j], a0[j]);
+                            a1[j] = fma((double)v[u].y, sP[(i + u) * jn + j], a1[j]);
+                        }
+            }
+            for (; i < m; ++i) {
+                const float2 v = *reinterpret_cast<const float2*>(Z.q[i] + e);
 #pragma unroll
                 for (int j = 0; j < 16; ++j)
-                    if (j < jn) acc[j] = fma(zv, sP[i * jn + j], acc[j]);
+                    if (j < jn) {
+                        a0[j] = fma((double)v.x, sP[i * jn + j], a0[j]);
+                        a1[j] = fma((double)v.y, sP[i * jn + j], a1[j]);
+                    }
             }
 #pragma unroll
             for (int j = 0; j < 16; ++j)
-                if (j < jn) Out.q[j0 + j][e] = (float)acc[j];
+                if (j < jn) *reinterpret_cast<float2*>(Out.q[j0 + j] + e) = make_float2((float)a0[j], (float)a1[j]);
+        } else {
+            for (int64_t f = e; f < e + 2 && f < len; ++f) {
+                double acc[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    const double zv = (double)Z.q[i][f];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        if (j < jn) acc[j] = fma(zv, sP[i * jn + j], acc[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j < jn) Out.q[j0 + j][f] = (float)acc[j];
+            }
         }
     }
 }
@@ -314,8 +370,13 @@ int launch_combine_add_f32(const bsp_ctx* ctx, int64_t len, int m, const float* 
     for (int j0 = 0; j0 < m; j0 += kMaxBasis) {
         const int mm = std::min(kMaxBasis, m - j0);
         FPtrs t{};
-        for (int j = 0; j < mm; ++j) t.q[j] = Z[j0 + j];
-        hipLaunchKernelGGL(k_combine_add_f32, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, mm, t, c + j0, x);
+        bool vec = (uintptr_t)x % 16 == 0;
+        for (int j = 0; j < mm; ++j) {
+            t.q[j] = Z[j0 + j];
+            vec = vec && (uintptr_t)Z[j0 + j] % 8 == 0;
+        }
+        hipLaunchKernelGGL(k_combine_add_f32, dim3(grid_for((len + 1) / 2)), dim3(kBlock), 0, ctx->stream, len, mm, t,
+                           c + j0, x, vec ? 1 : 0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(ctx, BSP_E_HIP, hipGetErrorString(e));
     }
@@ -327,9 +388,17 @@ int launch_basis_rotate_f32(const bsp_ctx* ctx, int64_t len, int m, int kp, cons
     if (m > kMaxBasis || kp > kMaxBasis) return fail(ctx, BSP_E_UNSUPPORTED, "basis larger than 64 vectors");
     FPtrs t{};
     FOut o{};
-    for (int i = 0; i < m; ++i) t.q[i] = Z[i];
-    for (int j = 0; j < kp; ++j) o.q[j] = Zout[j];
-    hipLaunchKernelGGL(k_basis_rotate_f32, dim3(grid_for(len)), dim3(kBlock), 0, ctx->stream, len, m, kp, t, P, o);
+    bool vec = true;
+    for (int i = 0; i < m; ++i) {
+        t.q[i] = Z[i];
+        vec = vec && (uintptr_t)Z[i] % 8 == 0;
+    }
+    for (int j = 0; j < kp; ++j) {
+        o.q[j] = Zout[j];
+        vec = vec && (uintptr_t)Zout[j] % 8 == 0;
+    }
+    hipLaunchKernelGGL(k_basis_rotate_f32, dim3(grid_for((len + 1) / 2)), dim3(kBlock), 0, ctx->stream, len, m, kp, t,
+                       P, o, vec ? 1 : 0);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
 }
