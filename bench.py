@@ -166,6 +166,44 @@ def tight_reference(bsp, ctx, A, b, p, colour):
     return xref, f"all-FP64 coloured FGMRES to true relres {st.true_relres:.1e} (oracle bsmv: {rel:.1e})"
 
 
+def bench_bsmm(bsp, ctx, A, p, mfma=1, reps=5):
+    """C = A A: both passes on the device; the numeric pass timed alone.  Algorithmic bytes:
+    A's values read twice (as left and right factor), C's written once, and the three block
+    structures (row pointers, column indices)."""
+    ctx.set_option("bsmm_mfma", mfma)
+    t0 = time.perf_counter()
+    A2 = bsp.bsmm(ctx, A, A)
+    ctx.sync()
+    t_first = time.perf_counter() - t0
+    t_num = timed_ms(ctx, lambda: bsp.bsmm_numeric(ctx, A, A, A2), reps)
+    b2 = p.bs * p.bs
+    products = int(np.diff(np.asarray(p.rowptr, dtype=np.int64))[np.asarray(p.colidx)].sum())
+    alg = (2 * p.nnzb * b2 * 8 + A2.nnzb * b2 * 8 + 2 * (4 * p.nnzb + 4 * (p.n + 1))
+           + 4 * A2.nnzb + 4 * (p.n + 1))
+    gbps = alg / t_num / 1e6
+    out = {"matrix": p.name, "bs": p.bs, "nnzb_A": p.nnzb, "nnzb_C": A2.nnzb, "block_products": products,
+           "first_call_s": round(t_first, 3), "matrix_cores": bool(mfma), "numeric_us": round(t_num * 1e3, 1),
+           "algorithmic_bytes": alg, "numeric_GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4),
+           "numeric_GFLOPs": round(2.0 * products * p.bs ** 3 / t_num / 1e6, 1)}
+    log(f"bsmm(A,A) {p.name}: first call {t_first:.2f}s (both passes on the device), numeric "
+        f"{t_num * 1e3:.0f} us, {gbps:.0f} GB/s ({gbps / HBM_PEAK_GBPS:.3f} of HBM)")
+    del A2
+    ctx.set_option("bsmm_mfma", 1)
+    return out
+
+
+def bench_bsmm_all(bsp, pr, ctx):
+    """C = A A on the 2D (512^2 cells) and 3D (64^3 cells) DG matrices, 8 x 8 blocks, on the
+    matrix cores and on the FMA row kernel."""
+    res = []
+    for p in (pr.dg2d_veclap(512, 512, seed=1), pr.dg3d_lap(64, 64, 64, seed=4)):
+        A = bsp.BSR.from_problem(ctx, p)
+        for mf in (1, 0):
+            res.append(bench_bsmm(bsp, ctx, A, p, mfma=mf))
+        del A
+    return {"bsmm": res}
+
+
 def bench_bf16(bsp, pr, ctx, solve=True):
     """3D DG Laplacian 64^3 (262144 block rows, bs 8, 7 blocks/row): BF16 MFMA block SpMM
     with 32 N(0,1) right-hand sides against FP64, and the mixed-precision solve against
@@ -433,7 +471,7 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
     ap.add_argument("--skip-extras", action="store_true", help="skip the 3D BF16, batched and learned workloads")
-    ap.add_argument("--only", choices=["bf16", "batch", "learned", "pcg", "c1"], default=None,
+    ap.add_argument("--only", choices=["bf16", "batch", "learned", "pcg", "c1", "bsmm"], default=None,
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -456,7 +494,8 @@ def main():
               "batch": lambda: bench_batch(bsp, pr, ctx, baseline_every=512),
               "learned": lambda: bench_learned(bsp, pr, ctx, train_only=True),
               "pcg": lambda: bench_pcg(bsp, pr, ctx),
-              "c1": lambda: bench_c1(bsp, pr, ctx)}[args.only]
+              "c1": lambda: bench_c1(bsp, pr, ctx),
+              "bsmm": lambda: bench_bsmm_all(bsp, pr, ctx)}[args.only]
         print(json.dumps(fn()))
         return
 
@@ -477,19 +516,7 @@ def main():
     roofline = {"kernel": "k_bsmv_perm<8> (chunk-major values, XCD-aware rows, streaming loads)", "bound": "hbm", "achieved": round(bsmv_gbps, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(bsmv_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "algorithmic_bytes": bsmv_bytes, "avg_us": round(bsmv_ms * 1e3, 2)}
-    # bsmm(A, A): symbolic pass on the host (first-touch order), numeric pass on the device
-    t0 = time.perf_counter()
-    A2 = bsp.bsmm(ctx, A, A)
-    ctx.sync()
-    t_bsmm_total = time.perf_counter() - t0
-    t_num = timed_ms(ctx, lambda: bsp.bsmm_numeric(ctx, A, A, A2), 5)
-    n_pairs = int(sum((p.rowptr[c + 1] - p.rowptr[c]) for c in p.colidx))
-    bsmm_bytes = 2 * p.nnzb * p.bs * p.bs * 8 + A2.nnzb * p.bs * p.bs * 8 + n_pairs * 8 + (A2.nnzb + 1) * 4
-    bsmm = {"nnzb_C": A2.nnzb, "pairs": n_pairs, "first_call_s": round(t_bsmm_total, 3),
-            "numeric_us": round(t_num * 1e3, 1), "numeric_GBps_unique": round(bsmm_bytes / t_num / 1e6, 1),
-            "numeric_GFLOPs": round(2.0 * n_pairs * p.bs ** 3 / t_num / 1e6, 1)}
-    log(f"bsmm(A,A): first call {t_bsmm_total:.2f}s (symbolic on host), numeric {t_num * 1e3:.0f} us")
-    del A2
+    bsmm = bench_bsmm(bsp, ctx, A, p)
     if args.bsmv_only:
         # calibration kernel of known traffic for the FETCH_SIZE / WRITE_SIZE counters:
         # axpy on 2^21 doubles reads 2 x 16 MiB and writes 16 MiB per launch

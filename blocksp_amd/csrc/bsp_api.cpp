@@ -152,6 +152,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "batch_ell") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "batch_ell must be 0 or 1");
         ctx->batch_ell = (int)value;
+    } else if (k == "bsmm_mfma") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "bsmm_mfma must be 0 or 1");
+        ctx->bsmm_mfma = (int)value;
     } else if (k == "deflate_device") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "deflate_device must be 0 or 1");
         ctx->deflate_device = (int)value;

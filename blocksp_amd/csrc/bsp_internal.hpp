@@ -143,6 +143,7 @@ struct bsp_ctx {
                                      // Gram-Schmidt steps in one single-workgroup kernel
     int sweep_lds = 1;               // such sweeps keep the solution vector in LDS when it fits
                                      // (at most 64 KiB and the device's limit per workgroup)
+    int bsmm_mfma = 1;               // bsmm numeric pass for 8 x 8 blocks: 1 on the matrix cores (FP64 MFMA), 0 FMA
     int deflate_device = 0;          // GMRES restarts: 1 solve the harmonic-Ritz eigenproblem in one workgroup on the
                                      // device (~1.6 ms per restart at 30 columns, latency-bound), 0 on the host
                                      // (~0.1 ms); least squares, residual vector and rotations stay on the device
@@ -192,9 +193,10 @@ struct bsp_bsr {
     mutable bool h_val_stale = false;  // values produced on the device (bsmm): copy back on first host use
     // lazily built transpose (for bsmv_transpose)
     mutable std::unique_ptr<bsp_bsr> transposed;
-    // C = A B (or A^T B) from bsp_bsmm: contributing (A block, B block) pairs per block of C
-    bsp::DevMem mm_pair_ptr, mm_pairs;
+    // C = A B (or A^T B) from bsp_bsmm: the factors its numeric pass reads, and (8 x 8 blocks)
+    // each row's product schedule over the matrix-core streams (mm_soff[i]: row i's first entry)
     const bsp_bsr* mm_A = nullptr;
+    bsp::DevMem mm_soff, mm_sched;
     const bsp_bsr* mm_B = nullptr;
     bool mm_Atran = false;             // C = A^T B: the pairs index A's explicit transpose
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)

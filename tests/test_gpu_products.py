@@ -159,3 +159,70 @@ def test_bsmm_transA_rejects_mismatched_rows(ctx):
     Bm = bsp.BSR(ctx, 6, 4, 2, rp2, ci2, v2)
     with pytest.raises(Exception):
         bsp.bsmm(ctx, A, Bm, Atran=True)
+
+
+@pytest.mark.parametrize("bs,width", [(1, 12), (4, 9), (8, 7), (16, 6), (5, 11)])
+def test_bsmm_wide_rows_and_block_sizes(ctx, bs, width):
+    """Rows of C with more blocks than one register pass holds (16), empty rows, and the block
+    sizes of the row kernel: structure bit-exact (first-touch order), values to 1e-13 of |A||B|."""
+    rng = np.random.default_rng(bs * 100 + width)
+    n, m, k = 40, 37, 45
+
+    def rand(nr, nc, w):
+        rows = []
+        for i in range(nr):
+            cnt = 0 if i % 13 == 5 else int(rng.integers(1, w + 1))
+            rows.append(rng.choice(nc, size=cnt, replace=False))   # unsorted: insertion order
+        rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+        ci = np.concatenate(rows).astype(np.int32)
+        return rp, ci, rng.standard_normal((len(ci), bs, bs))
+
+    arp, aci, av = rand(n, m, width)
+    brp, bci, bv = rand(m, k, width)
+    Cm = bsp.bsmm(ctx, bsp.BSR(ctx, n, m, bs, arp, aci, av), bsp.BSR(ctx, m, k, bs, brp, bci, bv))
+    crp, cci, cv = Cm.download()
+    orp, oci, ov = po.bsmm(n, bs, arp, aci, av, brp, bci, bv)
+    np.testing.assert_array_equal(crp, orp)
+    np.testing.assert_array_equal(cci, oci)
+    assert np.diff(orp).max() > 16        # some rows need more than one pass
+    _, _, bound = po.bsmm(n, bs, arp, aci, np.abs(av), brp, bci, np.abs(bv))
+    assert np.all(np.abs(cv - ov) <= 1e-13 * bound + 1e-300)
+
+
+@pytest.mark.parametrize("mfma", [0, 1])
+def test_bsmm_8x8_fma_and_matrix_core_paths(ctx, mfma):
+    """The 8 x 8 numeric pass on the matrix cores (v_mfma_f64_16x16x4f64, two product streams
+    per 16 x 16 tile) and the FMA row kernel both meet the oracle's bound, on the DG operator,
+    on random insertion-ordered rows wider than one register pass and on dense block rows."""
+    ctx.set_option("bsmm_mfma", mfma)
+    try:
+        p = pr.dg3d_lap(5, 4, 6, seed=2)
+        A = bsp.BSR.from_problem(ctx, p)
+        rp, ci, v = bsp.bsmm(ctx, A, A).download()
+        orp, oci, ov = po.bsmm(p.n, 8, p.rowptr, p.colidx, p.vals, p.rowptr, p.colidx, p.vals)
+        _, _, bound = po.bsmm(p.n, 8, p.rowptr, p.colidx, np.abs(p.vals), p.rowptr, p.colidx, np.abs(p.vals))
+        np.testing.assert_array_equal(rp, orp)
+        np.testing.assert_array_equal(ci, oci)
+        assert np.all(np.abs(v - ov) <= 1e-13 * bound + 1e-300)
+        n, m, k = 33, 29, 31
+        arp, aci, av = _rand_bsr(n, m, 8, seed=91, dens=0.4)
+        brp, bci, bv = _rand_bsr(m, k, 8, seed=92, dens=0.4)
+        rp, ci, v = bsp.bsmm(ctx, bsp.BSR(ctx, n, m, 8, arp, aci, av), bsp.BSR(ctx, m, k, 8, brp, bci, bv)).download()
+        orp, oci, ov = po.bsmm(n, 8, arp, aci, av, brp, bci, bv)
+        _, _, bound = po.bsmm(n, 8, arp, aci, np.abs(av), brp, bci, np.abs(bv))
+        np.testing.assert_array_equal(ci, oci)
+        assert np.all(np.abs(v - ov) <= 1e-13 * bound + 1e-300)
+        # dense block rows: 24 blocks of 24 products each per row of C (many steps per stream)
+        n = 24
+        rng = np.random.default_rng(93)
+        rp = (np.arange(n + 1) * n).astype(np.int32)
+        ci = np.concatenate([rng.permutation(n) for _ in range(n)]).astype(np.int32)
+        dv = rng.standard_normal((n * n, 8, 8))
+        D = bsp.BSR(ctx, n, n, 8, rp, ci, dv)
+        rp2, ci2, v2 = bsp.bsmm(ctx, D, D).download()
+        orp, oci, ov = po.bsmm(n, 8, rp, ci, dv, rp, ci, dv)
+        _, _, bound = po.bsmm(n, 8, rp, ci, np.abs(dv), rp, ci, np.abs(dv))
+        np.testing.assert_array_equal(ci2, oci)
+        assert np.all(np.abs(v2 - ov) <= 1e-13 * bound + 1e-300)
+    finally:
+        ctx.set_option("bsmm_mfma", 1)
