@@ -218,19 +218,24 @@ def bench_bf16(bsp, pr, ctx, solve=True):
     Yb = ctx.empty((p.n * nrhs * 8,), np.float32)
     Y64 = ctx.empty(X.shape)
     t_bf = timed_ms(ctx, lambda: bsp.bsmm_rhs_bf16(ctx, A, nrhs, Xb, Yb), 20)
-    t_64 = timed_ms(ctx, lambda: bsp.bsmm_rhs(ctx, A, nrhs, Xd, Y64), 5)
+    t_64 = timed_ms(ctx, lambda: bsp.bsmm_rhs(ctx, A, nrhs, Xd, Y64), 20)
     Y = ctx.empty(X.shape)
     bsp.rhs_unpack_f32(ctx, p.n, 8, nrhs, Yb, Y)
     a, b64 = Y.numpy(), Y64.numpy()
     nbytes = p.nnzb * 64 * 2 + p.n * 8 * nrhs * (2 + 4) + p.nnzb * 4 + (p.n + 1) * 4
     flops = 2.0 * p.nnzb * 64 * nrhs
+    # the same product in FP64 (values, X and Y in FP64, the same index arrays)
+    nbytes64 = p.nnzb * 64 * 8 + p.n * 8 * nrhs * (8 + 8) + p.nnzb * 4 + (p.n + 1) * 4
     out = {"workload": "dg3d_lap 64^3 Q1-SWIP-DG (BlockSp<double,8>, 262144 block rows, 7 blocks/row), 32 RHS N(0,1)",
            "bf16_spmm_us": round(t_bf * 1e3, 2), "bf16_spmm_GBps": round(nbytes / t_bf / 1e6, 1),
            "bf16_spmm_frac_hbm": round(nbytes / t_bf / 1e6 / HBM_PEAK_GBPS, 4),
            "bf16_spmm_TFLOPs": round(flops / t_bf / 1e9, 2), "algorithmic_bytes": nbytes,
-           "fp64_bsmm_rhs_us": round(t_64 * 1e3, 2), "speedup_vs_fp64_bsmm": round(t_64 / t_bf, 2),
+           "fp64_bsmm_rhs_us": round(t_64 * 1e3, 2), "fp64_bsmm_rhs_GBps": round(nbytes64 / t_64 / 1e6, 1),
+           "fp64_bsmm_rhs_frac_hbm": round(nbytes64 / t_64 / 1e6 / HBM_PEAK_GBPS, 4),
+           "fp64_algorithmic_bytes": nbytes64, "speedup_vs_fp64_bsmm": round(t_64 / t_bf, 2),
            "bf16_vs_fp64_rel_diff": float(np.linalg.norm(a - b64) / np.linalg.norm(b64))}
-    log(f"bf16 MFMA SpMM {t_bf * 1e3:.1f} us ({out['bf16_spmm_GBps']:.0f} GB/s), FP64 {t_64 * 1e3:.1f} us")
+    log(f"bf16 MFMA SpMM {t_bf * 1e3:.1f} us ({out['bf16_spmm_GBps']:.0f} GB/s), FP64 {t_64 * 1e3:.1f} us "
+        f"({out['fp64_bsmm_rhs_GBps']:.0f} GB/s)")
     if not solve:
         return out
     b = ctx.to_device(p.b)

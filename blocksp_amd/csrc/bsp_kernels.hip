@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_bsmv_dim(int n, int B, int D, int di
 }
 
 // Multi-RHS: Y[(i*bs + r)*nrhs + k] = sum_blocks sum_c a_rc X[(col*bs + c)*nrhs + k].
-// One thread per (row entry, rhs); RHS-minor layout keeps the X reads coalesced.
+// One thread per (row entry, rhs); for the block sizes k_bsmm_rhs_tile has no instance of.
 __global__ __launch_bounds__(kBlock) void k_bsmm_rhs(int n, int bs, int nrhs, const int* __restrict__ rp,
                                                      const int* __restrict__ ci, const double* __restrict__ val,
                                                      const double* __restrict__ X, double* __restrict__ Y) {
@@ -345,6 +345,201 @@ __global__ __launch_bounds__(kBlock) void k_bsmm_rhs(int n, int bs, int nrhs, co
     }
     Y[t] = acc;
 }
+
+// Multi-RHS product on the RHS tile mapping: one wavefront per (block row i, tile of KT
+// right-hand sides), RG = 64 / KT lane groups.  Batches of NB blocks of the row are staged:
+// their values go through LDS with one coalesced streaming load per block (read once, so they
+// bypass the caches), their X rows load straight into registers (coalesced over k; X is
+// cached -- neighbouring rows read the same blocks); the row's column indices come in one
+// vector load, 64 at a time.  Two lane-group layouts:
+//  * k_bsmm_rhs_splitc (BS a multiple of RG): lane (g, k) sums the columns c of group g's
+//    share for every row r of the block -- each X entry is loaded by one lane only -- and the
+//    RG partial sums of an output are added across the groups at the end;
+//  * k_bsmm_rhs_tile (other BS): lane (g, k) owns rows g R .. g R + R - 1 and all columns;
+//    each output adds s = sum_c a_rc x_c (FMA in c order) per block, bsmv's order.
+template <int BS>
+struct RhsShape {
+    static constexpr int NB = BS <= 8 ? 2 : 1;
+};
+
+template <int BS, int KT, int NB>
+__global__ __launch_bounds__(kBlock) void k_bsmm_rhs_tile(int64_t nwaves, int nrhs, int ntiles,
+                                                          const int* __restrict__ rp, const int* __restrict__ ci,
+                                                          const double* __restrict__ val,
+                                                          const double* __restrict__ X, double* __restrict__ Y) {
+    constexpr int RG = kWave / KT, R = (BS + RG - 1) / RG, B2 = BS * BS;
+    constexpr int WPB = kBlock / kWave;
+    __shared__ double As[WPB][NB][B2];
+    const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    // XCD-aware order: XCD x (workgroups x, x + 8, ...) takes one contiguous range of rows
+    const int64_t nwg = (nwaves + WPB - 1) / WPB, per = (nwg + 7) / 8;
+    const int64_t wg = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int64_t gw = wg * WPB + wv;
+    if (wg >= nwg || gw >= nwaves) return;   // whole wavefronts leave; no workgroup barrier follows
+    const int i = (int)(gw / ntiles), tile = (int)(gw % ntiles);
+    const int kk = l % KT, g = l / KT;
+    const int k = tile * KT + kk;
+    const bool kv = k < nrhs;
+    double acc[R];
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) acc[rr] = 0.0;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    // the row's column indices, 64 at a time, one per lane
+    int cl = 0;
+    for (int pb = p0; pb < p1; pb += NB) {
+        const int nb = min(NB, p1 - pb);
+        if (((pb - p0) & (kWave - 1)) == 0) cl = pb + l < p1 ? ci[pb + l] : 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nb)
+                for (int e = l; e < B2; e += kWave)
+                    As[wv][j][e] = __builtin_nontemporal_load(val + (size_t)(pb + j) * B2 + e);
+        double xv[NB][BS];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int col = __shfl(cl, (pb - p0 + (j < nb ? j : 0)) & (kWave - 1), kWave);
+            const double* xc = X + (size_t)col * BS * nrhs + k;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) xv[j][c] = (j < nb && kv) ? xc[(size_t)c * nrhs] : 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j >= nb) break;
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const int r = g * R + rr;
+                if (R * RG == BS || r < BS) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) sum = fma(As[wv][j][r * BS + c], xv[j][c], sum);
+                    acc[rr] += sum;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (kv) {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int r = g * R + rr;
+            if (R * RG == BS || r < BS) __builtin_nontemporal_store(acc[rr], Y + ((size_t)i * BS + r) * nrhs + k);
+        }
+    }
+}
+
+template <int BS, int KT, int NB>
+__global__ __launch_bounds__(kBlock) void k_bsmm_rhs_splitc(int64_t nwaves, int nrhs, int ntiles,
+                                                            const int* __restrict__ rp, const int* __restrict__ ci,
+                                                            const double* __restrict__ val,
+                                                            const double* __restrict__ X, double* __restrict__ Y) {
+    constexpr int RG = kWave / KT, CG = BS / RG, B2 = BS * BS;
+    constexpr int WPB = kBlock / kWave;
+    __shared__ double As[WPB][NB][B2];
+    const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    const int64_t nwg = (nwaves + WPB - 1) / WPB, per = (nwg + 7) / 8;
+    const int64_t wg = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int64_t gw = wg * WPB + wv;
+    if (wg >= nwg || gw >= nwaves) return;   // whole wavefronts leave; no workgroup barrier follows
+    const int i = (int)(gw / ntiles), tile = (int)(gw % ntiles);
+    const int kk = l % KT, g = l / KT;
+    const int k = tile * KT + kk;
+    const bool kv = k < nrhs;
+    double acc[BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+    const int p0 = rp[i], p1 = rp[i + 1];
+    int cl = 0;
+    for (int pb = p0; pb < p1; pb += NB) {
+        const int nb = min(NB, p1 - pb);
+        if (((pb - p0) & (kWave - 1)) == 0) cl = pb + l < p1 ? ci[pb + l] : 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nb)
+                for (int e = l; e < B2; e += kWave)
+                    As[wv][j][e] = __builtin_nontemporal_load(val + (size_t)(pb + j) * B2 + e);
+        double xv[NB][CG];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int col = __shfl(cl, (pb - p0 + (j < nb ? j : 0)) & (kWave - 1), kWave);
+            const double* xc = X + ((size_t)col * BS + g * CG) * nrhs + k;
+#pragma unroll
+            for (int c = 0; c < CG; ++c) xv[j][c] = (j < nb && kv) ? xc[(size_t)c * nrhs] : 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j >= nb) break;
+#pragma unroll
+            for (int r = 0; r < BS; ++r) {
+                double sum = 0.0;
+#pragma unroll
+                for (int c = 0; c < CG; ++c) sum = fma(As[wv][j][r * BS + g * CG + c], xv[j][c], sum);
+                acc[r] += sum;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+#pragma unroll
+    for (int r = 0; r < BS; ++r) {
+        double v = acc[r];
+#pragma unroll
+        for (int d = KT; d < kWave; d <<= 1) v += __shfl_xor(v, d, kWave);
+        acc[r] = v;
+    }
+    if (kv && g == 0) {
+#pragma unroll
+        for (int r = 0; r < BS; ++r) __builtin_nontemporal_store(acc[r], Y + ((size_t)i * BS + r) * nrhs + k);
+    }
+}
+
+// RHS tile width: the smallest of 8, 16, 32, 64 lanes that covers nrhs (64 for more, in
+// tiles), and at least 64 / BS so that every lane has a row
+template <int BS>
+struct BsmmRhsL {
+    using Kern = void (*)(int64_t, int, int, const int*, const int*, const double*, const double*, double*);
+    template <int KT, int NB>
+    static Kern pick() {
+        if constexpr (BS % (kWave / KT) == 0)
+            return k_bsmm_rhs_splitc<BS, KT, NB>;
+        else
+            return k_bsmm_rhs_tile<BS, KT, NB>;
+    }
+    static void run(hipStream_t st, int n, int nrhs, const int* rp, const int* ci, const double* val,
+                    const double* X, double* Y) {
+        constexpr int kmin = BS >= 8 ? 8 : (BS >= 4 ? 16 : (BS >= 2 ? 32 : 64));
+        int kt = kmin;
+        while (kt < 64 && kt < nrhs) kt *= 2;
+        const int ntiles = (nrhs + kt - 1) / kt;
+        const int64_t nwaves = (int64_t)n * ntiles;
+        const int64_t nwg = (nwaves + kBlock / kWave - 1) / (kBlock / kWave);
+        const dim3 grid((unsigned)((nwg + 7) / 8 * 8));
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, st, nwaves, nrhs, ntiles, rp, ci, val, X, Y);
+        };
+        constexpr int NB = RhsShape<BS>::NB;
+        switch (kt) {
+            case 8:
+                if constexpr (kmin <= 8) go(pick<8, NB>());
+                break;
+            case 16:
+                if constexpr (kmin <= 16) go(pick<16, NB>());
+                break;
+            case 32:
+                if constexpr (kmin <= 32) go(pick<32, NB>());
+                break;
+            default: go(pick<64, NB>()); break;
+        }
+    }
+};
 
 // x_i <- D_i x_i for every block row (Diag preconditioner).
 template <int BS>
@@ -846,8 +1041,11 @@ int launch_bsmv_dim(const bsp_ctx* ctx, int n, int B, int D, int dim, const int*
 int launch_bsmm_rhs(const bsp_ctx* ctx, int n, int bs, int nrhs, const int* rp, const int* ci, const double* val,
                     const double* X, double* Y) {
     if (n <= 0) return BSP_OK;
-    hipLaunchKernelGGL(k_bsmm_rhs, dim3(grid_for((int64_t)n * bs * nrhs)), dim3(kBlock), 0, ctx->stream, n, bs, nrhs,
-                       rp, ci, val, X, Y);
+    if (nrhs <= 0) return BSP_OK;
+    if (!dispatch_bs<BsmmRhsL>(bs, ctx->stream, n, nrhs, rp, ci, val, X, Y)) {
+        hipLaunchKernelGGL(k_bsmm_rhs, dim3(grid_for((int64_t)n * bs * nrhs)), dim3(kBlock), 0, ctx->stream, n, bs,
+                           nrhs, rp, ci, val, X, Y);
+    }
     BSP_LAUNCH_CHECK();
     return BSP_OK;
 }

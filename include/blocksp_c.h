@@ -121,8 +121,8 @@ int bsp_bsr_load(bsp_ctx* ctx, const char* path, bsp_bsr** out);
  * (X[(row*bs + r)*nrhs + k]); no single reference call, it is nrhs bsmv calls. */
 int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X_dev, double* Y_dev);
 /* C = A * B (bsmm, blocksp_multiply.hpp:51-136): structure in the reference's first-touch
- * order (symbolic pass on the host, bit-exact), values computed on the device.  A and B must
- * outlive C for bsp_bsmm_numeric. */
+ * order (bit-exact), both passes on the device (8 x 8 blocks: numeric pass on the FP64
+ * matrix cores).  A and B must outlive C for bsp_bsmm_numeric. */
 int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** C);
 /* C = A^T * B (bsmm with Atran = true, blocksp_multiply.hpp:99-134): row j of C in the
  * order the reference inserts it (rows i of A holding column j ascending, then row i of B in

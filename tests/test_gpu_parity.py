@@ -100,9 +100,15 @@ def test_bsmv_dim(ctx, B, D):
         check_bsmv(yd.numpy(), yref, absax)
 
 
-def test_bsmm_rhs(ctx):
-    n, bs, k = 200, 8, 5
-    rp, ci, v = random_bsr(n, bs, seed=11)
+@pytest.mark.parametrize("bs,k", [(8, 5), (8, 32), (8, 70), (1, 3), (3, 33), (4, 16), (16, 9), (32, 2),
+                                  (3, 20), (5, 12), (7, 30), (6, 10), (12, 4)])
+def test_bsmm_rhs(ctx, bs, k):
+    """Multi-RHS product on the RHS tile kernels -- columns split over lane groups (bs a
+    multiple of the group count) or rows split (bs 3, 5, 7 and 6 at 16-wide tiles), tiles of
+    8-64 right-hand sides, several per row when nrhs > 64 -- and the per-output kernel (bs 12):
+    each column within bsmv's bound."""
+    n = 200 if bs <= 16 else 40
+    rp, ci, v = random_bsr(n, bs, seed=11 + bs)
     A = bsp.BSR(ctx, n, n, bs, rp, ci, v)
     X = np.random.default_rng(2).standard_normal((n * bs, k))
     Yd = ctx.zeros(n * bs * k)
