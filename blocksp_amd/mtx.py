@@ -20,18 +20,42 @@ import numpy as np
 __all__ = ["entries_to_bsr", "read_mtx_entries", "read_mtx_vector", "vector_to_solarray"]
 
 
-def read_mtx_entries(path):
-    """Parse a BlockSp-style .mtx file (size line first, 1-based entries)."""
+def _goto_line(f, start_line):
+    """Skip start_line - 1 lines, as GotoLine does (blocksp_tests.hpp:582-590): the size line
+    is line start_line (1-based; 0 and 1 both mean the first line)."""
+    for _ in range(max(int(start_line) - 1, 0)):
+        f.readline()
+
+
+def read_mtx_entries(path, start_line=0):
+    """Parse a BlockSp-style .mtx matrix (blocksp_tests.hpp:595-616): "M N nnz" on line
+    start_line, then nnz lines "i j value", 1-based.  Returns M, N and 0-based i, j, values
+    in file order."""
     with open(path) as f:
-        M, N, nnz = (int(t) for t in f.readline().split())
-        data = np.loadtxt(f, dtype=np.float64, max_rows=nnz, ndmin=2)
+        _goto_line(f, start_line)
+        head = f.readline().split()
+        if len(head) < 3:
+            raise ValueError(f"{path}: no 'M N nnz' line at line {max(int(start_line), 1)}")
+        M, N, nnz = (int(t) for t in head[:3])
+        data = np.loadtxt(f, dtype=np.float64, max_rows=nnz, ndmin=2) if nnz else np.zeros((0, 3))
+    if data.shape[0] != nnz:
+        raise ValueError(f"{path}: {data.shape[0]} entries, the size line says {nnz}")
     return M, N, data[:, 0].astype(np.int64) - 1, data[:, 1].astype(np.int64) - 1, data[:, 2].copy()
 
 
-def read_mtx_vector(path):
+def read_mtx_vector(path, start_line=0):
+    """Parse a BlockSp-style .mtx vector (blocksp_tests.hpp:652-675): "M 1" on line
+    start_line, then M values."""
     with open(path) as f:
-        M = int(f.readline().split()[0])
-        return np.loadtxt(f, dtype=np.float64, max_rows=M, ndmin=1)
+        _goto_line(f, start_line)
+        head = f.readline().split()
+        if not head:
+            raise ValueError(f"{path}: no 'M N' line at line {max(int(start_line), 1)}")
+        M = int(head[0])
+        v = np.loadtxt(f, dtype=np.float64, max_rows=M, ndmin=1) if M else np.zeros(0)
+    if v.shape[0] != M:
+        raise ValueError(f"{path}: {v.shape[0]} values, the size line says {M}")
+    return v
 
 
 def entries_to_bsr(M, N, i, j, v, bs, symmetric=False):

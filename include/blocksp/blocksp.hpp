@@ -26,11 +26,13 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -1221,5 +1223,123 @@ solArray<T, B, D> ichol0_pCG(const BlockSp<T, B * D>& A, const solArray<T, B, D>
 }
 
 }  // namespace ez
+
+// ===========================================================================
+// Matrix-Market converters (blocksp_tests.hpp:578-699, namespace tests)
+// ===========================================================================
+
+namespace tests {
+
+/// GotoLine (blocksp_tests.hpp:582-590): rewinds and skips num - 1 lines.
+inline std::ifstream& GotoLine(std::ifstream& file, int num) {
+    file.seekg(0, std::ios::beg);
+    for (int i = 0; i < num - 1; ++i) file.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+    return file;
+}
+
+/// mtxMatrix_to_BlockSp (blocksp_tests.hpp:595-649): line start_line holds "M N nnz", the
+/// nnz lines after it "i j value" (complex T: "i j re im"), 1-based.  Every entry goes
+/// through setFromGlobal, so a block row lists its blocks in first-touch order of the
+/// file; with symmetric the mirrored entry (j, i) -- conjugated for complex T -- is written
+/// right after (i, j) when i != j; a repeated entry keeps the last value.  As in the
+/// reference, base_zero is accepted and not used: indices are always read as 1-based.
+/// Unlike the reference (an assert, or garbage from an unopened stream), a file that cannot
+/// be read or a size that B does not divide throws BlockSp::Error.
+template <typename T, int B = 1>
+BlockSp<T, B> mtxMatrix_to_BlockSp(const std::string& mat_loc, unsigned int start_line = 0, bool symmetric = false,
+                                   bool base_zero = false, bool printMatSize = false) {
+    (void)base_zero;
+    std::ifstream f(mat_loc, std::ios::in);
+    if (!f) throw Error(BSP_E_INVALID, "mtxMatrix_to_BlockSp: cannot open " + mat_loc);
+    GotoLine(f, (int)start_line);
+    std::string size;
+    std::getline(f, size);
+    std::stringstream ss(size);
+    int M = -1, N = -1, nnz = -1;
+    ss >> M >> N >> nnz;
+    if (!ss || M < 0 || N < 0 || nnz < 0)
+        throw Error(BSP_E_INVALID, "mtxMatrix_to_BlockSp: no \"M N nnz\" line in " + mat_loc);
+    if (printMatSize) {
+        std::cout << M << ' ' << N << ' ' << nnz;
+        if (symmetric) std::cout << " Symmetric";
+        std::cout << std::endl;
+    }
+    if (M % B != 0 || N % B != 0)
+        throw Error(BSP_E_INVALID, "mtxMatrix_to_BlockSp: block size " + std::to_string(B) +
+                                       " does not divide " + std::to_string(M) + " x " + std::to_string(N));
+    BlockSp<T, B> A(M / B, N / B);
+    for (int e = 0; e < nnz; ++e) {
+        std::string entry;
+        if (!std::getline(f, entry)) throw Error(BSP_E_INVALID, "mtxMatrix_to_BlockSp: file ends early: " + mat_loc);
+        std::stringstream es(entry);
+        int i = 0, j = 0;
+        es >> i >> j;
+        --i;
+        --j;
+        if (!es || i < 0 || j < 0 || i >= M || j >= N)
+            throw Error(BSP_E_INVALID, "mtxMatrix_to_BlockSp: bad entry \"" + entry + "\"");
+        T val;
+        if constexpr (is_complex_v<T>) {
+            ScalarType_t<T> re = 0, im = 0;
+            es >> re >> im;
+            val = T(re, im);
+        } else {
+            es >> val;
+        }
+        if (!es) throw Error(BSP_E_INVALID, "mtxMatrix_to_BlockSp: bad value \"" + entry + "\"");
+        util::setFromGlobal(A, i, j, val);
+        if (symmetric && i != j) {
+            if constexpr (is_complex_v<T>)
+                util::setFromGlobal(A, j, i, std::conj(val));
+            else
+                util::setFromGlobal(A, j, i, val);
+        }
+    }
+    return A;
+}
+
+/// mtxArray_to_solArray (blocksp_tests.hpp:652-699): line start_line holds "M 1", the M lines
+/// after it one value each (complex T: "re im"); value i goes to solArray element
+/// (i / (B D), dim, ib) with i % (B D) = dim B + ib, the layout setFromGlobal uses
+/// (blocksp_utility.hpp:630-639).  The reference computes the in-element index as i % B
+/// (blocksp_tests.hpp:679), which for D > 1 writes every value to dim 0; for D = 1 the two
+/// agree.  Errors throw BlockSp::Error as in mtxMatrix_to_BlockSp.
+template <typename T, int B = 1, int D = 1>
+solArray<T, B, D> mtxArray_to_solArray(const std::string& vec_loc, unsigned int start_line = 0,
+                                       bool printVecSize = false) {
+    std::ifstream f(vec_loc, std::ios::in);
+    if (!f) throw Error(BSP_E_INVALID, "mtxArray_to_solArray: cannot open " + vec_loc);
+    GotoLine(f, (int)start_line);
+    std::string size;
+    std::getline(f, size);
+    std::stringstream ss(size);
+    int M = -1, N = -1;
+    ss >> M >> N;
+    if (!ss || M < 0) throw Error(BSP_E_INVALID, "mtxArray_to_solArray: no \"M N\" line in " + vec_loc);
+    if (printVecSize) std::cout << M << ' ' << N << std::endl;
+    constexpr int BD = B * D;
+    if (M % BD != 0 || N != 1)
+        throw Error(BSP_E_INVALID, "mtxArray_to_solArray: need one column and B*D = " + std::to_string(BD) +
+                                       " dividing " + std::to_string(M));
+    solArray<T, B, D> x(M / BD);
+    for (int i = 0; i < M; ++i) {
+        std::string entry;
+        if (!std::getline(f, entry)) throw Error(BSP_E_INVALID, "mtxArray_to_solArray: file ends early: " + vec_loc);
+        std::stringstream es(entry);
+        T val;
+        if constexpr (is_complex_v<T>) {
+            long double re = 0, im = 0;
+            es >> re >> im;
+            val = T((ScalarType_t<T>)re, (ScalarType_t<T>)im);
+        } else {
+            es >> val;
+        }
+        if (!es) throw Error(BSP_E_INVALID, "mtxArray_to_solArray: bad value \"" + entry + "\"");
+        util::setFromGlobal(x, i, val);
+    }
+    return x;
+}
+
+}  // namespace tests
 
 }  // namespace BlockSp
