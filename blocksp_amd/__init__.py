@@ -66,6 +66,9 @@ _SIGS = {
     "bsp_event_record": [_VP, C.c_int],
     "bsp_event_elapsed_ms": [_VP, C.c_int, C.c_int, C.POINTER(C.c_float)],
     "bsp_bsr_create": [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_bsr_create_complex": [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
+    "bsp_bsr_complex_bs": [_VP, C.POINTER(C.c_int)],
+    "bsp_bsr_download_complex": [_VP, _VP, _VP, _VP, _VP],
     "bsp_bsr_destroy": [_VP],
     "bsp_bsr_info": [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)],
     "bsp_bsr_download": [_VP, _VP, _VP, _VP, _VP],
@@ -265,7 +268,11 @@ class DeviceArray:
 
 
 class BSR:
-    """BlockSp<T,B*D> on the device (bsp_bsr)."""
+    """BlockSp<T,B*D> on the device (bsp_bsr).  A complex matrix (BSR.from_complex) has
+    cbs = its complex block size and bs = 2 cbs (the real-equivalent blocks the device holds);
+    its vectors are complex128 device arrays of n_rows * cbs entries."""
+
+    cbs = 0
 
     def __init__(self, ctx: Context, n_rows, n_cols, bs, rowptr, colidx, vals):
         self.ctx = ctx
@@ -284,6 +291,22 @@ class BSR:
         return cls(ctx, p.n, p.n, p.bs, p.rowptr, p.colidx, p.vals)
 
     @classmethod
+    def from_complex(cls, ctx, n_rows, n_cols, bs, rowptr, colidx, vals):
+        """BlockSp<std::complex<double>, bs>: vals complex, (nnzb, bs, bs) row-major blocks in
+        stored order (bsp_bsr_create_complex)."""
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        ci = np.ascontiguousarray(colidx, np.int32)
+        v = np.ascontiguousarray(vals, np.complex128)
+        h = C.c_void_p()
+        ctx.check(ctx.L.bsp_bsr_create_complex(ctx.h, int(n_rows), int(n_cols), int(bs), _ptr(rp), _ptr(ci), _ptr(v),
+                                               C.byref(h)), "bsr_create_complex")
+        return cls._wrap(ctx, h)
+
+    @property
+    def is_complex(self):
+        return self.cbs > 0
+
+    @classmethod
     def _wrap(cls, ctx, h, keep=()):
         """A BSR object around a handle the library created (e.g. by bsp_bsmm)."""
         self = cls.__new__(cls)
@@ -291,6 +314,9 @@ class BSR:
         n, m, bs, nz = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
         ctx.check(ctx.L.bsp_bsr_info(h, C.byref(n), C.byref(m), C.byref(bs), C.byref(nz)), "bsr_info")
         self.n_rows, self.n_cols, self.bs, self.nnzb = n.value, m.value, bs.value, nz.value
+        cbs = C.c_int()
+        ctx.check(ctx.L.bsp_bsr_complex_bs(h, C.byref(cbs)), "bsr_complex_bs")
+        self.cbs = cbs.value
         return self
 
     def save(self, path):
@@ -306,6 +332,11 @@ class BSR:
     def download(self):
         rp = np.empty(self.n_rows + 1, np.int32)
         ci = np.empty(self.nnzb, np.int32)
+        if self.cbs:
+            v = np.empty((self.nnzb, self.cbs, self.cbs), np.complex128)
+            self.ctx.check(self.ctx.L.bsp_bsr_download_complex(self.ctx.h, self.h, _ptr(rp), _ptr(ci), _ptr(v)),
+                           "bsr_download_complex")
+            return rp, ci, v
         v = np.empty((self.nnzb, self.bs, self.bs))
         self.ctx.check(self.ctx.L.bsp_bsr_download(self.ctx.h, self.h, _ptr(rp), _ptr(ci), _ptr(v)), "bsr_download")
         return rp, ci, v
@@ -512,8 +543,9 @@ class IChol0(_Precond):
 
 
 def pGMRES(ctx, A: BSR, b: DeviceArray, tol, M, maxIters=500, restart=-1, k=-1, tol_mode=TOL_REFERENCE, x=None):
-    """pGMRES (blocksp_solvers.hpp:156-295). Returns (x DeviceArray, Stats, return code)."""
-    x = x if x is not None else ctx.empty(b.shape)
+    """pGMRES (blocksp_solvers.hpp:156-295). Returns (x DeviceArray, Stats, return code).
+    For a complex A, b and x are complex128 arrays."""
+    x = x if x is not None else ctx.empty(b.shape, b.dtype)
     st = Stats()
     rc = ctx.L.bsp_pgmres(ctx.h, A.h, M.h, b.ptr, x.ptr, float(tol), int(maxIters), int(restart), int(k),
                           int(tol_mode), C.byref(st))

@@ -280,6 +280,61 @@ int bsp_bsr_create(bsp_ctx* ctx, int n_rows, int n_cols, int bs, const int32_t* 
     return BSP_OK;
 }
 
+int bsp_bsr_create_complex(bsp_ctx* ctx, int n_rows, int n_cols, int bs, const int32_t* rowptr, const int32_t* colidx,
+                           const double* vals, bsp_bsr** out) {
+    if (!ctx || !out) return BSP_E_INVALID;
+    *out = nullptr;
+    BSP_CHECK(n_rows >= 0 && n_cols >= 0 && bs >= 1 && bs <= 32, BSP_E_INVALID,
+              "bad complex matrix dimensions (block size 1-32)");
+    BSP_CHECK(rowptr && rowptr[0] == 0 && rowptr[n_rows] >= 0, BSP_E_INVALID, "bad rowptr");
+    const int64_t nnzb = rowptr[n_rows];
+    BSP_CHECK(nnzb == 0 || vals, BSP_E_INVALID, "vals is null");
+    // complex entry z_rc = a + ib -> rows 2r, 2r+1 and columns 2c, 2c+1 of [[a, -b], [b, a]]
+    const int rb = 2 * bs;
+    std::vector<double> re((size_t)nnzb * rb * rb);
+    for (int64_t p = 0; p < nnzb; ++p) {
+        const double* z = vals + (size_t)p * bs * bs * 2;
+        double* R = re.data() + (size_t)p * rb * rb;
+        for (int r = 0; r < bs; ++r)
+            for (int c = 0; c < bs; ++c) {
+                const double a = z[(r * bs + c) * 2], bi = z[(r * bs + c) * 2 + 1];
+                R[(2 * r) * rb + 2 * c] = a;
+                R[(2 * r) * rb + 2 * c + 1] = -bi;
+                R[(2 * r + 1) * rb + 2 * c] = bi;
+                R[(2 * r + 1) * rb + 2 * c + 1] = a;
+            }
+    }
+    int rc = bsp_bsr_create(ctx, n_rows, n_cols, rb, rowptr, colidx, re.data(), out);
+    if (rc) return rc;
+    (*out)->cbs = bs;
+    return BSP_OK;
+}
+
+int bsp_bsr_complex_bs(const bsp_bsr* A, int* cbs) {
+    if (!A || !cbs) return BSP_E_INVALID;
+    *cbs = A->cbs;
+    return BSP_OK;
+}
+
+int bsp_bsr_download_complex(bsp_ctx* ctx, const bsp_bsr* A, int32_t* rowptr, int32_t* colidx, double* vals) {
+    if (!ctx || !A) return BSP_E_INVALID;
+    BSP_CHECK(A->cbs > 0, BSP_E_INVALID, "bsr_download_complex: not a complex matrix");
+    const int bs = A->cbs, rb = A->bs;
+    std::vector<double> re(vals ? (size_t)A->nnzb * rb * rb : 0);
+    int rc = bsp_bsr_download(ctx, A, rowptr, colidx, vals ? re.data() : nullptr);
+    if (rc || !vals) return rc;
+    for (int64_t p = 0; p < A->nnzb; ++p) {
+        const double* R = re.data() + (size_t)p * rb * rb;
+        double* z = vals + (size_t)p * bs * bs * 2;
+        for (int r = 0; r < bs; ++r)
+            for (int c = 0; c < bs; ++c) {
+                z[(r * bs + c) * 2] = R[(2 * r) * rb + 2 * c];
+                z[(r * bs + c) * 2 + 1] = R[(2 * r + 1) * rb + 2 * c];
+            }
+    }
+    return BSP_OK;
+}
+
 int bsp_bsr_destroy(bsp_bsr* A) {
     delete A;
     return BSP_OK;
@@ -343,6 +398,7 @@ int bsp_bsmv_dim(bsp_ctx* ctx, const bsp_bsr* A, int D, int dim, const double* x
 
 int bsp_bsmm_rhs(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const double* X, double* Y) {
     if (!ctx || !A) return BSP_E_INVALID;
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "the multi-RHS product does not take complex matrices");
     BSP_CHECK(nrhs >= 1 && X && Y && X != Y, BSP_E_INVALID, "bad multi-RHS arguments");
     return launch_bsmm_rhs(ctx, A->n_rows, A->bs, nrhs, A->rp.as<int>(), A->ci.as<int>(), A->val.as<double>(), X, Y);
 }

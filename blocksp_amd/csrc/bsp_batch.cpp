@@ -293,6 +293,7 @@ int bsp_batch_create_ex(bsp_ctx* ctx, const bsp_bsr* A, int nsys, const int32_t*
     if (!ctx || !A || !sys_rowptr || !out || nsys <= 0 || nrhs <= 0) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "batched systems must be square");
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "batched pCG does not take complex matrices");
     BSP_CHECK(sys_rowptr[0] == 0 && sys_rowptr[nsys] == A->n_rows, BSP_E_INVALID,
               "system row pointers must cover the matrix");
     BSP_CHECK(batch::supported(A->bs, 1), BSP_E_UNSUPPORTED, "batched pCG supports block sizes 1, 2, 4 and 8");

@@ -265,6 +265,7 @@ int bsp_rhs_unpack_f32(bsp_ctx* ctx, int64_t rows, int bs, int nrhs, const float
 
 int bsp_bsmm_rhs_bf16(bsp_ctx* ctx, const bsp_bsr* A, int nrhs, const uint16_t* Xb_dev, float* Yb_dev) {
     if (!ctx || !A || !Xb_dev || !Yb_dev) return BSP_E_INVALID;
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "the BF16 block SpMM does not take complex matrices");
     BSP_CHECK(A->bs == 8, BSP_E_UNSUPPORTED, "the BF16 MFMA block SpMM needs 8 x 8 blocks");
     BSP_CHECK(nrhs > 0 && nrhs % 16 == 0, BSP_E_INVALID, "the BF16 MFMA block SpMM needs a multiple of 16 RHS");
     BSP_HIP(hipSetDevice(ctx->device));

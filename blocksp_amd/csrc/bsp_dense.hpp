@@ -162,7 +162,7 @@ inline void givens(cplx f, cplx g, double& c, cplx& s) {
     s = (f / af) * std::conj(g) / r;
 }
 
-// Complex Schur decomposition A = Z T Z^H of a real n x n matrix.
+// Complex Schur decomposition A = Z T Z^H of a real or complex n x n matrix.
 struct Schur {
     int n = 0;
     std::vector<cplx> T, Z;
@@ -190,7 +190,9 @@ struct Schur {
         }
     }
 
-    bool compute(int nn, const double* A) {
+    // A: n x n row-major, real (double) or complex (cplx)
+    template <class S>
+    bool compute(int nn, const S* A) {
         n = nn;
         T.assign((size_t)n * n, 0.0);
         Z.assign((size_t)n * n, 0.0);

@@ -7,6 +7,7 @@
 #include <limits>
 
 #include "bsp_dense.hpp"
+#include "bsp_zdense.hpp"
 #include "bsp_internal.hpp"
 
 using namespace bsp;
@@ -32,9 +33,16 @@ int bsp_pgmres_ritz(bsp_ctx* ctx, double* re, double* im, int cap, int* count, i
     if (count) *count = 0;
     if (harmonic) *harmonic = ctx->last_restarted ? 1 : 0;
     const int m = ctx->last_Hm, n = ctx->last_Hn;
+    std::vector<dense::cplx> ev;
+    if (!ctx->last_Hc.empty()) {   // the last run solved a complex system
+        BSP_CHECK(n > 0 && m >= n && (int64_t)ctx->last_Hc.size() >= (int64_t)m * n, BSP_E_INVALID,
+                  "no pGMRES Hessenberg recorded on this context");
+        BSP_CHECK(dense::zritz_values(m, n, ctx->last_Hc, ctx->last_restarted, ev), BSP_E_NUMERIC,
+                  "Hessenberg eigenproblem failed");
+        return write_values(ev, re, im, cap, count);
+    }
     BSP_CHECK(n > 0 && m >= n && (int64_t)ctx->last_H.size() >= (int64_t)m * n, BSP_E_INVALID,
               "no pGMRES Hessenberg recorded on this context");
-    std::vector<dense::cplx> ev;
     // the square part of H; harmonic Ritz values when the run restarted
     BSP_CHECK(dense::ritz_values(m, n, ctx->last_H.data(), ctx->last_restarted, ev), BSP_E_NUMERIC,
               "Hessenberg eigenproblem failed");
@@ -45,6 +53,7 @@ int bsp_arnoldi_eigenvalues(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M
                             double* re, double* im, int cap, int* count) {
     if (!ctx || !A || iters < 1 || cap < 0) return BSP_E_INVALID;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "arnoldi_eigenvalues needs a square matrix");
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "arnoldi_eigenvalues: complex matrices are not supported");
     BSP_CHECK(!M || (M->n == A->n_rows && M->bs == A->bs), BSP_E_INVALID, "preconditioner does not match");
     BSP_HIP(hipSetDevice(ctx->device));
     const int n = A->n_rows, bs = A->bs;

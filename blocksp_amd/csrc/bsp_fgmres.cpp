@@ -388,6 +388,7 @@ int bsp_fgmres_fallback(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, co
                         double inner_reduction, bsp_stats* stats) {
     if (!ctx || !A || !M || !b || !x) return BSP_E_INVALID;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "FGMRES needs a square matrix");
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "the mixed-precision FGMRES does not take complex matrices");
     BSP_CHECK(M->n == A->n_rows && M->bs == A->bs, BSP_E_INVALID, "preconditioner does not match the matrix");
     BSP_CHECK(!M_fallback || (M_fallback->n == A->n_rows && M_fallback->bs == A->bs), BSP_E_INVALID,
               "fallback preconditioner does not match the matrix");

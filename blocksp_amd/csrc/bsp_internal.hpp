@@ -8,6 +8,7 @@
 #include <ctime>
 #include <memory>
 #include <string>
+#include <complex>
 #include <vector>
 
 #include "../../include/blocksp_c.h"
@@ -170,8 +171,10 @@ struct bsp_ctx {
     bsp::DevMem scalars;             // double [kScalarSlots]
     bsp::DevMem err_flag;            // uint32 [1], sync-free wait timeouts
     double* pinned = nullptr;        // host pinned [kScalarSlots]
-    // final Hessenberg of the last bsp_pgmres (m x n row-major) for the Ritz-value diagnostics
+    // final Hessenberg of the last bsp_pgmres (m x n row-major) for the Ritz-value diagnostics;
+    // last_Hc instead of last_H after a complex system
     std::vector<double> last_H;
+    std::vector<std::complex<double>> last_Hc;
     int last_Hm = 0, last_Hn = 0;
     bool last_restarted = false;
     // Krylov basis recorder (bsp_ctx_record_basis)
@@ -188,6 +191,8 @@ struct bsp_bsr {
     int n_rows = 0, n_cols = 0, bs = 0;
     int64_t nnzb = 0;
     bsp::DevMem rp, ci, val;
+    int cbs = 0;                       // complex matrix (bsp_bsr_create_complex): its complex block size;
+                                       // bs = 2 cbs holds the real-equivalent blocks
     std::vector<int32_t> h_rp, h_ci;   // host copy of the structure
     mutable std::vector<double> h_val; // host copy of the values (factorisations start here)
     mutable bool h_val_stale = false;  // values produced on the device (bsmm): copy back on first host use
@@ -354,6 +359,11 @@ int sync_host_values(bsp_ctx* ctx, const bsp_bsr* A);
 int ensure_transpose(bsp_ctx* ctx, const bsp_bsr* A);
 
 // ||b - A x|| / bnorm (FP64, on the device) and the sync-free timeout check (bsp_krylov.cpp)
+// pGMRES core for complex systems (bsp_zgmres.cpp), the contract of gmres_core
+int zgmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* b, double* x, double tol,
+                bool tol_is_relative, int maxIters, int restart, int k, int& iters_out, int& restarts_out,
+                double& res0_out, double& res_out, bool& converged, int& applies, int stag_window,
+                bool& stagnated);
 int true_relres_ext(bsp_ctx* ctx, const bsp_bsr* A, const double* b, const double* x, double* work, double bnorm,
                     double& out);
 int check_sweep_errors_ext(bsp_ctx* ctx);

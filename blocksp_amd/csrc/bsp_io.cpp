@@ -34,7 +34,9 @@ int bsp_bsr_save(bsp_ctx* ctx, const bsp_bsr* A, const char* path) {
     File out;
     out.f = std::fopen(path, "wb");
     BSP_CHECK(out.f, BSP_E_INVALID, std::string("cannot open ") + path + " for writing");
-    const int32_t hdr[4] = {A->n_rows, A->n_cols, A->bs, 0};
+    // the fourth header word: 0 for a real matrix, the complex block size for a complex one
+    // (whose values are then its real-equivalent blocks)
+    const int32_t hdr[4] = {A->n_rows, A->n_cols, A->bs, A->cbs};
     const int64_t nnzb = A->nnzb;
     const size_t nval = (size_t)nnzb * A->bs * A->bs;
     bool ok = std::fwrite(kMagic, 1, 8, out.f) == 8 && std::fwrite(hdr, sizeof(int32_t), 4, out.f) == 4 &&
@@ -59,8 +61,9 @@ int bsp_bsr_load(bsp_ctx* ctx, const char* path, bsp_bsr** out) {
               std::string(path) + " is not a BSR cache file");
     BSP_CHECK(std::fread(hdr, sizeof(int32_t), 4, in.f) == 4 && std::fread(&nnzb, sizeof(int64_t), 1, in.f) == 1,
               BSP_E_INVALID, std::string("truncated header in ") + path);
-    const int n = hdr[0], m = hdr[1], bs = hdr[2];
-    BSP_CHECK(n >= 0 && m >= 0 && bs >= 1 && nnzb >= 0, BSP_E_INVALID, std::string("bad header in ") + path);
+    const int n = hdr[0], m = hdr[1], bs = hdr[2], cbs = hdr[3];
+    BSP_CHECK(n >= 0 && m >= 0 && bs >= 1 && nnzb >= 0 && (cbs == 0 || bs == 2 * cbs), BSP_E_INVALID,
+              std::string("bad header in ") + path);
     std::vector<int32_t> rp(n + 1), ci(nnzb);
     std::vector<double> v((size_t)nnzb * bs * bs);
     const bool ok = std::fread(rp.data(), sizeof(int32_t), rp.size(), in.f) == rp.size() &&
@@ -68,7 +71,9 @@ int bsp_bsr_load(bsp_ctx* ctx, const char* path, bsp_bsr** out) {
                     std::fread(v.data(), sizeof(double), v.size(), in.f) == v.size();
     BSP_CHECK(ok, BSP_E_INVALID, std::string("truncated data in ") + path);
     BSP_CHECK(rp[0] == 0 && rp[n] == nnzb, BSP_E_INVALID, std::string("inconsistent row pointers in ") + path);
-    return bsp_bsr_create(ctx, n, m, bs, rp.data(), ci.data(), v.data(), out);
+    int rc = bsp_bsr_create(ctx, n, m, bs, rp.data(), ci.data(), v.data(), out);
+    if (rc == BSP_OK) (*out)->cbs = cbs;
+    return rc;
 }
 
 }  // extern "C"

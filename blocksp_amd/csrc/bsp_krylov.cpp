@@ -284,6 +284,7 @@ int gmres_core(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     std::vector<double> Hh;
     if ((rc = ls_download_h(ctx, ls, nc, Hh))) return rc;
     ctx->last_H = Hh;
+    ctx->last_Hc.clear();
     ctx->last_Hm = nc + 1;
     ctx->last_Hn = nc;
     ctx->last_restarted = ls.restarted;
@@ -328,8 +329,10 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
     // factor 2) and restarts from the current iterate with the exact residual.
     const bool true_mode = tol_mode == BSP_TOL_TRUE_RESIDUAL;
     const int stag_window = true_mode && restart > 0 ? 8 : 0;
-    int rc = gmres_core(ctx, A, M, b, x, tol, true, maxIters, restart, k, iters, restarts, res0, res, conv, applies,
-                        stag_window, stagnated);
+    // complex systems: the same driver over the complex-arithmetic core
+    auto core = A->cbs > 0 ? zgmres_core : gmres_core;
+    int rc = core(ctx, A, M, b, x, tol, true, maxIters, restart, k, iters, restarts, res0, res, conv, applies,
+                  stag_window, stagnated);
     if (rc < 0) return rc;
     double bnorm = 0.0, rr = 0.0;
     DevMem work;
@@ -346,8 +349,8 @@ int bsp_pgmres(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doubl
             int it2 = 0, rs2 = 0;
             double r02 = 0.0, res2 = 0.0;
             bool c2 = false;
-            rc = gmres_core(ctx, A, M, b, x, pass_tol, true, maxIters - total_iters, restart, k, it2, rs2, r02, res2,
-                            c2, applies, stag_window, stagnated);
+            rc = core(ctx, A, M, b, x, pass_tol, true, maxIters - total_iters, restart, k, it2, rs2, r02, res2, c2,
+                      applies, stag_window, stagnated);
             if (rc < 0) return rc;
             total_iters += it2;
             total_restarts += rs2;
@@ -382,6 +385,8 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
             int maxIters, int tol_mode, bsp_stats* stats) {
     if (!ctx || !A || !M || !b || !x) return BSP_E_INVALID;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "pCG needs a square matrix");
+    // the reference's pCG solvers are real only (SOR_pCG, IChol0_pCG: blocksp_solvers.hpp:411, 485)
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "pCG: complex matrices are not supported");
     BSP_CHECK(M->n == A->n_rows && M->bs == A->bs, BSP_E_INVALID, "preconditioner does not match the matrix");
     BSP_CHECK(b != x, BSP_E_INVALID, "b and x must not alias");
     BSP_HIP(hipSetDevice(ctx->device));

@@ -296,6 +296,7 @@ int bsp_mpnn_graph_create(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, 
     if (!ctx || !A || !M || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "the learned correction needs a square matrix");
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "the learned correction does not take complex matrices");
     BSP_CHECK(A->bs >= 1 && A->bs <= 8, BSP_E_UNSUPPORTED, "the learned correction supports block sizes 1..8");
     BSP_CHECK(M->kind == BSP_PRE_ILU0 && M->n == A->n_rows && M->bs == A->bs && (!M->f_val.empty() || M->fdev.p),
               BSP_E_INVALID, "the factor to correct must be an ILU0 preconditioner of A");
@@ -552,6 +553,7 @@ int bsp_precond_create_learned(bsp_ctx* ctx, const bsp_bsr* A, const bsp_mpnn* n
     *out = nullptr;
     BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
               "bad precision");
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "the learned correction does not take complex matrices");
     BSP_CHECK(net->bs == A->bs, BSP_E_INVALID, "network and matrix block sizes differ");
     BSP_HIP(hipSetDevice(ctx->device));
     std::vector<int32_t> elim, pos;

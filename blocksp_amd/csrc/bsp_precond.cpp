@@ -245,6 +245,8 @@ int bsp_precond_create_diag(bsp_ctx* ctx, const bsp_bsr* A, int inv_kind, bsp_pr
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "Diag needs a square matrix");
+    // CholeskyFactor is real only (dense_solvers.hpp:123, 196)
+    BSP_CHECK(A->cbs == 0 || inv_kind == 0, BSP_E_UNSUPPORTED, "complex matrices take LU block inverses only");
     BSP_HIP(hipSetDevice(ctx->device));
     auto M = std::make_unique<bsp_precond>();
     M->ctx = ctx;
@@ -271,6 +273,7 @@ int bsp_precond_create_sor_ex(bsp_ctx* ctx, const bsp_bsr* A, double w, int nu, 
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "SOR needs a square matrix");
+    BSP_CHECK(A->cbs == 0 || inv_kind == 0, BSP_E_UNSUPPORTED, "complex matrices take LU block inverses only");
     BSP_CHECK(nu >= 0, BSP_E_INVALID, "nu must be >= 0");
     BSP_HIP(hipSetDevice(ctx->device));
     auto M = std::make_unique<bsp_precond>();
@@ -841,6 +844,8 @@ int bsp_precond_create_ilu0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* co
     if (!ctx) return BSP_E_INVALID;
     BSP_CHECK(precision == BSP_PREC_F64 || precision == BSP_PREC_F32 || precision == BSP_PREC_BF16, BSP_E_INVALID,
               "precision must be F64, F32 or BF16");
+    BSP_CHECK(!A || A->cbs == 0 || precision == BSP_PREC_F64, BSP_E_UNSUPPORTED,
+              "complex matrices take an FP64 ILU0 only");
     return create_ilu0_impl(ctx, A, colour, precision, out);
 }
 
@@ -852,6 +857,7 @@ int bsp_precond_create_ichol0_ex(bsp_ctx* ctx, const bsp_bsr* A, const int32_t* 
     if (!ctx || !A || !out) return BSP_E_INVALID;
     *out = nullptr;
     BSP_CHECK(A->n_rows == A->n_cols, BSP_E_INVALID, "IChol0 needs a square matrix");
+    BSP_CHECK(A->cbs == 0, BSP_E_UNSUPPORTED, "IChol0 is real only (IChol0_pCG, blocksp_solvers.hpp:485)");
     BSP_HIP(hipSetDevice(ctx->device));
     if (factor_device_supported(A->bs) && ctx->factor_device) return ichol0_create_device(ctx, A, colour, out);
     if (int rc = sync_host_values(ctx, A)) return rc;

@@ -575,8 +575,12 @@ int bsp_bsmm(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** out) {
     *out = nullptr;
     BSP_CHECK(A->n_cols == B->n_rows, BSP_E_INVALID, "bsmm: inner dimensions differ");
     BSP_CHECK(A->bs == B->bs, BSP_E_INVALID, "bsmm: block sizes differ");
+    BSP_CHECK(A->cbs == B->cbs, BSP_E_INVALID, "bsmm: one factor is complex, the other real");
     BSP_HIP(hipSetDevice(ctx->device));
-    return bsmm_build(ctx, A, A, false, B, out);
+    int rc = bsmm_build(ctx, A, A, false, B, out);
+    // real-equivalent blocks multiply as the complex ones do: C is complex with A and B
+    if (rc == BSP_OK) (*out)->cbs = A->cbs;
+    return rc;
 }
 
 int bsp_bsmm_trans_a(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr** out) {
@@ -584,10 +588,15 @@ int bsp_bsmm_trans_a(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr**
     *out = nullptr;
     BSP_CHECK(A->n_rows == B->n_rows, BSP_E_INVALID, "bsmm_trans_a: A and B have different row counts");
     BSP_CHECK(A->bs == B->bs, BSP_E_INVALID, "bsmm_trans_a: block sizes differ");
+    BSP_CHECK(A->cbs == B->cbs, BSP_E_INVALID, "bsmm_trans_a: one factor is complex, the other real");
     BSP_HIP(hipSetDevice(ctx->device));
     int rc = ensure_transpose(ctx, A);
     if (rc) return rc;
-    return bsmm_build(ctx, A->transposed.get(), A, true, B, out);
+    // a complex A's real-equivalent transpose is that of A^H: C = A^H B, the reference's
+    // conjugate-transposed product for complex T (dense_multiply.hpp:155-175)
+    rc = bsmm_build(ctx, A->transposed.get(), A, true, B, out);
+    if (rc == BSP_OK) (*out)->cbs = A->cbs;
+    return rc;
 }
 
 int bsp_bsmm_numeric(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr* C) {

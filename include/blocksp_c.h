@@ -95,6 +95,30 @@ int bsp_bsr_download(bsp_ctx* ctx, const bsp_bsr* A, int32_t* rowptr_host, int32
 /* Replace the values (same structure) from a device buffer. */
 int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev);
 
+/* ---- complex matrices: BlockSp<std::complex<double>, B> (blocksp_containers.hpp:59-83) ----
+ * vals_host: nnzb * bs * bs complex numbers as (re, im) pairs -- the memory of
+ * std::complex<double> -- in row-major blocks, stored order.  The device holds each complex
+ * entry a + ib as the real 2 x 2 block [[a, -b], [b, a]], so the matrix is a real BSR of block
+ * size 2 bs (bsp_bsr_info reports 2 bs) and a complex vector is its interleaved (re, im)
+ * memory, 2 n_rows bs doubles.  On such a matrix: bsp_bsmv / bsp_bsmv_dim give the complex
+ * product; bsp_bsmv_transpose and bsp_bsmm_trans_a the conjugate-transposed ones (the
+ * reference's transposed products for complex T, dense_multiply.hpp:80-89, 155-175); bsp_bsmm
+ * of two complex matrices is complex; the NoPre, Diag and SOR (LU block inverses) and ILU0
+ * (FP64, any elimination order) preconditioners and user host preconditioners (on the
+ * interleaved vector) apply to complex vectors; bsp_pgmres runs the complex pGMRES (complex
+ * inner products, least squares and harmonic-Ritz restart; solvers.hpp:109-295 with T
+ * complex) and bsp_pgmres_ritz reports its Ritz values.  Cholesky block inverses, IChol0,
+ * pCG (real only in the reference's solvers, blocksp_solvers.hpp:411, 485), the FP32/BF16
+ * factors and the solvers and products with no reference counterpart return BSP_E_UNSUPPORTED.
+ * bsp_bsr_save records the complex block size in the cache header's fourth word. */
+int bsp_bsr_create_complex(bsp_ctx* ctx, int n_rows, int n_cols, int bs, const int32_t* rowptr_host,
+                           const int32_t* colidx_host, const double* vals_host, bsp_bsr** out);
+/* The complex block size of A (0 for a real matrix). */
+int bsp_bsr_complex_bs(const bsp_bsr* A, int* cbs);
+/* As bsp_bsr_download, the values as complex blocks (vals_host: nnzb * cbs * cbs (re, im) pairs). */
+int bsp_bsr_download_complex(bsp_ctx* ctx, const bsp_bsr* A, int32_t* rowptr_host, int32_t* colidx_host,
+                             double* vals_host);
+
 /* ---- products ---- */
 
 /* y (+)= A x.  Replaces bsmv(A,x) / bsmv(b,A,x) for BlockSp<T,B*D> (blocksp_multiply.hpp:365-390,

@@ -48,6 +48,19 @@ class ZBsr:
         return D
 
 
+def complexify(p, seed=0, scale=0.3):
+    """A complex test matrix on the block pattern of the real problem p: every block times
+    (1 + i s u), u ~ U(-1, 1) per entry, plus i s/2 (mean |diagonal|) on the diagonal blocks --
+    a non-Hermitian, shifted operator.  Returns (rowptr, colidx, complex vals)."""
+    rng = np.random.default_rng(seed)
+    v = p.vals * (1.0 + 1j * scale * rng.uniform(-1, 1, p.vals.shape))
+    for i in range(p.n):
+        for q in range(p.rowptr[i], p.rowptr[i + 1]):
+            if p.colidx[q] == i:
+                v[q] = v[q] + 0.5j * scale * np.abs(np.diag(p.vals[q])).mean() * np.eye(p.bs)
+    return np.asarray(p.rowptr), np.asarray(p.colidx), v
+
+
 def bsmv(A: ZBsr, x):
     """y_i = sum over row i's blocks, in stored order, of A_ij x_j (multiply.hpp:365-390)."""
     bs = A.bs

@@ -20,15 +20,8 @@ from blocksp_amd import problems as pr
 
 
 def complexify(p, seed=0, scale=0.3):
-    """Complex matrix on p's pattern: every block times (1 + i s u), u ~ U(-1, 1) per entry,
-    plus i s/2 on the block diagonal (a non-Hermitian, shifted operator)."""
-    rng = np.random.default_rng(seed)
-    v = p.vals * (1.0 + 1j * scale * rng.uniform(-1, 1, p.vals.shape))
-    for i in range(p.n):
-        for q in range(p.rowptr[i], p.rowptr[i + 1]):
-            if p.colidx[q] == i:
-                v[q] = v[q] + 0.5j * scale * np.abs(np.diag(p.vals[q])).mean() * np.eye(p.bs)
-    return zo.ZBsr(p.n, p.bs, p.rowptr, p.colidx, v)
+    rp, ci, v = zo.complexify(p, seed, scale)
+    return zo.ZBsr(p.n, p.bs, rp, ci, v)
 
 
 def rhs(n, seed=1):
