@@ -12,7 +12,8 @@
 // the GPU through the C ABI in blocksp_c.h.  A matrix is flattened to BSR and
 // uploaded once and cached until it is modified (non-const access bumps a
 // version stamp).  Device arithmetic is FP64: float matrices/vectors are widened
-// on upload and narrowed on download.  Complex T is not supported on the device.
+// on upload and narrowed on download.  Complex T (std::complex<float/double>) uploads through
+// bsp_bsr_create_complex; its vectors travel as interleaved (re, im) doubles.
 //
 // Link with -lblocksp_amd (blocksp_amd/libblocksp_amd.so).
 #pragma once
@@ -524,7 +525,13 @@ public:
 
 inline bsp_ctx* ctx() { return Context::instance().get(); }
 
-/// Device vector of doubles.
+/// Doubles that n values of T occupy on the device (2 per complex value).
+template <typename T>
+constexpr size_t doubles(size_t n) {
+    return is_complex_v<T> ? 2 * n : n;
+}
+
+/// Device vector of doubles; complex values are stored as (re, im) pairs.
 class DVec {
     double* p_ = nullptr;
     size_t n_ = 0;
@@ -542,9 +549,12 @@ public:
     DVec(DVec&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
     double* get() const { return p_; }
     size_t size() const { return n_; }
+    // std::complex<S> is laid out as S[2], so complex arrays move as their scalar parts
     template <typename T>
-    void upload(const T* src) {
-        if constexpr (std::is_same_v<T, double>) {
+    void upload(const T* src_t) {
+        using S = ScalarType_t<T>;
+        const S* src = reinterpret_cast<const S*>(src_t);
+        if constexpr (std::is_same_v<S, double>) {
             check(ctx(), bsp_memcpy_h2d(ctx(), p_, src, sizeof(double) * n_), "h2d");
         } else {
             std::vector<double> tmp(src, src + n_);
@@ -552,13 +562,15 @@ public:
         }
     }
     template <typename T>
-    void download(T* dst) const {
-        if constexpr (std::is_same_v<T, double>) {
+    void download(T* dst_t) const {
+        using S = ScalarType_t<T>;
+        S* dst = reinterpret_cast<S*>(dst_t);
+        if constexpr (std::is_same_v<S, double>) {
             check(ctx(), bsp_memcpy_d2h(ctx(), dst, p_, sizeof(double) * n_), "d2h");
         } else {
             std::vector<double> tmp(n_);
             check(ctx(), bsp_memcpy_d2h(ctx(), tmp.data(), p_, sizeof(double) * n_), "d2h");
-            for (size_t i = 0; i < n_; ++i) dst[i] = static_cast<T>(tmp[i]);
+            for (size_t i = 0; i < n_; ++i) dst[i] = static_cast<S>(tmp[i]);
         }
     }
 };
@@ -585,26 +597,38 @@ public:
     }
     template <typename T, int M>
     std::shared_ptr<DMatrix> get(const BlockSp<T, M>& A) {
-        static_assert(!is_complex_v<T>, "complex matrices are not supported on the device");
         std::lock_guard<std::mutex> g(mu_);
         auto it = map_.find(A.id());
         if (it != map_.end() && it->second.first == A.version()) return it->second.second;
         // flatten to BSR in insertion order
         std::vector<int32_t> rp(A.m() + 1, 0), ci;
-        std::vector<double> vals;
+        std::vector<double> vals;   // complex entries as (re, im) pairs
         ci.reserve(A.nz());
-        vals.reserve((size_t)A.nz() * M * M);
+        vals.reserve(doubles<T>((size_t)A.nz() * M * M));
         for (int i = 0; i < A.m(); ++i) {
             const auto& row = A.get_data(i);
             const auto& cols = A.get_colIndex(i);
             for (size_t k = 0; k < cols.size(); ++k) {
                 ci.push_back(cols[k]);
-                for (int e = 0; e < M * M; ++e) vals.push_back(static_cast<double>(row[k].data()[e]));
+                for (int e = 0; e < M * M; ++e) {
+                    const T v = row[k].data()[e];
+                    if constexpr (is_complex_v<T>) {
+                        vals.push_back(static_cast<double>(v.real()));
+                        vals.push_back(static_cast<double>(v.imag()));
+                    } else {
+                        vals.push_back(static_cast<double>(v));
+                    }
+                }
             }
             rp[i + 1] = (int32_t)ci.size();
         }
         auto d = std::make_shared<DMatrix>();
-        check(ctx(), bsp_bsr_create(ctx(), A.m(), A.n(), M, rp.data(), ci.data(), vals.data(), &d->h), "bsr_create");
+        if constexpr (is_complex_v<T>)
+            check(ctx(), bsp_bsr_create_complex(ctx(), A.m(), A.n(), M, rp.data(), ci.data(), vals.data(), &d->h),
+                  "bsr_create_complex");
+        else
+            check(ctx(), bsp_bsr_create(ctx(), A.m(), A.n(), M, rp.data(), ci.data(), vals.data(), &d->h),
+                  "bsr_create");
         d->n_rows = A.m();
         d->n_cols = A.n();
         d->bs = M;
@@ -638,8 +662,7 @@ template <typename T, int M, int B, int D>
 void bsmv_device(solArray<T, B, D>& b, const BlockSp<T, M>& A, const solArray<T, B, D>& x, bool accumulate,
                  bool transpose, int dim_mode, int dim) {
     auto dA = gpu::device_matrix(A);
-    const size_t nx = x.size_flat();
-    gpu::DVec dx(nx), dy(b.size_flat());
+    gpu::DVec dx(gpu::doubles<T>(x.size_flat())), dy(gpu::doubles<T>(b.size_flat()));
     dx.upload(x.data());
     if (accumulate) dy.upload(b.data());
     int rc;
@@ -715,7 +738,8 @@ BlockSp<T, M> blkSpIdent(int size) {
 }
 
 /// Transpose with transposed blocks (multiply.hpp:31-44): row j lists the rows i holding a
-/// block in column j in ascending i.
+/// block in column j in ascending i; blocks are conjugate-transposed for complex T, as
+/// dense::transpose is (dense_multiply.hpp:207-213).
 template <typename T, int M>
 BlockSp<T, M> blkSpTranspose(const BlockSp<T, M>& A) {
     BlockSp<T, M> tr(A.n(), A.m());
@@ -725,7 +749,12 @@ BlockSp<T, M> blkSpTranspose(const BlockSp<T, M>& A) {
             const auto& src = A.get_data(im)[in];
             auto& dst = tr(j, im);
             for (int r = 0; r < M; ++r)
-                for (int c = 0; c < M; ++c) dst(c, r) = src(r, c);
+                for (int c = 0; c < M; ++c) {
+                    if constexpr (is_complex_v<T>)
+                        dst(c, r) = std::conj(src(r, c));
+                    else
+                        dst(c, r) = src(r, c);
+                }
         }
     return tr;
 }
@@ -746,14 +775,24 @@ BlockSp<T, M> blkSpMatMat(const BlockSp<T, M>& A, const BlockSp<T, M>& B, bool A
     int64_t nz = 0;
     gpu::check(gpu::ctx(), bsp_bsr_info(h, &nr, &nc, &bs, &nz), "bsr_info");
     std::vector<int32_t> rp(nr + 1), ci(std::max<int64_t>(nz, 1));
-    std::vector<double> v(std::max<size_t>((size_t)nz * M * M, 1));
-    gpu::check(gpu::ctx(), bsp_bsr_download(gpu::ctx(), h, rp.data(), ci.data(), v.data()), "bsr_download");
+    std::vector<double> v(std::max<size_t>(gpu::doubles<T>((size_t)nz * M * M), 1));
+    if constexpr (is_complex_v<T>)
+        gpu::check(gpu::ctx(), bsp_bsr_download_complex(gpu::ctx(), h, rp.data(), ci.data(), v.data()),
+                   "bsr_download_complex");
+    else
+        gpu::check(gpu::ctx(), bsp_bsr_download(gpu::ctx(), h, rp.data(), ci.data(), v.data()), "bsr_download");
     BlockSp<T, M> C(nr, nc);
     for (int i = 0; i < nr; ++i) {
         C.reserve(i, rp[i + 1] - rp[i]);
         for (int p = rp[i]; p < rp[i + 1]; ++p) {
             auto& blk = C(i, ci[p]);
-            for (int e = 0; e < M * M; ++e) blk.data()[e] = static_cast<T>(v[(size_t)p * M * M + e]);
+            for (int e = 0; e < M * M; ++e) {
+                const size_t q = (size_t)p * M * M + e;
+                if constexpr (is_complex_v<T>)
+                    blk.data()[e] = T((ScalarType_t<T>)v[2 * q], (ScalarType_t<T>)v[2 * q + 1]);
+                else
+                    blk.data()[e] = static_cast<T>(v[q]);
+            }
         }
     }
     return C;
@@ -868,7 +907,7 @@ public:
     bsp_precond* handle() const { return pre_ ? pre_->h : nullptr; }
     void operator()(solArray<T, B, D>& r) const {
         if (!pre_) return;
-        gpu::DVec dr(r.size_flat());
+        gpu::DVec dr(gpu::doubles<T>(r.size_flat()));
         dr.upload(r.data());
         gpu::check(gpu::ctx(), bsp_precond_apply(gpu::ctx(), pre_->h, dr.get()), "precond_apply");
         dr.download(r.data());
@@ -961,12 +1000,16 @@ concept DevicePrecond = requires(const P& p) { p.handle(); };
 template <typename T, int B, int D, typename PreCond>
 struct HostPrecondThunk {
     const PreCond* pc;
+    // r: len doubles, (re, im) pairs for complex T
     static void call(double* r, int64_t len, void* user) {
+        using S = ScalarType_t<T>;
+        constexpr int64_t per = is_complex_v<T> ? 2 : 1;
         auto* self = static_cast<HostPrecondThunk*>(user);
-        solArray<T, B, D> sa((int)(len / (B * D)));
-        for (int64_t i = 0; i < len; ++i) sa.data()[i] = static_cast<T>(r[i]);
+        solArray<T, B, D> sa((int)(len / per / (B * D)));
+        S* s = reinterpret_cast<S*>(sa.data());
+        for (int64_t i = 0; i < len; ++i) s[i] = static_cast<S>(r[i]);
         (*self->pc)(sa);
-        for (int64_t i = 0; i < len; ++i) r[i] = static_cast<double>(sa.data()[i]);
+        for (int64_t i = 0; i < len; ++i) r[i] = static_cast<double>(s[i]);
     }
 };
 
@@ -992,7 +1035,7 @@ solArray<T, B, D> run(const BlockSp<T, B * D>& A, const solArray<T, B, D>& b, co
         }
         h = wrapper.h;
     }
-    gpu::DVec db(b.size_flat()), dx(b.size_flat());
+    gpu::DVec db(gpu::doubles<T>(b.size_flat())), dx(gpu::doubles<T>(b.size_flat()));
     db.upload(b.data());
     fn(dA->h, h, db.get(), dx.get());
     solArray<T, B, D> x(b.len());

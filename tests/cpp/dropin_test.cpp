@@ -3,6 +3,7 @@
 // structs, ez:: one-call solvers and a user lambda preconditioner, checking the
 // true residual of every solve on the host.  Prints one line per check and
 // exits non-zero on the first failure.
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -81,6 +82,52 @@ double relres(const BSM<double, B * D>& A, const solArray<double, B, D>& x, cons
     for (size_t i = 0; i < r.size(); ++i) {
         nr += r[i] * r[i];
         nb += b.data()[i] * b.data()[i];
+    }
+    return std::sqrt(nr / nb);
+}
+
+using Z = std::complex<double>;
+
+// complex 2D 5-point block stencil: shifted, non-Hermitian blocks (a Helmholtz-like operator)
+template <int CB>
+BSM<Z, CB> zstencil(int nx, int ny) {
+    BSM<Z, CB> A(nx * ny);
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            const int i = iy * nx + ix;
+            TinyMatrix<Z, CB, CB> d(Z(0.0));
+            for (int r = 0; r < CB; ++r) {
+                d(r, r) = Z(4.5, 0.8);
+                if (r + 1 < CB) d(r, r + 1) = Z(0.2 * urand(), 0.2 * urand());
+            }
+            A(i, i) = d;
+            const int nb[4] = {ix > 0 ? i - 1 : -1, ix < nx - 1 ? i + 1 : -1, iy > 0 ? i - nx : -1,
+                               iy < ny - 1 ? i + nx : -1};
+            for (int j : nb) {
+                if (j < 0) continue;
+                TinyMatrix<Z, CB, CB> o(Z(0.0));
+                for (int r = 0; r < CB; ++r) o(r, r) = Z(-1.0, 0.1 * urand());
+                A(i, j) = o;
+            }
+        }
+    A.shrink_to_fit();
+    return A;
+}
+
+template <int CB>
+double zrelres(const BSM<Z, CB>& A, const solArray<Z, CB>& x, const solArray<Z, CB>& b) {
+    std::vector<Z> r(b.data(), b.data() + b.size_flat());
+    for (int i = 0; i < A.m(); ++i)
+        for (size_t k = 0; k < A.colSize(i); ++k) {
+            const int j = A.colInd(i, (int)k);
+            const auto& blk = A.get_data(i)[k];
+            for (int rr = 0; rr < CB; ++rr)
+                for (int c = 0; c < CB; ++c) r[i * CB + rr] -= blk(rr, c) * x.data()[j * CB + c];
+        }
+    double nr = 0, nb = 0;
+    for (size_t i = 0; i < r.size(); ++i) {
+        nr += std::norm(r[i]);
+        nb += std::norm(b.data()[i]);
     }
     return std::sqrt(nr / nb);
 }
@@ -274,6 +321,52 @@ int main() {
         const auto Sumd = dense(Sum);
         for (size_t i = 0; i < Sd.size(); ++i) esplit = std::max(esplit, std::abs(Sumd[i] - Sd[i]));
         report("upTri_plus_strict_lowTri", esplit, 0.0);
+    }
+    // ---- complex T: BlockSp<std::complex<double>, 3> (the qc_test path, blocksp_tests.hpp:386-401)
+    {
+        constexpr int CB = 3;
+        auto Az = zstencil<CB>(14, 12);
+        solArray<Z, CB> bz(Az.m());
+        for (size_t i = 0; i < bz.size_flat(); ++i) bz.data()[i] = Z(urand(), urand());
+        auto yz = bsmv(Az, bz);
+        double err = 0.0, scale = 0.0;
+        for (int i = 0; i < Az.m(); ++i)
+            for (int r = 0; r < CB; ++r) {
+                Z acc(0.0);
+                for (size_t k = 0; k < Az.colSize(i); ++k)
+                    for (int c = 0; c < CB; ++c)
+                        acc += Az.get_data(i)[k](r, c) * bz.data()[Az.colInd(i, (int)k) * CB + c];
+                err = std::max(err, std::abs(acc - yz.data()[i * CB + r]));
+                scale = std::max(scale, std::abs(acc));
+            }
+        report("complex_bsmv", err / scale, 1e-14);
+        report("complex_ilu0_pgmres_dr", zrelres<CB>(Az, ez::ilu0_pGMRES_dr<Z, CB>(Az, bz), bz), 1e-6);
+        report("complex_ilu0_pgmres", zrelres<CB>(Az, ILU0_pGMRES<Z, CB>(Az, 1e-10, 5000, -1, -1).solve(Az, bz), bz),
+               1e-8);
+        report("complex_sor_pgmres_robust", zrelres<CB>(Az, ez::sor_pGMRES_robust<Z, CB>(Az, bz), bz), 1e-6);
+        // a host lambda preconditioner on complex vectors (block Jacobi with the diagonal's
+        // diagonal) through the host callback
+        auto zjac = [&](solArray<Z, CB>& r) {
+            for (int i = 0; i < Az.m(); ++i)
+                for (int c = 0; c < CB; ++c) r.data()[i * CB + c] /= Az(i, i)(c, c);
+        };
+        report("complex_pgmres_lambda_precond", zrelres<CB>(Az, pGMRES<Z, CB, 1>(Az, bz, 1e-10, zjac, 5000, 20, 5), bz),
+               1e-8);
+        // A^H A on the device against the host conjugate transpose
+        auto AhA = BlockSp::bsmm(Az, Az, true);
+        auto Ah = BlockSp::bs_tr(Az);
+        auto AhA2 = BlockSp::bsmm(Ah, Az);
+        double eh = 0.0, sh = 0.0;
+        for (int i = 0; i < AhA.m(); ++i)
+            for (size_t k = 0; k < AhA.colSize(i); ++k) {
+                const int j = AhA.colInd(i, (int)k);
+                for (int r = 0; r < CB; ++r)
+                    for (int c = 0; c < CB; ++c) {
+                        eh = std::max(eh, std::abs(AhA.get_data(i)[k](r, c) - AhA2(i, j)(r, c)));
+                        sh = std::max(sh, std::abs(AhA2(i, j)(r, c)));
+                    }
+            }
+        report("complex_bsmm_AhA_equals_bs_tr_path", eh / sh, 1e-14);
     }
     std::printf("%s\n", failures ? "DROPIN FAILED" : "DROPIN PASSED");
     return failures ? 1 : 0;
