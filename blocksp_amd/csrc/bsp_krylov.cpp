@@ -425,7 +425,12 @@ int bsp_pcg(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const double* 
         double pn = 0.0;
         rc = reduce(ctx, 1, len, p, p, SLOT_TMP0, &pn);
         if (rc) return rc;
-        if (outer == 0) res0 = pn;
+        if (outer == 0) {
+            res0 = pn;
+            // as pGMRES: a right-hand side (or preconditioner) that gives a non-finite initial
+            // residual is an error, not a run that stops at once
+            if (!std::isfinite(pn)) return fail(ctx, BSP_E_NUMERIC, "initial residual is not finite");
+        }
         double inner_tol = tol;
         if (outer > 0) inner_tol = std::max(0.25 * target / rr, 1e-15);
         const double tolerance = pn * inner_tol;
