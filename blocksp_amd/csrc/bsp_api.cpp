@@ -18,7 +18,7 @@ using namespace bsp;
 namespace bsp {
 
 int bsmv_matrix(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int accumulate) {
-    if (ctx->bsmv_variant >= 3 && bsmv_perm_supported(A->bs)) {
+    if (ctx->bsmv_layout == 1 && bsmv_perm_supported(A->bs)) {
         if (!A->valp.p) {
             BSP_HIP(A->valp.alloc(sizeof(double) * std::max<int64_t>(A->nnzb * A->bs * A->bs, 1)));
             int rc = launch_permute_blocks(ctx, A->nnzb, A->bs, A->val.as<double>(), A->valp.as<double>());
@@ -120,18 +120,16 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
     } else if (k == "sweep_wg_per_cu") {
         BSP_CHECK(value >= 1 && value <= 64, BSP_E_INVALID, "sweep_wg_per_cu must be in [1, 64]");
         ctx->sweep_wg_per_cu = (int)value;
-    } else if (k == "bsmv_variant") {
-        BSP_CHECK(value >= 0 && value <= 5, BSP_E_INVALID, "bsmv_variant must be in [0, 5]");
-        ctx->bsmv_variant = (int)value;
+    } else if (k == "bsmv_layout") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "bsmv_layout must be 0 or 1");
+        ctx->bsmv_layout = (int)value;
     } else if (k == "fgmres_ortho") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_ortho must be 0 (CGS2) or 1 (MGS)");
         ctx->fgmres_ortho = (int)value;
     } else if (k == "fgmres_basis_f32") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_basis_f32 must be 0 or 1");
         ctx->fgmres_basis_f32 = (int)value;
-    } else if (k == "bf16_variant") {
-        BSP_CHECK(value >= 0 && value <= 3, BSP_E_INVALID, "bf16_variant must be 0, 1, 2 or 3");
-        ctx->bf16_variant = (int)value;
+
     } else if (k == "sweep_graphs") {
         ctx->sweep_graphs = value != 0;
     } else if (k == "sweep_ell") {

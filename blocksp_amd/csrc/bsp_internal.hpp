@@ -152,19 +152,13 @@ struct bsp_ctx {
                                      // host loop (the method of the reference, one row at a time)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
-    int bsmv_variant = 5;            // FP64 bsmv on the 512^2 DG matrix (bs 8): 5 chunk-major value copy +
-                                     // XCD-aware row ranges + streaming value loads (117.6 us, 6.04 TB/s, the
-                                     // default for even bs <= 8); 4 chunk-major + XCD (126.3 us); 3 chunk-major
-                                     // (127.1 us); 0 row-major lane = row (131.2 us); 1 row-major streaming
-                                     // (2.2x slower); 2 row-major coalesced 16-byte chunks (slower)
+    int bsmv_layout = 1;             // solver bsmv for even bs <= 8: 1 a chunk-major copy of the values read by
+                                     // XCD-aware row ranges with streaming loads (117.6 us on the 512^2 DG
+                                     // matrix, 6.04 TB/s), 0 the values as stored (131.2 us)
     int fgmres_ortho = 0;            // mixed FGMRES orthogonalisation: 0 classical Gram-Schmidt with conditional
                                      // reorthogonalisation, 1 modified Gram-Schmidt (the reference's)
     int fgmres_basis_f32 = 0;        // mixed FGMRES Arnoldi basis storage: 0 FP64, 1 FP32 (FP64 arithmetic,
                                      // classical Gram-Schmidt only)
-    int bf16_variant = 0;            // BF16 MFMA SpMM on the 64^3 DG matrix, 32 RHS: 0 loads of 4 slots of a
-                                     // row pair issued before their MFMAs + streaming matrix loads / result
-                                     // stores (176 us); 1 slot by slot, streaming (193 us); 2 slot by slot,
-                                     // cached (211 us); 3 as 0 with 8 slots per batch (182 us)
     // reduction scratch
     bsp::DevMem red_partials;        // double [kMaxRedBlocks]
     bsp::DevMem red_counter;         // uint32 [kRedSlots]

@@ -99,21 +99,19 @@ typedef double d2v_p __attribute__((ext_vector_type(2)));
 // stored chunk-major: entry (r, c) of a BS x BS block sits at ((c/2) * BS + r) * 2 + c%2,
 // so the k-th 16-byte load of the BS lanes of a block row covers one contiguous
 // BS*16-byte run (in the row-major layout the same load touches BS separate rows).
-// XCD: workgroups are remapped so each of the 8 XCDs walks one contiguous range of block
-// rows (neighbouring x blocks stay in that XCD's L2).  NT: streaming loads for the values.
-template <int BS, bool XCD, bool NT>
+// Workgroups are remapped so each of the 8 XCDs walks one contiguous range of block rows
+// (neighbouring x blocks stay in that XCD's L2), and the values -- read once -- are streamed
+// with non-temporal loads so that they do not evict x.
+template <int BS>
 __global__ __launch_bounds__(kBlock) void k_bsmv_perm(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const double* __restrict__ valp, const double* __restrict__ x,
                                                       double* __restrict__ y, int accumulate, int nwg) {
     static_assert(BS % 2 == 0, "chunk-major layout needs an even block size");
     constexpr int RPW = kWave / BS;
     const int lane = threadIdx.x & (kWave - 1);
-    int wg = blockIdx.x;
-    if constexpr (XCD) {
-        const int per = (nwg + 7) / 8;
-        wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
-        if (wg >= nwg) return;
-    }
+    const int per = (nwg + 7) / 8;
+    const int wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (wg >= nwg) return;
     const int wave = (wg * kBlock + threadIdx.x) / kWave;
     const int g = lane / BS;
     const int r = lane - g * BS;
@@ -126,16 +124,9 @@ __global__ __launch_bounds__(kBlock) void k_bsmv_perm(int n, const int* __restri
         const double2* s2 = reinterpret_cast<const double2*>(valp + (size_t)p * BS * BS) + r;
 #pragma unroll
         for (int k = 0; k < BS / 2; ++k) {
-            double2 t;
-            if constexpr (NT) {
-                const d2v_p u = __builtin_nontemporal_load(reinterpret_cast<const d2v_p*>(s2 + k * BS));
-                t.x = u.x;
-                t.y = u.y;
-            } else {
-                t = s2[k * BS];
-            }
-            a[2 * k] = t.x;
-            a[2 * k + 1] = t.y;
+            const d2v_p u = __builtin_nontemporal_load(reinterpret_cast<const d2v_p*>(s2 + k * BS));
+            a[2 * k] = u.x;
+            a[2 * k + 1] = u.y;
         }
     };
     int p = p0;
@@ -167,110 +158,6 @@ __global__ void k_permute_blocks(int64_t total, int bs, const double* __restrict
     const int e = (int)(t - p * b2);
     const int r = e / bs, c = e - r * bs;
     valp[p * b2 + ((c / 2) * bs + r) * 2 + (c & 1)] = val[t];
-}
-
-// Same mapping; the matrix values (read exactly once) are streamed with
-// non-temporal loads so that they do not evict x from the caches, and four
-// blocks of a row are loaded before any is used (more bytes in flight).
-typedef double d2v __attribute__((ext_vector_type(2)));
-
-template <int BS>
-__device__ __forceinline__ void load_vec_nt(const double* __restrict__ src, double (&a)[BS]) {
-    if constexpr (BS % 2 == 0) {
-        const d2v* s2 = reinterpret_cast<const d2v*>(src);
-#pragma unroll
-        for (int c = 0; c < BS / 2; ++c) {
-            const d2v t = __builtin_nontemporal_load(s2 + c);
-            a[2 * c] = t.x;
-            a[2 * c + 1] = t.y;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < BS; ++c) a[c] = __builtin_nontemporal_load(src + c);
-    }
-}
-
-template <int BS>
-__global__ __launch_bounds__(kBlock) void k_bsmv_nt(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                    const double* __restrict__ val, const double* __restrict__ x,
-                                                    double* __restrict__ y, int accumulate) {
-    constexpr int RPW = kWave / BS;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const int g = lane / BS;
-    const int r = lane - g * BS;
-    if (g >= RPW) return;
-    const int i = wave * RPW + g;
-    if (i >= n) return;
-    double acc = accumulate ? y[(size_t)i * BS + r] : 0.0;
-    const int p0 = rp[i], p1 = rp[i + 1];
-    // all blocks of the row (up to 8 per pass) are requested before any is used
-    constexpr int U = BS <= 8 ? 8 : 2;
-    for (int pb = p0; pb < p1; pb += U) {
-        const int cnt = min(U, p1 - pb);
-        int cols[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) cols[u] = u < cnt ? ci[pb + u] : 0;
-        double a[U][BS], xv[U][BS];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u < cnt) load_vec<BS>(val + ((size_t)(pb + u) * BS + r) * BS, a[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u < cnt) load_vec<BS>(x + (size_t)cols[u] * BS, xv[u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u < cnt) acc += row_dot<BS>(a[u], xv[u]);
-    }
-    y[(size_t)i * BS + r] = acc;
-}
-
-// Coalesced mapping (even BS <= 8): a block row is owned by BS*BS/2 lanes, each
-// holding one 16-byte chunk (two adjacent entries of one row) of a block, so one
-// load instruction streams whole contiguous blocks.  Lane l of a group takes row
-// r = l / (BS/2), columns c = 2 (l % (BS/2)) and c+1 of every block of the row,
-// accumulates over the blocks, and the BS/2 lanes of a row are summed with
-// cross-lane shuffles.  All blocks of the row are loaded before any is used.
-template <int BS>
-__global__ __launch_bounds__(kBlock) void k_bsmv_co(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                    const double* __restrict__ val, const double* __restrict__ x,
-                                                    double* __restrict__ y, int accumulate) {
-    constexpr int LPB = BS * BS / 2;   // lanes per block row
-    constexpr int HPR = BS / 2;        // lanes per matrix row
-    constexpr int RPW = kWave / LPB;   // block rows per wavefront
-    constexpr int MAXB = 8;            // blocks loaded ahead
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const int g = lane / LPB;
-    const int l = lane - g * LPB;
-    const int r = l / HPR;
-    const int c = (l - r * HPR) * 2;
-    const int i = wave * RPW + g;
-    const bool active = i < n;
-    double acc = 0.0;
-    if (active) {
-        const int p0 = rp[i], p1 = rp[i + 1];
-        const double2* v2 = reinterpret_cast<const double2*>(val);
-        for (int pb = p0; pb < p1; pb += MAXB) {
-            const int cnt = min(MAXB, p1 - pb);
-            double2 a[MAXB], xv[MAXB];
-#pragma unroll
-            for (int u = 0; u < MAXB; ++u)
-                if (u < cnt) {
-                    a[u] = v2[(size_t)(pb + u) * LPB + l];
-                    xv[u] = *reinterpret_cast<const double2*>(x + (size_t)ci[pb + u] * BS + c);
-                }
-#pragma unroll
-            for (int u = 0; u < MAXB; ++u)
-                if (u < cnt) acc += fma(a[u].x, xv[u].x, a[u].y * xv[u].y);
-        }
-    }
-#pragma unroll
-    for (int o = 1; o < HPR; o <<= 1) acc += __shfl_xor(acc, o, kWave);
-    if (active && l % HPR == 0) {
-        double* yi = y + (size_t)i * BS + r;
-        *yi = accumulate ? *yi + acc : acc;
-    }
 }
 
 // Runtime block size (bs <= 64): same mapping, scalar loads.
@@ -928,25 +815,11 @@ bool dispatch_bs(int bs, Args&&... args) {
 
 template <int BS>
 struct BsmvL {
-    static void run(hipStream_t st, int variant, int n, const int* rp, const int* ci, const double* val,
-                    const double* x, double* y, int acc) {
+    static void run(hipStream_t st, int n, const int* rp, const int* ci, const double* val, const double* x, double* y,
+                    int acc) {
         constexpr int RPW = kWave / BS;
         const int64_t waves = (n + RPW - 1) / RPW;
-        if constexpr (BS % 2 == 0 && BS <= 8) {
-            if (variant == 2) {
-                constexpr int RPWC = kWave / (BS * BS / 2);
-                const int64_t wc = (n + RPWC - 1) / RPWC;
-                hipLaunchKernelGGL(k_bsmv_co<BS>, dim3(grid_for(wc * kWave)), dim3(kBlock), 0, st, n, rp, ci, val, x,
-                                   y, acc);
-                return;
-            }
-        }
-        if (variant == 1)
-            hipLaunchKernelGGL(k_bsmv_nt<BS>, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, st, n, rp, ci, val, x, y,
-                               acc);
-        else
-            hipLaunchKernelGGL(k_bsmv<BS>, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, st, n, rp, ci, val, x, y,
-                               acc);
+        hipLaunchKernelGGL(k_bsmv<BS>, dim3(grid_for(waves * kWave)), dim3(kBlock), 0, st, n, rp, ci, val, x, y, acc);
     }
 };
 
@@ -974,7 +847,7 @@ struct DiagL {
 int launch_bsmv(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* val,
                 const double* x, double* y, int accumulate) {
     if (n <= 0) return BSP_OK;
-    if (!dispatch_bs<BsmvL>(bs, ctx->stream, ctx->bsmv_variant, n, rp, ci, val, x, y, accumulate)) {
+    if (!dispatch_bs<BsmvL>(bs, ctx->stream, n, rp, ci, val, x, y, accumulate)) {
         BSP_CHECK(bs <= kWave, BSP_E_UNSUPPORTED, "block size > 64 is not supported");
         const int rpw = kWave / bs;
         const int64_t waves = (n + rpw - 1) / rpw;
@@ -996,33 +869,23 @@ int launch_permute_blocks(const bsp_ctx* ctx, int64_t nnzb, int bs, const double
 }
 
 template <int BS>
-void launch_perm_bs(hipStream_t st, int mode, int n, const int* rp, const int* ci, const double* valp,
-                    const double* x, double* y, int accumulate) {
+void launch_perm_bs(hipStream_t st, int n, const int* rp, const int* ci, const double* valp, const double* x,
+                    double* y, int accumulate) {
     constexpr int rpw = kWave / BS;
     const int64_t waves = (n + rpw - 1) / rpw;
     const int nwg = (int)grid_for(waves * kWave);
     const int grid8 = (nwg + 7) / 8 * 8;
-    if (mode == 1)
-        hipLaunchKernelGGL((k_bsmv_perm<BS, true, false>), dim3(grid8), dim3(kBlock), 0, st, n, rp, ci, valp, x, y,
-                           accumulate, nwg);
-    else if (mode == 2)
-        hipLaunchKernelGGL((k_bsmv_perm<BS, true, true>), dim3(grid8), dim3(kBlock), 0, st, n, rp, ci, valp, x, y,
-                           accumulate, nwg);
-    else
-        hipLaunchKernelGGL((k_bsmv_perm<BS, false, false>), dim3(nwg), dim3(kBlock), 0, st, n, rp, ci, valp, x, y,
-                           accumulate, nwg);
+    hipLaunchKernelGGL(k_bsmv_perm<BS>, dim3(grid8), dim3(kBlock), 0, st, n, rp, ci, valp, x, y, accumulate, nwg);
 }
 
 int launch_bsmv_perm(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* valp,
                      const double* x, double* y, int accumulate) {
     if (n <= 0) return BSP_OK;
-    // variant 3: plain order; 4: XCD-aware row ranges; 5: XCD-aware + streaming value loads
-    const int mode = ctx->bsmv_variant == 4 ? 1 : ctx->bsmv_variant == 5 ? 2 : 0;
     switch (bs) {
-        case 2: launch_perm_bs<2>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
-        case 4: launch_perm_bs<4>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
-        case 6: launch_perm_bs<6>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
-        case 8: launch_perm_bs<8>(ctx->stream, mode, n, rp, ci, valp, x, y, accumulate); break;
+        case 2: launch_perm_bs<2>(ctx->stream, n, rp, ci, valp, x, y, accumulate); break;
+        case 4: launch_perm_bs<4>(ctx->stream, n, rp, ci, valp, x, y, accumulate); break;
+        case 6: launch_perm_bs<6>(ctx->stream, n, rp, ci, valp, x, y, accumulate); break;
+        case 8: launch_perm_bs<8>(ctx->stream, n, rp, ci, valp, x, y, accumulate); break;
         default: return fail(ctx, BSP_E_UNSUPPORTED, "chunk-major bsmv needs block size 2, 4, 6 or 8");
     }
     BSP_LAUNCH_CHECK();

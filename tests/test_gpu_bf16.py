@@ -40,14 +40,9 @@ def _ref(rp, ci, Ab, Xb, n, nrhs):
     return Y
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("n,nrhs", [(37, 32), (36, 16), (11, 48)])
-def test_bf16_spmm_matches_numpy(ctx, n, nrhs, variant):
-    ctx.set_option("bf16_variant", variant)
-    try:
-        _check_spmm(ctx, n, nrhs)
-    finally:
-        ctx.set_option("bf16_variant", 0)
+def test_bf16_spmm_matches_numpy(ctx, n, nrhs):
+    _check_spmm(ctx, n, nrhs)
 
 
 def _check_spmm(ctx, n, nrhs):
