@@ -361,6 +361,28 @@ def bench_batch(bsp, pr, ctx, baseline_every=16):
     return out
 
 
+def bench_batch_solve(bsp, pr, ctx, wg=1024, rhs_per_item=1):
+    """The 512-system batch of bench_batch with fixed tiles (the tuned 1024 x 1) and two
+    complete IChol0-pCG solves (the first captures the batch's HIP graph), nothing else -- the
+    run the batch's FETCH_SIZE / WRITE_SIZE counters are read from."""
+    probs = pr.batch_blocklap(nsys=512, lo=1e3, hi=5e4, seed=2, nrhs=4)
+    rp, ci, vals, sysp = bsp.block_diagonal(probs)
+    n = int(sysp[-1])
+    Bh = np.stack([np.concatenate([q.meta["rhs"][k] for q in probs]) for k in range(4)])
+    A = bsp.BSR(ctx, n, n, 4, rp, ci, vals)
+    del rp, ci, vals
+    bat = bsp.Batch(ctx, A, sysp, 4)
+    bat.set_tiles(wg, rhs_per_item)
+    B = ctx.to_device(Bh)
+    X = ctx.zeros(Bh.shape)
+    bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    its, rel, ms, rc = bat.pcg(B, X, 1e-10, maxIters=5000, tol_mode=bsp.TOL_TRUE_RESIDUAL)
+    log(f"batch solve (tiles {wg} x {rhs_per_item}): {ms:.0f} ms, iterations mean {its.mean():.1f} max {its.max()}")
+    return {"batch_solve": {"batched_s": round(ms / 1e3, 4), "tiles": {"wg": wg, "rhs_per_item": rhs_per_item},
+                            "iters_mean": round(float(its.mean()), 1), "iters_max": int(its.max()),
+                            "converged": rc == 0}}
+
+
 def bench_c1(bsp, pr, ctx):
     """The reference's CPU-sized case: 64 x 64 block stencil (B=2, 4096 block rows),
     SOR(w=1.2, nu=3)-pGMRES(30, k=10) to tol 1e-10 with the reference's stopping rule, on the GPU
@@ -476,7 +498,7 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
     ap.add_argument("--skip-extras", action="store_true", help="skip the 3D BF16, batched and learned workloads")
-    ap.add_argument("--only", choices=["bf16", "batch", "learned", "pcg", "c1", "bsmm"], default=None,
+    ap.add_argument("--only", choices=["bf16", "batch", "batch_solve", "learned", "pcg", "c1", "bsmm"], default=None,
                     help="run one secondary workload alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -500,7 +522,8 @@ def main():
               "learned": lambda: bench_learned(bsp, pr, ctx, train_only=True),
               "pcg": lambda: bench_pcg(bsp, pr, ctx),
               "c1": lambda: bench_c1(bsp, pr, ctx),
-              "bsmm": lambda: bench_bsmm_all(bsp, pr, ctx)}[args.only]
+              "bsmm": lambda: bench_bsmm_all(bsp, pr, ctx),
+              "batch_solve": lambda: bench_batch_solve(bsp, pr, ctx)}[args.only]
         print(json.dumps(fn()))
         return
 
