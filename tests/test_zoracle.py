@@ -60,25 +60,43 @@ def test_real_data_matches_real_oracle(small, kind, kw, restart, k):
     np.testing.assert_allclose(xz.real, xr, rtol=0, atol=1e-9 * np.abs(xr).max())
 
 
+def property_failures(A, b, M):
+    """The complex-only properties of a pGMRES solve; returns the names of those that fail."""
+    bad = []
+    x, st = zo.pgmres(A, b, 1e-10, M, maxIters=600, restart=8, k=3)
+    if st.restarts < 1:
+        bad.append("no restart")
+    # solves the dense complex system
+    xd = np.linalg.solve(A.dense(), b)
+    if not np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd):
+        bad.append("dense solution")
+    # the least-squares residual estimate is the true preconditioned residual
+    true = zo.vnorm(M(b - zo.bsmv(A, x)))
+    if not (abs(true - st.res_est) <= 1e-6 * st.res0 + 1e-3 * true and st.res_est <= 1e-10 * st.res0):
+        bad.append("residual estimate")
+    # b -> c b with |c| = 1 gives x -> c x in the same number of iterations
+    c = np.exp(0.7j)
+    xc, stc = zo.pgmres(A, c * b, 1e-10, M, maxIters=600, restart=8, k=3)
+    if stc.iters != st.iters or not np.allclose(xc, c * x, rtol=0, atol=1e-10 * np.abs(x).max()):
+        bad.append("unit-modulus scaling")
+    return bad
+
+
 @pytest.mark.parametrize("kind,kw", [(zo.PRE_ILU0, {}), (zo.PRE_SOR, dict(w=1.0, nu=2)), (zo.PRE_DIAG, {})])
 def test_complex_solve_properties(small, kind, kw):
     A = complexify(small, seed=2)
     b = rhs(A.n * A.bs, 3)
-    M = zo.Precond(A, kind, **kw)
-    x, st = zo.pgmres(A, b, 1e-10, M, maxIters=600, restart=8, k=3)
-    assert st.restarts >= 1
-    # solves the dense complex system
-    xd = np.linalg.solve(A.dense(), b)
-    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
-    # the least-squares residual estimate is the true preconditioned residual
-    true = zo.vnorm(M(b - zo.bsmv(A, x)))
-    assert abs(true - st.res_est) <= 1e-6 * st.res0 + 1e-3 * true
-    assert st.res_est <= 1e-10 * st.res0
-    # b -> c b with |c| = 1 gives x -> c x in the same number of iterations
-    c = np.exp(0.7j)
-    xc, stc = zo.pgmres(A, c * b, 1e-10, M, maxIters=600, restart=8, k=3)
-    assert stc.iters == st.iters
-    np.testing.assert_allclose(xc, c * x, rtol=0, atol=1e-10 * np.abs(x).max())
+    assert property_failures(A, b, zo.Precond(A, kind, **kw)) == []
+
+
+@pytest.mark.parametrize("wrong", ["no conjugate", "conjugate on the other operand"])
+def test_planted_conjugation_mistakes_fail_the_pins(small, monkeypatch, wrong):
+    """A plausible mistake planted in the oracle's inner product is caught by the properties."""
+    fn = (lambda a, b: np.dot(a, b)) if wrong == "no conjugate" else (lambda a, b: np.vdot(b, a))
+    monkeypatch.setattr(zo, "vdot", fn)
+    A = complexify(small, seed=2)
+    b = rhs(A.n * A.bs, 3)
+    assert property_failures(A, b, zo.Precond(A, zo.PRE_ILU0)) != []
 
 
 def test_ilu0_defining_property_and_tridiagonal_exactness():
