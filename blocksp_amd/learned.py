@@ -27,17 +27,18 @@ def record_krylov(ctx: Context, A: BSR, M, b, count: int, restart: int = 30, k: 
     return buf, K
 
 
-def training_set(ctx: Context, problems, count: int = 250, colour_fn=None):
+def training_set(ctx: Context, problems, count: int = 250, colour_fn=None, krylov_colour=False):
     """(A, M, graph, Z, K) for each problem: M is the ILU0 whose factor the network corrects
     (elimination order from colour_fn(problem), natural when None); Z holds the recorded
-    Krylov vectors of an FP64 natural-order ILU0-pGMRES solve."""
+    Krylov vectors of an FP64 ILU0-pGMRES solve -- with the natural-order ILU0, or with M itself
+    when krylov_colour (the directions the solver with the coloured factor explores)."""
     out = []
     for p in problems:
         A = BSR.from_problem(ctx, p)
-        M0 = ILU0(ctx, A)
-        Z, K = record_krylov(ctx, A, M0, p.b, count)
         col = colour_fn(p) if colour_fn else None
+        M0 = ILU0(ctx, A) if col is None or not krylov_colour else None
         M = M0 if col is None else ILU0Ex(ctx, A, colour=col, precision=PREC_F64)
+        Z, K = record_krylov(ctx, A, M if krylov_colour else M0, p.b, count)
         g = MPNNGraph(ctx, A, M)
         out.append((A, M, g, Z, K))
     return out

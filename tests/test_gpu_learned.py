@@ -148,6 +148,38 @@ def test_learned_correction_cuts_iterations(ctx):
     assert sum(learned_its) <= 0.9 * sum(plain_its), (plain_its, learned_its)
 
 
+def test_learned_correction_on_the_coloured_factor(ctx):
+    """Ablation on the parity-coloured factor (the mixed solver's elimination order): an untrained
+    network reproduces the coloured ILU0 bit for bit, and a network trained on Krylov vectors of
+    coloured ILU0-pGMRES solves converges without the fallback in at most the plain coloured
+    FGMRES's iterations + 5% -- its gain on this factor is a few percent: the fill that colouring
+    drops couples rows of one colour, which a block-diagonal correction cannot represent."""
+    p = _grf(12, 100)
+    A = bsp.BSR.from_problem(ctx, p)
+    col = _colour(p)
+    Mc = bsp.ILU0Ex(ctx, A, colour=col)
+    untrained = bsp.LearnedILU0(ctx, A, bsp.MPNN(ctx, 8, seed=0), colour=col)
+    np.testing.assert_array_equal(untrained.factor()[2], Mc.factor()[2])
+    data = learned.training_set(ctx, [_grf(8, s) for s in range(8)], count=64, colour_fn=_colour, krylov_colour=True)
+    net, losses = learned.train(ctx, data, epochs=20, batch=16, lr=1e-3)
+    assert losses[-1] <= losses[0], losses
+    plain_its, learned_its = [], []
+    for s in (100, 101, 102):
+        p = _grf(12, s)
+        A = bsp.BSR.from_problem(ctx, p)
+        b = ctx.to_device(p.b)
+        Mc = bsp.ILU0Ex(ctx, A, colour=col)
+        Ml = bsp.LearnedILU0(ctx, A, net, colour=col)
+        _, st0, rc0 = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mc, spmv_precision=bsp.PREC_F64)
+        _, st1, rc1 = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Ml, spmv_precision=bsp.PREC_F64, fallback=Mc)
+        assert rc0 == 0 and rc1 == 0 and st0.true_relres <= 1e-10 and st1.true_relres <= 1e-10
+        assert st1.fallback == 0
+        plain_its.append(st0.iters)
+        learned_its.append(st1.iters)
+    print("coloured ablation iterations plain", plain_its, "learned", learned_its, "loss", losses[0], "->", losses[-1])
+    assert sum(learned_its) <= 1.05 * sum(plain_its), (plain_its, learned_its)
+
+
 def test_nonfinite_preconditioner_falls_back(ctx):
     p = _grf(6, 9)
     A = bsp.BSR.from_problem(ctx, p)
