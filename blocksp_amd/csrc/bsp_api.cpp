@@ -130,6 +130,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_basis_f32 must be 0 or 1");
         ctx->fgmres_basis_f32 = (int)value;
 
+    } else if (k == "bf16_ell") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "bf16_ell must be 0 or 1");
+        ctx->bf16_ell = (int)value;
     } else if (k == "sweep_graphs") {
         ctx->sweep_graphs = value != 0;
     } else if (k == "sweep_ell") {
@@ -370,6 +373,7 @@ int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev) {
     // reduced-precision copies and the transpose are rebuilt from the new values on next use
     A->val32.release();
     A->val16.release();
+    A->ci16.release();
     A->valp.release();
     A->transposed.reset();
     return BSP_OK;

@@ -157,6 +157,9 @@ struct bsp_ctx {
                                      // matrix, 6.04 TB/s), 0 the values as stored (131.2 us)
     int fgmres_ortho = 0;            // mixed FGMRES orthogonalisation: 0 classical Gram-Schmidt with conditional
                                      // reorthogonalisation, 1 modified Gram-Schmidt (the reference's)
+    int bf16_ell = 1;                // BF16 MFMA SpMM: 1 reads an ELL copy of the BF16 values where rows hold at
+                                     // most 8 blocks (persistent kernel, 136-140 us on the 64^3 DG matrix with 32
+                                     // right-hand sides), 0 a copy in CSR order (178-180 us)
     int fgmres_basis_f32 = 0;        // mixed FGMRES Arnoldi basis storage: 0 FP64, 1 FP32 (FP64 arithmetic,
                                      // classical Gram-Schmidt only)
     // reduction scratch
@@ -201,6 +204,9 @@ struct bsp_bsr {
     mutable bsp::DevMem val32;         // FP32 copy of the values (mixed-precision solves)
     mutable bool val32_chunked = false; // val32 is in the chunk-major order of k_bsmv_lp_perm
     mutable bsp::DevMem val16;         // BF16 copy of the values (MFMA block SpMM)
+    mutable bsp::DevMem ci16;          // its column indices when val16 is an ELL copy
+    mutable int val16_w = -1;          // ELL width for val16 (0: no ELL copy, -1: not yet decided)
+    mutable bool val16_ell = false;
     mutable bsp::DevMem valp;          // chunk-major copy of the values (bsmv variant 3)
 };
 

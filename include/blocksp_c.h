@@ -69,7 +69,9 @@ int bsp_ctx_sync(bsp_ctx* ctx);
  * 0: reads it from global memory), "mgs_onewg" (1: the modified Gram-Schmidt of an Arnoldi step
  * on at most 8192 entries and 64 basis vectors runs in one single-workgroup kernel / 0: one
  * launch per step; bitwise the same),
- * "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards).
+ * "batch_ell" (1 / 0: ELL or CSR sweeps for batches created afterwards), "bf16_ell" (1: the
+ * BF16 MFMA SpMM reads an ELL copy of the BF16 values when no row holds more than 8 blocks /
+ * 0: a copy in CSR order; the same results).
  * Unknown names and out-of-range values return BSP_E_INVALID. */
 int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value);
 

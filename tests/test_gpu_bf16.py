@@ -19,11 +19,16 @@ def bf16_value(bits):
     return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def _random_bsr(n, seed):
+def _random_bsr(n, seed, even=False):
+    """Random block pattern; row 5 empty.  even=False: 1-7 blocks per row and row 7 with 11 (more
+    than one round of slots); even=True: 5 or 6 blocks per row (rows padded to an ELL copy)."""
     rng = np.random.default_rng(seed)
     rows = []
     for i in range(n):
-        k = 0 if i == 5 else (11 if i == 7 else rng.integers(1, 8))   # row 5 empty, row 7 longer than 8
+        if even:
+            k = 0 if i == 5 else rng.integers(5, 7)
+        else:
+            k = 0 if i == 5 else (11 if i == 7 else rng.integers(1, 8))
         rows.append(np.sort(rng.choice(n, size=k, replace=False)))
     rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
     ci = np.concatenate(rows).astype(np.int32)
@@ -40,13 +45,22 @@ def _ref(rp, ci, Ab, Xb, n, nrhs):
     return Y
 
 
+@pytest.fixture(params=[1, 0])
+def kernel(ctx, request):
+    """bf16_ell = 1: ELL copy and persistent kernel where rows allow it; 0: CSR copy."""
+    ctx.set_option("bf16_ell", request.param)
+    yield request.param
+    ctx.set_option("bf16_ell", 1)
+
+
+@pytest.mark.parametrize("even", [False, True])
 @pytest.mark.parametrize("n,nrhs", [(37, 32), (36, 16), (11, 48)])
-def test_bf16_spmm_matches_numpy(ctx, n, nrhs):
-    _check_spmm(ctx, n, nrhs)
+def test_bf16_spmm_matches_numpy(ctx, kernel, n, nrhs, even):
+    _check_spmm(ctx, n, nrhs, even)
 
 
-def _check_spmm(ctx, n, nrhs):
-    rp, ci, vals = _random_bsr(n, seed=n + nrhs)
+def _check_spmm(ctx, n, nrhs, even=False):
+    rp, ci, vals = _random_bsr(n, seed=n + nrhs, even=even)
     A = bsp.BSR(ctx, n, n, 8, rp, ci, vals)
     X = np.random.default_rng(1).standard_normal((n * 8, nrhs))     # RHS-minor
     Xd = ctx.to_device(X)
@@ -68,7 +82,7 @@ def _check_spmm(ctx, n, nrhs):
     assert np.all(Y.numpy()[5 * 8:6 * 8] == 0.0)
 
 
-def test_bf16_spmm_close_to_fp64_on_dg_matrix(ctx):
+def test_bf16_spmm_close_to_fp64_on_dg_matrix(ctx, kernel):
     p = pr.dg3d_lap(6, 6, 6, seed=4)
     A = bsp.BSR.from_problem(ctx, p)
     nrhs = 32
@@ -94,3 +108,29 @@ def test_bf16_spmm_rejects_bad_shapes(ctx):
     Yb = ctx.empty((p.n * 16 * 2,), np.float32)
     with pytest.raises(bsp.BspError):
         bsp.bsmm_rhs_bf16(ctx, A, 16, Xb, Yb)     # 2 x 2 blocks
+
+
+def test_bf16_copy_follows_option_and_new_values(ctx):
+    """The BF16 copy of the values is rebuilt when the bf16_ell option changes and after new
+    values; both copies give the same products (same FP32 sums)."""
+    rp, ci, vals = _random_bsr(40, seed=7, even=True)
+    n, nrhs = 40, 32
+    A = bsp.BSR(ctx, n, n, 8, rp, ci, vals)
+    X = np.random.default_rng(4).standard_normal((n * 8, nrhs))
+    Xb = ctx.empty((n * nrhs * 8,), np.uint16)
+    bsp.rhs_pack_bf16(ctx, n, 8, nrhs, ctx.to_device(X), Xb)
+    Yb = ctx.empty((n * nrhs * 8,), np.float32)
+    outs = []
+    try:
+        for ell in (1, 0, 1):
+            ctx.set_option("bf16_ell", ell)
+            bsp.bsmm_rhs_bf16(ctx, A, nrhs, Xb, Yb)
+            outs.append(Yb.numpy().copy())
+        np.testing.assert_array_equal(outs[0], outs[1])
+        np.testing.assert_array_equal(outs[0], outs[2])
+        v2 = ctx.to_device(np.ascontiguousarray(2.0 * vals))
+        ctx.check(ctx.L.bsp_bsr_set_values_dev(ctx.h, A.h, v2.ptr), "set_values")
+        bsp.bsmm_rhs_bf16(ctx, A, nrhs, Xb, Yb)
+        np.testing.assert_array_equal(Yb.numpy(), 2.0 * outs[0])
+    finally:
+        ctx.set_option("bf16_ell", 1)

@@ -53,7 +53,9 @@ def _bf16_value(bits):
     return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def test_dg3d_bf16_spmm_32rhs_matches_numpy_bf16(ctx):
+@pytest.mark.parametrize("ell", [1, 0])
+def test_dg3d_bf16_spmm_32rhs_matches_numpy_bf16(ctx, ell):
+    ctx.set_option("bf16_ell", ell)
     p = pr.dg3d_lap(8, 8, 8, seed=4)
     n, nrhs = p.n, 32
     A = bsp.BSR.from_problem(ctx, p)
@@ -75,6 +77,7 @@ def test_dg3d_bf16_spmm_32rhs_matches_numpy_bf16(ctx):
     np.add.at(ref, rows, np.einsum("prc,pck->prk", Ab, Xv[p.colidx]))
     np.add.at(bound, rows, np.einsum("prc,pck->prk", np.abs(Ab), np.abs(Xv[p.colidx])))
     got = Y.numpy().reshape(n, 8, nrhs)
+    ctx.set_option("bf16_ell", 1)
     # FP32 accumulation of 7 blocks x 8 products of exactly representable BF16 operands
     assert np.all(np.abs(got - ref) <= 8e-6 * bound + 1e-30)
 
