@@ -563,7 +563,7 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and args.nx == 512:
         traffic = json.load(open(TRAFFIC_FILE)).get("traffic_bytes_per_launch")
-    roofline = {"kernel": "k_bsmv_perm<8> (chunk-major values, XCD-aware rows, streaming loads)", "bound": "hbm", "achieved": round(bsmv_gbps, 1), "peak": HBM_PEAK_GBPS,
+    roofline = {"kernel": "k_bsmv_ell<8> (ELL copy of chunk-major values, XCD-aware rows, streaming loads)", "bound": "hbm", "achieved": round(bsmv_gbps, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(bsmv_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "algorithmic_bytes": bsmv_bytes, "avg_us": round(bsmv_ms * 1e3, 2)}
     bsmm = bench_bsmm(bsp, ctx, A, p)

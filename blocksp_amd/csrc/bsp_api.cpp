@@ -17,8 +17,35 @@ using namespace bsp;
 
 namespace bsp {
 
+int ell_width(const bsp_bsr* A) {
+    if (A->ell_w < 0) {
+        int W = 0;
+        for (int i = 0; i < A->n_rows; ++i) W = std::max(W, A->h_rp[i + 1] - A->h_rp[i]);
+        A->ell_w = W >= 1 && W <= 8 && (int64_t)W * A->n_rows <= A->nnzb + A->nnzb / 4 ? W : 0;
+    }
+    return A->ell_w;
+}
+
+int ensure_ell_columns(bsp_ctx* ctx, const bsp_bsr* A) {
+    if (A->ell_ci.p || ell_width(A) == 0) return BSP_OK;
+    BSP_HIP(A->ell_ci.alloc(sizeof(int) * std::max<int64_t>((int64_t)A->n_rows * A->ell_w, 1)));
+    return launch_ell_columns(ctx, A->n_rows, A->ell_w, A->rp.as<int>(), A->ci.as<int>(), A->ell_ci.as<int>());
+}
+
 int bsmv_matrix(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int accumulate) {
-    if (ctx->bsmv_layout == 1 && bsmv_perm_supported(A->bs)) {
+    if (ctx->bsmv_layout == 2 && bsmv_perm_supported(A->bs) && ell_width(A) > 0) {
+        int rc = ensure_ell_columns(ctx, A);
+        if (rc) return rc;
+        if (!A->valpe.p) {
+            BSP_HIP(A->valpe.alloc(sizeof(double) * A->bs * A->bs * std::max<int64_t>((int64_t)A->n_rows * A->ell_w, 1)));
+            rc = launch_ell_values_f64(ctx, A->n_rows, A->bs, A->ell_w, A->rp.as<int>(), A->val.as<double>(),
+                                       A->valpe.as<double>());
+            if (rc) return rc;
+        }
+        return launch_bsmv_ell(ctx, A->n_rows, A->bs, A->ell_w, A->ell_ci.as<int>(), A->valpe.as<double>(), x, y,
+                               accumulate);
+    }
+    if (ctx->bsmv_layout >= 1 && bsmv_perm_supported(A->bs)) {
         if (!A->valp.p) {
             BSP_HIP(A->valp.alloc(sizeof(double) * std::max<int64_t>(A->nnzb * A->bs * A->bs, 1)));
             int rc = launch_permute_blocks(ctx, A->nnzb, A->bs, A->val.as<double>(), A->valp.as<double>());
@@ -121,7 +148,7 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
         BSP_CHECK(value >= 1 && value <= 64, BSP_E_INVALID, "sweep_wg_per_cu must be in [1, 64]");
         ctx->sweep_wg_per_cu = (int)value;
     } else if (k == "bsmv_layout") {
-        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "bsmv_layout must be 0 or 1");
+        BSP_CHECK(value >= 0 && value <= 2, BSP_E_INVALID, "bsmv_layout must be 0, 1 or 2");
         ctx->bsmv_layout = (int)value;
     } else if (k == "fgmres_ortho") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_ortho must be 0 (CGS2) or 1 (MGS)");
@@ -375,6 +402,7 @@ int bsp_bsr_set_values_dev(bsp_ctx* ctx, bsp_bsr* A, const double* vals_dev) {
     A->val16.release();
     A->ci16.release();
     A->valp.release();
+    A->valpe.release();
     A->transposed.reset();
     return BSP_OK;
 }

@@ -22,12 +22,23 @@
 
 namespace bsp {
 
-// FP32 copy of A's values for the inner SpMV, in the chunk-major order of the chunked kernel
-// when the block size allows it.
+// FP32 copy of A's values for the inner SpMV: in the ELL order of A's ELL structure (chunk-major
+// blocks) when the block size allows chunks and the context's bsmv_layout is 2, else in the
+// chunk-major order of the chunked kernel when the block size allows it.
 int ensure_values_f32(bsp_ctx* ctx, const bsp_bsr* A) {
     if (A->val32.p) return BSP_OK;
+    if (ctx->bsmv_layout == 2 && bsmv_f32_chunked_supported(A->bs) && ell_width(A) > 0) {
+        int rc = ensure_ell_columns(ctx, A);
+        if (rc) return rc;
+        BSP_HIP(A->val32.alloc(sizeof(float) * A->bs * A->bs * std::max<int64_t>((int64_t)A->n_rows * A->ell_w, 1)));
+        A->val32_ell = true;
+        A->val32_chunked = false;
+        return launch_ell_values_f32(ctx, A->n_rows, A->bs, A->ell_w, A->rp.as<int>(), A->val.as<double>(),
+                                     A->val32.as<float>());
+    }
     const int64_t cnt = A->nnzb * (int64_t)A->bs * A->bs;
     BSP_HIP(A->val32.alloc(sizeof(float) * std::max<int64_t>(cnt, 1)));
+    A->val32_ell = false;
     A->val32_chunked = bsmv_f32_chunked_supported(A->bs);
     if (A->val32_chunked) return launch_d2f_chunked(ctx, A->nnzb, A->bs, A->val.as<double>(), A->val32.as<float>());
     return launch_d2f(ctx, cnt, A->val.as<double>(), A->val32.as<float>());
@@ -214,7 +225,9 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
         if constexpr (z64) {
             r2 = bsmv_matrix(ctx, A, z, w, 0);
         } else if (spmv_f32) {
-            if (A->val32_chunked)
+            if (A->val32_ell)
+                r2 = launch_bsmv_f32_ell(ctx, n, bs, A->ell_w, A->ell_ci.as<int>(), A->val32.as<float>(), z, w);
+            else if (A->val32_chunked)
                 r2 = launch_bsmv_f32_chunked(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);
             else
                 r2 = launch_bsmv_f32(ctx, n, bs, A->rp.as<int>(), A->ci.as<int>(), A->val32.as<float>(), z, w);

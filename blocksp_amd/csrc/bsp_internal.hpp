@@ -152,9 +152,11 @@ struct bsp_ctx {
                                      // host loop (the method of the reference, one row at a time)
     int batch_ell = 1;               // batched pCG: slot-ordered ELL copies of the factor sweeps
                                      // (used when no row holds more than 4 strict blocks)
-    int bsmv_layout = 1;             // solver bsmv for even bs <= 8: 1 a chunk-major copy of the values read by
-                                     // XCD-aware row ranges with streaming loads (117.6 us on the 512^2 DG
-                                     // matrix, 6.04 TB/s), 0 the values as stored (131.2 us)
+    int bsmv_layout = 2;             // solver bsmv for even bs <= 8 (and the mixed solver's FP32 product for bs 4,
+                                     // 8, 16): 2 an ELL copy of the chunk-major values where rows hold at most 8
+                                     // blocks (111-112 us on the 512^2 DG matrix, 6.35 TB/s; else as 1), 1 a
+                                     // chunk-major copy read by XCD-aware row ranges with streaming loads (117 us),
+                                     // 0 the values as stored (131 us)
     int fgmres_ortho = 0;            // mixed FGMRES orthogonalisation: 0 classical Gram-Schmidt with conditional
                                      // reorthogonalisation, 1 modified Gram-Schmidt (the reference's)
     int bf16_ell = 1;                // BF16 MFMA SpMM: 1 reads an ELL copy of the BF16 values where rows hold at
@@ -208,6 +210,13 @@ struct bsp_bsr {
     mutable int val16_w = -1;          // ELL width for val16 (0: no ELL copy, -1: not yet decided)
     mutable bool val16_ell = false;
     mutable bsp::DevMem valp;          // chunk-major copy of the values (bsmv variant 3)
+    // ELL structure shared by the ELL value copies of the products: W = ell_w slots per row (the
+    // widest row, at most 8, padding at most a quarter of the blocks; 0: none; -1: not yet decided),
+    // ell_ci[row * W + s] the row's s-th block column or -1
+    mutable int ell_w = -1;
+    mutable bsp::DevMem ell_ci;
+    mutable bsp::DevMem valpe;         // FP64 values in that ELL order, chunk-major blocks (k_bsmv_ell)
+    mutable bool val32_ell = false;    // val32 is in that ELL order (chunk-major FP32 blocks)
 };
 
 struct bsp_precond {
@@ -253,6 +262,16 @@ int bsmv_matrix(bsp_ctx* ctx, const bsp_bsr* A, const double* x, double* y, int 
 // chunk-major copy of the values for k_bsmv_perm (even block sizes <= 8)
 bool bsmv_perm_supported(int bs);
 int launch_permute_blocks(const bsp_ctx* ctx, int64_t nnzb, int bs, const double* val, double* valp);
+int launch_bsmv_ell(const bsp_ctx* ctx, int n, int bs, int W, const int* cie, const double* vale, const double* x,
+                    double* y, int accumulate);
+int launch_ell_columns(const bsp_ctx* ctx, int n, int W, const int* rp, const int* ci, int* cie);
+int launch_ell_values_f64(const bsp_ctx* ctx, int n, int bs, int W, const int* rp, const double* val, double* vale);
+int launch_ell_values_f32(const bsp_ctx* ctx, int n, int bs, int W, const int* rp, const double* val, float* vale);
+// ELL width of A (computed once from the host structure) and its device column array
+int ell_width(const bsp_bsr* A);
+int ensure_ell_columns(bsp_ctx* ctx, const bsp_bsr* A);
+int launch_bsmv_f32_ell(const bsp_ctx* ctx, int n, int bs, int W, const int* cie, const float* vale, const float* x,
+                        double* y);
 int launch_bsmv_perm(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* valp,
                      const double* x, double* y, int accumulate);
 

@@ -150,6 +150,89 @@ __global__ __launch_bounds__(kBlock) void k_bsmv_perm(int n, const int* __restri
     y[(size_t)i * BS + r] = acc;
 }
 
+// The same product over an ELL copy of the chunk-major values (W <= 8 slots per row, column -1 and
+// zero blocks on padding): a row's column indices load from its row number, all at once, so the
+// blocks and the x entries wait on one load instead of the row pointers and then the columns.
+// Real blocks are added in the row's stored order and padding is skipped: the sums are bitwise
+// those of k_bsmv_perm.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bsmv_ell(int n, int W, const int* __restrict__ cie,
+                                                     const double* __restrict__ vale, const double* __restrict__ x,
+                                                     double* __restrict__ y, int accumulate, int nwg) {
+    static_assert(BS % 2 == 0, "chunk-major layout needs an even block size");
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int per = (nwg + 7) / 8;
+    const int wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (wg >= nwg) return;
+    const int wave = (wg * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    double acc = accumulate ? y[(size_t)i * BS + r] : 0.0;
+    int col[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) col[s] = s < W ? cie[(size_t)i * W + s] : -1;
+    auto load_row = [&](int q, double (&a)[BS]) {
+        const double2* s2 = reinterpret_cast<const double2*>(vale + (size_t)q * BS * BS) + r;
+#pragma unroll
+        for (int k = 0; k < BS / 2; ++k) {
+            const d2v_p u = __builtin_nontemporal_load(reinterpret_cast<const d2v_p*>(s2 + k * BS));
+            a[2 * k] = u.x;
+            a[2 * k + 1] = u.y;
+        }
+    };
+    const size_t q0 = (size_t)i * W;
+#pragma unroll
+    for (int s = 0; s < 8; s += 2) {
+        if (col[s] < 0) break;   // padding is at the end of a row
+        double a0[BS], x0[BS];
+        load_row(q0 + s, a0);
+        load_vec<BS>(x + (size_t)col[s] * BS, x0);
+        if (col[s + 1] >= 0) {
+            double a1[BS], x1[BS];
+            load_row(q0 + s + 1, a1);
+            load_vec<BS>(x + (size_t)col[s + 1] * BS, x1);
+            acc += row_dot<BS>(a0, x0);
+            acc += row_dot<BS>(a1, x1);
+        } else {
+            acc += row_dot<BS>(a0, x0);
+            break;
+        }
+    }
+    y[(size_t)i * BS + r] = acc;
+}
+
+// Column indices of the ELL structure: slot (row, s) holds the row's s-th block column, -1 past
+// the row's end.
+__global__ void k_ell_columns(int64_t slots, int W, const int* __restrict__ rp, const int* __restrict__ ci,
+                              int* __restrict__ cie) {
+    const int64_t qs = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (qs >= slots) return;
+    const int64_t row = qs / W, s = qs - row * W;
+    const int p0 = rp[row], len = rp[row + 1] - p0;
+    cie[qs] = s < len ? ci[p0 + s] : -1;
+}
+
+// ELL copy of the values in the chunk-major order of k_bsmv_perm (FP64, chunks of 2) or of
+// k_bsmv_lp_perm (FP32, chunks of 4): thread t writes entry t of the copy; zero on padding.
+template <typename T, int CH>
+__global__ void k_ell_values(int64_t total, int bs, int W, const int* __restrict__ rp, const double* __restrict__ val,
+                             T* __restrict__ vale) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int b2 = bs * bs;
+    const int64_t qs = t / b2;
+    const int e = (int)(t - qs * b2);
+    const int64_t row = qs / W, s = qs - row * W;
+    const int p0 = rp[row], len = rp[row + 1] - p0;
+    const int chunk = e / (CH * bs), rem = e - chunk * CH * bs;
+    const int r = rem / CH, c = chunk * CH + rem % CH;
+    vale[t] = s < len ? (T)val[(size_t)(p0 + s) * b2 + r * bs + c] : (T)0;
+}
+
 __global__ void k_permute_blocks(int64_t total, int bs, const double* __restrict__ val, double* __restrict__ valp) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -876,6 +959,56 @@ void launch_perm_bs(hipStream_t st, int n, const int* rp, const int* ci, const d
     const int nwg = (int)grid_for(waves * kWave);
     const int grid8 = (nwg + 7) / 8 * 8;
     hipLaunchKernelGGL(k_bsmv_perm<BS>, dim3(grid8), dim3(kBlock), 0, st, n, rp, ci, valp, x, y, accumulate, nwg);
+}
+
+template <int BS>
+void launch_ell_bs(hipStream_t st, int n, int W, const int* cie, const double* vale, const double* x, double* y,
+                   int accumulate) {
+    constexpr int rpw = kWave / BS;
+    const int64_t waves = (n + rpw - 1) / rpw;
+    const int nwg = (int)grid_for(waves * kWave);
+    const int grid8 = (nwg + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_bsmv_ell<BS>, dim3(grid8), dim3(kBlock), 0, st, n, W, cie, vale, x, y, accumulate, nwg);
+}
+
+int launch_bsmv_ell(const bsp_ctx* ctx, int n, int bs, int W, const int* cie, const double* vale, const double* x,
+                    double* y, int accumulate) {
+    if (n <= 0) return BSP_OK;
+    switch (bs) {
+        case 2: launch_ell_bs<2>(ctx->stream, n, W, cie, vale, x, y, accumulate); break;
+        case 4: launch_ell_bs<4>(ctx->stream, n, W, cie, vale, x, y, accumulate); break;
+        case 6: launch_ell_bs<6>(ctx->stream, n, W, cie, vale, x, y, accumulate); break;
+        case 8: launch_ell_bs<8>(ctx->stream, n, W, cie, vale, x, y, accumulate); break;
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "ELL bsmv needs block size 2, 4, 6 or 8");
+    }
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_ell_columns(const bsp_ctx* ctx, int n, int W, const int* rp, const int* ci, int* cie) {
+    const int64_t slots = (int64_t)n * W;
+    if (slots <= 0) return BSP_OK;
+    hipLaunchKernelGGL(k_ell_columns, dim3(grid_for(slots)), dim3(kBlock), 0, ctx->stream, slots, W, rp, ci, cie);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_ell_values_f64(const bsp_ctx* ctx, int n, int bs, int W, const int* rp, const double* val, double* vale) {
+    const int64_t total = (int64_t)n * W * bs * bs;
+    if (total <= 0) return BSP_OK;
+    hipLaunchKernelGGL((k_ell_values<double, 2>), dim3(grid_for(total)), dim3(kBlock), 0, ctx->stream, total, bs, W,
+                       rp, val, vale);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
+}
+
+int launch_ell_values_f32(const bsp_ctx* ctx, int n, int bs, int W, const int* rp, const double* val, float* vale) {
+    const int64_t total = (int64_t)n * W * bs * bs;
+    if (total <= 0) return BSP_OK;
+    hipLaunchKernelGGL((k_ell_values<float, 4>), dim3(grid_for(total)), dim3(kBlock), 0, ctx->stream, total, bs, W,
+                       rp, val, vale);
+    BSP_LAUNCH_CHECK();
+    return BSP_OK;
 }
 
 int launch_bsmv_perm(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const double* valp,

@@ -155,6 +155,71 @@ __global__ __launch_bounds__(kBlock) void k_bsmv_lp_perm(int n, const int* __res
     y[(size_t)i * BS + r] = (double)acc;
 }
 
+// The same product over an ELL copy of the chunk-major FP32 values (W <= 8 slots per row, column
+// -1 and zero blocks on padding): the column indices load from the row number, all at once, so
+// the blocks and x entries wait on one load.  Real blocks in stored order, padding skipped: the
+// results are the bits of k_bsmv_lp_perm.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_bsmv_lp_ell(int n, int W, const int* __restrict__ cie,
+                                                        const float* __restrict__ vale, const float* __restrict__ x,
+                                                        double* __restrict__ y, int nwg) {
+    static_assert(BS % 4 == 0, "chunk-major FP32 layout needs a block size divisible by 4");
+    constexpr int RPW = kWave / BS;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int per = (nwg + 7) / 8;
+    const int wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (wg >= nwg) return;
+    const int wave = (wg * kBlock + threadIdx.x) / kWave;
+    const int g = lane / BS;
+    const int r = lane - g * BS;
+    if (g >= RPW) return;
+    const int i = wave * RPW + g;
+    if (i >= n) return;
+    int col[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) col[s] = s < W ? cie[(size_t)i * W + s] : -1;
+    auto load_row = [&](size_t q, float (&a)[BS]) {
+        const f4v* s4 = reinterpret_cast<const f4v*>(vale + q * BS * BS) + r;
+#pragma unroll
+        for (int k = 0; k < BS / 4; ++k) {
+            const f4v t = __builtin_nontemporal_load(s4 + k * BS);
+            a[4 * k] = t.x;
+            a[4 * k + 1] = t.y;
+            a[4 * k + 2] = t.z;
+            a[4 * k + 3] = t.w;
+        }
+    };
+    float acc = 0.0f;
+    const size_t q0 = (size_t)i * W;
+#pragma unroll
+    for (int s = 0; s < 8; s += 2) {
+        if (col[s] < 0) break;   // padding is at the end of a row
+        float a0[BS], x0[BS];
+        load_row(q0 + s, a0);
+        load_vec<BS, float>(x + (size_t)col[s] * BS, x0);
+        if (col[s + 1] >= 0) {
+            float a1[BS], x1[BS];
+            load_row(q0 + s + 1, a1);
+            load_vec<BS, float>(x + (size_t)col[s + 1] * BS, x1);
+            float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                s0 = fmaf(a0[c], x0[c], s0);
+                s1 = fmaf(a1[c], x1[c], s1);
+            }
+            acc += s0;
+            acc += s1;
+        } else {
+            float s0 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) s0 = fmaf(a0[c], x0[c], s0);
+            acc += s0;
+            break;
+        }
+    }
+    y[(size_t)i * BS + r] = (double)acc;
+}
+
 // FP64 row-major blocks -> FP32 chunk-major blocks (the layout of k_bsmv_lp_perm)
 __global__ __launch_bounds__(kBlock) void k_d2f_chunked(int64_t total, int bs, const double* __restrict__ val,
                                                         float* __restrict__ valp) {
@@ -323,6 +388,27 @@ int launch_d2f_chunked(const bsp_ctx* ctx, int64_t nnzb, int bs, const double* v
     hipLaunchKernelGGL(k_d2f_chunked, dim3(grid_for(total)), dim3(kBlock), 0, ctx->stream, total, bs, val, valp);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+}
+
+int launch_bsmv_f32_ell(const bsp_ctx* ctx, int n, int bs, int W, const int* cie, const float* vale, const float* x,
+                        double* y) {
+    if (n <= 0) return BSP_OK;
+    auto launch = [&](auto bsc) {
+        constexpr int BS = decltype(bsc)::value;
+        constexpr int RPW = kWave / BS;
+        const int64_t waves = (n + RPW - 1) / RPW;
+        const int nwg = (int)grid_for(waves * kWave);
+        const int grid = (nwg + 7) / 8 * 8;
+        hipLaunchKernelGGL((k_bsmv_lp_ell<BS>), dim3(grid), dim3(kBlock), 0, ctx->stream, n, W, cie, vale, x, y, nwg);
+    };
+    switch (bs) {
+        case 4: launch(std::integral_constant<int, 4>{}); break;
+        case 8: launch(std::integral_constant<int, 8>{}); break;
+        case 16: launch(std::integral_constant<int, 16>{}); break;
+        default: return fail(ctx, BSP_E_UNSUPPORTED, "ELL FP32 bsmv: block size 4, 8 or 16");
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, std::string("bsmv_lp_ell launch: ") + hipGetErrorString(e));
 }
 
 int launch_bsmv_f32_chunked(const bsp_ctx* ctx, int n, int bs, const int* rp, const int* ci, const float* valp,

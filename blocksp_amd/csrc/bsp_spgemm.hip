@@ -620,6 +620,7 @@ int bsp_bsmm_numeric(bsp_ctx* ctx, const bsp_bsr* A, const bsp_bsr* B, bsp_bsr* 
     C->val32.release();
     C->val16.release();
     C->valp.release();
+    C->valpe.release();
     C->transposed.reset();
     return BSP_OK;
 }

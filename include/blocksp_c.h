@@ -58,8 +58,9 @@ int bsp_ctx_sync(bsp_ctx* ctx);
 /* Set a tuning option by name: "trsv" (0 level launches, default / 1 single-launch
  * sync-free sweeps), "sweep_graphs" (1: replay many-level sweeps from HIP graphs),
  * "sweep_wg_per_cu", "spin_limit" (polls before a sync-free wait gives up), "verbose",
- * "bsmv_layout" (1: solver products for even block sizes up to 8 read a chunk-major copy of the
- * values / 0: the values as stored), "fgmres_ortho" (0 classical / 1 modified
+ * "bsmv_layout" (2: solver products for even block sizes up to 8 read an ELL copy of chunk-major
+ * values when no row holds more than 8 blocks, else as 1 / 1: a chunk-major copy of the values /
+ * 0: the values as stored; the same results), "fgmres_ortho" (0 classical / 1 modified
  * Gram-Schmidt), "fgmres_basis_f32", "sweep_ell" (1: sweeps read their ELL copy where
  * built / 0: CSR), "sweep_onewg_max" (a sweep of more than 8 levels, each at most 1024 / bs
  * rows, runs in one workgroup when n * bs is at most this; default 4194304, 0: never),

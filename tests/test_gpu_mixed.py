@@ -273,3 +273,24 @@ def test_fgmres_mixed_rejects_bad_restart(ctx, restart, k):
     M = bsp.ILU0Ex(ctx, A, None, bsp.PREC_F32)
     with pytest.raises(bsp.BspError):
         bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, restart=restart, k=k)
+
+
+@pytest.mark.parametrize("prec", [bsp.PREC_F32, bsp.PREC_BF16])
+def test_fgmres_mixed_ell_spmv_gives_the_chunk_major_iterates(ctx, prec):
+    """The mixed solver's FP32 product over the ELL copy (bsmv_layout 2, the default) gives bitwise
+    the iterates of the product over the chunk-major copy (bsmv_layout 1)."""
+    p = pr.dg2d_veclap(24, 20, seed=2)
+    col = pr.parity_colours((24, 20))
+    res = {}
+    try:
+        for layout in (1, 2):
+            ctx.set_option("bsmv_layout", layout)
+            A = bsp.BSR.from_problem(ctx, p)      # a new matrix: its FP32 copy follows the option
+            M = bsp.ILU0Ex(ctx, A, col, prec)
+            xd, st, rc = bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, maxIters=5000)
+            assert rc == 0
+            res[layout] = (st.iters, xd.numpy())
+    finally:
+        ctx.set_option("bsmv_layout", 2)
+    assert res[1][0] == res[2][0]
+    np.testing.assert_array_equal(res[1][1], res[2][1])

@@ -319,11 +319,12 @@ def test_full_size_bsmv_properties(ctx):
     assert np.max(np.abs(y1.numpy() - p.bsmv(x1))) <= 1e-12 * np.max(np.abs(p.bsmv(np.abs(x1))))
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("bs", [2, 3, 4, 8])
 def test_bsmv_layouts(ctx, bs, layout):
-    """The solvers' bsmv on the values as stored (0) and on their chunk-major copy (1, the
-    default for even bs <= 8; bs 3 always reads the stored values)."""
+    """The solvers' bsmv on the values as stored (0), on their chunk-major copy (1) and on an
+    ELL copy of that (2, the default for even bs <= 8; these random rows pad too much for an
+    ELL copy, so 2 reads the chunk-major copy; bs 3 always reads the stored values)."""
     ctx.set_option("bsmv_layout", layout)
     try:
         n = 257
@@ -336,7 +337,48 @@ def test_bsmv_layouts(ctx, bs, layout):
         absax = po.bsmv_arrays(n, bs, rp, ci, np.abs(v), np.abs(x))
         check_bsmv(yd.numpy(), yref, absax)
     finally:
-        ctx.set_option("bsmv_layout", 1)   # the default
+        ctx.set_option("bsmv_layout", 2)   # the default
+
+
+def uniform_bsr(n, bs, seed):
+    """Rows of 4 or 5 blocks in shuffled insertion order, one empty row: an ELL copy of width 5
+    pads about a tenth of the blocks."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        if i == 3:
+            rows.append([])
+            continue
+        cols = list(rng.choice(n, size=int(rng.integers(4, 6)), replace=False))
+        rng.shuffle(cols)
+        rows.append(cols)
+    rowptr = np.zeros(n + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    colidx = np.array([c for r in rows for c in r], np.int32)
+    return rowptr, colidx, rng.standard_normal((colidx.shape[0], bs, bs))
+
+
+@pytest.mark.parametrize("bs", [2, 4, 6, 8])
+def test_bsmv_ell_copy_matches_oracle_and_chunk_major_bitwise(ctx, bs):
+    """bsmv over the ELL copy (layout 2): within the oracle's bound and bitwise the product over
+    the chunk-major copy (layout 1) -- the same blocks added in the same order."""
+    n = 301
+    rp, ci, v = uniform_bsr(n, bs, seed=60 + bs)
+    x = np.random.default_rng(5).standard_normal(n * bs)
+    out = {}
+    try:
+        for layout in (1, 2):
+            ctx.set_option("bsmv_layout", layout)
+            A = bsp.BSR(ctx, n, n, bs, rp, ci, v)
+            yd = ctx.zeros(n * bs)
+            bsp.bsmv(ctx, A, ctx.to_device(x), yd)
+            out[layout] = yd.numpy()
+    finally:
+        ctx.set_option("bsmv_layout", 2)
+    yref = po.bsmv_arrays(n, bs, rp, ci, v, x)
+    absax = po.bsmv_arrays(n, bs, rp, ci, np.abs(v), np.abs(x))
+    check_bsmv(out[2], yref, absax)
+    np.testing.assert_array_equal(out[2], out[1])
 
 
 @pytest.mark.parametrize("mode", [0, 1])
