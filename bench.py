@@ -518,6 +518,8 @@ def main():
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=0, help="run N iterations of each solver and exit")
+    ap.add_argument("--profile-mixed-only", action="store_true",
+                    help="with --profile-iters: only the mixed solver (for counter runs)")
     ap.add_argument("--skip-extras", action="store_true", help="skip the 3D BF16, batched and learned workloads")
     ap.add_argument("--only", choices=["bf16", "batch", "batch_solve", "learned", "learned_solve", "pcg", "c1", "bsmm"], default=None,
                     help="run one secondary workload alone (for profiling) and print its JSON")
@@ -587,6 +589,8 @@ def main():
         Mh = bsp.ILU0Ex(ctx, A, colour, bsp.PREC_BF16)
         _, st, _ = bsp.FGMRES_mixed(ctx, A, b, 1e-10, Mh, maxIters=args.profile_iters, restart=30, k=10)
         log(f"mixed: {st.iters} iterations in {st.solve_ms:.1f} ms")
+        if args.profile_mixed_only:
+            return
         M = bsp.ILU0(ctx, A)
         for mode in (0, 1):
             ctx.set_option("trsv", mode)
