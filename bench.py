@@ -166,6 +166,25 @@ def tight_reference(bsp, ctx, A, b, p, colour):
     return xref, f"all-FP64 coloured FGMRES to true relres {st.true_relres:.1e} (oracle bsmv: {rel:.1e})"
 
 
+def bench_bsmv_configs(bsp, pr, ctx):
+    """The FP64 bsmv on the other named matrices: the 64 x 64 block stencil (B=2), the 64^3 DG
+    Laplacian and the 48^3 random-field DG matrix (bs 8), each the median of 3 timings of 20
+    launches; algorithmic bytes as for the headline matrix."""
+    out = {}
+    for name, p in (("stencil2d_b2_64x64", pr.stencil2d_b2(64, 64, seed=0)),
+                    ("dg3d_lap_64cubed", pr.dg3d_lap(64, 64, 64, seed=4)),
+                    ("dg3d_grf_48cubed", pr.dg3d_grf(48, 48, 48, seed=3))):
+        A = bsp.BSR.from_problem(ctx, p)
+        ms = sorted(measure_bsmv(bsp, ctx, A, p, reps=20)[0] for _ in range(3))[1]
+        nbytes = bsmv_algorithmic_bytes(p.n, p.bs, p.nnzb)
+        out[name] = {"n_block_rows": p.n, "bs": p.bs, "nnzb": p.nnzb, "us": round(ms * 1e3, 2),
+                     "algorithmic_bytes": nbytes, "GBps": round(nbytes / ms / 1e6, 1),
+                     "frac_hbm": round(nbytes / ms / 1e6 / HBM_PEAK_GBPS, 4)}
+        log(f"bsmv {name}: {ms * 1e3:.1f} us, {nbytes / ms / 1e6:.0f} GB/s")
+        del A
+    return out
+
+
 def bench_bsmm(bsp, ctx, A, p, mfma=1, reps=5):
     """C = A A: both passes on the device; the numeric pass timed alone.  Algorithmic bytes:
     A's values read twice (as left and right factor), C's written once, and the three block
@@ -556,8 +575,18 @@ def main():
     colour = pr.parity_colours((args.nx, args.nx))
     log(f"generated {p.name}: n={p.n} bs={p.bs} nnzb={p.nnzb} in {time.perf_counter() - t0:.1f}s")
     ctx = bsp.Context(0)
+    # host handoff of the drop-in path: the matrix from host arrays (structure checks, host copy, upload)
+    # and the right-hand side up, then (after the solves) a solution vector down
+    ctx.sync()
+    t0 = time.perf_counter()
     A = bsp.BSR.from_problem(ctx, p)
+    ctx.sync()
+    t_bsr = time.perf_counter() - t0
+    t0 = time.perf_counter()
     b = ctx.to_device(p.b)
+    ctx.sync()
+    t_b = time.perf_counter() - t0
+    mat_bytes = p.nnzb * p.bs * p.bs * 8 + 4 * p.nnzb + 4 * (p.n + 1)
 
     bsmv_ms, bsmv_bytes = measure_bsmv(bsp, ctx, A, p)
     bsmv_gbps = bsmv_bytes / (bsmv_ms * 1e-3) / 1e9
@@ -687,9 +716,18 @@ def main():
     if not args.skip_extras:
         extras = {"dg3d_bf16_mixed": bench_bf16(bsp, pr, ctx), "batched_ichol0_pcg": bench_batch(bsp, pr, ctx),
                   "ichol0_pcg_single": bench_pcg(bsp, pr, ctx), "learned_ilu0": bench_learned(bsp, pr, ctx),
-                  "sor_pgmres_c1": bench_c1(bsp, pr, ctx)}
+                  "sor_pgmres_c1": bench_c1(bsp, pr, ctx), "bsmv_other_matrices": bench_bsmv_configs(bsp, pr, ctx)}
 
     value = ms_per_step / 1e3
+    t0 = time.perf_counter()
+    _x_host = _x_last[0].numpy()
+    t_x = time.perf_counter() - t0
+    host_handoff = {"bsr_create_s": round(t_bsr, 3), "matrix_bytes": mat_bytes,
+                    "bsr_create_GBps": round(mat_bytes / t_bsr / 1e9, 2), "rhs_upload_ms": round(t_b * 1e3, 2),
+                    "solution_download_ms": round(t_x * 1e3, 2),
+                    "mixed_with_handoff_s": round(value + setup_mixed + t_bsr + t_b + t_x, 4),
+                    "plain_with_handoff_s": (round(plain["time_s"] + plain["setup_s"] + t_bsr + t_b + t_x, 4)
+                                             if plain else None)}
     out = {
         "metric": METRIC,
         "value": round(value, 6),
@@ -723,6 +761,7 @@ def main():
         "speedup_vs_plain_fp64": round(plain["time_s"] / value, 3) if plain else None,
         "speedup_vs_plain_fp64_with_setup": (round((plain["time_s"] + plain["setup_s"]) / (value + setup_mixed), 3)
                                              if plain else None),
+        "host_handoff": host_handoff,
         "bsmv_GBps": round(bsmv_gbps, 1),
         "bsmm_AA": bsmm,
         "roofline": roofline,

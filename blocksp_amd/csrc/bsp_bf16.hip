@@ -89,9 +89,10 @@ __global__ __launch_bounds__(kBlock) void k_bsmm_bf16_k32(int n, int nrhs, int W
     const bool row_ok = row < n;
     const bool a_live = top == (m < 8);
     const int arow = m & 7, par = g & 1;
-    int base, len, len_pair;
+    int64_t base;
+    int len, len_pair;
     if (ELL) {
-        base = row * W;
+        base = (int64_t)row * W;
         len = row_ok ? W : 0;
         len_pair = W;
     } else {
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_bsmm_bf16_ell_persist(int n, int nrh
     }
     while (true) {
         const int row = 2 * pair + (top ? 0 : 1);
-        const int base = row * W;
+        const int64_t base = (int64_t)row * W;
         v8bf a[NP], b0[NP], b1[NP];
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
