@@ -19,8 +19,9 @@
 //     (3 workgroups of 4 wavefronts per CU; 142 us at 5, 156 us at 2).
 // Measured and dropped: chunks of rows taken in turn from slabs of an XCD's range (keeps the
 // z-neighbour right-hand sides in L2; 143-149 us), two pairs in flight per wavefront with the
-// next pair's operands loaded before the current pair's MFMAs (130 VGPRs, 157-188 us), and two
-// wavefronts per pair splitting the right-hand sides (161-177 us).
+// next pair's operands loaded before the current pair's MFMAs (130 VGPRs, 157-188 us), two
+// wavefronts per pair splitting the right-hand sides (161-177 us), and cached instead of streaming
+// matrix loads or result stores (141-153 us).
 #include "bsp_internal.hpp"
 
 #include <algorithm>
