@@ -100,8 +100,8 @@ def test_bsmv_dim(ctx, B, D):
         check_bsmv(yd.numpy(), yref, absax)
 
 
-@pytest.mark.parametrize("bs,k", [(8, 5), (8, 32), (8, 70), (1, 3), (3, 33), (4, 16), (16, 9), (32, 2),
-                                  (3, 20), (5, 12), (7, 30), (6, 10), (12, 4)])
+@pytest.mark.parametrize("bs,k", [(8, 5), (8, 32), (8, 70), (8, 16), (8, 48), (8, 96), (1, 3), (3, 33), (4, 16),
+                                  (16, 9), (32, 2), (3, 20), (5, 12), (7, 30), (6, 10), (12, 4)])
 def test_bsmm_rhs(ctx, bs, k):
     """Multi-RHS product on the RHS tile kernels -- columns split over lane groups (bs a
     multiple of the group count) or rows split (bs 3, 5, 7 and 6 at 16-wide tiles), tiles of
