@@ -157,6 +157,9 @@ int bsp_ctx_set_option(bsp_ctx* ctx, const char* name, int64_t value) {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "fgmres_basis_f32 must be 0 or 1");
         ctx->fgmres_basis_f32 = (int)value;
 
+    } else if (k == "cgs_pythagoras") {
+        BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "cgs_pythagoras must be 0 or 1");
+        ctx->cgs_pythagoras = (int)value;
     } else if (k == "bf16_ell") {
         BSP_CHECK(value == 0 || value == 1, BSP_E_INVALID, "bf16_ell must be 0 or 1");
         ctx->bf16_ell = (int)value;

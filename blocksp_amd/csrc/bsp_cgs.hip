@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 namespace bsp {
 
@@ -250,6 +251,105 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t len, double* __re
     if (threadIdx.x == 0) part_norm[blockIdx.x] = acc[0];
 }
 
+// Pass B of a first pass whose caller reads the H column on the host, with the normalisation folded
+// in: the new norm comes from the dots pass by Pythagoras, ||w - Q h||^2 = ||w||^2 - ||h||^2 (Q
+// orthonormal, h = Q^T w), which every workgroup computes from the same values in the same order;
+// each element is then updated, stored (the vector a second pass would start from) and scaled:
+// q = w / ||w||, z (optional) = FP32 copy of q.  Workgroup 0 writes h, the norm and the
+// reorthogonalisation flag to hcol, to the host copy hcol_host and to the scratch.  The formula
+// loses relative accuracy ~eps ||w_old||^2 / ||w_new||^2 <= 1e4 eps on steps without a second
+// pass (below that ratio the flag asks for the second pass, which recomputes the norm as a sum).
+template <typename QT>
+__global__ __launch_bounds__(kBlock) void k_cgs_update_normalize(int64_t len, double* __restrict__ w, QPtrs<QT> Q,
+                                                                 int m, Scratch sc, QT* __restrict__ q,
+                                                                 float* __restrict__ z, double* __restrict__ hcol,
+                                                                 double* hcol_host, int* __restrict__ flag_out) {
+    constexpr int N = QVec<QT>::N;
+    __shared__ const QT* qs[kMaxBasis];
+    __shared__ double hs[kMaxBasis];
+    __shared__ double s_norm;
+    for (int s = threadIdx.x; s < m; s += kBlock) {
+        qs[s] = Q.q[s];
+        hs[s] = sc.st[s];
+    }
+    if (threadIdx.x == 0) {
+        const double wn2 = sc.st[2 * m];
+        double hh = 0.0;
+        for (int s = 0; s < m; ++s) hh = fma(sc.st[s], sc.st[s], hh);
+        const double v = wn2 - hh;
+        const double nrm = sqrt(fabs(v));
+        s_norm = nrm;
+        if (blockIdx.x == 0) {
+            const bool reorth = v < kReorth2 * wn2;
+            *flag_out = reorth ? 1 : 0;
+            for (int s = 0; s < m; ++s) {
+                hcol[s] = sc.st[s];
+                hcol_host[s] = sc.st[s];
+            }
+            hcol[m] = nrm;
+            hcol[m + 1] = reorth ? 1.0 : 0.0;
+            hcol_host[m] = nrm;
+            hcol_host[m + 1] = reorth ? 1.0 : 0.0;
+            sc.st[2 * m + 1] = nrm;
+            __threadfence_system();
+        }
+    }
+    __syncthreads();
+    const double h = s_norm;
+    const int64_t nvec = len / N;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    auto out = [&](int64_t e0, const double* wv) {
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            const QT v = (QT)(wv[e] / h);
+            q[e0 + e] = v;
+            if (z) z[e0 + e] = (float)v;
+        }
+    };
+    // two element groups per step (each keeps the one-group order of its updates)
+    for (; i + stride < nvec; i += 2 * stride) {
+        double wa[N], wb[N];
+        load_w<N>(w, i, wa);
+        load_w<N>(w, i + stride, wb);
+#pragma unroll 4
+        for (int s = 0; s < m; ++s) {
+            double qa[N], qb[N];
+            load_q(qs[s], i, qa);
+            load_q(qs[s], i + stride, qb);
+#pragma unroll
+            for (int e = 0; e < N; ++e) wa[e] = fma(-hs[s], qa[e], wa[e]);
+#pragma unroll
+            for (int e = 0; e < N; ++e) wb[e] = fma(-hs[s], qb[e], wb[e]);
+        }
+        store_w<N>(w, i, wa);
+        store_w<N>(w, i + stride, wb);
+        out(i * N, wa);
+        out((i + stride) * N, wb);
+    }
+    if (i < nvec) {
+        double wi[N];
+        load_w<N>(w, i, wi);
+        for (int s = 0; s < m; ++s) {
+            double qi[N];
+            load_q(qs[s], i, qi);
+#pragma unroll
+            for (int e = 0; e < N; ++e) wi[e] = fma(-hs[s], qi[e], wi[e]);
+        }
+        store_w<N>(w, i, wi);
+        out(i * N, wi);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int64_t e = nvec * N; e < len; ++e) {
+            double wi = w[e];
+            for (int s = 0; s < m; ++s) wi = fma(-hs[s], (double)qs[s][e], wi);
+            w[e] = wi;
+            const QT v = (QT)(wi / h);
+            q[e] = v;
+            if (z) z[e] = (float)v;
+        }
+}
+
 // Fixed-order sums of the partials.  hcol has m + 2 entries.
 //   stage 0 (after A):  h1[s] = sum part[s];  wn2 = sum part[m]
 //   stage 1 (after B):  n2 = sum part_norm; reorth = (n2 < 1e-4 wn2); *flag_out = reorth;
@@ -373,6 +473,15 @@ int arnoldi_cgs(const bsp_ctx* ctx, int64_t len, double* w, const QT* const* Q, 
                            (const int*)nullptr);
         hipLaunchKernelGGL(k_cgs_reduce, dim3(m + 1), dim3(kBlock), 0, ctx->stream, 0, m, sc, nb, (const int*)nullptr,
                            flag, hcol);
+        // (an FP32-stored basis is orthonormal to ~1e-7 only: the Pythagorean norm would be off by
+        // that times ||w_old||^2 / ||w_new||^2, so that basis keeps the summed norm)
+        if (std::is_same_v<QT, double> && second == 0 && hcol_host && ctx->cgs_pythagoras) {
+            hipLaunchKernelGGL(k_cgs_update_normalize<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc,
+                               qnew, znext, hcol, hcol_host, flag);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return fail(ctx, BSP_E_HIP, std::string("cgs launch: ") + hipGetErrorString(e));
+            return BSP_OK;
+        }
         hipLaunchKernelGGL(k_cgs_update<QT>, dim3(nb), dim3(kBlock), 0, ctx->stream, len, w, qp, m, sc.st,
                            sc.part_norm, (const int*)nullptr);
         if (second == 0 && hcol_host) {

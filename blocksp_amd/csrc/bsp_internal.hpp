@@ -159,6 +159,9 @@ struct bsp_ctx {
                                      // 0 the values as stored (131 us)
     int fgmres_ortho = 0;            // mixed FGMRES orthogonalisation: 0 classical Gram-Schmidt with conditional
                                      // reorthogonalisation, 1 modified Gram-Schmidt (the reference's)
+    int cgs_pythagoras = 1;          // classical Gram-Schmidt (FP64 basis, first pass read by the host): 1 the new
+                                     // norm as ||w||^2 - ||h||^2, normalised inside the update pass (838-843 ms
+                                     // headline), 0 summed after the update in a separate kernel (876-899 ms)
     int bf16_ell = 1;                // BF16 MFMA SpMM: 1 reads an ELL copy of the BF16 values where rows hold at
                                      // most 8 blocks (persistent kernel, 136-140 us on the 64^3 DG matrix with 32
                                      // right-hand sides), 0 a copy in CSR order (178-180 us)

@@ -61,7 +61,9 @@ int bsp_ctx_sync(bsp_ctx* ctx);
  * "bsmv_layout" (2: solver products for even block sizes up to 8 read an ELL copy of chunk-major
  * values when no row holds more than 8 blocks, else as 1 / 1: a chunk-major copy of the values /
  * 0: the values as stored; the same results), "fgmres_ortho" (0 classical / 1 modified
- * Gram-Schmidt), "fgmres_basis_f32", "sweep_ell" (1: sweeps read their ELL copy where
+ * Gram-Schmidt), "cgs_pythagoras" (1: classical Gram-Schmidt of an FP64 basis takes the new norm
+ * as ||w||^2 - ||h||^2 and normalises in the update pass / 0: sums it after the update),
+ * "fgmres_basis_f32", "sweep_ell" (1: sweeps read their ELL copy where
  * built / 0: CSR), "sweep_onewg_max" (a sweep of more than 8 levels, each at most 1024 / bs
  * rows, runs in one workgroup when n * bs is at most this; default 4194304, 0: never),
  * "sweep_ahead" (1: such a sweep over an ELL copy, blocks up to 4 x 4, loads each level's

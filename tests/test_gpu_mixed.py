@@ -294,3 +294,28 @@ def test_fgmres_mixed_ell_spmv_gives_the_chunk_major_iterates(ctx, prec):
         ctx.set_option("bsmv_layout", 2)
     assert res[1][0] == res[2][0]
     np.testing.assert_array_equal(res[1][1], res[2][1])
+
+
+@pytest.mark.parametrize("pyth", [1, 0])
+def test_fgmres_mixed_pythagorean_norm(ctx, pyth):
+    """Classical Gram-Schmidt with the new norm taken as ||w||^2 - ||h||^2 and the normalisation in
+    the update pass (cgs_pythagoras = 1) or summed after the update (0): both reach the FP64 residual
+    and the oracle's solution, in iteration counts within a few percent of each other."""
+    dims = (40, 36)
+    p = pr.dg2d_veclap(*dims, seed=3)
+    xo, sto = po.pgmres(p, p.b, 1e-12, po.PRE_ILU0, maxIters=5000, restart=30, k=10)
+    its = {}
+    try:
+        for mode in (pyth, 1 - pyth):
+            ctx.set_option("cgs_pythagoras", mode)
+            A = bsp.BSR.from_problem(ctx, p)
+            M = bsp.ILU0Ex(ctx, A, pr.parity_colours(dims), bsp.PREC_BF16)
+            xd, st, rc = bsp.FGMRES_mixed(ctx, A, ctx.to_device(p.b), 1e-10, M, maxIters=5000, restart=30, k=10)
+            x = xd.numpy()
+            relres = np.linalg.norm(p.b - po.bsmv(p, x)) / np.linalg.norm(p.b)
+            assert rc == 0 and relres <= 1e-10, (mode, rc, relres, st.iters)
+            assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+            its[mode] = st.iters
+    finally:
+        ctx.set_option("cgs_pythagoras", 1)
+    assert abs(its[0] - its[1]) <= max(3, 0.05 * its[0]), its
