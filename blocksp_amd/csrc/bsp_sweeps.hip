@@ -376,6 +376,14 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg(SweepArgs a, const in
 // (triangular solves) or the right-hand side and the previous iterate's rows (SOR).  After the
 // barrier only the dependency columns' new values are read.  The arithmetic is sweep_chunk's,
 // in the same order, so results are bitwise the same.
+//
+// A level here is one wavefront's instruction stream between two barriers, and its time is set
+// by the number of instructions the wavefront issues (loading further ahead does not shorten
+// it), so the per-slot code is kept short: no branches (padding entries and missing rows read
+// row 0 and are masked by selects), KM entries per slot (2 when no level is wider), operand
+// addresses as 32-bit byte offsets from kernel-argument bases, and the diagonal block's row
+// values moved between the lanes of a block row by DPP where the block size divides a quad.
+// With the vector in LDS, its rows go to memory once, after the last level.
 struct EllIdx {
     bool have;         // the lane group holds a slot of the level
     int i;
@@ -385,7 +393,7 @@ struct EllIdx {
 
 template <int BS, typename VT>
 struct EllRow {
-    int i;
+    int i;                // the slot's row; -1: none (its operands are then row 0's, unused)
     int cq[kEllMax];
     VT av[kEllMax][BS];
     VT xv[kEllMax][BS];   // SOR: rows of the previous iterate (columns on the other side)
@@ -394,20 +402,30 @@ struct EllRow {
     VT xold;
 };
 
-// Lanes without a slot read the level's first slot (every level holds a row there) and are
-// masked by `have` when the operands are loaded, so the loads need no per-lane branch.
-template <bool SOR>
-__device__ __forceinline__ void ell_idx_load(const SweepArgs& a, int slot, int first, bool have, int K,
-                                             int64_t base, EllIdx& t) {
+// An element of `base` at a 32-bit byte offset (the host keeps every operand array of these
+// sweeps below 4 GiB): the loads take the base from scalar registers and need no 64-bit
+// address arithmetic.
+template <typename T>
+__device__ __forceinline__ const T* at32(const void* base, uint32_t byte_off) {
+    return reinterpret_cast<const T*>(static_cast<const char*>(base) + byte_off);
+}
+
+// The indices of a level of width K whose first slot is `first` and whose entries start at
+// `ebase`.  Lanes without a slot read the level's first slot (every level holds a row there)
+// and are masked by `have` when the operands are loaded.
+template <bool SOR, int KM>
+__device__ __forceinline__ void ell_idx_load(const SweepArgs& a, int slot, int first, bool have, int K, int ebase,
+                                             EllIdx& t) {
     const int sl = have ? slot : first;
     t.have = have;
-    t.i = a.order[sl];
+    t.i = *at32<int>(a.order, (uint32_t)sl * 4u);
+    const uint32_t e0 = (uint32_t)(ebase + (sl - first) * K);
 #pragma unroll
     for (int k = 0; k < kEllMax; ++k) {
-        if (k < K) {
-            const int64_t e = base + (int64_t)sl * K + k;
-            t.cq[k] = a.ecol[e];
-            t.pq[k] = SOR ? a.epos[e] : (int)e;
+        if (k < KM && k < K) {
+            const uint32_t e = e0 + (uint32_t)k;
+            t.cq[k] = *at32<int>(a.ecol, e * 4u);
+            t.pq[k] = SOR ? *at32<int>(a.epos, e * 4u) : (int)e;
         } else {
             t.cq[k] = -1;
             t.pq[k] = 0;
@@ -415,116 +433,168 @@ __device__ __forceinline__ void ell_idx_load(const SweepArgs& a, int slot, int f
     }
 }
 
-template <int BS, typename VT, typename MT, bool SOR>
+template <int BS, typename VT, typename MT, bool SOR, int KM>
 __device__ __forceinline__ void ell_row_load(const SweepArgs& a, const EllIdx& t, EllRow<BS, VT>& q) {
-    const int r = (threadIdx.x & (kWave - 1)) % BS;
-    const MT* __restrict__ val = static_cast<const MT*>(SOR ? a.val : a.eval);
-    const MT* __restrict__ dinv = static_cast<const MT*>(a.dinv);
-    const VT* xo = static_cast<const VT*>(a.xo);
-    q.i = t.have ? t.i : -1;
+    const uint32_t r = (uint32_t)((threadIdx.x & (kWave - 1)) % BS);
+    // The indices were loaded a level ago; the empty asm statements keep the compiler from moving
+    // the address arithmetic on them back to their loads (which would wait there for each of them).
+    EllIdx u = t;
+    asm volatile("" : "+v"(u.i));
+#pragma unroll
+    for (int k = 0; k < kEllMax; ++k) asm volatile("" : "+v"(u.cq[k]), "+v"(u.pq[k]));
+    q.i = u.have ? u.i : -1;
+    const uint32_t ir = (uint32_t)max(q.i, 0);
+    const void* val = SOR ? a.val : a.eval;
+    // without a previous iterate (zero) the right-hand side stands in for it; the products with
+    // it are replaced by zeros in ell_row_finish
+    const void* xo = a.xo ? a.xo : a.b;
 #pragma unroll
     for (int k = 0; k < kEllMax; ++k) {
-        q.cq[k] = t.have ? t.cq[k] : -1;
-        if (q.cq[k] == -1) continue;
-        load_mrow<BS, MT, VT>(val + ((size_t)t.pq[k] * BS + r) * BS, q.av[k]);
+        q.cq[k] = u.have ? u.cq[k] : -1;
+        if (k >= KM) continue;
+        const uint32_t p = q.cq[k] == -1 ? 0u : (uint32_t)u.pq[k];
+        load_mrow<BS, MT, VT>(at32<MT>(val, (p * BS + r) * BS * (uint32_t)sizeof(MT)), q.av[k]);
         if constexpr (SOR) {
-            if (q.cq[k] < 0) {
-                if (xo) {
-                    load_row<BS, VT>(xo + (size_t)(-2 - q.cq[k]) * BS, q.xv[k]);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < BS; ++c) q.xv[k][c] = VT(0);
-                }
-            }
+            const uint32_t c = q.cq[k] < -1 ? (uint32_t)(-2 - q.cq[k]) : 0u;
+            load_row<BS, VT>(at32<VT>(xo, c * BS * (uint32_t)sizeof(VT)), q.xv[k]);
         }
     }
-    q.s = VT(0);
-    q.xold = VT(0);
-    if (q.i >= 0) {
-        if (dinv) load_mrow<BS, MT, VT>(dinv + ((size_t)q.i * BS + r) * BS, q.d);
-        if constexpr (SOR) {
-            q.s = static_cast<const VT*>(a.b)[(size_t)q.i * BS + r];
-            q.xold = xo ? xo[(size_t)q.i * BS + r] : VT(0);
-        } else {
-            q.s = static_cast<const VT*>(a.x)[(size_t)q.i * BS + r];
-        }
+    if (a.dinv) {
+        load_mrow<BS, MT, VT>(at32<MT>(a.dinv, (ir * BS + r) * BS * (uint32_t)sizeof(MT)), q.d);
+    } else {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) q.d[c] = VT(0);
+    }
+    if constexpr (SOR) {
+        q.s = *at32<VT>(a.b, (ir * BS + r) * (uint32_t)sizeof(VT));
+        const VT xi = *at32<VT>(xo, (ir * BS + r) * (uint32_t)sizeof(VT));
+        q.xold = a.xo ? xi : VT(0);
+    } else {
+        q.s = *at32<VT>(a.x, (ir * BS + r) * (uint32_t)sizeof(VT));
+        q.xold = VT(0);
     }
 }
 
-template <int BS, typename VT, bool SOR, bool LX>
-__device__ __forceinline__ void ell_row_finish(const SweepArgs& a, EllRow<BS, VT>& q, bool have_dinv, VT* xs) {
+// v from lane (lane - lane % BS + c) of the wavefront, for a block size that divides a quad
+// (DPP quad permutation) or any other (bpermute).
+template <int BS>
+__device__ __forceinline__ double block_lane(double v, int c, int src_lane) {
+    if constexpr (BS == 1) {
+        return v;
+    } else if constexpr (BS == 2 || BS == 4) {
+        const int2 w = __builtin_bit_cast(int2, v);
+        int lo, hi;
+        // quad_perm: lane j of each quad reads lane (j - j % BS + c) of that quad
+#define BSP_QP(c0, c1, c2, c3) ((c0) | ((c1) << 2) | ((c2) << 4) | ((c3) << 6))
+        switch (BS * 4 + c) {
+            case 8: lo = __builtin_amdgcn_mov_dpp(w.x, BSP_QP(0, 0, 2, 2), 0xf, 0xf, false);
+                    hi = __builtin_amdgcn_mov_dpp(w.y, BSP_QP(0, 0, 2, 2), 0xf, 0xf, false); break;
+            case 9: lo = __builtin_amdgcn_mov_dpp(w.x, BSP_QP(1, 1, 3, 3), 0xf, 0xf, false);
+                    hi = __builtin_amdgcn_mov_dpp(w.y, BSP_QP(1, 1, 3, 3), 0xf, 0xf, false); break;
+            case 16: lo = __builtin_amdgcn_mov_dpp(w.x, BSP_QP(0, 0, 0, 0), 0xf, 0xf, false);
+                     hi = __builtin_amdgcn_mov_dpp(w.y, BSP_QP(0, 0, 0, 0), 0xf, 0xf, false); break;
+            case 17: lo = __builtin_amdgcn_mov_dpp(w.x, BSP_QP(1, 1, 1, 1), 0xf, 0xf, false);
+                     hi = __builtin_amdgcn_mov_dpp(w.y, BSP_QP(1, 1, 1, 1), 0xf, 0xf, false); break;
+            case 18: lo = __builtin_amdgcn_mov_dpp(w.x, BSP_QP(2, 2, 2, 2), 0xf, 0xf, false);
+                     hi = __builtin_amdgcn_mov_dpp(w.y, BSP_QP(2, 2, 2, 2), 0xf, 0xf, false); break;
+            default: lo = __builtin_amdgcn_mov_dpp(w.x, BSP_QP(3, 3, 3, 3), 0xf, 0xf, false);
+                     hi = __builtin_amdgcn_mov_dpp(w.y, BSP_QP(3, 3, 3, 3), 0xf, 0xf, false); break;
+        }
+#undef BSP_QP
+        return __builtin_bit_cast(double, make_int2(lo, hi));
+    } else {
+        return __shfl(v, src_lane, kWave);
+    }
+}
+
+template <int BS>
+__device__ __forceinline__ float block_lane(float v, int c, int src_lane) {
+    if constexpr (BS == 1) {
+        return v;
+    } else if constexpr (BS == 2 || BS == 4) {
+        const double d = block_lane<BS>(__builtin_bit_cast(double, make_float2(v, 0.f)), c, src_lane);
+        return __builtin_bit_cast(float2, d).x;
+    } else {
+        return __shfl(v, src_lane, kWave);
+    }
+}
+
+template <int BS, typename VT, bool SOR, bool LX, int KM>
+__device__ __forceinline__ void ell_row_finish(const SweepArgs& a, const EllRow<BS, VT>& q, bool have_dinv,
+                                               bool have_xo, VT* xs) {
     constexpr int RPW = kWave / BS;
     const int lane = threadIdx.x & (kWave - 1);
     const int g = lane / BS;
     const int r = lane - g * BS;
-    VT* xg = static_cast<VT*>(a.x);
-    VT* x = LX ? xs : xg;   // where the dependency columns' new values are read
     const bool active = q.i >= 0;
+    // the dependency columns' new values (from LDS: row 0 for the other entries, not used; from
+    // memory only where the entry is a dependency -- loads of row 0 there measured 8% slower)
+    VT xn[kEllMax][BS];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        const uint32_t c = (uint32_t)max(q.cq[k], 0);
+        if constexpr (LX) {
+            load_row<BS, VT>(xs + c * BS, xn[k]);
+        } else if (q.cq[k] >= 0) {
+            load_row<BS, VT>(at32<VT>(a.x, c * BS * (uint32_t)sizeof(VT)), xn[k]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < BS; ++e) xn[k][e] = VT(0);
+        }
+    }
     VT s;
-    if constexpr (SOR && BS * sizeof(VT) > 24) {
-        // (4 x 4 FP64 blocks: the registers of the form below spill at 1024 threads)
-#pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
+    if constexpr (SOR) {
+        // each entry's product with the new (dependency) or previous (other side) row, added in
+        // the row's order; without a previous iterate the other side's products are zero
         VT ph = VT(0);
 #pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] != -1) ph += row_dot<BS, VT>(q.av[k], q.xv[k]);
-        s = active ? q.s - ph : VT(0);
-    } else if constexpr (SOR) {
-        // the previous iterate's products first (their rows were loaded a level ago), which
-        // frees their registers, so the new values' loads all issue together; the products are
-        // then added in the row's order
-        VT rd[kEllMax];
+        for (int k = 0; k < KM; ++k) {
+            const bool dep = q.cq[k] >= 0;
+            VT xk[BS];
 #pragma unroll
-        for (int k = 0; k < kEllMax; ++k) rd[k] = q.cq[k] < -1 ? row_dot<BS, VT>(q.av[k], q.xv[k]) : VT(0);
-#pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
-#pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] >= 0) rd[k] = row_dot<BS, VT>(q.av[k], q.xv[k]);
-        VT ph = VT(0);
-#pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] != -1) ph += rd[k];
+            for (int c = 0; c < BS; ++c) xk[c] = dep ? xn[k][c] : q.xv[k][c];
+            VT pk = row_dot<BS, VT>(q.av[k], xk);
+            pk = (dep || have_xo) ? pk : VT(0);
+            ph = q.cq[k] != -1 ? ph + pk : ph;
+        }
         s = active ? q.s - ph : VT(0);
     } else {
-#pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] >= 0) load_row<BS, VT>(x + (size_t)q.cq[k] * BS, q.xv[k]);
         s = q.s;
 #pragma unroll
-        for (int k = 0; k < kEllMax; ++k)
-            if (q.cq[k] >= 0) s -= row_dot<BS, VT>(q.av[k], q.xv[k]);
+        for (int k = 0; k < KM; ++k) {
+            const VT pk = row_dot<BS, VT>(q.av[k], xn[k]);
+            s = q.cq[k] >= 0 ? s - pk : s;
+        }
     }
     VT out = s;
     if (have_dinv) {
         const int base = (g < RPW ? g : 0) * BS;
         out = VT(0);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) {
-            const VT sc = __shfl(s, base + c, kWave);
-            if (active) out = fma(q.d[c], sc, out);
-        }
+        for (int c = 0; c < BS; ++c) out = fma(q.d[c], block_lane<BS>(s, c, base + c), out);
     }
     if constexpr (SOR) {
         const VT w = (VT)a.w;
         out = q.xold + (-w * q.xold + w * out);
     }
     if (active) {
-        xg[(size_t)q.i * BS + r] = out;
-        if constexpr (LX) xs[(size_t)q.i * BS + r] = out;
+        // with the vector in LDS the rows go to memory after the last level
+        if constexpr (LX)
+            xs[(size_t)q.i * BS + r] = out;
+        else
+            static_cast<VT*>(a.x)[(size_t)q.i * BS + r] = out;
     }
 }
 
-template <int BS, typename VT, typename MT, bool SOR, bool LX>
+template <int BS, typename VT, typename MT, bool SOR, bool LX, int KM>
 __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, const int* __restrict__ level_ptr,
                                                                const int* __restrict__ ell_kl,
                                                                const int64_t* __restrict__ ell_off, int l0, int l1,
-                                                               int nx) {
-    // LX: the solution vector's rows written by this sweep are also kept in LDS (nx entries), and
-    // dependency columns are read from there; rows of levels before l0 are copied in first
+                                                               int nx, int whole) {
+    // LX: the solution vector's rows written by this sweep are kept in LDS (nx entries) and
+    // dependency columns are read from there; rows of levels before l0 are copied in first, and
+    // the rows of levels [l0, l1) are copied out after the last level (all nx entries when
+    // `whole`: the sweep covers every level)
     extern __shared__ __align__(16) unsigned char sweep_lds[];
     VT* xs = reinterpret_cast<VT*>(sweep_lds);
     if constexpr (LX) {
@@ -540,30 +610,41 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, cons
     const int off = (int)(threadIdx.x / kWave) * RPW + g;   // the lane group's slot within a level
     const bool grp = g < RPW;
     const bool have_dinv = a.dinv != nullptr;
+    const bool have_xo = a.xo != nullptr;
     if (l0 >= l1) return;
+    // the indices of level L (clamped to the last level; lanes of levels past it hold no slot)
+    auto idx = [&](int L, EllIdx& t) {
+        const int Lc = min(L, l1 - 1);
+        const int s0 = level_ptr[Lc], s1 = level_ptr[Lc + 1], K = ell_kl[Lc];
+        ell_idx_load<SOR, KM>(a, s0 + off, s0, grp && L < l1 && s0 + off < s1, K, (int)ell_off[Lc], t);
+    };
     EllIdx t;
     EllRow<BS, VT> q;
-    int s0 = level_ptr[l0], s1 = level_ptr[l0 + 1], K = ell_kl[l0];
-    ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l0] - (int64_t)s0 * K, t);
-    ell_row_load<BS, VT, MT, SOR>(a, t, q);
-    if (l0 + 1 < l1) {   // (s0, s1): the slots of the level whose indices t holds
-        s0 = s1;
-        s1 = level_ptr[l0 + 2];
-        K = ell_kl[l0 + 1];
-        ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l0 + 1] - (int64_t)s0 * K, t);
+    idx(l0, t);
+    ell_row_load<BS, VT, MT, SOR, KM>(a, t, q);
+    idx(l0 + 1, t);
+    for (int l = l0;; ++l) {
+        ell_row_finish<BS, VT, SOR, LX, KM>(a, q, have_dinv, have_xo, xs);
+        if (l + 1 >= l1) break;
+        ell_row_load<BS, VT, MT, SOR, KM>(a, t, q);
+        idx(l + 2, t);
+        __syncthreads();
     }
-    for (int l = l0; l < l1; ++l) {
-        ell_row_finish<BS, VT, SOR, LX>(a, q, have_dinv, xs);
-        if (l + 1 < l1) {
-            ell_row_load<BS, VT, MT, SOR>(a, t, q);
-            if (l + 2 < l1) {
-                s0 = s1;
-                s1 = level_ptr[l + 3];
-                K = ell_kl[l + 2];
-                ell_idx_load<SOR>(a, s0 + off, s0, grp && s0 + off < s1, K, ell_off[l + 2] - (int64_t)s0 * K, t);
+    if constexpr (LX) {
+        __syncthreads();
+        VT* xg = static_cast<VT*>(a.x);
+        if (whole) {
+            for (int e = threadIdx.x; e < nx; e += blockDim.x) xg[e] = xs[e];
+        } else {
+            const int sb = level_ptr[l0];
+            const int ne = (level_ptr[l1] - sb) * BS;
+            for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+                const int i = a.order[sb + e / BS];   // -1: a padding slot of the level
+                if (i < 0) continue;
+                const size_t j = (size_t)i * BS + e % BS;
+                xg[j] = xs[j];
             }
         }
-        __syncthreads();
     }
 }
 
@@ -582,22 +663,34 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
         constexpr int SPP = (kOneBlock / kWave) * (kWave / BS);
         bool ahead = false;
         if constexpr (BS <= kSorEllMaxBs) {
-            // (ELL entry indices are held as int)
-            if (a.ecol && S.max_level_slots <= SPP && ctx->sweep_ahead && S.ell_off.back() < (int64_t(1) << 31)) {
+            // (ELL entry indices are held as int; operand byte offsets as 32-bit)
+            const int64_t blocks = std::max({S.nnzb, S.ell_off.back(), (int64_t)S.n, (int64_t)S.nslots});
+            if (a.ecol && S.max_level_slots <= SPP && ctx->sweep_ahead && S.ell_off.back() < (int64_t(1) << 31) &&
+                blocks * BS * BS * (int64_t)sizeof(double) < (int64_t(1) << 32)) {
                 ahead = true;
                 // as many wavefronts as the widest level needs (idle ones would only add loads)
                 const int waves = std::max(1, (S.max_level_slots + kWave / BS - 1) / (kWave / BS));
                 const int nx = S.n * BS;
                 const int64_t lds = (int64_t)nx * sizeof(VT);
                 const int lend = l1 < 0 ? S.nlevels : l1;
-                if (ctx->sweep_lds && lds <= std::min<int64_t>(ctx->lds_per_block, 65536))
-                    hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR, true>), dim3(1), dim3(waves * kWave),
-                                       (size_t)lds, ctx->stream, a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(),
-                                       S.dell_off.as<int64_t>(), l0, lend, nx);
-                else
-                    hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR, false>), dim3(1), dim3(waves * kWave), 0,
-                                       ctx->stream, a, S.dlevel_ptr.as<int>(), S.dell_kl.as<int>(),
-                                       S.dell_off.as<int64_t>(), l0, lend, nx);
+                const int whole = l0 == 0 && lend == S.nlevels;
+                const bool lx = ctx->sweep_lds && lds <= std::min<int64_t>(ctx->lds_per_block, 65536);
+                // levels at most 2 entries wide (a triangular factor of a 2D stencil) take the
+                // kernel that holds 2 entries per slot
+                auto go = [&](auto lxc, auto kmc) {
+                    constexpr bool LXc = decltype(lxc)::value;
+                    constexpr int KMc = decltype(kmc)::value;
+                    hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR, LXc, KMc>), dim3(1), dim3(waves * kWave),
+                                       LXc ? (size_t)lds : 0, ctx->stream, a, S.dlevel_ptr.as<int>(),
+                                       S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0, lend, nx, whole);
+                };
+                using T2 = std::integral_constant<int, 2>;
+                using T4 = std::integral_constant<int, kEllMax>;
+                if (lx) {
+                    if (S.ell_k <= 2) go(std::true_type{}, T2{}); else go(std::true_type{}, T4{});
+                } else {
+                    if (S.ell_k <= 2) go(std::false_type{}, T2{}); else go(std::false_type{}, T4{});
+                }
             }
         }
         if (!ahead)
