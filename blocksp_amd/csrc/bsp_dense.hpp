@@ -387,9 +387,10 @@ inline void hessenberg_q(int n, std::vector<double>& H, std::vector<double>& Q) 
     for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
     std::vector<double> v(n);
     for (int k = 0; k + 2 < n; ++k) {
-        double nrm2 = 0.0;
-        for (int i = k + 1; i < n; ++i) nrm2 += H[(size_t)i * n + k] * H[(size_t)i * n + k];
-        if (nrm2 == 0.0) continue;
+        double tail = 0.0;   // column k below the subdiagonal: nothing to reduce when zero
+        for (int i = k + 2; i < n; ++i) tail += H[(size_t)i * n + k] * H[(size_t)i * n + k];
+        if (tail == 0.0) continue;
+        const double nrm2 = tail + H[(size_t)(k + 1) * n + k] * H[(size_t)(k + 1) * n + k];
         const double x0 = H[(size_t)(k + 1) * n + k];
         const double alpha = x0 > 0 ? -std::sqrt(nrm2) : std::sqrt(nrm2);
         for (int i = 0; i < n; ++i) v[i] = 0.0;
@@ -529,31 +530,44 @@ inline bool hqr_eigs(int n, std::vector<double>& H, std::vector<cplx>& ev) {
 }
 
 // Solve (H - theta I) x = b for upper Hessenberg H (complex shift) by Gaussian elimination
-// with partial pivoting between adjacent rows; tiny pivots are replaced by eps * scale.
-inline void hess_shift_solve(int n, const std::vector<double>& H, cplx theta, std::vector<cplx>& x, double scale) {
-    std::vector<cplx> M((size_t)n * n);
+// with partial pivoting between adjacent rows; tiny pivots are replaced by eps * scale.  The
+// complex arithmetic is written out on real and imaginary parts (one real division per
+// pivot, no library calls), and `M` is the caller's workspace of n * n entries.
+inline void hess_shift_solve(int n, const std::vector<double>& H, cplx theta, std::vector<cplx>& x, double scale,
+                             std::vector<cplx>& M) {
+    M.resize((size_t)n * n);
+    auto m = [&](int i, int j) -> cplx& { return M[(size_t)i * n + j]; };
+    // only the Hessenberg part is read: row i from column i - 1 on
     for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) M[(size_t)i * n + j] = H[(size_t)i * n + j] - (i == j ? theta : cplx(0.0));
+        for (int j = std::max(i - 1, 0); j < n; ++j)
+            m(i, j) = cplx(H[(size_t)i * n + j] - (i == j ? theta.real() : 0.0), i == j ? -theta.imag() : 0.0);
     const double tiny = std::numeric_limits<double>::epsilon() * scale;
+    const double tiny2 = tiny * tiny;
+    auto nrm2 = [](cplx z) { return z.real() * z.real() + z.imag() * z.imag(); };
+    auto mul = [](cplx a, cplx b) {
+        return cplx(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+    };
+    auto inv = [&](cplx z) {
+        const double d = nrm2(z);
+        return cplx(z.real() / d, -z.imag() / d);
+    };
     for (int k = 0; k + 1 < n; ++k) {
-        cplx& a = M[(size_t)k * n + k];
-        cplx& b = M[(size_t)(k + 1) * n + k];
-        if (std::abs(b) > std::abs(a)) {
-            for (int j = k; j < n; ++j) std::swap(M[(size_t)k * n + j], M[(size_t)(k + 1) * n + j]);
+        if (nrm2(m(k + 1, k)) > nrm2(m(k, k))) {
+            for (int j = k; j < n; ++j) std::swap(m(k, j), m(k + 1, j));
             std::swap(x[k], x[k + 1]);
         }
-        if (std::abs(M[(size_t)k * n + k]) < tiny) M[(size_t)k * n + k] = tiny;
-        const cplx f = M[(size_t)(k + 1) * n + k] / M[(size_t)k * n + k];
+        if (nrm2(m(k, k)) < tiny2) m(k, k) = tiny;
+        const cplx f = mul(m(k + 1, k), inv(m(k, k)));
         if (f != cplx(0.0)) {
-            for (int j = k; j < n; ++j) M[(size_t)(k + 1) * n + j] -= f * M[(size_t)k * n + j];
-            x[k + 1] -= f * x[k];
+            for (int j = k; j < n; ++j) m(k + 1, j) -= mul(f, m(k, j));
+            x[k + 1] -= mul(f, x[k]);
         }
     }
-    if (std::abs(M[(size_t)(n - 1) * n + n - 1]) < tiny) M[(size_t)(n - 1) * n + n - 1] = tiny;
+    if (nrm2(m(n - 1, n - 1)) < tiny2) m(n - 1, n - 1) = tiny;
     for (int i = n - 1; i >= 0; --i) {
         cplx s = x[i];
-        for (int j = i + 1; j < n; ++j) s -= M[(size_t)i * n + j] * x[j];
-        x[i] = s / M[(size_t)i * n + i];
+        for (int j = i + 1; j < n; ++j) s -= mul(m(i, j), x[j]);
+        x[i] = mul(s, inv(m(i, i)));
     }
 }
 
@@ -578,6 +592,7 @@ inline int invariant_subspace_fast(int n, const double* A, int k, std::vector<do
     if (scale == 0.0) scale = 1.0;
     std::vector<std::vector<double>> cand;
     std::vector<char> done(n, 0);
+    std::vector<cplx> work;
     const double zt = 10.0 * n * n * std::numeric_limits<double>::epsilon();
     for (int i = 0; i < n; ++i) {
         if (!sel[i] || done[i]) continue;
@@ -592,7 +607,7 @@ inline int invariant_subspace_fast(int n, const double* A, int k, std::vector<do
                 }
         std::vector<cplx> x(n, cplx(1.0));
         for (int it = 0; it < 2; ++it) {
-            hess_shift_solve(n, H, ev[i], x, scale);
+            hess_shift_solve(n, H, ev[i], x, scale, work);
             double nr = 0.0;
             for (auto& e : x) nr += std::norm(e);
             nr = std::sqrt(nr);

@@ -157,7 +157,11 @@ int fgmres_pass(bsp_ctx* ctx, const bsp_bsr* A, const bsp_precond* M, const doub
                 if (x) (void)hipEventDestroy(x);
         }
     } ev;
-    for (hipEvent_t& x : ev.e) BSP_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    // e[0..1] only order the second stream after the main one on this device: no system-scope
+    // fence (host-visible results are written to pinned memory by the kernels themselves, and
+    // e[2..3], which the host waits on, keep the fence); measured 6.6 -> 5.1 us between steps
+    for (int i = 0; i < 4; ++i)
+        BSP_HIP(hipEventCreateWithFlags(&ev.e[i], hipEventDisableTiming | (i < 2 ? hipEventDisableSystemFence : 0)));
     // the least-squares updates run on a second stream: the next Arnoldi step does not need them,
     // only the host (the residual estimate) and the next update do
     if (!ctx->aux_stream) BSP_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
