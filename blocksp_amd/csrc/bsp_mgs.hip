@@ -230,8 +230,8 @@ struct Tile2 {
 __global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg_reg(int64_t len, double* __restrict__ v, QList Q,
                                                                        int j, int nparts, double* __restrict__ hcol,
                                                                        double* __restrict__ qnew) {
-    __shared__ double sh[kOneGroups][kBlock / kWave];
-    __shared__ double part[2][kOneGroups];
+    // per wavefront sums of a step, double-buffered: one barrier per step
+    __shared__ double shw[2][kOneGroups * (kBlock / kWave)];
     const int grp = threadIdx.x / kBlock, tid = threadIdx.x % kBlock;
     int64_t base[2];
     bool own[2];
@@ -249,10 +249,19 @@ __global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg_reg(int64_t l
             cur[tt] = ld(Q.q[0], tt);
         }
     }
-    auto sum_parts = [&](const double* pin) {
-        double s = 0.0;
-        for (int b = tid; b < nparts; b += kBlock) s += pin[b];
-        return group_sum(s, sh[grp], tid);
+    // The step's sum from the wavefront sums, in the order k_mgs2 / k_mgs_scale take: each group's
+    // four wavefront sums in order (block_sum), then the group sums as the 64-lane xor butterfly of
+    // block_sum adds them -- values first plus +0 (the zeros of the idle lanes), then (p0 + p2) +
+    // (p1 + p3) -- so the result has the same bits as the launch per step.
+    auto sum_parts = [&](const double* w) {
+        double p[kOneGroups];
+#pragma unroll
+        for (int g = 0; g < kOneGroups; ++g) {
+            const double* x = w + g * (kBlock / kWave);
+            p[g] = g < nparts ? ((x[0] + x[1]) + x[2]) + x[3] : 0.0;
+            p[g] = p[g] + 0.0;
+        }
+        return (p[0] + p[2]) + (p[1] + p[3]);
     };
     for (int st = 0; st <= j + 1; ++st) {
         if (st + 1 <= j) {
@@ -262,7 +271,7 @@ __global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg_reg(int64_t l
         }
         double mh = 0.0;
         if (st > 0) {
-            const double h = sum_parts(part[(st + 1) & 1]);
+            const double h = sum_parts(shw[(st + 1) & 1]);
             if (grp == 0 && tid == 0) hcol[st - 1] = h;
             mh = -h;
         }
@@ -292,8 +301,10 @@ __global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg_reg(int64_t l
                 s1 = fma(a1.y, a1.y, s1);
             }
         }
-        const double ps = group_sum(s0 + s1, sh[grp], tid);
-        if (tid == 0) part[st & 1][grp] = ps;
+        double ps = s0 + s1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) ps += __shfl_xor(ps, o, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0) shw[st & 1][threadIdx.x / kWave] = ps;
         __syncthreads();
 #pragma unroll
         for (int tt = 0; tt < 2; ++tt) {
@@ -301,7 +312,7 @@ __global__ __launch_bounds__(kBlock * kOneGroups) void k_mgs_onewg_reg(int64_t l
             cur[tt] = nxt[tt];
         }
     }
-    const double h = sqrt(fabs(sum_parts(part[(j + 1) & 1])));
+    const double h = sqrt(fabs(sum_parts(shw[(j + 1) & 1])));
     if (grp == 0 && tid == 0) hcol[j + 1] = h;
 #pragma unroll
     for (int tt = 0; tt < 2; ++tt) {
