@@ -95,6 +95,8 @@ struct Sweep {
     DevMem epos;
     int sor_forward = -1;
     DevMem dell_kl, dell_off;    // ell_kl / ell_off on the device (one-workgroup sweeps)
+    DevMem dlevel_info;          // int4 per level: first slot, end slot, ELL width, first entry (when
+                                 // the entries fit int; the one-workgroup ELL sweeps, one load a level)
     DevMem flags;                // uint32 [n], epoch-stamped completion flags
     DevMem ticket;               // uint64 workgroup ticket counter
     // level-launch sweeps replayed from captured HIP graphs, keyed by their operands

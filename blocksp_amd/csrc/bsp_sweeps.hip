@@ -544,10 +544,8 @@ __device__ __forceinline__ void ell_row_finish(const SweepArgs& a, const EllRow<
 }
 
 template <int BS, typename VT, typename MT, bool SOR, bool LX, int KM>
-__global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, const int* __restrict__ level_ptr,
-                                                               const int* __restrict__ ell_kl,
-                                                               const int64_t* __restrict__ ell_off, int l0, int l1,
-                                                               int nx, int whole) {
+__global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, const int4* __restrict__ level_info,
+                                                               int l0, int l1, int nx, int whole) {
     // LX: the solution vector's rows written by this sweep are kept in LDS (nx entries) and
     // dependency columns are read from there; rows of levels before l0 are copied in first, and
     // the rows of levels [l0, l1) are copied out after the last level (all nx entries when
@@ -571,9 +569,8 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, cons
     if (l0 >= l1) return;
     // the indices of level L (clamped to the last level; lanes of levels past it hold no slot)
     auto idx = [&](int L, EllIdx& t) {
-        const int Lc = min(L, l1 - 1);
-        const int s0 = level_ptr[Lc], s1 = level_ptr[Lc + 1], K = ell_kl[Lc];
-        ell_idx_load<SOR, KM>(a, s0 + off, s0, grp && L < l1 && s0 + off < s1, K, (int)ell_off[Lc], t);
+        const int4 li = level_info[min(L, l1 - 1)];   // first slot, end slot, width, first entry
+        ell_idx_load<SOR, KM>(a, li.x + off, li.x, grp && L < l1 && li.x + off < li.y, li.z, li.w, t);
     };
     EllIdx t;
     EllRow<BS, VT> q;
@@ -593,8 +590,8 @@ __global__ __launch_bounds__(kOneBlock) void k_sweep_onewg_ell(SweepArgs a, cons
         if (whole) {
             for (int e = threadIdx.x; e < nx; e += blockDim.x) xg[e] = xs[e];
         } else {
-            const int sb = level_ptr[l0];
-            const int ne = (level_ptr[l1] - sb) * BS;
+            const int sb = level_info[l0].x;
+            const int ne = (level_info[l1 - 1].y - sb) * BS;
             for (int e = threadIdx.x; e < ne; e += blockDim.x) {
                 const int i = a.order[sb + e / BS];   // -1: a padding slot of the level
                 if (i < 0) continue;
@@ -621,8 +618,10 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
         bool ahead = false;
         if constexpr (BS <= kSorEllMaxBs) {
             // (ELL entry indices are held as int; operand byte offsets as 32-bit)
-            const int64_t blocks = std::max({S.nnzb, S.ell_off.back(), (int64_t)S.n, (int64_t)S.nslots});
-            if (a.ecol && S.max_level_slots <= SPP && ctx->sweep_ahead && S.ell_off.back() < (int64_t(1) << 31) &&
+            // (ell_off is empty when the sweep has no ELL copy)
+            const int64_t entries = S.ell_off.empty() ? 0 : S.ell_off.back();
+            const int64_t blocks = std::max({S.nnzb, entries, (int64_t)S.n, (int64_t)S.nslots});
+            if (a.ecol && S.dlevel_info.p && S.max_level_slots <= SPP && ctx->sweep_ahead &&
                 blocks * BS * BS * (int64_t)sizeof(double) < (int64_t(1) << 32)) {
                 ahead = true;
                 // as many wavefronts as the widest level needs (idle ones would only add loads)
@@ -638,8 +637,8 @@ int run_sweep(bsp_ctx* ctx, Sweep& S, SweepArgs a, int l0, int l1) {
                     constexpr bool LXc = decltype(lxc)::value;
                     constexpr int KMc = decltype(kmc)::value;
                     hipLaunchKernelGGL((k_sweep_onewg_ell<BS, VT, MT, SOR, LXc, KMc>), dim3(1), dim3(waves * kWave),
-                                       LXc ? (size_t)lds : 0, ctx->stream, a, S.dlevel_ptr.as<int>(),
-                                       S.dell_kl.as<int>(), S.dell_off.as<int64_t>(), l0, lend, nx, whole);
+                                       LXc ? (size_t)lds : 0, ctx->stream, a, S.dlevel_info.as<int4>(), l0, lend,
+                                       nx, whole);
                 };
                 using T2 = std::integral_constant<int, 2>;
                 using T4 = std::integral_constant<int, kEllMax>;
@@ -832,6 +831,15 @@ hipError_t set_ell_levels(Sweep& S, const std::vector<int>& kl, int64_t& entries
     if (e == hipSuccess) e = hipMemcpy(S.dell_kl.p, kl.data(), sizeof(int) * kl.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(S.dell_off.p, S.ell_off.data(), sizeof(int64_t) * S.ell_off.size(), hipMemcpyHostToDevice);
+    S.dlevel_info.release();
+    if (e == hipSuccess && S.ell_off.back() < (int64_t(1) << 31)) {
+        std::vector<int4> info(S.nlevels);
+        for (int l = 0; l < S.nlevels; ++l)
+            info[l] = make_int4(S.level_ptr[l], S.level_ptr[l + 1], kl[l], (int)S.ell_off[l]);
+        e = S.dlevel_info.alloc(sizeof(int4) * std::max(S.nlevels, 1));
+        if (e == hipSuccess && S.nlevels > 0)
+            e = hipMemcpy(S.dlevel_info.p, info.data(), sizeof(int4) * info.size(), hipMemcpyHostToDevice);
+    }
     return e;
 }
 
