@@ -10,6 +10,8 @@
 // (blocksp_multiply.hpp:289-325, blocksp_dense_multiply.hpp:29-48), with FMA.
 #include "bsp_internal.hpp"
 
+#include <utility>
+
 #include <algorithm>
 
 namespace bsp {
@@ -765,6 +767,45 @@ __global__ __launch_bounds__(kBlock) void k_combine_add(int64_t len, int m, PtrT
 // Qout_j = sum_i Q_i P_ij (deflated_restart.hpp:250-258), m <= 64, kp <= 64.  Two consecutive
 // entries per thread (16-byte loads and stores), the basis read once per 16 output vectors,
 // four input vectors' loads issued before their FMAs.
+// Qout_j = sum_i Q_i P_ij for kp = JN <= 16 output vectors, two entries per thread, eight input
+// vectors' loads issued before their FMAs, P read with uniform (scalar) loads: the same sums in
+// the same order as k_basis_rotate, without its LDS staging and per-output masks.  len is even
+// and every vector 16-byte aligned.
+template <int JN>
+__global__ __launch_bounds__(kBlock) void k_basis_rotate_jn(int64_t len, int m, PtrTable Q,
+                                                            const double* __restrict__ P, OutTable Out) {
+    const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 2;
+    if (e >= len) return;
+    double a0[JN], a1[JN];
+#pragma unroll
+    for (int j = 0; j < JN; ++j) a0[j] = a1[j] = 0.0;
+    int i = 0;
+    for (; i + 8 <= m; i += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(Q.q[i + u] + e);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int j = 0; j < JN; ++j) {
+                const double pv = P[(i + u) * JN + j];
+                a0[j] = fma(v[u].x, pv, a0[j]);
+                a1[j] = fma(v[u].y, pv, a1[j]);
+            }
+    }
+    for (; i < m; ++i) {
+        const double2 v = *reinterpret_cast<const double2*>(Q.q[i] + e);
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const double pv = P[i * JN + j];
+            a0[j] = fma(v.x, pv, a0[j]);
+            a1[j] = fma(v.y, pv, a1[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JN; ++j) *reinterpret_cast<double2*>(Out.q[j] + e) = make_double2(a0[j], a1[j]);
+}
+
 __global__ __launch_bounds__(kBlock) void k_basis_rotate(int64_t len, int m, int kp, PtrTable Q,
                                                          const double* __restrict__ P, OutTable Out, int vec) {
     __shared__ double sP[kMaxBasis * 16];
@@ -1142,6 +1183,22 @@ int launch_basis_rotate(const bsp_ctx* ctx, int64_t len, int m, int kp, const do
     for (int j = 0; j < kp; ++j) {
         o.q[j] = Qout[j];
         vec = vec && (uintptr_t)Qout[j] % 16 == 0;
+    }
+    if (vec && len % 2 == 0 && kp >= 1 && kp <= 16) {
+        bool done = false;
+        auto go = [&](auto jn) {
+            constexpr int JN = decltype(jn)::value;
+            if (JN != kp) return;
+            hipLaunchKernelGGL(k_basis_rotate_jn<JN>, dim3(grid_for(len / 2)), dim3(kBlock), 0, ctx->stream, len, m,
+                               t, P, o);
+            done = true;
+        };
+        [&]<int... J>(std::integer_sequence<int, J...>) { (go(std::integral_constant<int, J + 1>{}), ...); }(
+            std::make_integer_sequence<int, 16>{});
+        if (done) {
+            BSP_LAUNCH_CHECK();
+            return BSP_OK;
+        }
     }
     hipLaunchKernelGGL(k_basis_rotate, dim3(grid_for((len + 1) / 2)), dim3(kBlock), 0, ctx->stream, len, m, kp, t, P, o,
                        vec ? 1 : 0);

@@ -8,6 +8,8 @@
 //    FP32 basis vectors Z_j (the solution update of flexible GMRES).
 #include "bsp_internal.hpp"
 
+#include <utility>
+
 #include <algorithm>
 
 namespace bsp {
@@ -283,6 +285,44 @@ __global__ __launch_bounds__(kBlock) void k_combine_add_f32(int64_t len, int m, 
     }
 }
 
+// The same for kp = JN <= 16 outputs: eight input vectors' loads issued before their FMAs, P
+// read with uniform (scalar) loads, the sums in k_basis_rotate_f32's order; len even, vectors
+// 8-byte aligned.
+template <int JN>
+__global__ __launch_bounds__(kBlock) void k_basis_rotate_f32_jn(int64_t len, int m, FPtrs Z,
+                                                                const double* __restrict__ P, FOut Out) {
+    const int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 2;
+    if (e >= len) return;
+    double a0[JN], a1[JN];
+#pragma unroll
+    for (int j = 0; j < JN; ++j) a0[j] = a1[j] = 0.0;
+    int i = 0;
+    for (; i + 8 <= m; i += 8) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float2*>(Z.q[i + u] + e);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int j = 0; j < JN; ++j) {
+                const double pv = P[(i + u) * JN + j];
+                a0[j] = fma((double)v[u].x, pv, a0[j]);
+                a1[j] = fma((double)v[u].y, pv, a1[j]);
+            }
+    }
+    for (; i < m; ++i) {
+        const float2 v = *reinterpret_cast<const float2*>(Z.q[i] + e);
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const double pv = P[i * JN + j];
+            a0[j] = fma((double)v.x, pv, a0[j]);
+            a1[j] = fma((double)v.y, pv, a1[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JN; ++j) *reinterpret_cast<float2*>(Out.q[j] + e) = make_float2((float)a0[j], (float)a1[j]);
+}
+
 // Zout_j = sum_i Z_i P_ij (FP64 accumulation, FP32 storage), m, kp <= 64; two entries per
 // thread, four input vectors' loads issued before their FMAs.
 __global__ __launch_bounds__(kBlock) void k_basis_rotate_f32(int64_t len, int m, int kp, FPtrs Z,
@@ -482,6 +522,22 @@ int launch_basis_rotate_f32(const bsp_ctx* ctx, int64_t len, int m, int kp, cons
     for (int j = 0; j < kp; ++j) {
         o.q[j] = Zout[j];
         vec = vec && (uintptr_t)Zout[j] % 8 == 0;
+    }
+    if (vec && len % 2 == 0 && kp >= 1 && kp <= 16) {
+        bool done = false;
+        auto go = [&](auto jn) {
+            constexpr int JN = decltype(jn)::value;
+            if (JN != kp) return;
+            hipLaunchKernelGGL(k_basis_rotate_f32_jn<JN>, dim3(grid_for(len / 2)), dim3(kBlock), 0, ctx->stream, len,
+                               m, t, P, o);
+            done = true;
+        };
+        [&]<int... J>(std::integer_sequence<int, J...>) { (go(std::integral_constant<int, J + 1>{}), ...); }(
+            std::make_integer_sequence<int, 16>{});
+        if (done) {
+            hipError_t e = hipGetLastError();
+            return e == hipSuccess ? BSP_OK : fail(ctx, BSP_E_HIP, hipGetErrorString(e));
+        }
     }
     hipLaunchKernelGGL(k_basis_rotate_f32, dim3(grid_for((len + 1) / 2)), dim3(kBlock), 0, ctx->stream, len, m, kp, t,
                        P, o, vec ? 1 : 0);
