@@ -265,12 +265,27 @@ __device__ __forceinline__ void sweep_chunk(const SweepArgs& a, int slot_begin, 
             }
         } else if (!a.ecol) {
             s = x[(size_t)i * BS + r];
-            for (int p = p0; p < p1; ++p) {
-                const int c = a.ci[p];
-                VT av[BS], xv[BS];
-                load_mrow<BS, MT, VT>(val + ((size_t)p * BS + r) * BS, av);
-                load_row<BS, VT>(x + (size_t)c * BS, xv);
-                s -= row_dot<BS, VT>(av, xv);
+            // NB entries at a time: their columns, blocks and vector rows load together (one
+            // load latency per NB entries instead of one per entry), then the products are
+            // subtracted in the row's order.  FP64 only: the natural-order FP64 ILU0 of the 512^2
+            // DG matrix measured 4% faster, the BF16 sweeps of its coloured factor (rows of one or
+            // two entries) 25% slower (13.8 -> 17.3 us a level, the registers cost occupancy).
+            constexpr int W = 2 * BS * (int)sizeof(VT) / 4;   // registers an entry holds
+            constexpr int NB = !std::is_same_v<VT, double> || W >= 64 ? 1 : (64 / W >= 4 ? 4 : 64 / W);
+            for (int p = p0; p < p1; p += NB) {
+                int c[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) c[u] = p + u < p1 ? a.ci[p + u] : -1;
+                VT av[NB][BS], xv[NB][BS];
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+                    if (c[u] >= 0) {
+                        load_mrow<BS, MT, VT>(val + ((size_t)(p + u) * BS + r) * BS, av[u]);
+                        load_row<BS, VT>(x + (size_t)c[u] * BS, xv[u]);
+                    }
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+                    if (c[u] >= 0) s -= row_dot<BS, VT>(av[u], xv[u]);
             }
         }
     }
